@@ -1,0 +1,174 @@
+/*
+ * nii2mesh_amd.h -- C ABI of the MI355X (gfx950) voxel-to-mesh library.
+ *
+ * The first block declares the reference's own entry points with the same
+ * prototypes and semantics, so the nii2mesh CLI, obj2mesh and the mesh
+ * writers link against this library unchanged (see the per-symbol citations).
+ * Everything runs on the GPU; a missing or failing device is reported as
+ * EXIT_FAILURE (or, for functions without a status, a message on stderr),
+ * never silently computed on the CPU.
+ *
+ * The second block (n2m_*) is new: stage entry points used by parity tests,
+ * a device-resident context API (inputs already in HBM, no PCIe in the timed
+ * path), the batched atlas entry and the adjoint.
+ */
+#ifndef NII2MESH_AMD_H
+#define NII2MESH_AMD_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+/* Same layout and include guard as the reference's src/meshtypes.h:1-11, so a
+ * translation unit may include both headers. */
+#ifndef MESHIFY_TYPES_H
+#define MESHIFY_TYPES_H
+typedef struct {
+	double x, y, z;
+} vec3d;
+typedef struct {
+	int x, y, z;
+} vec3i;
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Drop-in entry points (reference prototypes)
+ * ------------------------------------------------------------------------- */
+
+/* Replaces src/meshify.h:9 meshify() (implementation src/meshify.c:286-389):
+ * pre-smooth, range check, largest cluster / bubble fill, dilate, edge darken,
+ * marching cubes, vertex welding and degenerate-triangle removal.  img is
+ * mutated in place like the reference.  *t and *p are malloc'd; the caller
+ * frees them.  originalMC selects the Bourke table inside the Lewiner
+ * algorithm (src/MarchingCubes.c:471-477); which marching-cubes algorithm
+ * runs (classic src/oldcubes.c or Lewiner src/MarchingCubes.c, a compile-time
+ * choice in the reference, src/meshify.c:25-29) is chosen by
+ * n2m_set_cubes() or the N2M_CUBES environment variable ("classic" default,
+ * "lewiner"). */
+int meshify(float *img, short dim[3], int originalMC, float isolevel, vec3i **t, vec3d **p,
+            int *nt, int *np, bool preSmooth, bool onlyLargest, bool fillBubbles, bool verbose);
+
+/* Replaces src/quadric.h:7 laplacian_smoothHC() (src/quadric.c:343-394):
+ * HC Laplacian smoothing in FP64, border vertices reset when lockEdges. */
+void laplacian_smoothHC(vec3d *verts, vec3i *tris, int nvert, int ntri, double alpha, double beta,
+                        int iter, bool lockEdges);
+
+/* Replaces src/bwlabel.h:10 bwlabel() (src/bwlabel.c:478-543): connected
+ * components (conn 6/18/26) numbered by first voxel in x-fastest order,
+ * optional largest-cluster selection and bubble fill.  Returns the number of
+ * labels (1 after onlyLargest), 0 on error. */
+int bwlabel(float *img, int conn, size_t dim[3], bool onlyLargest, bool fillBubbles);
+
+/* Replaces src/oldcubes.h:9 / src/MarchingCubes.h:19 marchingCubes(): the
+ * flavour follows n2m_set_cubes().  Classic emits 3 private vertices per
+ * triangle (src/oldcubes.c:465-522); Lewiner shares edge vertices
+ * (src/MarchingCubes.c:1086-1143). */
+int marchingCubes(float *img, short dim[3], int lo[3], int hi[3], int originalMC, float isolevel,
+                  vec3d **vs, vec3i **ts, int *nv, int *nt);
+
+/* Replaces src/isolevel.h:10 setThreshold() (src/isolevel.c:245-277):
+ * histograms on the GPU, multi-Otsu search on the host. */
+float setThreshold(float *img, int nvox, int darkMediumBright123);
+
+/* ---------------------------------------------------------------------------
+ * Configuration and diagnostics
+ * ------------------------------------------------------------------------- */
+#define N2M_CUBES_CLASSIC 0 /* src/oldcubes.c */
+#define N2M_CUBES_LEWINER 1 /* src/MarchingCubes.c */
+
+/* Select the marching-cubes algorithm used by meshify()/marchingCubes() in
+ * this process.  Returns the previous setting. */
+int n2m_set_cubes(int flavour);
+int n2m_get_cubes(void);
+/* Text of the last error on the calling thread ("" if none). */
+const char *n2m_last_error(void);
+/* 1 if a gfx9 device is usable, else 0 (and n2m_last_error says why). */
+int n2m_device_ok(void);
+
+/* ---------------------------------------------------------------------------
+ * Stage entry points on host buffers (one H2D, one D2H each); each names the
+ * reference function whose result it reproduces.
+ * ------------------------------------------------------------------------- */
+
+/* src/meshify.c:170-216 quick_smooth(): returns EXIT_FAILURE (image
+ * untouched) if any dimension is < 5. */
+int n2m_quick_smooth(float *img, int nx, int ny, int nz);
+
+/* src/meshify.c:45-106 unify_vertices(): welds *pts in place (realloc'd with
+ * malloc), remaps tris, returns the new vertex count. */
+int n2m_unify_vertices(vec3d **pts, vec3i *tris, int npt, int ntri);
+
+/* src/meshify.c:113-168 remove_degenerate_triangles(): compacts *tris
+ * (re-malloc'd when triangles are removed), returns the new count. */
+int n2m_remove_degenerate_triangles(vec3d *pts, vec3i **tris, int ntri);
+
+/* src/quadric.c:316-341 laplacian_smooth(), applied iter times. */
+void n2m_laplacian_smooth(vec3d *verts, vec3i *tris, int nvert, int ntri, int iter);
+
+/* src/meshify.c:1025-1045 apply_sform() on the GPU (FP64). */
+void n2m_apply_sform(vec3i *t, vec3d *p, int nt, int np, float srow_x[4], float srow_y[4],
+                     float srow_z[4]);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident context: one HIP stream, reusable device buffers.
+ * All d_* pointers are device memory on the context's device.
+ * ------------------------------------------------------------------------- */
+typedef struct n2m_ctx n2m_ctx;
+
+typedef struct {
+	int cubes;        /* N2M_CUBES_CLASSIC or N2M_CUBES_LEWINER */
+	int originalMC;   /* Lewiner only: Bourke table */
+	float isolevel;
+	int preSmooth;    /* bool */
+	int onlyLargest;  /* bool */
+	int fillBubbles;  /* bool */
+	int verbose;      /* bool: print the reference's stage lines */
+} n2m_params;
+
+n2m_ctx *n2m_ctx_create(void);
+void n2m_ctx_destroy(n2m_ctx *ctx);
+/* hipStream_t of the context (as void*). */
+void *n2m_ctx_stream(n2m_ctx *ctx);
+
+/* meshify() on a device volume (mutated in place, dims x-fastest).  The mesh
+ * stays on the device in the context.  Returns EXIT_SUCCESS/EXIT_FAILURE
+ * exactly where meshify() would. */
+int n2m_ctx_meshify(n2m_ctx *ctx, float *d_img, const int dim[3], const n2m_params *prm);
+
+/* Multi-isolevel pipeline: pre-smooth once (if preSmooth), then for each of
+ * niso isolevels run the rest of meshify() on a private copy of the smoothed
+ * volume followed by HC smoothing (alpha, beta, hcIter, lockEdges=true).
+ * Meshes are kept in the context (mesh index 0..niso-1).  d_img is not
+ * modified.  Equal to calling meshify()+laplacian_smoothHC() per isolevel on
+ * copies of the input volume. */
+int n2m_ctx_pipeline(n2m_ctx *ctx, const float *d_img, const int dim[3], const n2m_params *prm,
+                     int niso, const float *isolevels, double alpha, double beta, int hcIter);
+
+/* GPU histograms + host multi-Otsu on a device volume (setThreshold()). */
+float n2m_ctx_set_threshold(n2m_ctx *ctx, const float *d_img, int64_t nvox, int darkMediumBright123);
+
+/* HC smoothing of mesh `mesh` held by the context. */
+int n2m_ctx_smooth_hc(n2m_ctx *ctx, int mesh, double alpha, double beta, int iter, int lockEdges);
+
+int n2m_ctx_mesh_count(n2m_ctx *ctx);
+int n2m_ctx_mesh_size(n2m_ctx *ctx, int mesh, int *nt, int *np);
+/* Copy mesh `mesh` to caller-allocated host arrays (nt / np entries). */
+int n2m_ctx_download(n2m_ctx *ctx, int mesh, vec3i *t, vec3d *p);
+/* Device pointers of mesh `mesh` (valid until the next run on the context). */
+int n2m_ctx_device_mesh(n2m_ctx *ctx, int mesh, const vec3i **d_t, const vec3d **d_p);
+
+/* Per-kernel timing with HIP events on the context stream. */
+void n2m_ctx_enable_timing(n2m_ctx *ctx, int on);
+void n2m_ctx_reset_timing(n2m_ctx *ctx);
+/* Returns the number of timed kernels; fills entry idx if idx < count. */
+int n2m_ctx_kernel_stat(n2m_ctx *ctx, int idx, const char **name, double *ms_total, int *launches,
+                        double *bytes_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NII2MESH_AMD_H */
