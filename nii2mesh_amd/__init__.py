@@ -1,0 +1,173 @@
+"""nii2mesh_amd -- MI355X (gfx950) voxel-to-mesh hot path of nii2mesh.
+
+The product is the C-ABI shared library ``lib/libnii2mesh_amd.so`` (HIP
+kernels + host orchestration, see include/nii2mesh_amd.h).  This package is
+the Python side of that boundary:
+
+* `load()`      -- the ctypes handle; raises if the library is missing (there
+                   is no CPU fallback);
+* `lib()`       -- the reference's entry points (meshify, laplacian_smoothHC,
+                   bwlabel, setThreshold, marchingCubes, ...) bound through
+                   `abi.MeshLib`;
+* `DeviceContext` -- the device-resident API (volume already in HBM), used by
+                   bench.py;
+* `synth`       -- the seeded synthetic workloads.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'lib', 'libnii2mesh_amd.so')
+_lib = None
+
+
+def load():
+    """ctypes handle of the HIP library (built by nii2mesh_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError('nii2mesh_amd: %s is missing; run python -m nii2mesh_amd.build' % LIB_PATH)
+        _lib = C.CDLL(LIB_PATH)
+        _bind_ext(_lib)
+    return _lib
+
+
+class _Params(C.Structure):
+    _fields_ = [('cubes', C.c_int), ('originalMC', C.c_int), ('isolevel', C.c_float),
+                ('preSmooth', C.c_int), ('onlyLargest', C.c_int), ('fillBubbles', C.c_int),
+                ('verbose', C.c_int)]
+
+
+def _bind_ext(L):
+    L.n2m_set_cubes.argtypes = [C.c_int]
+    L.n2m_set_cubes.restype = C.c_int
+    L.n2m_get_cubes.restype = C.c_int
+    L.n2m_last_error.restype = C.c_char_p
+    L.n2m_device_ok.restype = C.c_int
+    L.n2m_ctx_create.restype = C.c_void_p
+    L.n2m_ctx_destroy.argtypes = [C.c_void_p]
+    L.n2m_ctx_stream.argtypes = [C.c_void_p]
+    L.n2m_ctx_stream.restype = C.c_void_p
+    L.n2m_ctx_meshify.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(_Params)]
+    L.n2m_ctx_pipeline.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(_Params), C.c_int,
+                                   C.POINTER(C.c_float), C.c_double, C.c_double, C.c_int]
+    L.n2m_ctx_set_threshold.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
+    L.n2m_ctx_set_threshold.restype = C.c_float
+    L.n2m_ctx_smooth_hc.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]
+    L.n2m_ctx_mesh_count.argtypes = [C.c_void_p]
+    L.n2m_ctx_mesh_size.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.n2m_ctx_download.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.n2m_ctx_enable_timing.argtypes = [C.c_void_p, C.c_int]
+    L.n2m_ctx_reset_timing.argtypes = [C.c_void_p]
+    L.n2m_ctx_kernel_stat.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.n2m_quick_smooth.argtypes = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                   C.c_int]
+    L.n2m_laplacian_smooth.argtypes = [np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'),
+                                       np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                       C.c_int]
+    L.n2m_laplacian_smooth.restype = None
+    f32 = C.POINTER(C.c_float)
+    L.n2m_apply_sform.argtypes = [np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'),
+                                  np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                  f32, f32, f32]
+    L.n2m_apply_sform.restype = None
+
+
+def lib(cubes=None):
+    """The reference entry points of the HIP library (abi.MeshLib)."""
+    L = load()
+    if cubes is not None:
+        L.n2m_set_cubes(int(cubes))
+    return abi.MeshLib(L)
+
+
+def last_error():
+    return load().n2m_last_error().decode()
+
+
+CUBES_CLASSIC = 0
+CUBES_LEWINER = 1
+
+
+class DeviceContext:
+    """Device-resident pipeline (one HIP stream, reusable HBM buffers)."""
+
+    def __init__(self):
+        self.L = load()
+        self.h = self.L.n2m_ctx_create()
+        if not self.h:
+            raise RuntimeError('n2m_ctx_create failed: ' + last_error())
+
+    def close(self):
+        if self.h:
+            self.L.n2m_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def params(isolevel, cubes=CUBES_CLASSIC, originalMC=0, preSmooth=True, onlyLargest=True,
+               fillBubbles=False, verbose=False):
+        return _Params(int(cubes), int(originalMC), float(isolevel), int(preSmooth), int(onlyLargest),
+                       int(fillBubbles), int(verbose))
+
+    @property
+    def stream(self):
+        return self.L.n2m_ctx_stream(self.h)
+
+    def meshify(self, d_ptr, dims, prm):
+        d = (C.c_int * 3)(*dims)
+        return self.L.n2m_ctx_meshify(self.h, C.c_void_p(d_ptr), d, C.byref(prm))
+
+    def pipeline(self, d_ptr, dims, prm, isolevels, alpha=0.1, beta=0.5, hc_iters=100):
+        d = (C.c_int * 3)(*dims)
+        iso = (C.c_float * len(isolevels))(*isolevels)
+        return self.L.n2m_ctx_pipeline(self.h, C.c_void_p(d_ptr), d, C.byref(prm), len(isolevels), iso,
+                                       float(alpha), float(beta), int(hc_iters))
+
+    def set_threshold(self, d_ptr, nvox, mode):
+        return float(self.L.n2m_ctx_set_threshold(self.h, C.c_void_p(d_ptr), int(nvox), int(mode)))
+
+    def smooth_hc(self, mesh=0, alpha=0.1, beta=0.5, iters=100, lock_edges=True):
+        return self.L.n2m_ctx_smooth_hc(self.h, mesh, alpha, beta, iters, int(lock_edges))
+
+    def mesh(self, k=0):
+        nt, npt = C.c_int(0), C.c_int(0)
+        if self.L.n2m_ctx_mesh_size(self.h, k, C.byref(nt), C.byref(npt)) != 0:
+            raise IndexError(k)
+        t = np.empty((nt.value, 3), np.int32)
+        p = np.empty((npt.value, 3), np.float64)
+        if self.L.n2m_ctx_download(self.h, k, t.ctypes.data, p.ctypes.data) != 0:
+            raise RuntimeError(last_error())
+        return t, p
+
+    def mesh_size(self, k=0):
+        nt, npt = C.c_int(0), C.c_int(0)
+        self.L.n2m_ctx_mesh_size(self.h, k, C.byref(nt), C.byref(npt))
+        return nt.value, npt.value
+
+    def timing(self, on=True):
+        self.L.n2m_ctx_enable_timing(self.h, int(on))
+
+    def reset_timing(self):
+        self.L.n2m_ctx_reset_timing(self.h)
+
+    def kernel_stats(self):
+        """{name: (ms_total, launches, algorithmic_bytes_total)}"""
+        out = {}
+        name = C.c_char_p()
+        ms, n, b = C.c_double(), C.c_int(), C.c_double()
+        cnt = self.L.n2m_ctx_kernel_stat(self.h, -1, C.byref(name), C.byref(ms), C.byref(n), C.byref(b))
+        for i in range(cnt):
+            self.L.n2m_ctx_kernel_stat(self.h, i, C.byref(name), C.byref(ms), C.byref(n), C.byref(b))
+            out[name.value.decode()] = (ms.value, n.value, b.value)
+        return out

@@ -1,0 +1,595 @@
+// k_mc.hip -- marching cubes on gfx950: classify -> scan -> emit.
+//
+// Classic (src/oldcubes.c:465-522): configuration bit p = corner p below the
+// isolevel; every triangle corner is its own vertex, interpolated in FP64
+// exactly as VertexInterp (src/oldcubes.c:22-40).  Triangles come out in cube
+// raster order, table order inside a cube, like the reference loop.
+//
+// Lewiner MC33 (src/MarchingCubes.c:116-1143): the volume is read through
+// the reference's sub-box view (hi-lo+1 points per axis, img-iso, clamped
+// away from 0 by FLT_EPSILON, wrap of the extra column/row, min-iso padding),
+// edge vertices are numbered in point raster order (x, y, z edge of each
+// point), centre vertices after them in cube order, triangles in cube order
+// with the winding reversed (v3,v2,v1).
+//
+// Row segments: the Lewiner kernels work on segments of up to SEG points of
+// one x-row; per-segment counts are scanned on the device, and inside a
+// segment a workgroup-level scan gives each point/cube its offset.
+#include <float.h>
+
+#include "n2m_device.cuh"
+#include "n2m_internal.h"
+
+#define N2M_TABLE_QUAL static __constant__ const
+#include "mc_tables.h"
+
+namespace n2m {
+
+// corner p -> (dx,dy,dz), src/oldcubes.c:419-426 / MarchingCubes.c:131
+__constant__ int8_t c_cx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__constant__ int8_t c_cy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+__constant__ int8_t c_cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+// classic edge e interpolates from corner A[e] toward B[e]
+__constant__ int8_t c_ea[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
+__constant__ int8_t c_eb[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
+
+__device__ __forceinline__ int classic_ntri(uint64_t w)
+{
+	int n = 0;
+	while (n < 5 && ((w >> (12 * n)) & 0xF) != 0xF)
+		n++;
+	return n;
+}
+
+// ---------------------------------------------------------------------------
+// classic
+// ---------------------------------------------------------------------------
+struct ClsGeo {
+	int nx, ny, nz;
+	int lo0, lo1, lo2;
+	int cx, cy, cz; // cube counts
+	float iso;
+};
+
+constexpr int MC_BS = 256;
+
+__global__ void __launch_bounds__(MC_BS)
+k_classic_count(const float *__restrict__ img, ClsGeo g, uint8_t *__restrict__ cfgs,
+                uint32_t *__restrict__ blockSums)
+{
+	__shared__ uint32_t scratch[MC_BS / N2M_WAVE + 1];
+	size_t ncube = (size_t)g.cx * g.cy * g.cz;
+	size_t c = blockIdx.x * (size_t)MC_BS + threadIdx.x;
+	uint32_t nt = 0;
+	if (c < ncube) {
+		int x = g.lo0 + (int)(c % g.cx);
+		size_t r = c / g.cx;
+		int y = g.lo1 + (int)(r % g.cy), z = g.lo2 + (int)(r / g.cy);
+		const size_t nxy = (size_t)g.nx * g.ny;
+		const float *b = img + (size_t)x + (size_t)y * g.nx + (size_t)z * nxy;
+		int cfg = 0;
+#pragma unroll
+		for (int p = 0; p < 8; p++) {
+			float v = b[c_cx[p] + (size_t)c_cy[p] * g.nx + (size_t)c_cz[p] * nxy];
+			if (v < g.iso)
+				cfg |= 1 << p;
+		}
+		cfgs[c] = (uint8_t)cfg;
+		nt = classic_ntri(n2m_classic_tri[cfg]);
+	}
+	uint32_t s = block_sum_u32<MC_BS>(nt, scratch);
+	if (threadIdx.x == 0)
+		blockSums[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(MC_BS)
+k_classic_emit(const float *__restrict__ img, ClsGeo g, const uint8_t *__restrict__ cfgs,
+               const uint32_t *__restrict__ blockOffs, double *__restrict__ pts)
+{
+	__shared__ uint32_t scratch[MC_BS / N2M_WAVE + 1];
+	size_t ncube = (size_t)g.cx * g.cy * g.cz;
+	size_t c = blockIdx.x * (size_t)MC_BS + threadIdx.x;
+	int cfg = c < ncube ? cfgs[c] : 0;
+	uint64_t w = n2m_classic_tri[cfg];
+	uint32_t nt = classic_ntri(w), tot;
+	uint32_t off = block_excl_scan<MC_BS>(nt, scratch, &tot) + blockOffs[blockIdx.x];
+	if (!nt)
+		return;
+	int x = g.lo0 + (int)(c % g.cx);
+	size_t r = c / g.cx;
+	int y = g.lo1 + (int)(r % g.cy), z = g.lo2 + (int)(r / g.cy);
+	const size_t nxy = (size_t)g.nx * g.ny;
+	const float *b = img + (size_t)x + (size_t)y * g.nx + (size_t)z * nxy;
+	double val[8];
+#pragma unroll
+	for (int p = 0; p < 8; p++)
+		val[p] = b[c_cx[p] + (size_t)c_cy[p] * g.nx + (size_t)c_cz[p] * nxy];
+	const double iso = g.iso;
+	double *o = pts + (size_t)off * 9;
+	for (uint32_t k = 0; k < 3 * nt; k++) {
+		int e = (int)((w >> (4 * k)) & 0xF);
+		int a = c_ea[e], bb = c_eb[e];
+		double mu = (iso - val[a]) / (val[bb] - val[a]);
+		double ax = x + c_cx[a], ay = y + c_cy[a], az = z + c_cz[a];
+		double bx = x + c_cx[bb], by = y + c_cy[bb], bz = z + c_cz[bb];
+		o[3 * k + 0] = ax + mu * (bx - ax);
+		o[3 * k + 1] = ay + mu * (by - ay);
+		o[3 * k + 2] = az + mu * (bz - az);
+	}
+}
+
+size_t classic_blocks(int cx, int cy, int cz) { return ((size_t)cx * cy * cz + MC_BS - 1) / MC_BS; }
+
+void launch_classic_count(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3],
+                          float iso, uint8_t *cfgs, uint32_t *blockSums, hipStream_t s)
+{
+	ClsGeo g{nx, ny, nz, lo[0], lo[1], lo[2], hi[0] - 1 - lo[0], hi[1] - 1 - lo[1], hi[2] - 1 - lo[2], iso};
+	size_t nb = classic_blocks(g.cx, g.cy, g.cz);
+	hipLaunchKernelGGL(k_classic_count, dim3((unsigned)nb), dim3(MC_BS), 0, s, img, g, cfgs, blockSums);
+}
+
+void launch_classic_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3],
+                         float iso, const uint8_t *cfgs, const uint32_t *blockOffs, double *pts,
+                         hipStream_t s)
+{
+	ClsGeo g{nx, ny, nz, lo[0], lo[1], lo[2], hi[0] - 1 - lo[0], hi[1] - 1 - lo[1], hi[2] - 1 - lo[2], iso};
+	size_t nb = classic_blocks(g.cx, g.cy, g.cz);
+	hipLaunchKernelGGL(k_classic_emit, dim3((unsigned)nb), dim3(MC_BS), 0, s, img, g, cfgs, blockOffs, pts);
+}
+
+// ---------------------------------------------------------------------------
+// Lewiner MC33
+// ---------------------------------------------------------------------------
+struct LwGeo {
+	int nx;
+	long long nxy, nxyz;
+	int lo0, lo1, lo2;
+	int sx, sy, sz; // sub-box points
+	float iso, mnv; // mnv = min(img) - iso
+	int originalMC;
+	int nseg;       // segments per row
+};
+
+constexpr int LW_SEG = 1024, LW_BS = 256;
+
+__device__ __forceinline__ float lw_clamp(float v) { return fabsf(v) < FLT_EPSILON ? FLT_EPSILON : v; }
+
+// value of the sub-box copy at point (x,y,z), src/MarchingCubes.c:1097-1115
+__device__ __forceinline__ float lw_data(const float *__restrict__ img, const LwGeo &g, int x, int y, int z)
+{
+	long long j = (long long)g.lo0 + x + (long long)(y + g.lo1) * g.nx + (long long)(z + g.lo2) * g.nxy;
+	float v = j < g.nxyz ? img[j] - g.iso : g.mnv;
+	return lw_clamp(v);
+}
+
+// test_face(), src/MarchingCubes.c:276-295
+__constant__ int8_t c_face[6][4] = {{0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3},
+                                   {3, 7, 4, 0}, {0, 3, 2, 1}, {4, 7, 6, 5}};
+__device__ __forceinline__ bool lw_face(const float *c, int face)
+{
+	int f = face < 0 ? -face : face;
+	float A = 0.f, B = 0.f, C = 0.f, D = 0.f;
+	if (f >= 1 && f <= 6) {
+		A = c[c_face[f - 1][0]];
+		B = c[c_face[f - 1][1]];
+		C = c[c_face[f - 1][2]];
+		D = c[c_face[f - 1][3]];
+	}
+	float det = A * C - B * D;
+	if (fabsf(det) < FLT_EPSILON)
+		return face >= 0;
+	float lhs = (float)face * A;
+	lhs = lhs * det;
+	return lhs >= 0.f;
+}
+
+// test_interior(), src/MarchingCubes.c:301-453; the reference edge row gives
+// (p, q) for t = c[p]/(c[p]-c[q]) and the (from, to) corners of B, C, D.
+__constant__ int8_t c_iedge[12][8] = {
+    {0, 1, 3, 2, 7, 6, 4, 5}, {1, 2, 0, 3, 4, 7, 5, 6}, {2, 3, 1, 0, 5, 4, 6, 7},
+    {3, 0, 2, 1, 6, 5, 7, 4}, {4, 5, 7, 6, 3, 2, 0, 1}, {5, 6, 4, 7, 0, 3, 1, 2},
+    {6, 7, 5, 4, 1, 0, 2, 3}, {7, 4, 6, 5, 2, 1, 3, 0}, {0, 4, 3, 7, 2, 6, 1, 5},
+    {1, 5, 0, 4, 3, 7, 2, 6}, {2, 6, 1, 5, 0, 4, 3, 7}, {3, 7, 2, 6, 1, 5, 0, 4}};
+
+__device__ bool lw_interior(const float *c, int cs, int cf, int sub, int s)
+{
+	float t, At = 0.f, Bt = 0.f, Ct = 0.f, Dt = 0.f;
+	if (cs == 4 || cs == 10) {
+		float a = (c[4] - c[0]) * (c[6] - c[2]) - (c[7] - c[3]) * (c[5] - c[1]);
+		float b = c[2] * (c[4] - c[0]) + c[0] * (c[6] - c[2]) - c[1] * (c[7] - c[3]) - c[3] * (c[5] - c[1]);
+		t = -b / (2.f * a);
+		if (t < 0.f || t > 1.f)
+			return s > 0;
+		At = c[0] + (c[4] - c[0]) * t;
+		Bt = c[3] + (c[7] - c[3]) * t;
+		Ct = c[2] + (c[6] - c[2]) * t;
+		Dt = c[1] + (c[5] - c[1]) * t;
+	} else {
+		int edge;
+		if (cs == 6)
+			edge = n2m_lw_test6[cf * 3 + 2];
+		else if (cs == 7)
+			edge = n2m_lw_test7[cf * 5 + 4];
+		else if (cs == 12)
+			edge = n2m_lw_test12[cf * 4 + 3];
+		else
+			edge = n2m_lw_edges[n2m_lw_row_off[N2M_LW_T13_5_1 + cf * 4 + sub]];
+		if (edge >= 0 && edge < 12) {
+			const int8_t *e = c_iedge[edge];
+			t = c[e[0]] / (c[e[0]] - c[e[1]]);
+			At = 0.f;
+			Bt = c[e[2]] + (c[e[3]] - c[e[2]]) * t;
+			Ct = c[e[4]] + (c[e[5]] - c[e[4]]) * t;
+			Dt = c[e[6]] + (c[e[7]] - c[e[6]]) * t;
+		}
+	}
+	int test = (At >= 0.f) + 2 * (Bt >= 0.f) + 4 * (Ct >= 0.f) + 8 * (Dt >= 0.f);
+	switch (test) {
+	case 5:
+		return (At * Ct - Bt * Dt < FLT_EPSILON) ? s > 0 : s < 0;
+	case 10:
+		return (At * Ct - Bt * Dt >= FLT_EPSILON) ? s > 0 : s < 0;
+	case 7:
+	case 11:
+	case 13:
+	case 14:
+	case 15:
+		return s < 0;
+	default:
+		return s > 0;
+	}
+}
+
+// process_cube() decision tree, src/MarchingCubes.c:458-795 -> tiling row or -1
+__device__ int lw_select(const float c[8], int lut, int originalMC)
+{
+	if (originalMC)
+		return N2M_LW_CLASSIC + lut;
+	const int cs = n2m_lw_case[lut], cf = n2m_lw_config[lut];
+	int sub = 0;
+	switch (cs) {
+	case 0:
+		return -1;
+	case 1:
+		return N2M_LW_T1 + cf;
+	case 2:
+		return N2M_LW_T2 + cf;
+	case 3:
+		return lw_face(c, n2m_lw_test3[cf]) ? N2M_LW_T3_2 + cf : N2M_LW_T3_1 + cf;
+	case 4:
+		return lw_interior(c, 4, cf, 0, n2m_lw_test4[cf]) ? N2M_LW_T4_1 + cf : N2M_LW_T4_2 + cf;
+	case 5:
+		return N2M_LW_T5 + cf;
+	case 6:
+		if (lw_face(c, n2m_lw_test6[cf * 3 + 0]))
+			return N2M_LW_T6_2 + cf;
+		return lw_interior(c, 6, cf, 0, n2m_lw_test6[cf * 3 + 1]) ? N2M_LW_T6_1_1 + cf : N2M_LW_T6_1_2 + cf;
+	case 7:
+		for (int i = 0; i < 3; i++)
+			if (lw_face(c, n2m_lw_test7[cf * 5 + i]))
+				sub += 1 << i;
+		switch (sub) {
+		case 0: return N2M_LW_T7_1 + cf;
+		case 1: return N2M_LW_T7_2 + cf * 3 + 0;
+		case 2: return N2M_LW_T7_2 + cf * 3 + 1;
+		case 3: return N2M_LW_T7_3 + cf * 3 + 0;
+		case 4: return N2M_LW_T7_2 + cf * 3 + 2;
+		case 5: return N2M_LW_T7_3 + cf * 3 + 1;
+		case 6: return N2M_LW_T7_3 + cf * 3 + 2;
+		default:
+			return lw_interior(c, 7, cf, sub, n2m_lw_test7[cf * 5 + 3]) ? N2M_LW_T7_4_2 + cf : N2M_LW_T7_4_1 + cf;
+		}
+	case 8:
+		return N2M_LW_T8 + cf;
+	case 9:
+		return N2M_LW_T9 + cf;
+	case 10: {
+		bool f0 = lw_face(c, n2m_lw_test10[cf * 3 + 0]), f1 = lw_face(c, n2m_lw_test10[cf * 3 + 1]);
+		if (f0)
+			return (f1 ? N2M_LW_T10_1_1_ : N2M_LW_T10_2) + cf;
+		if (f1)
+			return N2M_LW_T10_2_ + cf;
+		return (lw_interior(c, 10, cf, 0, n2m_lw_test10[cf * 3 + 2]) ? N2M_LW_T10_1_1 : N2M_LW_T10_1_2) + cf;
+	}
+	case 11:
+		return N2M_LW_T11 + cf;
+	case 12: {
+		bool f0 = lw_face(c, n2m_lw_test12[cf * 4 + 0]), f1 = lw_face(c, n2m_lw_test12[cf * 4 + 1]);
+		if (f0)
+			return (f1 ? N2M_LW_T12_1_1_ : N2M_LW_T12_2) + cf;
+		if (f1)
+			return N2M_LW_T12_2_ + cf;
+		return (lw_interior(c, 12, cf, 0, n2m_lw_test12[cf * 4 + 2]) ? N2M_LW_T12_1_1 : N2M_LW_T12_1_2) + cf;
+	}
+	case 13: {
+		for (int i = 0; i < 6; i++)
+			if (lw_face(c, n2m_lw_test13[cf * 7 + i]))
+				sub += 1 << i;
+		int s = n2m_lw_subconfig13[sub];
+		if (s == 0)
+			return N2M_LW_T13_1 + cf;
+		if (s >= 1 && s <= 6)
+			return N2M_LW_T13_2 + cf * 6 + (s - 1);
+		if (s >= 7 && s <= 18)
+			return N2M_LW_T13_3 + cf * 12 + (s - 7);
+		if (s >= 19 && s <= 22)
+			return N2M_LW_T13_4 + cf * 4 + (s - 19);
+		if (s >= 23 && s <= 26) {
+			int k = s - 23;
+			return lw_interior(c, 13, cf, k, n2m_lw_test13[cf * 7 + 6]) ? N2M_LW_T13_5_1 + cf * 4 + k
+			                                                            : N2M_LW_T13_5_2 + cf * 4 + k;
+		}
+		if (s >= 27 && s <= 38)
+			return N2M_LW_T13_3_ + cf * 12 + (s - 27);
+		if (s >= 39 && s <= 44)
+			return N2M_LW_T13_2_ + cf * 6 + (s - 39);
+		if (s == 45)
+			return N2M_LW_T13_1_ + cf;
+		return -1;
+	}
+	case 14:
+		return N2M_LW_T14 + cf;
+	}
+	return -1;
+}
+
+// edge-vertex flags of a point: bit a set when the a-edge (x,y,z) carries a
+// vertex, src/MarchingCubes.c:238-269
+__device__ __forceinline__ uint32_t lw_point_flags(const float *__restrict__ img, const LwGeo &g, int x, int y,
+                                                   int z, float c0)
+{
+	float cn[3];
+	cn[0] = x < g.sx - 1 ? lw_data(img, g, x + 1, y, z) : c0;
+	cn[1] = y < g.sy - 1 ? lw_data(img, g, x, y + 1, z) : c0;
+	cn[2] = z < g.sz - 1 ? lw_data(img, g, x, y, z + 1) : c0;
+	uint32_t f = 0;
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+		bool cross = c0 < 0.f ? cn[a] > 0.f : cn[a] < 0.f;
+		f |= (uint32_t)cross << a;
+	}
+	return f;
+}
+
+// Pass A: per point flags, per cube tiling row; per segment counts of edge
+// vertices, triangles and centre vertices.
+__global__ void __launch_bounds__(LW_BS)
+k_lw_classify(const float *__restrict__ img, LwGeo g, uint8_t *__restrict__ pflags,
+              uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt /* 3 arrays of nseg*sy*sz */)
+{
+	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
+	const int seg = blockIdx.x, y = blockIdx.y, z = blockIdx.z;
+	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
+	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
+	uint32_t nv = 0, nt = 0, nc = 0;
+	const size_t rowbase = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
+	for (int x = xs + threadIdx.x; x < xe; x += LW_BS) {
+		float c0 = lw_data(img, g, x, y, z);
+		uint32_t f = lw_point_flags(img, g, x, y, z, c0);
+		pflags[rowbase + x] = (uint8_t)f;
+		nv += __popc(f);
+		uint16_t cr = 0;
+		if (cuberow && x < g.sx - 1) {
+			float c[8];
+			int lut = 0;
+#pragma unroll
+			for (int p = 0; p < 8; p++) {
+				c[p] = lw_data(img, g, x + c_cx[p], y + c_cy[p], z + c_cz[p]);
+				if (c[p] > 0.f)
+					lut += 1 << p;
+			}
+			int row = lw_select(c, lut, g.originalMC);
+			if (row >= 0) {
+				cr = (uint16_t)(row + 1);
+				nt += n2m_lw_row_ntri[row];
+				nc += (!g.originalMC && n2m_lw_row_centre[row]) ? 1u : 0u;
+			}
+		}
+		crow[rowbase + x] = cr;
+	}
+	nv = block_sum_u32<LW_BS>(nv, scratch);
+	nt = block_sum_u32<LW_BS>(nt, scratch);
+	nc = block_sum_u32<LW_BS>(nc, scratch);
+	if (threadIdx.x == 0) {
+		size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
+		size_t ns = (size_t)g.nseg * g.sy * g.sz;
+		segcnt[si] = nv;
+		segcnt[ns + si] = nt;
+		segcnt[2 * ns + si] = nc;
+	}
+}
+
+// edge vertex position in sub-box coordinates, src/MarchingCubes.c:935-1026
+__device__ __forceinline__ void lw_edge_pos(const float *__restrict__ img, const LwGeo &g, int x, int y, int z,
+                                            int a, float *p)
+{
+	float c0 = lw_data(img, g, x, y, z);
+	float c1 = lw_data(img, g, x + (a == 0), y + (a == 1), z + (a == 2));
+	float denom = c0 - c1;
+	float u = denom != 0.0f ? c0 / denom : 0.5f;
+	p[0] = (float)x;
+	p[1] = (float)y;
+	p[2] = (float)z;
+	p[a] = p[a] + u;
+}
+
+// cube edge code -> (point dx, dy, dz, axis), src/MarchingCubes.c:811-824
+__constant__ int8_t c_code[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1},
+                                    {0, 0, 1, 0}, {1, 0, 1, 1}, {0, 1, 1, 0}, {0, 0, 1, 1},
+                                    {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
+
+// Pass B: write edge vertices of this segment, and the triangles and centre
+// vertices of its cubes.  bases: 3 arrays (vertex, triangle, centre) of
+// exclusive segment offsets; nedge = total edge vertices.
+__global__ void __launch_bounds__(LW_BS)
+k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
+          const uint16_t *__restrict__ crow, const uint32_t *__restrict__ bases, uint32_t nedge,
+          double *__restrict__ pts, int *__restrict__ tris)
+{
+	__shared__ uint32_t s_pre[4][LW_SEG + 1]; // popcount prefix for the 4 point rows
+	__shared__ uint32_t s_base[4];
+	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
+	const int seg = blockIdx.x, y = blockIdx.y, z = blockIdx.z;
+	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
+	const int len = xe - xs; // points owned; prefix covers [xs, xe]
+	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
+	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
+	const int nrows = cuberow ? 4 : 1;
+	// row r: (y + (r&1), z + (r>>1))
+	if (threadIdx.x < nrows) {
+		int r = threadIdx.x;
+		int yy = y + (r & 1), zz = z + (r >> 1);
+		s_base[r] = bases[(size_t)seg + (size_t)g.nseg * ((size_t)yy + (size_t)g.sy * zz)];
+	}
+	for (int r = 0; r < nrows; r++) {
+		int yy = y + (r & 1), zz = z + (r >> 1);
+		const size_t rb = (size_t)g.sx * ((size_t)yy + (size_t)g.sy * zz);
+		uint32_t carry = 0;
+		for (int x0 = 0; x0 <= len; x0 += LW_BS) {
+			int x = xs + x0 + threadIdx.x;
+			uint32_t f = (x0 + (int)threadIdx.x < len) ? __popc(pflags[rb + x]) : 0u;
+			uint32_t tot;
+			uint32_t ex = block_excl_scan<LW_BS>(f, scratch, &tot);
+			if (x0 + (int)threadIdx.x <= len)
+				s_pre[r][x0 + threadIdx.x] = carry + ex;
+			carry += tot;
+		}
+	}
+	__syncthreads();
+	// edge vertices of the owned points
+	{
+		const size_t rb = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
+		for (int i = threadIdx.x; i < len; i += LW_BS) {
+			int x = xs + i;
+			uint32_t f = pflags[rb + x];
+			uint32_t id = s_base[0] + s_pre[0][i];
+			for (int a = 0; a < 3; a++) {
+				if (!((f >> a) & 1u))
+					continue;
+				float p[3];
+				lw_edge_pos(img, g, x, y, z, a, p);
+				double *o = pts + (size_t)id * 3;
+				o[0] = (double)(p[0] + (float)g.lo0);
+				o[1] = (double)(p[1] + (float)g.lo1);
+				o[2] = (double)(p[2] + (float)g.lo2);
+				id++;
+			}
+		}
+	}
+	if (!cuberow)
+		return;
+	const uint32_t tbase = bases[ns + (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z)];
+	const uint32_t cbase = nedge + bases[2 * ns + (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z)];
+	const size_t rb0 = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
+	uint32_t tcarry = 0, ccarry = 0;
+	const int ncube = min(xe, g.sx - 1) - xs;
+	for (int i0 = 0; i0 < ncube; i0 += LW_BS) {
+		int i = i0 + threadIdx.x;
+		int row = -1;
+		if (i < ncube)
+			row = (int)crow[rb0 + xs + i] - 1;
+		uint32_t nt = row >= 0 ? n2m_lw_row_ntri[row] : 0u;
+		uint32_t hc = (row >= 0 && !g.originalMC && n2m_lw_row_centre[row]) ? 1u : 0u;
+		uint32_t ttot, ctot;
+		uint32_t toff = block_excl_scan<LW_BS>(nt, scratch, &ttot);
+		uint32_t coff = block_excl_scan<LW_BS>(hc, scratch, &ctot);
+		if (row >= 0) {
+			const int x = xs + i;
+			// vertex id of cube edge code e
+			uint32_t ids[12];
+#pragma unroll
+			for (int e = 0; e < 12; e++) {
+				int dx = c_code[e][0], dy = c_code[e][1], dz = c_code[e][2], a = c_code[e][3];
+				int r = dy + 2 * dz;
+				size_t pb = (size_t)g.sx * ((size_t)(y + dy) + (size_t)g.sy * (z + dz));
+				uint32_t f = pflags[pb + x + dx];
+				ids[e] = ((f >> a) & 1u) ? s_base[r] + s_pre[r][i + dx] + __popc(f & ((1u << a) - 1u))
+				                         : 0xffffffffu;
+			}
+			uint32_t cid = 0xffffffffu;
+			if (hc) { // add_c_vertex(): mean of the cube's edge vertices, in float
+				float sxv = 0.f, syv = 0.f, szv = 0.f, cnt = 0.f;
+				for (int e = 0; e < 12; e++) {
+					if (ids[e] == 0xffffffffu)
+						continue;
+					float p[3];
+					lw_edge_pos(img, g, x + c_code[e][0], y + c_code[e][1], z + c_code[e][2], c_code[e][3], p);
+					++cnt;
+					sxv += p[0];
+					syv += p[1];
+					szv += p[2];
+				}
+				if (cnt > 0.f) {
+					sxv /= cnt;
+					syv /= cnt;
+					szv /= cnt;
+				}
+				cid = cbase + ccarry + coff;
+				double *o = pts + (size_t)cid * 3;
+				o[0] = (double)(sxv + (float)g.lo0);
+				o[1] = (double)(syv + (float)g.lo1);
+				o[2] = (double)(szv + (float)g.lo2);
+			}
+			const uint8_t *codes = &n2m_lw_edges[n2m_lw_row_off[row]];
+			int *t = tris + (size_t)(tbase + tcarry + toff) * 3;
+			for (uint32_t k = 0; k < nt; k++) {
+				uint32_t v[3];
+				for (int j = 0; j < 3; j++) {
+					int e = codes[3 * k + j];
+					v[j] = e == 12 ? cid : ids[e];
+				}
+				t[3 * k + 0] = (int)v[2];
+				t[3 * k + 1] = (int)v[1];
+				t[3 * k + 2] = (int)v[0];
+			}
+		}
+		tcarry += ttot;
+		ccarry += ctot;
+	}
+}
+
+void lw_geo(LwGeo &g, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso, float mnv,
+            int originalMC)
+{
+	g.nx = nx;
+	g.nxy = (long long)nx * ny;
+	g.nxyz = (long long)nx * ny * nz;
+	g.lo0 = lo[0];
+	g.lo1 = lo[1];
+	g.lo2 = lo[2];
+	g.sx = hi[0] - lo[0] + 1;
+	g.sy = hi[1] - lo[1] + 1;
+	g.sz = hi[2] - lo[2] + 1;
+	g.iso = iso;
+	g.mnv = mnv;
+	g.originalMC = originalMC;
+	g.nseg = (g.sx + LW_SEG - 1) / LW_SEG;
+}
+
+size_t lw_nseg_total(int nx, int ny, int nz, const int lo[3], const int hi[3])
+{
+	LwGeo g;
+	lw_geo(g, nx, ny, nz, lo, hi, 0.f, 0.f, 0);
+	return (size_t)g.nseg * g.sy * g.sz;
+}
+
+void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
+                        float mnv, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
+                        hipStream_t s)
+{
+	LwGeo g;
+	lw_geo(g, nx, ny, nz, lo, hi, iso, mnv, originalMC);
+	hipLaunchKernelGGL(k_lw_classify, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags, crow, segcnt);
+}
+
+void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
+                    float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow,
+                    const uint32_t *bases, uint32_t nedge, double *pts, int *tris, hipStream_t s)
+{
+	LwGeo g;
+	lw_geo(g, nx, ny, nz, lo, hi, iso, mnv, originalMC);
+	hipLaunchKernelGGL(k_lw_emit, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags, crow, bases,
+	                   nedge, pts, tris);
+}
+
+} // namespace n2m

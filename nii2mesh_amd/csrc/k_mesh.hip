@@ -1,0 +1,672 @@
+// k_mesh.hip -- surface stages on gfx950: scan, stable radix sort, vertex
+// welding (src/meshify.c:45-106), degenerate-triangle removal
+// (src/meshify.c:113-168), Laplacian / HC smoothing (src/quadric.c:316-394),
+// border detection (src/quadric.c:188-215) and apply_sform
+// (src/meshify.c:1021-1045).
+#include <float.h>
+
+#include "n2m_device.cuh"
+#include "n2m_internal.h"
+
+namespace n2m {
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of uint32 (multi-level: reduce, scan the partials, apply).
+// ---------------------------------------------------------------------------
+constexpr int SC_BS = 256, SC_IPT = 8, SC_TILE = SC_BS * SC_IPT;
+
+__global__ void __launch_bounds__(SC_BS) k_scan_reduce(const uint32_t *__restrict__ in, size_t n, uint32_t *sums)
+{
+	__shared__ uint32_t scratch[SC_BS / N2M_WAVE + 1];
+	size_t base = blockIdx.x * (size_t)SC_TILE;
+	uint32_t s = 0;
+	for (int k = 0; k < SC_IPT; k++) {
+		size_t i = base + k * SC_BS + threadIdx.x;
+		if (i < n)
+			s += in[i];
+	}
+	s = block_sum_u32<SC_BS>(s, scratch);
+	if (threadIdx.x == 0)
+		sums[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(SC_BS)
+k_scan_apply(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, size_t n, const uint32_t *offs,
+             uint32_t *total)
+{
+	__shared__ uint32_t scratch[SC_BS / N2M_WAVE + 1];
+	size_t base = blockIdx.x * (size_t)SC_TILE + (size_t)threadIdx.x * SC_IPT;
+	uint32_t v[SC_IPT], s = 0;
+#pragma unroll
+	for (int k = 0; k < SC_IPT; k++) {
+		size_t i = base + k;
+		v[k] = i < n ? in[i] : 0u;
+		s += v[k];
+	}
+	uint32_t tot;
+	uint32_t run = block_excl_scan<SC_BS>(s, scratch, &tot) + (offs ? offs[blockIdx.x] : 0u);
+#pragma unroll
+	for (int k = 0; k < SC_IPT; k++) {
+		size_t i = base + k;
+		if (i < n)
+			out[i] = run;
+		run += v[k];
+	}
+	if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == SC_BS - 1)
+		*total = run;
+}
+
+size_t scan_scratch_words(size_t n)
+{
+	size_t w = 0;
+	while (n > SC_TILE) {
+		n = (n + SC_TILE - 1) / SC_TILE;
+		w += n;
+	}
+	return w + 1;
+}
+
+void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, uint32_t *scratch,
+                   hipStream_t s)
+{
+	if (n == 0) {
+		if (total)
+			hipMemsetAsync(total, 0, sizeof(uint32_t), s);
+		return;
+	}
+	size_t nb = (n + SC_TILE - 1) / SC_TILE;
+	if (nb == 1) {
+		hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SC_BS), 0, s, in, out, n, (const uint32_t *)nullptr, total);
+		return;
+	}
+	uint32_t *sums = scratch;
+	hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_BS), 0, s, in, n, sums);
+	scan_excl_u32(sums, sums, nb, nullptr, scratch + nb, s);
+	hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SC_BS), 0, s, in, out, n, (const uint32_t *)sums,
+	                   total);
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (uint32 key, uint32 value), 8-bit digits.
+// Each workgroup ranks a tile of RS_TILE keys in RS_IPT rounds of RS_BS keys
+// (tile order is preserved: round-major, thread order inside a round).
+// ---------------------------------------------------------------------------
+constexpr int RS_BS = 256, RS_IPT = 8, RS_TILE = RS_BS * RS_IPT, RS_NW = RS_BS / N2M_WAVE;
+
+__global__ void __launch_bounds__(RS_BS)
+k_radix_hist(const uint32_t *__restrict__ keys, size_t n, int shift, uint32_t *__restrict__ counts, size_t nblocks)
+{
+	__shared__ uint32_t h[256];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	size_t base = blockIdx.x * (size_t)RS_TILE;
+	for (int k = 0; k < RS_IPT; k++) {
+		size_t i = base + k * RS_BS + threadIdx.x;
+		if (i < n)
+			atomicAdd(&h[(keys[i] >> shift) & 0xFF], 1u);
+	}
+	__syncthreads();
+	counts[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(RS_BS)
+k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint32_t *__restrict__ kout,
+                uint32_t *__restrict__ vout, size_t n, int shift, const uint32_t *__restrict__ offs, size_t nblocks)
+{
+	__shared__ uint32_t run[256];
+	__shared__ uint32_t wcnt[RS_NW][256];
+	__shared__ uint32_t gbase[256];
+	run[threadIdx.x] = 0;
+	gbase[threadIdx.x] = offs[(size_t)threadIdx.x * nblocks + blockIdx.x];
+	for (int w = 0; w < RS_NW; w++)
+		wcnt[w][threadIdx.x] = 0;
+	__syncthreads();
+	const size_t base = blockIdx.x * (size_t)RS_TILE;
+	const int lane = lane_id(), wv = wave_id();
+	const unsigned long long lt = (1ull << lane) - 1ull;
+	for (int k = 0; k < RS_IPT; k++) {
+		size_t i = base + k * RS_BS + threadIdx.x;
+		bool act = i < n;
+		uint32_t key = act ? kin[i] : 0u, val = act ? vin[i] : 0u;
+		uint32_t d = (key >> shift) & 0xFF;
+		unsigned long long m = __ballot(act);
+#pragma unroll
+		for (int b = 0; b < 8; b++) {
+			bool bit = (d >> b) & 1u;
+			unsigned long long bb = __ballot(bit);
+			m &= bit ? bb : ~bb;
+		}
+		uint32_t rk = __popcll(m & lt);
+		if (act && rk == 0)
+			wcnt[wv][d] = __popcll(m);
+		__syncthreads();
+		if (act) {
+			uint32_t pos = run[d] + rk;
+			for (int w = 0; w < wv; w++)
+				pos += wcnt[w][d];
+			uint32_t o = gbase[d] + pos;
+			kout[o] = key;
+			vout[o] = val;
+		}
+		__syncthreads();
+		uint32_t add = 0;
+		for (int w = 0; w < RS_NW; w++) {
+			add += wcnt[w][threadIdx.x];
+			wcnt[w][threadIdx.x] = 0;
+		}
+		run[threadIdx.x] += add;
+		__syncthreads();
+	}
+}
+
+// Sorts (k0,v0) using (k1,v1) as the ping-pong buffers; returns true when
+// the result is in (k1,v1).  counts/offs hold 256*nblocks words each.
+bool radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int bits,
+                    uint32_t *counts, uint32_t *scan_scratch, hipStream_t s)
+{
+	if (n <= 1)
+		return false;
+	size_t nb = (n + RS_TILE - 1) / RS_TILE;
+	bool flip = false;
+	for (int shift = 0; shift < bits; shift += 8) {
+		const uint32_t *ki = flip ? k1 : k0, *vi = flip ? v1 : v0;
+		uint32_t *ko = flip ? k0 : k1, *vo = flip ? v0 : v1;
+		hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(RS_BS), 0, s, ki, n, shift, counts, nb);
+		scan_excl_u32(counts, counts, 256 * nb, nullptr, scan_scratch, s);
+		hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(RS_BS), 0, s, ki, vi, ko, vo, n, shift,
+		                   counts, nb);
+		flip = !flip;
+	}
+	return flip;
+}
+
+size_t radix_counts_words(size_t n) { return 256 * ((n + RS_TILE - 1) / RS_TILE) + 1; }
+
+// ---------------------------------------------------------------------------
+// Vertex welding, src/meshify.c:45-106.  The reference sorts FP32 distances
+// to vertex 0 (stable radix sort), then scans the sorted list: an unassigned
+// vertex seeds a new class and every vertex in its window (key within 1e-5,
+// FP32) that lies within 1e-5 in 3-D (FP64) joins that class, even one a
+// previous seed claimed.  A window can never cross a gap where
+// fl(key[k]-key[k-1]) >= tol, so the sorted list splits at such gaps into
+// independent segments: one thread runs the reference's loop per segment;
+// class numbers = number of seeds before, in sorted order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dist3(const double *a, const double *b)
+{
+	double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+	return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__global__ void k_weld_keys(const double *__restrict__ pts, uint32_t n, uint32_t *keys, uint32_t *vals)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	float d = (float)dist3(pts, pts + (size_t)i * 3);
+	keys[i] = __float_as_uint(d); // d >= +0: bit order is value order
+	vals[i] = i;
+}
+
+__global__ void k_weld_segflags(const uint32_t *__restrict__ keys, uint32_t n, uint32_t *flags)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const float tol = 0.00001f;
+	uint32_t f = 1;
+	if (i > 0)
+		f = (__uint_as_float(keys[i]) - __uint_as_float(keys[i - 1])) >= tol;
+	flags[i] = f;
+}
+
+// segment starts from the exclusive scan of the flags
+__global__ void k_weld_segstarts(const uint32_t *__restrict__ flags, const uint32_t *__restrict__ segid,
+                                 uint32_t n, uint32_t *starts)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && flags[i])
+		starts[segid[i]] = i;
+}
+
+__global__ void k_weld_greedy(const uint32_t *__restrict__ starts, uint32_t nseg, uint32_t n,
+                              const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+                              const double *__restrict__ pts, int *__restrict__ lcls, uint32_t *__restrict__ seeds)
+{
+	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= nseg)
+		return;
+	const float tol = 0.00001f;
+	uint32_t b = starts[s], e = s + 1 < nseg ? starts[s + 1] : n;
+	for (uint32_t i = b; i < e; i++)
+		lcls[i] = -1;
+	int ns = 0;
+	for (uint32_t i = b; i < e; i++) {
+		if (lcls[i] >= 0)
+			continue;
+		const float k0 = __uint_as_float(keys[i]);
+		const double *seed = pts + (size_t)vals[i] * 3;
+		for (uint32_t j = i; j < e && (__uint_as_float(keys[j]) - k0) < tol; j++)
+			if (dist3(seed, pts + (size_t)vals[j] * 3) < (double)tol)
+				lcls[j] = ns;
+		ns++;
+	}
+	seeds[s] = (uint32_t)ns;
+}
+
+__global__ void k_weld_assign(const uint32_t *__restrict__ vals, const int *__restrict__ lcls,
+                              const uint32_t *__restrict__ flags, const uint32_t *__restrict__ segid,
+                              const uint32_t *__restrict__ seedbase, uint32_t n, int *__restrict__ cls,
+                              int *__restrict__ rep)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	uint32_t sg = segid[i] + flags[i] - 1; // inclusive segment index
+	int c = (int)seedbase[sg] + lcls[i];
+	uint32_t v = vals[i];
+	cls[v] = c;
+	atomicMax(&rep[c], (int)v); // last member in index order provides the position
+}
+
+__global__ void k_gather_pts(const double *__restrict__ pts, const int *__restrict__ rep, uint32_t n,
+                             double *__restrict__ out)
+{
+	uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= n)
+		return;
+	const double *p = pts + (size_t)rep[c] * 3;
+	out[(size_t)c * 3 + 0] = p[0];
+	out[(size_t)c * 3 + 1] = p[1];
+	out[(size_t)c * 3 + 2] = p[2];
+}
+
+// tris[t][k] = cls[tris[t][k]]; implicit (classic 3t+k) when tris_in == null.
+// An id outside [0, npt) (a corner the tables left without a vertex) stays -1.
+__global__ void k_remap_tris(const int *__restrict__ tris_in, const int *__restrict__ cls, uint32_t nt,
+                             uint32_t npt, int *__restrict__ tris_out)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= 3 * nt)
+		return;
+	int v = tris_in ? tris_in[i] : (int)i;
+	tris_out[i] = (v >= 0 && (uint32_t)v < npt) ? cls[v] : -1;
+}
+
+static unsigned g1(size_t n) { return (unsigned)((n + 255) / 256); }
+
+__global__ void k_iota(int *__restrict__ out, uint32_t n)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		out[i] = (int)i;
+}
+void launch_iota(int *out, uint32_t n, hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_iota, dim3(g1(n)), dim3(256), 0, s, out, n);
+}
+
+void launch_weld_keys(const double *pts, uint32_t n, uint32_t *keys, uint32_t *vals, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_weld_keys, dim3(g1(n)), dim3(256), 0, s, pts, n, keys, vals);
+}
+void launch_weld_segflags(const uint32_t *keys, uint32_t n, uint32_t *flags, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_weld_segflags, dim3(g1(n)), dim3(256), 0, s, keys, n, flags);
+}
+void launch_weld_segstarts(const uint32_t *flags, const uint32_t *segid, uint32_t n, uint32_t *starts,
+                           hipStream_t s)
+{
+	hipLaunchKernelGGL(k_weld_segstarts, dim3(g1(n)), dim3(256), 0, s, flags, segid, n, starts);
+}
+void launch_weld_greedy(const uint32_t *starts, uint32_t nseg, uint32_t n, const uint32_t *keys,
+                        const uint32_t *vals, const double *pts, int *lcls, uint32_t *seeds, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_weld_greedy, dim3(g1(nseg)), dim3(256), 0, s, starts, nseg, n, keys, vals, pts, lcls,
+	                   seeds);
+}
+void launch_weld_assign(const uint32_t *vals, const int *lcls, const uint32_t *flags, const uint32_t *segid,
+                        const uint32_t *seedbase, uint32_t n, int *cls, int *rep, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_weld_assign, dim3(g1(n)), dim3(256), 0, s, vals, lcls, flags, segid, seedbase, n, cls,
+	                   rep);
+}
+void launch_gather_pts(const double *pts, const int *rep, uint32_t n, double *out, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_gather_pts, dim3(g1(n)), dim3(256), 0, s, pts, rep, n, out);
+}
+void launch_remap_tris(const int *tris_in, const int *cls, uint32_t nt, uint32_t npt, int *tris_out,
+                       hipStream_t s)
+{
+	hipLaunchKernelGGL(k_remap_tris, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris_in, cls, nt, npt, tris_out);
+}
+
+// ---------------------------------------------------------------------------
+// Degenerate triangles, src/meshify.c:113-168 (FP64, same operation order).
+// ---------------------------------------------------------------------------
+__global__ void k_degen_flags(const double *__restrict__ pts, const int *__restrict__ tris, uint32_t nt,
+                              uint32_t np, uint32_t *__restrict__ keep)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nt)
+		return;
+	if ((uint32_t)tris[3 * i] >= np || (uint32_t)tris[3 * i + 1] >= np || (uint32_t)tris[3 * i + 2] >= np) {
+		keep[i] = 0; // invalid vertex id: never read out of bounds
+		return;
+	}
+	const double *A = pts + (size_t)tris[3 * i] * 3, *B = pts + (size_t)tris[3 * i + 1] * 3,
+	             *C = pts + (size_t)tris[3 * i + 2] * 3;
+	double l = dist3(A, B), m = dist3(A, C), n = dist3(B, C);
+	double c = fmin(fmin(l, m), n);
+	double a = fmax(fmax(l, m), n);
+	double b = l + m + n - a - c;
+	uint32_t k = 1;
+	if ((c - (a - b)) <= 0.0) {
+		k = 0;
+	} else {
+		double area4 = 0.25 * sqrt((a + (b + c)) * (c - (a - b)) * (c + (a - b)) * (a + (b - c)));
+		if (area4 < FLT_EPSILON)
+			k = 0;
+	}
+	keep[i] = k;
+}
+
+__global__ void k_compact_tris(const int *__restrict__ tris, const uint32_t *__restrict__ keep,
+                               const uint32_t *__restrict__ pos, uint32_t nt, int *__restrict__ out)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nt || !keep[i])
+		return;
+	uint32_t o = pos[i];
+	out[3 * o] = tris[3 * i];
+	out[3 * o + 1] = tris[3 * i + 1];
+	out[3 * o + 2] = tris[3 * i + 2];
+}
+
+void launch_degen_flags(const double *pts, const int *tris, uint32_t nt, uint32_t np, uint32_t *keep,
+                        hipStream_t s)
+{
+	hipLaunchKernelGGL(k_degen_flags, dim3(g1(nt)), dim3(256), 0, s, pts, tris, nt, np, keep);
+}
+void launch_compact_tris(const int *tris, const uint32_t *keep, const uint32_t *pos, uint32_t nt, int *out,
+                         hipStream_t s)
+{
+	hipLaunchKernelGGL(k_compact_tris, dim3(g1(nt)), dim3(256), 0, s, tris, keep, pos, nt, out);
+}
+
+// ---------------------------------------------------------------------------
+// Smoothing.  CSR over triangle corners: for vertex v, its references
+// (t*3+k) in ascending order -- the order src/quadric.c:321-332 adds them --
+// stored as the pair of the other two corners (lower corner first).
+// ---------------------------------------------------------------------------
+__global__ void k_csr_count(const int *__restrict__ tris, uint32_t nt, uint32_t *__restrict__ cnt)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < 3 * nt)
+		atomicAdd(&cnt[tris[i]], 1u);
+}
+
+__global__ void k_csr_fill(const int *__restrict__ tris, uint32_t nt, const uint32_t *__restrict__ start,
+                           uint32_t *__restrict__ cursor, uint32_t *__restrict__ ref)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= 3 * nt)
+		return;
+	int v = tris[i];
+	uint32_t o = start[v] + atomicAdd(&cursor[v], 1u);
+	ref[o] = i;
+}
+
+// sort each vertex's references, then expand to neighbour pairs
+__global__ void k_csr_pairs(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restrict__ start,
+                            uint32_t *__restrict__ ref, int2 *__restrict__ pairs)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= nv)
+		return;
+	uint32_t b = start[v], e = start[v + 1];
+	for (uint32_t i = b + 1; i < e; i++) { // insertion sort (valence is small)
+		uint32_t x = ref[i];
+		uint32_t j = i;
+		while (j > b && ref[j - 1] > x) {
+			ref[j] = ref[j - 1];
+			j--;
+		}
+		ref[j] = x;
+	}
+	for (uint32_t i = b; i < e; i++) {
+		uint32_t r = ref[i], t = r / 3, k = r % 3;
+		int c0 = tris[3 * t], c1 = tris[3 * t + 1], c2 = tris[3 * t + 2];
+		int2 p;
+		p.x = k == 0 ? c1 : c0;
+		p.y = k == 2 ? c1 : c2;
+		pairs[i] = p;
+	}
+}
+
+// Border vertices as update_mesh(iteration 0) marks them: for vertex v,
+// every id in the triangles referencing v (once per reference) is counted;
+// ids counted exactly once are border.
+__global__ void k_border(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restrict__ start,
+                         const uint32_t *__restrict__ ref, uint8_t *__restrict__ border)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= nv)
+		return;
+	uint32_t b = start[v], e = start[v + 1];
+	for (uint32_t i = b; i < e; i++) {
+		uint32_t t = ref[i] / 3;
+		for (int k = 0; k < 3; k++) {
+			int id = tris[3 * t + k];
+			bool first = true;
+			for (uint32_t i2 = b; i2 <= i && first; i2++) {
+				uint32_t t2 = ref[i2] / 3;
+				int kend = i2 == i ? k : 3;
+				for (int k2 = 0; k2 < kend; k2++)
+					if (tris[3 * t2 + k2] == id) {
+						first = false;
+						break;
+					}
+			}
+			if (!first)
+				continue;
+			uint32_t count = 0;
+			for (uint32_t i2 = b; i2 < e; i2++) {
+				uint32_t t2 = ref[i2] / 3;
+				for (int k2 = 0; k2 < 3; k2++)
+					count += tris[3 * t2 + k2] == id;
+			}
+			if (count == 1)
+				border[id] = 1;
+		}
+	}
+}
+
+struct d3 {
+	double x, y, z;
+};
+
+// p' = L(p); b = p' - (o*alpha + p*alpha1)
+__global__ void __launch_bounds__(256)
+k_hc_a(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const d3 *__restrict__ o,
+       const d3 *__restrict__ p, d3 *__restrict__ pn, d3 *__restrict__ bq, uint32_t nv, double alpha, double alpha1)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= nv)
+		return;
+	uint32_t b = start[v], e = start[v + 1];
+	d3 pv = p[v], L;
+	if (e > b) {
+		double sx = 0.0, sy = 0.0, sz = 0.0;
+		for (uint32_t i = b; i < e; i++) {
+			int2 q = pairs[i];
+			d3 A = p[q.x], B = p[q.y];
+			sx = sx + (A.x + B.x);
+			sy = sy + (A.y + B.y);
+			sz = sz + (A.z + B.z);
+		}
+		double num = (double)(2 * (e - b));
+		L.x = sx / num;
+		L.y = sy / num;
+		L.z = sz / num;
+	} else {
+		L = pv;
+	}
+	d3 ov = o[v], bb;
+	bb.x = L.x - (ov.x * alpha + pv.x * alpha1);
+	bb.y = L.y - (ov.y * alpha + pv.y * alpha1);
+	bb.z = L.z - (ov.z * alpha + pv.z * alpha1);
+	pn[v] = L;
+	bq[v] = bb;
+}
+
+// p = p' - (b*beta + L(b)*beta1)
+__global__ void __launch_bounds__(256)
+k_hc_b(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const d3 *__restrict__ pn,
+       const d3 *__restrict__ bq, d3 *__restrict__ pout, uint32_t nv, double beta, double beta1)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= nv)
+		return;
+	uint32_t b = start[v], e = start[v + 1];
+	d3 bv = bq[v], L;
+	if (e > b) {
+		double sx = 0.0, sy = 0.0, sz = 0.0;
+		for (uint32_t i = b; i < e; i++) {
+			int2 q = pairs[i];
+			d3 A = bq[q.x], B = bq[q.y];
+			sx = sx + (A.x + B.x);
+			sy = sy + (A.y + B.y);
+			sz = sz + (A.z + B.z);
+		}
+		double num = (double)(2 * (e - b));
+		L.x = sx / num;
+		L.y = sy / num;
+		L.z = sz / num;
+	} else {
+		L = bv;
+	}
+	d3 pv = pn[v], r;
+	r.x = pv.x - (bv.x * beta + L.x * beta1);
+	r.y = pv.y - (bv.y * beta + L.y * beta1);
+	r.z = pv.z - (bv.z * beta + L.z * beta1);
+	pout[v] = r;
+}
+
+// plain Laplacian step (src/quadric.c:316-341): pout = L(p)
+__global__ void __launch_bounds__(256)
+k_laplace(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const d3 *__restrict__ p,
+          d3 *__restrict__ pout, uint32_t nv)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= nv)
+		return;
+	uint32_t b = start[v], e = start[v + 1];
+	d3 L = p[v];
+	if (e > b) {
+		double sx = 0.0, sy = 0.0, sz = 0.0;
+		for (uint32_t i = b; i < e; i++) {
+			int2 q = pairs[i];
+			d3 A = p[q.x], B = p[q.y];
+			sx = sx + (A.x + B.x);
+			sy = sy + (A.y + B.y);
+			sz = sz + (A.z + B.z);
+		}
+		double num = (double)(2 * (e - b));
+		L.x = sx / num;
+		L.y = sy / num;
+		L.z = sz / num;
+	}
+	pout[v] = L;
+}
+
+__global__ void k_hc_final(const d3 *__restrict__ o, const d3 *__restrict__ p, const uint8_t *__restrict__ border,
+                           uint32_t nv, d3 *__restrict__ out)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= nv)
+		return;
+	out[v] = (border && border[v]) ? o[v] : p[v];
+}
+
+void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_csr_count, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, cnt);
+}
+void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, uint32_t *cursor, uint32_t *ref,
+                     hipStream_t s)
+{
+	hipLaunchKernelGGL(k_csr_fill, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, start, cursor, ref);
+}
+void launch_csr_pairs(const int *tris, uint32_t nv, const uint32_t *start, uint32_t *ref, int2 *pairs,
+                      hipStream_t s)
+{
+	hipLaunchKernelGGL(k_csr_pairs, dim3(g1(nv)), dim3(256), 0, s, tris, nv, start, ref, pairs);
+}
+void launch_border(const int *tris, uint32_t nv, const uint32_t *start, const uint32_t *ref, uint8_t *border,
+                   hipStream_t s)
+{
+	hipLaunchKernelGGL(k_border, dim3(g1(nv)), dim3(256), 0, s, tris, nv, start, ref, border);
+}
+void launch_hc_a(const uint32_t *start, const int2 *pairs, const double *o, const double *p, double *pn,
+                 double *bq, uint32_t nv, double alpha, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_hc_a, dim3(g1(nv)), dim3(256), 0, s, start, pairs, (const d3 *)o, (const d3 *)p,
+	                   (d3 *)pn, (d3 *)bq, nv, alpha, 1.0 - alpha);
+}
+void launch_hc_b(const uint32_t *start, const int2 *pairs, const double *pn, const double *bq, double *pout,
+                 uint32_t nv, double beta, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_hc_b, dim3(g1(nv)), dim3(256), 0, s, start, pairs, (const d3 *)pn, (const d3 *)bq,
+	                   (d3 *)pout, nv, beta, 1.0 - beta);
+}
+void launch_laplace(const uint32_t *start, const int2 *pairs, const double *p, double *pout, uint32_t nv,
+                    hipStream_t s)
+{
+	hipLaunchKernelGGL(k_laplace, dim3(g1(nv)), dim3(256), 0, s, start, pairs, (const d3 *)p, (d3 *)pout, nv);
+}
+void launch_hc_final(const double *o, const double *p, const uint8_t *border, uint32_t nv, double *out,
+                     hipStream_t s)
+{
+	hipLaunchKernelGGL(k_hc_final, dim3(g1(nv)), dim3(256), 0, s, (const d3 *)o, (const d3 *)p, border, nv,
+	                   (d3 *)out);
+}
+
+// ---------------------------------------------------------------------------
+// apply_sform, src/meshify.c:1021-1045 (positions only; the winding flip is
+// decided on the host from the same float sums).
+// ---------------------------------------------------------------------------
+__global__ void k_sform(double *__restrict__ p, uint32_t np, float x0, float x1, float x2, float x3, float y0,
+                        float y1, float y2, float y3, float z0, float z1, float z2, float z3)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= np)
+		return;
+	double vx = p[3 * (size_t)i], vy = p[3 * (size_t)i + 1], vz = p[3 * (size_t)i + 2];
+	p[3 * (size_t)i + 0] = (vx * x0) + (vy * x1) + (vz * x2) + x3;
+	p[3 * (size_t)i + 1] = (vx * y0) + (vy * y1) + (vz * y2) + y3;
+	p[3 * (size_t)i + 2] = (vx * z0) + (vy * z1) + (vz * z2) + z3;
+}
+
+__global__ void k_swap_winding(int *__restrict__ t, uint32_t nt)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nt)
+		return;
+	int a = t[3 * i];
+	t[3 * i] = t[3 * i + 1];
+	t[3 * i + 1] = a;
+}
+
+void launch_sform(double *p, uint32_t np, const float *sx, const float *sy, const float *sz, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_sform, dim3(g1(np)), dim3(256), 0, s, p, np, sx[0], sx[1], sx[2], sx[3], sy[0], sy[1],
+	                   sy[2], sy[3], sz[0], sz[1], sz[2], sz[3]);
+}
+void launch_swap_winding(int *t, uint32_t nt, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_swap_winding, dim3(g1(nt)), dim3(256), 0, s, t, nt);
+}
+
+} // namespace n2m

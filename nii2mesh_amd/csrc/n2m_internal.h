@@ -1,0 +1,108 @@
+// n2m_internal.h -- host-side declarations shared by the kernel files and the
+// C-ABI layer (api.cpp).  Every launch_* enqueues work on stream s and
+// returns immediately.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace n2m {
+
+// inverse of the order-preserving float -> uint32 map used by the device
+// min/max reductions
+inline float ord2f_host(uint32_t o)
+{
+	uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+	float f;
+	__builtin_memcpy(&f, &u, 4);
+	return f;
+}
+
+struct CclTimer {
+	virtual void begin(const char *name) = 0;
+	virtual void end() = 0;
+	virtual ~CclTimer() {}
+};
+
+// k_volume.hip
+void launch_smooth(const float *in, float *out, int nx, int ny, int nz, uint32_t *range, float iso,
+                   uint32_t *bits, hipStream_t s);
+void launch_minmax(const float *img, size_t n, uint32_t *range, hipStream_t s);
+void launch_mask_bits(const float *img, int nx, int ny, int nz, float iso, int nonzero, uint32_t *bits,
+                      hipStream_t s);
+void launch_dilate_bits(const uint32_t *in, uint32_t *out, int nx, int ny, int nz, hipStream_t s);
+void launch_apply(float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax, const uint32_t *keep,
+                  const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s);
+void launch_hist_range(const float *img, size_t n, uint32_t *stats, hipStream_t s);
+void launch_hist_bins(const float *img, size_t n, float mn, float scl, int nbins, uint32_t *hist, hipStream_t s);
+
+// k_ccl.hip
+void launch_ccl(const uint32_t *bits, int nx, int ny, int nz, int conn, int invert, uint32_t *L, uint32_t *AUX,
+                int mode, uint32_t *nroots, uint32_t *rootlist, hipStream_t s, CclTimer *timer);
+void launch_ccl_argmax(const uint32_t *rootlist, const uint32_t *nroots, const uint32_t *AUX,
+                       unsigned long long *best, hipStream_t s);
+void launch_select_root(const uint32_t *bits, int nx, int ny, int nz, const uint32_t *L,
+                        const unsigned long long *best, uint32_t *out, hipStream_t s);
+void launch_fill_bubbles(uint32_t *bits, int nx, int ny, int nz, const uint32_t *L, const uint32_t *AUX,
+                         hipStream_t s);
+void launch_write_labels(const uint32_t *bits, int nx, int ny, int nz, const uint32_t *L, const uint32_t *rank,
+                         float *img, hipStream_t s);
+void launch_scatter_rank(const uint32_t *sorted, uint32_t n, uint32_t *rank, hipStream_t s);
+void launch_bits_to_float(const uint32_t *bits, int nx, int ny, int nz, float *img, hipStream_t s);
+
+// k_mc.hip
+size_t classic_blocks(int cx, int cy, int cz);
+void launch_classic_count(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
+                          uint8_t *cfgs, uint32_t *blockSums, hipStream_t s);
+void launch_classic_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
+                         const uint8_t *cfgs, const uint32_t *blockOffs, double *pts, hipStream_t s);
+size_t lw_nseg_total(int nx, int ny, int nz, const int lo[3], const int hi[3]);
+void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
+                        float mnv, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
+                        hipStream_t s);
+void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
+                    float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow, const uint32_t *bases,
+                    uint32_t nedge, double *pts, int *tris, hipStream_t s);
+
+// k_mesh.hip
+size_t scan_scratch_words(size_t n);
+void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, uint32_t *scratch,
+                   hipStream_t s);
+size_t radix_counts_words(size_t n);
+bool radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int bits,
+                    uint32_t *counts, uint32_t *scan_scratch, hipStream_t s);
+void launch_weld_keys(const double *pts, uint32_t n, uint32_t *keys, uint32_t *vals, hipStream_t s);
+void launch_weld_segflags(const uint32_t *keys, uint32_t n, uint32_t *flags, hipStream_t s);
+void launch_weld_segstarts(const uint32_t *flags, const uint32_t *segid, uint32_t n, uint32_t *starts,
+                           hipStream_t s);
+void launch_weld_greedy(const uint32_t *starts, uint32_t nseg, uint32_t n, const uint32_t *keys,
+                        const uint32_t *vals, const double *pts, int *lcls, uint32_t *seeds, hipStream_t s);
+void launch_weld_assign(const uint32_t *vals, const int *lcls, const uint32_t *flags, const uint32_t *segid,
+                        const uint32_t *seedbase, uint32_t n, int *cls, int *rep, hipStream_t s);
+void launch_gather_pts(const double *pts, const int *rep, uint32_t n, double *out, hipStream_t s);
+void launch_iota(int *out, uint32_t n, hipStream_t s);
+void launch_remap_tris(const int *tris_in, const int *cls, uint32_t nt, uint32_t npt, int *tris_out,
+                       hipStream_t s);
+void launch_degen_flags(const double *pts, const int *tris, uint32_t nt, uint32_t np, uint32_t *keep,
+                        hipStream_t s);
+void launch_compact_tris(const int *tris, const uint32_t *keep, const uint32_t *pos, uint32_t nt, int *out,
+                         hipStream_t s);
+void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, hipStream_t s);
+void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, uint32_t *cursor, uint32_t *ref,
+                     hipStream_t s);
+void launch_csr_pairs(const int *tris, uint32_t nv, const uint32_t *start, uint32_t *ref, int2 *pairs,
+                      hipStream_t s);
+void launch_border(const int *tris, uint32_t nv, const uint32_t *start, const uint32_t *ref, uint8_t *border,
+                   hipStream_t s);
+void launch_hc_a(const uint32_t *start, const int2 *pairs, const double *o, const double *p, double *pn,
+                 double *bq, uint32_t nv, double alpha, hipStream_t s);
+void launch_hc_b(const uint32_t *start, const int2 *pairs, const double *pn, const double *bq, double *pout,
+                 uint32_t nv, double beta, hipStream_t s);
+void launch_laplace(const uint32_t *start, const int2 *pairs, const double *p, double *pout, uint32_t nv,
+                    hipStream_t s);
+void launch_hc_final(const double *o, const double *p, const uint8_t *border, uint32_t nv, double *out,
+                     hipStream_t s);
+void launch_sform(double *p, uint32_t np, const float *sx, const float *sy, const float *sz, hipStream_t s);
+void launch_swap_winding(int *t, uint32_t nt, hipStream_t s);
+
+} // namespace n2m

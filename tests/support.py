@@ -1,0 +1,177 @@
+"""Test helpers: bindings of the checkers (the oracle restatement and the
+reference compiled from its own sources) and small seeded volumes."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from nii2mesh_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, 'oracle', 'liborc.so')
+REF_SO = {k: os.path.join(ROOT, 'oracle', '_ref', 'libn2mref_%s.so' % k) for k in ('classic', 'lewiner')}
+
+_cache = {}
+
+
+class Oracle(abi.MeshLib):
+    """MeshLib over the oracle plus its stage functions."""
+
+    def __init__(self, handle):
+        super().__init__(handle, prefix='orc_', cubes_arg=True)
+        h = handle
+        h.orc_quick_smooth.argtypes = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                       C.c_int]
+        h.orc_laplacian_smooth.argtypes = [np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'),
+                                           np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int]
+        h.orc_laplacian_smooth.restype = None
+        h.orc_border_vertices.argtypes = [np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                          np.ctypeslib.ndpointer(np.uint8, flags='C_CONTIGUOUS')]
+        h.orc_border_vertices.restype = None
+        h.orc_otsu.argtypes = [np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                               C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        h.orc_lewiner_select.argtypes = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                         C.POINTER(C.c_int)]
+        f32 = C.POINTER(C.c_float)
+        h.orc_apply_sform.argtypes = [np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'),
+                                      np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                      f32, f32, f32]
+        h.orc_apply_sform.restype = None
+        self.mc_classic = _bind_mc(h, 'orc_mc_classic', classic=True)
+        self.mc_lewiner = _bind_mc(h, 'orc_mc_lewiner', classic=False)
+        self.unify = abi.bind_unify(h, 'orc_unify_vertices')
+        self.degenerate = abi.bind_degenerate(h, 'orc_remove_degenerate_triangles')
+
+    def quick_smooth(self, img):
+        nz, ny, nx = img.shape
+        return self.lib.orc_quick_smooth(img, nx, ny, nz)
+
+    def laplacian_smooth(self, pts, tris, iters=1):
+        for _ in range(iters):
+            self.lib.orc_laplacian_smooth(pts, np.ascontiguousarray(tris, np.int32), len(pts), len(tris))
+        return pts
+
+    def border(self, tris, nv):
+        b = np.zeros(nv + 1, np.uint8)
+        self.lib.orc_border_vertices(np.ascontiguousarray(tris, np.int32), nv, len(tris), b)
+        return b[:nv]
+
+    def otsu(self, hist, mode):
+        d, m, b = C.c_int(), C.c_int(), C.c_int()
+        t = self.lib.orc_otsu(np.ascontiguousarray(hist, np.int32), len(hist), mode, C.byref(d), C.byref(m),
+                              C.byref(b))
+        return t, d.value, m.value, b.value
+
+    def select(self, cube, lut, originalMC=0):
+        cs = C.c_int()
+        row = self.lib.orc_lewiner_select(np.ascontiguousarray(cube, np.float32), lut, originalMC, C.byref(cs))
+        return row, cs.value
+
+    def apply_sform(self, tris, pts, sx, sy, sz):
+        f = C.c_float * 4
+        self.lib.orc_apply_sform(tris, pts, len(tris), len(pts), f(*sx), f(*sy), f(*sz))
+
+
+def _bind_mc(h, name, classic):
+    f = getattr(h, name)
+    args = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'), C.POINTER(C.c_short), C.POINTER(C.c_int),
+            C.POINTER(C.c_int)]
+    if not classic:
+        args.append(C.c_int)
+    args += [C.c_float, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    f.argtypes = args
+    f.restype = C.c_int
+
+    def call(img, lo, hi, iso, originalMC=0):
+        nz, ny, nx = img.shape
+        dim = (C.c_short * 3)(nx, ny, nz)
+        vs, ts = C.c_void_p(), C.c_void_p()
+        nv, nt = C.c_int(0), C.c_int(0)
+        a = [img, dim, (C.c_int * 3)(*lo), (C.c_int * 3)(*hi)]
+        if not classic:
+            a.append(int(originalMC))
+        a += [float(iso), C.byref(vs), C.byref(ts), C.byref(nv), C.byref(nt)]
+        rc = f(*a)
+        if rc != 0:
+            return rc, np.zeros((0, 3), np.int32), np.zeros((0, 3), np.float64)
+        return rc, abi._take(ts.value, nt.value, np.int32), abi._take(vs.value, nv.value, np.float64)
+    return call
+
+
+class Ref(abi.MeshLib):
+    """MeshLib over the reference's own sources (oracle/_ref) plus statics."""
+
+    def __init__(self, handle):
+        super().__init__(handle)
+        h = handle
+        h.ref_quick_smooth.argtypes = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                       C.c_int]
+        h.ref_laplacian_smooth.argtypes = [np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'),
+                                           np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int]
+        h.ref_laplacian_smooth.restype = None
+        h.ref_dilate.argtypes = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'), C.POINTER(C.c_size_t)]
+        h.ref_dilate.restype = None
+        self.marchingCubes = abi.bind_marching_cubes(h)
+        self.unify = abi.bind_unify(h, 'ref_unify_vertices')
+        self.degenerate = abi.bind_degenerate(h, 'ref_remove_degenerate_triangles')
+        f32 = C.POINTER(C.c_float)
+        h.apply_sform.argtypes = [np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'),
+                                  np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                  f32, f32, f32]
+        h.apply_sform.restype = None
+
+    def quick_smooth(self, img):
+        nz, ny, nx = img.shape
+        return self.lib.ref_quick_smooth(img, nx, ny, nz)
+
+    def laplacian_smooth(self, pts, tris, iters=1):
+        for _ in range(iters):
+            self.lib.ref_laplacian_smooth(pts, np.ascontiguousarray(tris, np.int32), len(pts), len(tris))
+        return pts
+
+    def dilate(self, img):
+        nz, ny, nx = img.shape
+        self.lib.ref_dilate(img, (C.c_size_t * 3)(nx, ny, nz))
+
+    def apply_sform(self, tris, pts, sx, sy, sz):
+        f = C.c_float * 4
+        self.lib.apply_sform(tris, pts, len(tris), len(pts), f(*sx), f(*sy), f(*sz))
+
+
+def oracle():
+    if 'orc' not in _cache:
+        if not os.path.exists(ORACLE_SO):
+            pytest.skip('oracle not built (make -C oracle oracle)')
+        _cache['orc'] = Oracle(C.CDLL(ORACLE_SO))
+    return _cache['orc']
+
+
+def ref(kind):
+    if kind not in _cache:
+        if not os.path.exists(REF_SO[kind]):
+            pytest.skip('reference build absent (make -C oracle ref)')
+        _cache[kind] = Ref(C.CDLL(REF_SO[kind]))
+    return _cache[kind]
+
+
+def blobs(shape, seed, nblobs=6, noise=0.0, dtype=np.float32):
+    """Sum of Gaussian blobs on a grid (a smooth field with several
+    components, holes and saddles), values roughly 0..100."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing='ij')
+    v = np.zeros(shape, np.float64)
+    for _ in range(nblobs):
+        c = rng.uniform(0, 1, 3) * np.array([nz, ny, nx])
+        s = rng.uniform(1.5, max(2.0, 0.25 * max(shape)))
+        a = rng.uniform(-60, 100)
+        v += a * np.exp(-((z - c[0]) ** 2 + (y - c[1]) ** 2 + (x - c[2]) ** 2) / (2 * s * s))
+    if noise:
+        v += rng.normal(0, noise, shape)
+    return np.ascontiguousarray(v.astype(dtype))
+
+
+def random_mask(shape, seed, p=0.5):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return (rng.uniform(size=shape) < p).astype(np.float32)

@@ -1,0 +1,190 @@
+"""GPU parity: every stage of the HIP library against the oracle, bit-exact.
+
+Integer/index work (labels, cases, triangle lists, vertex numbering) must be
+identical; floating-point outputs must be bit-identical too, because both
+sides perform the reference's operations in the same order and precision
+(src/meshify.c, src/bwlabel.c, src/oldcubes.c, src/MarchingCubes.c,
+src/quadric.c, src/isolevel.c).
+"""
+import numpy as np
+import pytest
+
+from tests import support
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(7, 9, 11), (24, 31, 40), (48, 48, 48), (33, 65, 70)]
+
+
+def _eq(a, b):
+    return a.shape == b.shape and np.array_equal(a, b)
+
+
+@pytest.mark.parametrize('shape', SHAPES + [(5, 5, 5), (4, 9, 9)])
+@pytest.mark.parametrize('seed', [1, 2])
+def test_quick_smooth(gpu, oracle, shape, seed):
+    """src/meshify.c:170-216"""
+    img = support.blobs(shape, seed, noise=3.0)
+    a, b = img.copy(), img.copy()
+    ra = gpu.load().n2m_quick_smooth(a, shape[2], shape[1], shape[0])
+    rb = oracle.quick_smooth(b)
+    assert ra == rb
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize('shape', [(13, 17, 19), (40, 33, 70), (1, 9, 9), (3, 2, 40)])
+@pytest.mark.parametrize('p', [0.3, 0.55, 0.75])
+@pytest.mark.parametrize('conn', [6, 18, 26])
+@pytest.mark.parametrize('largest,fill', [(False, False), (True, False), (False, True), (True, True)])
+def test_bwlabel(gpu, oracle, shape, p, conn, largest, fill):
+    """src/bwlabel.c:429-543: labels numbered by first voxel, largest cluster,
+    bubble fill."""
+    m = support.random_mask(shape, int(p * 100) + conn + 7 * largest + 3 * fill, p)
+    a, b = m.copy(), m.copy()
+    na = gpu.lib().bwlabel(a, conn, largest, fill)
+    nb = oracle.bwlabel(b, conn, largest, fill)
+    assert na == nb
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize('shape', SHAPES)
+@pytest.mark.parametrize('seed', [3, 4])
+@pytest.mark.parametrize('flavour', ['classic', 'lewiner', 'lewiner_o1'])
+def test_marching_cubes(gpu, oracle, shape, seed, flavour):
+    """src/oldcubes.c:465-522 and src/MarchingCubes.c:1086-1143 on the whole
+    box and on an interior sub-box (Lewiner reads past hi with wrap)."""
+    img = support.blobs(shape, seed, noise=0.5)
+    nz, ny, nx = shape
+    iso = float(np.float32(np.percentile(img, 60)))
+    L = gpu.load()
+    import nii2mesh_amd.abi as abi
+    mc = abi.bind_marching_cubes(L)
+    for lo, hi in [((0, 0, 0), (nx, ny, nz)), ((1, 2, 1), (nx - 1, ny - 2, nz))]:
+        if flavour == 'classic':
+            L.n2m_set_cubes(0)
+            got = mc(img, lo, hi, iso)
+            exp = oracle.mc_classic(img, lo, hi, iso)
+        else:
+            L.n2m_set_cubes(1)
+            o1 = 1 if flavour == 'lewiner_o1' else 0
+            got = mc(img, lo, hi, iso, o1)
+            exp = oracle.mc_lewiner(img, lo, hi, iso, o1)
+        assert got[0] == exp[0]
+        assert _eq(got[1], exp[1])
+        assert _eq(got[2], exp[2])
+
+
+def _raw_mesh(oracle, shape, seed, flavour):
+    img = support.blobs(shape, seed, noise=0.5)
+    nz, ny, nx = shape
+    iso = float(np.float32(np.percentile(img, 55)))
+    if flavour == 'classic':
+        return oracle.mc_classic(img, (0, 0, 0), (nx, ny, nz), iso)
+    return oracle.mc_lewiner(img, (0, 0, 0), (nx, ny, nz), iso, 0)
+
+
+@pytest.mark.parametrize('shape', SHAPES[1:])
+@pytest.mark.parametrize('flavour', ['classic', 'lewiner'])
+def test_unify_and_degenerate(gpu, oracle, shape, flavour):
+    """src/meshify.c:45-106 (weld) and :113-168 (degenerate filter)."""
+    import nii2mesh_amd.abi as abi
+    rc, tris, pts = _raw_mesh(oracle, shape, 5, flavour)
+    assert rc == 0
+    L = gpu.load()
+    gu = abi.bind_unify(L, 'n2m_unify_vertices')
+    gd = abi.bind_degenerate(L, 'n2m_remove_degenerate_triangles')
+    p1, t1 = gu(pts, tris)
+    p2, t2 = oracle.unify(pts, tris)
+    assert _eq(p1, p2) and _eq(t1, t2)
+    d1 = gd(p1, t1)
+    d2 = oracle.degenerate(p2, t2)
+    assert _eq(d1, d2)
+
+
+@pytest.mark.parametrize('flavour', ['classic', 'lewiner'])
+@pytest.mark.parametrize('lock', [True, False])
+def test_hc_smooth(gpu, oracle, flavour, lock):
+    """src/quadric.c:343-394: FP64, same summation order -> bit-identical."""
+    img = support.blobs((40, 44, 48), 9, noise=2.0)
+    rc, tris, pts = oracle.meshify(img.copy(), float(np.percentile(img, 50)), preSmooth=True,
+                                   onlyLargest=False, fillBubbles=False, cubes=0 if flavour == 'classic' else 1)
+    assert rc == 0
+    a, b = pts.copy(), pts.copy()
+    gpu.lib().laplacian_smoothHC(a, tris, 0.1, 0.5, 25, lock)
+    oracle.laplacian_smoothHC(b, tris, 0.1, 0.5, 25, lock)
+    assert np.array_equal(a, b)
+    # plain Laplacian (src/quadric.c:316-341)
+    c, d = pts.copy(), pts.copy()
+    gpu.load().n2m_laplacian_smooth(c, np.ascontiguousarray(tris), len(c), len(tris), 7)
+    oracle.laplacian_smooth(d, tris, 7)
+    assert np.array_equal(c, d)
+
+
+@pytest.mark.parametrize('seed', [11, 12, 13])
+@pytest.mark.parametrize('mode', [1, 2, 3])
+def test_set_threshold(gpu, oracle, seed, mode):
+    """src/isolevel.c:245-277 (GPU histograms, host Otsu)."""
+    img = support.blobs((30, 40, 50), seed, noise=4.0)
+    assert gpu.lib().setThreshold(img, mode) == oracle.setThreshold(img, mode)
+
+
+MESHIFY_CASES = [
+    # (preSmooth, onlyLargest, fillBubbles)
+    (True, True, False),
+    (True, True, True),
+    (True, False, True),
+    (False, True, False),
+    (True, False, False),
+    (False, False, False),
+]
+
+
+@pytest.mark.parametrize('shape', [(24, 31, 40), (48, 48, 48)])
+@pytest.mark.parametrize('flags', MESHIFY_CASES)
+@pytest.mark.parametrize('flavour', ['classic', 'lewiner', 'lewiner_o1'])
+def test_meshify_blobs(gpu, oracle, shape, flags, flavour):
+    """meshify() end to end (src/meshify.c:286-389), including the mutated
+    volume the reference leaves behind."""
+    img = support.blobs(shape, 21, noise=3.0)
+    iso = float(np.float32(np.percentile(img, 65)))
+    cubes = 0 if flavour == 'classic' else 1
+    o1 = 1 if flavour == 'lewiner_o1' else 0
+    a, b = img.copy(), img.copy()
+    g = gpu.lib(cubes)
+    ra = g.meshify(a, iso, *flags, originalMC=o1)
+    rb = oracle.meshify(b, iso, *flags, originalMC=o1, cubes=cubes)
+    assert ra[0] == rb[0]
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert _eq(ra[1], rb[1])
+    assert _eq(ra[2], rb[2])
+
+
+def test_meshify_iso_fallback_and_failures(gpu, oracle):
+    """Isolevel outside the range -> midpoint (src/meshify.c:316-319);
+    constant volume -> failure (:312-315)."""
+    img = support.blobs((20, 20, 20), 5)
+    for iso in (1e9, -1e9):
+        a, b = img.copy(), img.copy()
+        ra = gpu.lib(0).meshify(a, iso)
+        rb = oracle.meshify(b, iso, cubes=0)
+        assert ra[0] == rb[0] and _eq(ra[1], rb[1]) and _eq(ra[2], rb[2])
+    c = np.full((10, 10, 10), 3.0, np.float32)
+    assert gpu.lib(0).meshify(c.copy(), 1.0)[0] == 1
+
+
+@pytest.mark.parametrize('flavour,flags', [('classic', (True, True, False)), ('lewiner', (True, True, True))])
+def test_meshify_plain_u8_128(gpu, oracle, flavour, flags):
+    """The 128^3 uint8 workload at isolevel 128."""
+    from nii2mesh_amd import synth
+    img = synth.plain_u8()
+    cubes = 0 if flavour == 'classic' else 1
+    a, b = img.copy(), img.copy()
+    ra = gpu.lib(cubes).meshify(a, 128.0, *flags)
+    rb = oracle.meshify(b, 128.0, *flags, cubes=cubes)
+    assert ra[0] == rb[0] == 0
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert _eq(ra[1], rb[1]) and _eq(ra[2], rb[2])
+    pa, pb = ra[2].copy(), rb[2].copy()
+    gpu.lib().laplacian_smoothHC(pa, ra[1], 0.1, 0.5, 10, True)
+    oracle.laplacian_smoothHC(pb, rb[1], 0.1, 0.5, 10, True)
+    assert np.array_equal(pa, pb)
