@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""bench.py -- voxel-to-mesh throughput on one MI355X.
+
+Workload (one "step"): the 512x512x512 float32 volume of the large-pipeline
+workload (the 128^3 head generator scaled 4x, noise sigma 8), already in HBM:
+
+  1. dark / mid / bright multi-Otsu isolevels from GPU histograms
+     (setThreshold modes 1, 2, 3, src/isolevel.c:245-277);
+  2. for each isolevel: meshify() with pre-smooth, largest cluster and bubble
+     fill, Lewiner MC33 marching cubes, vertex welding and degenerate removal
+     (src/meshify.c:286-389) -- the pre-smooth is computed once and shared;
+  3. 100 HC smoothing iterations per mesh (alpha 0.1, beta 0.5, locked edges,
+     src/quadric.c:343-394).
+
+Metric: GVoxel/s = 512^3 voxels per step / step time.  The quadric
+reduction of the workload stays the reference's CPU code and is not part of
+the step.
+
+Also printed in the same JSON line:
+  roofline      the dominant kernel's algorithmic bytes per launch / its mean
+                launch time (HIP events on the library's stream, during the
+                timed steps), against the 8 TB/s HBM peak;
+  cpu_baseline  the reference compiled from its own sources, timed on this
+                host (1 core) on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N = 512
+HC_ITERS = 100
+MODES = [1, 2, 3]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(vol, isos, z0=224, nz=64):
+    """Reference CPU path (oracle/_ref, Lewiner build) on a z-slab of the same
+    volume: setThreshold for the three modes, then per isolevel meshify()
+    (pre-smooth, largest cluster, bubble fill) + 100 HC iterations."""
+    from tests import support  # checker bindings; the checker is only timed here
+    ref_so = support.REF_SO['lewiner']
+    kind = 'reference'
+    try:
+        if not os.path.exists(ref_so):
+            raise FileNotFoundError(ref_so)
+        lib = support.Ref(C.CDLL(ref_so))
+    except Exception:
+        lib = support.Oracle(C.CDLL(support.ORACLE_SO))
+        kind = 'port'
+    slab = np.ascontiguousarray(vol[z0:z0 + nz])
+    t0 = time.perf_counter()
+    for m in MODES:
+        lib.setThreshold(slab, m)
+    ntri = 0
+    for iso in isos:
+        img = slab.copy()
+        if kind == 'reference':
+            rc, tris, pts = lib.meshify(img, iso, True, True, True)
+        else:
+            rc, tris, pts = lib.meshify(img, iso, True, True, True, cubes=1)
+        if rc == 0:
+            lib.laplacian_smoothHC(pts, tris, 0.1, 0.5, HC_ITERS, True)
+            ntri += len(tris)
+    dt = time.perf_counter() - t0
+    vox = slab.size
+    return {'value': vox / dt / 1e9, 'unit': 'GVoxel/s', 'cores': 1, 'kind': kind,
+            'sample': '512x512x%d z-slab [%d,%d) of the same volume: setThreshold x3 + 3 isolevels x '
+                      '(meshify Lewiner, largest+bubbles, + %d HC iters), %.1f s, %d triangles'
+                      % (nz, z0, z0 + nz, HC_ITERS, dt, ntri)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU; --gpus %d is not supported (the path would '
+              'shard as z-slabs with halo exchange, not implemented here)' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import nii2mesh_amd as n2m
+    from nii2mesh_amd import synth
+
+    t = time.perf_counter()
+    vol = synth.large_f32()
+    log('generated 512^3 volume in %.1f s' % (time.perf_counter() - t))
+    d = torch.from_numpy(vol).to('cuda')
+    torch.cuda.synchronize()
+    nvox = vol.size
+    dims = (N, N, N)
+    ctx = n2m.DeviceContext()
+    prm = ctx.params(0.0, cubes=n2m.CUBES_LEWINER, preSmooth=True, onlyLargest=True, fillBubbles=True)
+    L = ctx.L
+    L.n2m_ctx_set_thresholds.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_float)]
+    modes = (C.c_int * 3)(*MODES)
+    isos_c = (C.c_float * 3)()
+
+    def step():
+        if L.n2m_ctx_set_thresholds(ctx.h, C.c_void_p(d.data_ptr()), nvox, 3, modes, isos_c) != 0:
+            raise RuntimeError(n2m.last_error())
+        isos = [float(v) for v in isos_c]
+        if ctx.pipeline(d.data_ptr(), dims, prm, isos, 0.1, 0.5, HC_ITERS) != 0:
+            raise RuntimeError('pipeline failed: ' + n2m.last_error())
+        return isos
+
+    for i in range(args.warmup):
+        isos = step()
+        log('warmup %d done, isolevels %s' % (i, isos))
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    ctx.reset_timing()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        isos = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.timing(False)
+    stats = ctx.kernel_stats()
+    ms_step = (t1 - t0) * 1000.0 / args.steps
+    sizes = [ctx.mesh_size(k) for k in range(3)]
+    ntri = sum(s[0] for s in sizes)
+    log('step %.2f ms; meshes (tris, verts): %s' % (ms_step, sizes))
+    tot_ms = sum(v[0] for v in stats.values())
+    tot_bytes = sum(v[2] for v in stats.values())
+    for k, (ms, n, b) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
+        log('  %-22s %9.3f ms/step %6d launches  %8.1f GB/s algorithmic' %
+            (k, ms / args.steps, n, (b / (ms * 1e-3) / 1e9) if ms > 0 else 0))
+    dom, (dms, dn, dbytes) = max(stats.items(), key=lambda kv: kv[1][0])
+    achieved = (dbytes / dn) / ((dms / dn) * 1e-3) / 1e9 if dn and dms else 0.0
+    roof = {'kernel': dom, 'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
+            'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
+            'bytes_per_launch': dbytes / dn if dn else 0, 'avg_launch_us': dms / dn * 1e3 if dn else 0,
+            'traffic': traffic_from_profile(dom)}
+    out = {
+        'metric': 'GVoxel/s mesh extraction at 512^3 (%HBM peak)',
+        'value': round(nvox / (ms_step * 1e-3) / 1e9, 4),
+        'unit': 'GVoxel/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(ms_step, 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': 'large pipeline 512^3 f32: 3 Otsu isolevels, Lewiner MC33, largest cluster + '
+                               'bubble fill, 100 HC iterations each', 'shape': [N, N, N],
+                   'isolevels': [round(v, 4) for v in isos], 'triangles_per_step': ntri},
+        'triangles_per_s': round(ntri / (ms_step * 1e-3), 1),
+        'pipeline_hbm_frac': round((tot_bytes / args.steps) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        'kernel_ms_per_step': round(tot_ms / args.steps, 3),
+        'roofline': roof,
+    }
+    if not args.no_cpu:
+        log('cpu baseline ...')
+        out['cpu_baseline'] = cpu_baseline(vol, isos)
+    print(json.dumps(out), flush=True)
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from profiles/traffic_<kernel>.txt
+    (FETCH_SIZE + WRITE_SIZE per dispatch, copy-kernel calibrated), if present."""
+    p = os.path.join(ROOT, 'profiles', 'traffic_%s.txt' % kernel)
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith('hbm_bytes_per_launch:'):
+            return float(line.split(':', 1)[1])
+    return None
+
+
+if __name__ == '__main__':
+    main()

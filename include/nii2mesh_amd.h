@@ -150,6 +150,10 @@ int n2m_ctx_pipeline(n2m_ctx *ctx, const float *d_img, const int dim[3], const n
 
 /* GPU histograms + host multi-Otsu on a device volume (setThreshold()). */
 float n2m_ctx_set_threshold(n2m_ctx *ctx, const float *d_img, int64_t nvox, int darkMediumBright123);
+/* Several isolevels (modes 1/2/3 = dark/mid/bright) from one set of
+ * histograms; out[i] for modes[i]. */
+int n2m_ctx_set_thresholds(n2m_ctx *ctx, const float *d_img, int64_t nvox, int n, const int *modes,
+                           float *out);
 
 /* HC smoothing of mesh `mesh` held by the context. */
 int n2m_ctx_smooth_hc(n2m_ctx *ctx, int mesh, double alpha, double beta, int iter, int lockEdges);

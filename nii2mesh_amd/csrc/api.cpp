@@ -114,6 +114,9 @@ enum Slot {
 	S_HC_B,
 	S_HC_Q,
 	S_HIST,
+	S_HC_TRIS,
+	S_HC_INV,
+	S_HC_OUT,
 	S_COUNT
 };
 
@@ -136,6 +139,8 @@ enum Small {
 	SM_VMIN = 12,  // 1 word
 	SM_TOTAL = 16, // 4 words of totals
 	SM_HSTAT = 24, // 3 words
+	SM_PBOX = 32,  // 6 words: float box of vertex positions
+	SM_FLAG = 40,  // 1 word
 	SM_WORDS = 64
 };
 
@@ -274,7 +279,7 @@ namespace {
 struct Timer : CclTimer {
 	n2m_ctx *c;
 	explicit Timer(n2m_ctx *c_) : c(c_) {}
-	void begin(const char *n) override { c->tbegin(n); }
+	void begin(const char *n, double bytes) override { c->tbegin(n, bytes); }
 	void end() override { c->tend(); }
 };
 
@@ -738,6 +743,59 @@ void check_tris(const int *tris, int ntri, int nvert)
 			fail("triangle %d references vertex %d (mesh has %d vertices)", i / 3, tris[i], nvert);
 }
 
+// Vertex renumbering used by the smoothers: Morton order of the positions.
+// perm[new] = old.  Returns false (identity) for small meshes.
+struct Relabel {
+	bool on = false;
+	const uint32_t *perm = nullptr;
+	const int *tris = nullptr;
+};
+
+Relabel morton_relabel(n2m_ctx *C, const double *pts, const int *tris, uint32_t nv, uint32_t nt)
+{
+	Relabel r;
+	if (nv < 32768)
+		return r;
+	hipStream_t s = C->stream;
+	C->tbegin("hc_relabel", 24.0 * nv * 2 + 16.0 * 4 * nv + 24.0 * nt);
+	uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+	HIPX(hipMemcpyAsync(C->d_small + SM_PBOX, init, sizeof init, hipMemcpyHostToDevice, s));
+	launch_pts_bbox(pts, nv, C->d_small + SM_PBOX, s);
+	uint32_t *k0 = C->buf<uint32_t>(S_K0, nv), *v0 = C->buf<uint32_t>(S_V0, nv);
+	uint32_t *k1 = C->buf<uint32_t>(S_K1, nv), *v1 = C->buf<uint32_t>(S_V1, nv);
+	launch_morton(pts, nv, C->d_small + SM_PBOX, k0, v0, s);
+	uint32_t *rc = C->buf<uint32_t>(S_RCOUNT, radix_counts_words(nv));
+	bool in1 = radix_sort_u32(k0, v0, k1, v1, nv, 32, rc, scan_scratch(C, radix_counts_words(nv)), s);
+	r.perm = in1 ? v1 : v0;
+	uint32_t *inv = C->buf<uint32_t>(S_HC_INV, nv);
+	launch_invert_perm(r.perm, nv, inv, s);
+	int *t2 = C->buf<int>(S_HC_TRIS, (size_t)3 * (nt ? nt : 1));
+	launch_relabel_tris(tris, inv, nt, t2, s);
+	C->tend();
+	r.tris = t2;
+	r.on = true;
+	return r;
+}
+
+// border vertices of update_mesh(iteration 0), src/quadric.c:188-215
+uint8_t *border_flags(n2m_ctx *C, const Csr &c, const int *tris, uint32_t nv, uint32_t nt)
+{
+	hipStream_t s = C->stream;
+	uint8_t *border = C->buf<uint8_t>(S_HC_BORDER, nv);
+	HIPX(hipMemsetAsync(C->d_small + SM_FLAG, 0, 4, s));
+	launch_tris_repeat(tris, nt, C->d_small + SM_FLAG, s);
+	bool repeats = C->fetch(SM_FLAG, 1)[0] != 0;
+	C->tbegin("border");
+	if (repeats) { // a triangle lists a vertex twice: count references literally
+		HIPX(hipMemsetAsync(border, 0, nv, s));
+		launch_border(tris, nv, c.start, c.ref, border, s);
+	} else {
+		launch_border_pairs(c.start, c.pairs, nv, border, s);
+	}
+	C->tend();
+	return border;
+}
+
 // laplacian_smoothHC on a device mesh in place, src/quadric.c:343-394
 void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint32_t nt, double alpha,
                       double beta, int iter, bool lockEdges)
@@ -745,12 +803,17 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 	if (nv == 0)
 		return;
 	hipStream_t s = C->stream;
-	Csr c = build_csr(C, tris, nv, nt);
+	Relabel R = morton_relabel(C, pts, tris, nv, nt);
+	const int *T = R.on ? R.tris : tris;
+	Csr c = build_csr(C, T, nv, nt);
 	size_t b = (size_t)24 * nv;
 	double *o = C->buf<double>(S_HC_O, 3 * (size_t)nv), *p = C->buf<double>(S_HC_P, 3 * (size_t)nv);
 	double *pn = C->buf<double>(S_HC_PN, 3 * (size_t)nv), *bq = C->buf<double>(S_HC_B, 3 * (size_t)nv);
-	HIPX(hipMemcpyAsync(o, pts, b, hipMemcpyDeviceToDevice, s));
-	HIPX(hipMemcpyAsync(p, pts, b, hipMemcpyDeviceToDevice, s));
+	if (R.on)
+		launch_gather_rows(pts, R.perm, nv, o, s);
+	else
+		HIPX(hipMemcpyAsync(o, pts, b, hipMemcpyDeviceToDevice, s));
+	HIPX(hipMemcpyAsync(p, o, b, hipMemcpyDeviceToDevice, s));
 	const double pair_bytes = 8.0 * 3 * nt;
 	for (int it = 0; it < iter; it++) {
 		C->tbegin("hc_a", 4.0 * nv + pair_bytes + 24.0 * nv * 2 + 48.0 * nv);
@@ -760,15 +823,14 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 		launch_hc_b(c.start, c.pairs, pn, bq, p, nv, beta, s);
 		C->tend();
 	}
-	uint8_t *border = nullptr;
-	if (lockEdges) { // border vertices keep their input position
-		border = C->buf<uint8_t>(S_HC_BORDER, nv);
-		HIPX(hipMemsetAsync(border, 0, nv, s));
-		C->tbegin("border");
-		launch_border(tris, nv, c.start, c.ref, border, s);
-		C->tend();
+	uint8_t *border = lockEdges ? border_flags(C, c, T, nv, nt) : nullptr;
+	if (R.on) {
+		double *out = C->buf<double>(S_HC_OUT, 3 * (size_t)nv);
+		launch_hc_final(o, p, border, nv, out, s);
+		launch_scatter_rows(out, R.perm, nv, pts, s);
+	} else {
+		launch_hc_final(o, p, border, nv, pts, s);
 	}
-	launch_hc_final(o, p, border, nv, pts, s);
 	C->check();
 }
 
@@ -776,19 +838,28 @@ void laplace_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint3
 {
 	if (nv == 0 || iter <= 0)
 		return;
-	Csr c = build_csr(C, tris, nv, nt);
-	double *p = C->buf<double>(S_HC_P, 3 * (size_t)nv);
-	double *cur = pts, *nxt = p;
+	hipStream_t s = C->stream;
+	Relabel R = morton_relabel(C, pts, tris, nv, nt);
+	Csr c = build_csr(C, R.on ? R.tris : tris, nv, nt);
+	double *p = C->buf<double>(S_HC_P, 3 * (size_t)nv), *q = C->buf<double>(S_HC_PN, 3 * (size_t)nv);
+	if (R.on)
+		launch_gather_rows(pts, R.perm, nv, q, s);
+	else
+		HIPX(hipMemcpyAsync(q, pts, (size_t)24 * nv, hipMemcpyDeviceToDevice, s));
+	double *cur = q, *nxt = p;
 	for (int it = 0; it < iter; it++) {
 		C->tbegin("laplace", 4.0 * nv + 8.0 * 3 * nt + 48.0 * nv);
-		launch_laplace(c.start, c.pairs, cur, nxt, nv, C->stream);
+		launch_laplace(c.start, c.pairs, cur, nxt, nv, s);
 		C->tend();
 		double *t = cur;
 		cur = nxt;
 		nxt = t;
 	}
-	if (cur != pts)
-		HIPX(hipMemcpyAsync(pts, cur, (size_t)24 * nv, hipMemcpyDeviceToDevice, C->stream));
+	if (R.on)
+		launch_scatter_rows(cur, R.perm, nv, pts, s);
+	else
+		HIPX(hipMemcpyAsync(pts, cur, (size_t)24 * nv, hipMemcpyDeviceToDevice, s));
+	C->check();
 }
 
 template <typename F> int guarded(F f, int on_error = EXIT_FAILURE)
@@ -814,6 +885,12 @@ template <typename T> T *host_copy_out(n2m_ctx *C, const void *d, size_t count)
 }
 
 int set_threshold_device(n2m_ctx *C, const float *d_img, int64_t nvox, int mode, float *result);
+struct ThrHist {
+	float mn, scl;
+	std::vector<int> h;
+};
+bool threshold_hist(n2m_ctx *C, const float *d_img, int64_t nvox, ThrHist &out);
+void thresholds_from(const ThrHist &H, int n, const int *modes, float *out);
 
 } // namespace
 
@@ -922,6 +999,20 @@ float n2m_ctx_set_threshold(n2m_ctx *C, const float *d_img, int64_t nvox, int mo
 	float r = 1.0f;
 	guarded([&] { return set_threshold_device(C, d_img, nvox, mode, &r); });
 	return r;
+}
+
+int n2m_ctx_set_thresholds(n2m_ctx *C, const float *d_img, int64_t nvox, int n, const int *modes, float *out)
+{
+	return guarded([&] {
+		ThrHist H;
+		if (!threshold_hist(C, d_img, nvox, H)) {
+			for (int i = 0; i < n; i++)
+				out[i] = 1.0f;
+			return EXIT_SUCCESS;
+		}
+		thresholds_from(H, n, modes, out);
+		return EXIT_SUCCESS;
+	});
 }
 
 int n2m_ctx_smooth_hc(n2m_ctx *C, int mesh, double alpha, double beta, int iter, int lockEdges)
@@ -1374,15 +1465,17 @@ int otsu(const int *H, int nBin, int mode, int *dark, int *mid, int *bright)
 
 // setThreshold(): GPU range + histograms, host percentiles and Otsu.
 // src/isolevel.c:35-139 (nifti_robust_range, ignoreZeroVoxels = 0), 245-277
-int set_threshold_device(n2m_ctx *C, const float *d_img, int64_t nvox, int mode, float *result)
+
+// range + the two histograms of setThreshold(); returns false where the
+// reference returns 1 early (nvox < 1)
+bool threshold_hist(n2m_ctx *C, const float *d_img, int64_t nvox, ThrHist &out)
 {
 	hipStream_t s = C->stream;
-	*result = 1.0f;
 	float mn, mx;
 	{
 		float pct2 = 0.0f, pct98 = 1.0f;
 		if (nvox < 1)
-			return 1;
+			return false;
 		uint32_t init[3] = {0xffffffffu, 0u, 0u};
 		HIPX(hipMemcpyAsync(C->d_small + SM_HSTAT, init, sizeof init, hipMemcpyHostToDevice, s));
 		C->tbegin("hist_range", 4.0 * nvox);
@@ -1457,15 +1550,42 @@ int set_threshold_device(n2m_ctx *C, const float *d_img, int64_t nvox, int mode,
 	std::vector<uint32_t> hu(kOtsuBins);
 	HIPX(hipMemcpyAsync(hu.data(), hist, 4 * kOtsuBins, hipMemcpyDeviceToHost, s));
 	C->sync();
-	std::vector<int> h(hu.begin(), hu.end());
-	int dark, mid, bright;
-	if (mode == 1 || mode == 3) {
-		otsu(h.data(), kOtsuBins, 5, &dark, &mid, &bright);
-		*result = mode == 1 ? (dark / scl) + mn : (bright / scl) + mn;
-		return 0;
+	out.mn = mn;
+	out.scl = scl;
+	out.h.assign(hu.begin(), hu.end());
+	return true;
+}
+
+// isolevels for several darkMediumBright123 modes from one set of histograms
+// (src/isolevel.c:264-276: modes 1 and 3 share the 3-threshold search)
+void thresholds_from(const ThrHist &H, int n, const int *modes, float *out)
+{
+	int d5 = -1, b5 = -1, m3 = -1, dark, mid, bright;
+	for (int i = 0; i < n; i++) {
+		if (modes[i] == 1 || modes[i] == 3) {
+			if (d5 < 0) {
+				otsu(H.h.data(), 256, 5, &dark, &mid, &bright);
+				d5 = dark;
+				b5 = bright;
+			}
+			out[i] = modes[i] == 1 ? (d5 / H.scl) + H.mn : (b5 / H.scl) + H.mn;
+		} else {
+			if (m3 < 0) {
+				otsu(H.h.data(), 256, 3, &dark, &mid, &bright);
+				m3 = mid;
+			}
+			out[i] = (m3 / H.scl) + H.mn;
+		}
 	}
-	otsu(h.data(), kOtsuBins, 3, &dark, &mid, &bright);
-	*result = (mid / scl) + mn;
+}
+
+int set_threshold_device(n2m_ctx *C, const float *d_img, int64_t nvox, int mode, float *result)
+{
+	ThrHist H;
+	*result = 1.0f;
+	if (!threshold_hist(C, d_img, nvox, H))
+		return 0;
+	thresholds_from(H, 1, &mode, result);
 	return 0;
 }
 

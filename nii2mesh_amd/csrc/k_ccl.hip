@@ -226,13 +226,16 @@ void launch_ccl(const uint32_t *bits, int nx, int ny, int nz, int conn, int inve
 {
 	Geo g{nx, ny, nz, (nx + 31) / 32};
 	dim3 grid = cc_grid(g);
-	if (timer) timer->begin("ccl_local");
+	const double nv = (double)nx * ny * nz;
+	// algorithmic bytes: mask bits in, labels out (local); mask bits
+	// (merge); labels in and out (flatten)
+	if (timer) timer->begin("ccl_local", nv / 8 + 4 * nv);
 	hipLaunchKernelGGL(k_ccl_local, grid, dim3(CC_BS), 0, s, bits, g, conn, invert, L, AUX);
 	if (timer) timer->end();
-	if (timer) timer->begin("ccl_merge");
+	if (timer) timer->begin("ccl_merge", nv / 8);
 	hipLaunchKernelGGL(k_ccl_merge, grid, dim3(CC_BS), 0, s, bits, g, conn, invert, L);
 	if (timer) timer->end();
-	if (timer) timer->begin("ccl_flatten");
+	if (timer) timer->begin("ccl_flatten", 8 * nv);
 	hipLaunchKernelGGL(k_ccl_flatten, grid, dim3(CC_BS), 0, s, bits, g, invert, L, AUX, mode, nroots,
 	                   rootlist);
 	if (timer) timer->end();

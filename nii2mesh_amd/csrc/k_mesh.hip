@@ -634,6 +634,174 @@ void launch_hc_final(const double *o, const double *p, const uint8_t *border, ui
 }
 
 // ---------------------------------------------------------------------------
+// Vertex relabelling for the smoothing gathers: vertices are renumbered in
+// Morton (Z-curve) order of their position so that a vertex's neighbours sit
+// at nearby addresses.  Triangle order is untouched, so every vertex still
+// sums its references in ascending triangle order: the result is unchanged.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_pts_bbox(const double *__restrict__ pts, uint32_t n, uint32_t *bb)
+{
+	__shared__ uint32_t s[6][4];
+	float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+	float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+		for (int k = 0; k < 3; k++) {
+			float v = (float)pts[3 * (size_t)i + k];
+			lo[k] = fminf(lo[k], v);
+			hi[k] = fmaxf(hi[k], v);
+		}
+	for (int k = 0; k < 3; k++) {
+		uint32_t a = wave_min_u32(f2ord(lo[k])), b = wave_max_u32(f2ord(hi[k]));
+		if (lane_id() == 0) {
+			s[k][wave_id()] = a;
+			s[3 + k][wave_id()] = b;
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x < 6) {
+		int k = threadIdx.x;
+		uint32_t r = s[k][0];
+		for (int w = 1; w < 4; w++)
+			r = k < 3 ? min(r, s[k][w]) : max(r, s[k][w]);
+		if (k < 3)
+			atomicMin(&bb[k], r);
+		else
+			atomicMax(&bb[k], r);
+	}
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+	v &= 0x3ffu;
+	v = (v | (v << 16)) & 0x030000ffu;
+	v = (v | (v << 8)) & 0x0300f00fu;
+	v = (v | (v << 4)) & 0x030c30c3u;
+	v = (v | (v << 2)) & 0x09249249u;
+	return v;
+}
+
+__global__ void k_morton(const double *__restrict__ pts, uint32_t n, const uint32_t *__restrict__ bb,
+                         uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	uint32_t key = 0;
+	for (int k = 0; k < 3; k++) {
+		float lo = ord2f(bb[k]), hi = ord2f(bb[3 + k]);
+		float ext = hi - lo;
+		float f = ext > 0.f ? ((float)pts[3 * (size_t)i + k] - lo) * (1023.0f / ext) : 0.f;
+		uint32_t q = f > 0.f ? (f < 1023.f ? (uint32_t)f : 1023u) : 0u; // NaN -> 0
+		key |= spread10(q) << k;
+	}
+	keys[i] = key;
+	vals[i] = i;
+}
+
+__global__ void k_invert_perm(const uint32_t *__restrict__ perm, uint32_t n, uint32_t *__restrict__ inv)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		inv[perm[i]] = i;
+}
+
+__global__ void k_gather_rows(const d3 *__restrict__ src, const uint32_t *__restrict__ perm, uint32_t n,
+                              d3 *__restrict__ dst)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		dst[i] = src[perm[i]];
+}
+
+__global__ void k_scatter_rows(const d3 *__restrict__ src, const uint32_t *__restrict__ perm, uint32_t n,
+                               d3 *__restrict__ dst)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		dst[perm[i]] = src[i];
+}
+
+__global__ void k_relabel_tris(const int *__restrict__ tris, const uint32_t *__restrict__ inv, uint32_t n3,
+                               int *__restrict__ out)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n3)
+		out[i] = (int)inv[tris[i]];
+}
+
+// 1 if any triangle repeats a vertex (then the general border rule applies)
+__global__ void k_tris_repeat(const int *__restrict__ tris, uint32_t nt, uint32_t *flag)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nt)
+		return;
+	int a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
+	if (a == b || b == c || a == c)
+		*flag = 1;
+}
+
+// Border for triangles with three distinct vertices: v is border when some
+// edge (v,a) lies in exactly one of v's triangles -- equivalent to the
+// update_mesh rule (a vertex in a single triangle has two such edges).
+__global__ void k_border_pairs(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, uint32_t nv,
+                               uint8_t *__restrict__ border)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= nv)
+		return;
+	uint32_t b = start[v], e = start[v + 1];
+	uint8_t r = 0;
+	for (uint32_t i = b; i < e && !r; i++) {
+		int2 q = pairs[i];
+		for (int s = 0; s < 2 && !r; s++) {
+			int a = s ? q.y : q.x;
+			uint32_t cnt = 0;
+			for (uint32_t j = b; j < e; j++) {
+				int2 w = pairs[j];
+				cnt += (w.x == a) + (w.y == a);
+			}
+			r = cnt == 1;
+		}
+	}
+	border[v] = r;
+}
+
+void launch_pts_bbox(const double *pts, uint32_t n, uint32_t *bb, hipStream_t s)
+{
+	unsigned b = g1(n);
+	b = b > 1024 ? 1024 : (b < 1 ? 1 : b);
+	hipLaunchKernelGGL(k_pts_bbox, dim3(b), dim3(256), 0, s, pts, n, bb);
+}
+void launch_morton(const double *pts, uint32_t n, const uint32_t *bb, uint32_t *keys, uint32_t *vals, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_morton, dim3(g1(n)), dim3(256), 0, s, pts, n, bb, keys, vals);
+}
+void launch_invert_perm(const uint32_t *perm, uint32_t n, uint32_t *inv, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_invert_perm, dim3(g1(n)), dim3(256), 0, s, perm, n, inv);
+}
+void launch_gather_rows(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_gather_rows, dim3(g1(n)), dim3(256), 0, s, (const d3 *)src, perm, n, (d3 *)dst);
+}
+void launch_scatter_rows(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_scatter_rows, dim3(g1(n)), dim3(256), 0, s, (const d3 *)src, perm, n, (d3 *)dst);
+}
+void launch_relabel_tris(const int *tris, const uint32_t *inv, uint32_t nt, int *out, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_relabel_tris, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, inv, 3 * nt, out);
+}
+void launch_tris_repeat(const int *tris, uint32_t nt, uint32_t *flag, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_tris_repeat, dim3(g1(nt)), dim3(256), 0, s, tris, nt, flag);
+}
+void launch_border_pairs(const uint32_t *start, const int2 *pairs, uint32_t nv, uint8_t *border, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_border_pairs, dim3(g1(nv)), dim3(256), 0, s, start, pairs, nv, border);
+}
+
+// ---------------------------------------------------------------------------
 // apply_sform, src/meshify.c:1021-1045 (positions only; the winding flip is
 // decided on the host from the same float sums).
 // ---------------------------------------------------------------------------

@@ -310,12 +310,14 @@ k_apply(float *__restrict__ img, int nx, int ny, int nz, int W, int xpad, float 
 	__shared__ int s_red[6][4];
 	__shared__ uint32_t s_min[4];
 	float lmin = __builtin_inff();
-	size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-	size_t rows = (size_t)ny * nz;
-	size_t row = t / xpad;
-	const int x = (int)(t % xpad);
 	int bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, bmax[3] = {-1, -1, -1};
-	if (row < rows && x < nx) {
+	const size_t rows = (size_t)ny * nz;
+	const size_t total = rows * xpad;
+	for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+		size_t row = t / xpad;
+		const int x = (int)(t % xpad);
+		if (x >= nx)
+			continue;
 		const int y = (int)(row % ny), z = (int)(row / ny);
 		size_t v = row * nx + x;
 		size_t wi = (size_t)(x >> 5) + (size_t)W * row;
@@ -325,43 +327,49 @@ k_apply(float *__restrict__ img, int nx, int ny, int nz, int W, int xpad, float 
 		if (dil && !((dil[wi] >> (x & 31)) & 1u))
 			val = mn;
 		if (val >= iso) {
-			bmin[0] = bmax[0] = x;
-			bmin[1] = bmax[1] = y;
-			bmin[2] = bmax[2] = z;
+			bmin[0] = min(bmin[0], x);
+			bmax[0] = max(bmax[0], x);
+			bmin[1] = min(bmin[1], y);
+			bmax[1] = max(bmax[1], y);
+			bmin[2] = min(bmin[2], z);
+			bmax[2] = max(bmax[2], z);
 		}
 		if (x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1)
 			val = fminf(edgeMax, val);
 		img[v] = val;
 		lmin = fminf(lmin, val);
 	}
-	uint32_t om = wave_min_u32(f2ord(lmin));
-	if (lane_id() == 0)
-		s_min[wave_id()] = om;
 	for (int k = 0; k < 3; k++) {
 		bmin[k] = wave_min_i32(bmin[k]);
 		bmax[k] = wave_max_i32(bmax[k]);
 	}
-	if (lane_id() == 0)
+	uint32_t om = wave_min_u32(f2ord(lmin));
+	if (lane_id() == 0) {
 		for (int k = 0; k < 3; k++) {
 			s_red[k][wave_id()] = bmin[k];
 			s_red[3 + k][wave_id()] = bmax[k];
 		}
+		s_min[wave_id()] = om;
+	}
 	__syncthreads();
 	if (threadIdx.x < 6) {
 		int k = threadIdx.x;
 		int r = s_red[k][0];
 		for (int w = 1; w < 4; w++)
 			r = k < 3 ? min(r, s_red[k][w]) : max(r, s_red[k][w]);
+		// a plain read first: most workgroups cannot improve the box
+		const int cur = (int)ld_agent((const uint32_t *)&bbox[k]);
 		if (k < 3) {
-			if (r != 0x7fffffff)
+			if (r != 0x7fffffff && r < cur)
 				atomicMin(&bbox[k], r);
-		} else if (r >= 0) {
+		} else if (r >= 0 && r > cur) {
 			atomicMax(&bbox[k], r);
 		}
 	}
 	if (threadIdx.x == 6) {
 		uint32_t m = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
-		atomicMin(vmin, m);
+		if (m < ld_agent(vmin))
+			atomicMin(vmin, m);
 	}
 }
 
@@ -370,8 +378,10 @@ void launch_apply(float *img, int nx, int ny, int nz, float iso, float mn, float
 {
 	int W = (nx + 31) / 32, xpad = (nx + 63) / 64 * 64;
 	size_t total = (size_t)xpad * ny * nz;
-	hipLaunchKernelGGL(k_apply, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, img, nx, ny, nz, W,
-	                   xpad, iso, mn, edgeMax, keep, dil, fill, bbox, vmin);
+	size_t blocks = (total + 255) / 256;
+	blocks = blocks > 4096 ? 4096 : (blocks < 1 ? 1 : blocks);
+	hipLaunchKernelGGL(k_apply, dim3((unsigned)blocks), dim3(256), 0, s, img, nx, ny, nz, W, xpad, iso, mn,
+	                   edgeMax, keep, dil, fill, bbox, vmin);
 }
 
 // ---------------------------------------------------------------------------

@@ -188,3 +188,34 @@ def test_meshify_plain_u8_128(gpu, oracle, flavour, flags):
     gpu.lib().laplacian_smoothHC(pa, ra[1], 0.1, 0.5, 10, True)
     oracle.laplacian_smoothHC(pb, rb[1], 0.1, 0.5, 10, True)
     assert np.array_equal(pa, pb)
+
+
+@pytest.mark.parametrize('lock', [True, False])
+def test_hc_large_mesh_relabelled(gpu, oracle, lock):
+    """Meshes above the relabelling threshold (Morton-ordered gathers) stay
+    bit-identical to the reference order of operations."""
+    img = support.blobs((96, 100, 104), 31, noise=4.0)
+    rc, tris, pts = oracle.meshify(img.copy(), float(np.percentile(img, 50)), True, False, False, cubes=1)
+    assert rc == 0 and len(pts) > 40000
+    a, b = pts.copy(), pts.copy()
+    gpu.lib().laplacian_smoothHC(a, tris, 0.1, 0.5, 15, lock)
+    oracle.laplacian_smoothHC(b, tris, 0.1, 0.5, 15, lock)
+    assert np.array_equal(a, b)
+    c, d = pts.copy(), pts.copy()
+    gpu.load().n2m_laplacian_smooth(c, np.ascontiguousarray(tris), len(c), len(tris), 4)
+    oracle.laplacian_smooth(d, tris, 4)
+    assert np.array_equal(c, d)
+
+
+@pytest.mark.parametrize('nv,nt', [(50, 80), (40000, 70000)])
+def test_hc_arbitrary_triangles(gpu, oracle, nv, nt):
+    """laplacian_smoothHC is a public entry (obj2mesh calls it): arbitrary
+    triangle lists, including repeated vertices, orphans and open borders."""
+    rng = np.random.Generator(np.random.PCG64(nv))
+    pts = rng.normal(size=(nv, 3)) * 10
+    tris = rng.integers(0, nv - 3, size=(nt, 3)).astype(np.int32)
+    tris[::7, 1] = tris[::7, 0]  # degenerate: repeated vertex
+    a, b = pts.copy(), pts.copy()
+    gpu.lib().laplacian_smoothHC(a, tris, 0.1, 0.5, 6, True)
+    oracle.laplacian_smoothHC(b, tris, 0.1, 0.5, 6, True)
+    assert np.array_equal(a, b)
