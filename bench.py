@@ -87,6 +87,9 @@ def main():
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--calib-copy', action='store_true',
+                    help='also run one device copy of the 512 MiB volume (a known-size kernel that '
+                         'calibrates FETCH_SIZE / WRITE_SIZE counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build measures one GPU; --gpus %d is not supported (the path would '
@@ -102,6 +105,11 @@ def main():
     log('generated 512^3 volume in %.1f s' % (time.perf_counter() - t))
     d = torch.from_numpy(vol).to('cuda')
     torch.cuda.synchronize()
+    if args.calib_copy:
+        e = torch.empty_like(d)
+        e.copy_(d)
+        torch.cuda.synchronize()
+        del e
     nvox = vol.size
     dims = (N, N, N)
     ctx = n2m.DeviceContext()

@@ -806,14 +806,11 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 	Relabel R = morton_relabel(C, pts, tris, nv, nt);
 	const int *T = R.on ? R.tris : tris;
 	Csr c = build_csr(C, T, nv, nt);
-	size_t b = (size_t)24 * nv;
-	double *o = C->buf<double>(S_HC_O, 3 * (size_t)nv), *p = C->buf<double>(S_HC_P, 3 * (size_t)nv);
-	double *pn = C->buf<double>(S_HC_PN, 3 * (size_t)nv), *bq = C->buf<double>(S_HC_B, 3 * (size_t)nv);
-	if (R.on)
-		launch_gather_rows(pts, R.perm, nv, o, s);
-	else
-		HIPX(hipMemcpyAsync(o, pts, b, hipMemcpyDeviceToDevice, s));
-	HIPX(hipMemcpyAsync(p, o, b, hipMemcpyDeviceToDevice, s));
+	const size_t n3 = 3 * (size_t)nv;
+	double *o = C->buf<double>(S_HC_O, n3), *p = C->buf<double>(S_HC_P, n3);
+	double *pn = C->buf<double>(S_HC_PN, n3), *bq = C->buf<double>(S_HC_B, n3);
+	launch_to4(pts, R.on ? R.perm : nullptr, nv, o, s);
+	HIPX(hipMemcpyAsync(p, o, 8 * n3, hipMemcpyDeviceToDevice, s));
 	const double pair_bytes = 8.0 * 3 * nt;
 	for (int it = 0; it < iter; it++) {
 		C->tbegin("hc_a", 4.0 * nv + pair_bytes + 24.0 * nv * 2 + 48.0 * nv);
@@ -824,13 +821,7 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 		C->tend();
 	}
 	uint8_t *border = lockEdges ? border_flags(C, c, T, nv, nt) : nullptr;
-	if (R.on) {
-		double *out = C->buf<double>(S_HC_OUT, 3 * (size_t)nv);
-		launch_hc_final(o, p, border, nv, out, s);
-		launch_scatter_rows(out, R.perm, nv, pts, s);
-	} else {
-		launch_hc_final(o, p, border, nv, pts, s);
-	}
+	launch_from4(p, o, border, R.on ? R.perm : nullptr, nv, pts, s);
 	C->check();
 }
 
@@ -841,11 +832,9 @@ void laplace_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint3
 	hipStream_t s = C->stream;
 	Relabel R = morton_relabel(C, pts, tris, nv, nt);
 	Csr c = build_csr(C, R.on ? R.tris : tris, nv, nt);
-	double *p = C->buf<double>(S_HC_P, 3 * (size_t)nv), *q = C->buf<double>(S_HC_PN, 3 * (size_t)nv);
-	if (R.on)
-		launch_gather_rows(pts, R.perm, nv, q, s);
-	else
-		HIPX(hipMemcpyAsync(q, pts, (size_t)24 * nv, hipMemcpyDeviceToDevice, s));
+	const size_t n3 = 3 * (size_t)nv;
+	double *p = C->buf<double>(S_HC_P, n3), *q = C->buf<double>(S_HC_PN, n3);
+	launch_to4(pts, R.on ? R.perm : nullptr, nv, q, s);
 	double *cur = q, *nxt = p;
 	for (int it = 0; it < iter; it++) {
 		C->tbegin("laplace", 4.0 * nv + 8.0 * 3 * nt + 48.0 * nv);
@@ -855,10 +844,7 @@ void laplace_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint3
 		cur = nxt;
 		nxt = t;
 	}
-	if (R.on)
-		launch_scatter_rows(cur, R.perm, nv, pts, s);
-	else
-		HIPX(hipMemcpyAsync(pts, cur, (size_t)24 * nv, hipMemcpyDeviceToDevice, s));
+	launch_from4(cur, cur, nullptr, R.on ? R.perm : nullptr, nv, pts, s);
 	C->check();
 }
 

@@ -23,23 +23,65 @@
 
 namespace n2m {
 
-constexpr int CC_TX = 32, CC_TY = 8, CC_TZ = 8, CC_BS = 256;
-constexpr int CC_N = CC_TX * CC_TY * CC_TZ;
+constexpr int CC_KW = 16, CC_TY = 4, CC_TZ = 4, CC_BS = CC_KW * CC_TY * CC_TZ; // 256
+constexpr int CC_TXV = CC_KW * 32;                 // voxels of x per tile (512)
+constexpr int CC_NODES = CC_TXV * CC_TY * CC_TZ;   // LDS label slots (8192)
 constexpr uint32_t NONE = 0xffffffffu;
 
-// backward neighbour offsets (dx,dy,dz): 6-, 18- and 26-connectivity, the
-// voxels earlier in raster order that src/bwlabel.c:94-131,185-210 inspect.
-__constant__ int8_t c_off[13][3] = {
-    {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},                                          // 6
-    {-1, -1, 0}, {1, -1, 0}, {-1, 0, -1}, {1, 0, -1}, {0, -1, -1}, {0, 1, -1},   // 18
-    {-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {1, 1, -1}};                         // 26
+// Run-based labelling.  A row word is split into maximal runs of set bits
+// (runs are also cut at word boundaries); a run is named by its first bit.
+// Two runs of neighbouring rows are connected when their bits overlap after
+// the x shift of a backward neighbour offset; every contiguous segment of
+// the overlap needs one union.  Backward neighbours of a row (y,z), with the
+// x shifts they allow (src/bwlabel.c:94-131,185-210):
+//   6:  (y-1,z) {0}, (y,z-1) {0}, and x-1 in the same row
+//   18: (y-1,z) {-1,0,1}, (y,z-1) {-1,0,1}, (y-1,z-1) {0}, (y+1,z-1) {0}
+//   26: 18 plus (y-1,z-1) {-1,1}, (y+1,z-1) {-1,1}
+struct NbrRow {
+	int8_t dy, dz, dxlo, dxhi;
+};
+__constant__ NbrRow c_nbr6[2] = {{-1, 0, 0, 0}, {0, -1, 0, 0}};
+__constant__ NbrRow c_nbr18[4] = {{-1, 0, -1, 1}, {0, -1, -1, 1}, {-1, -1, 0, 0}, {1, -1, 0, 0}};
+__constant__ NbrRow c_nbr26[4] = {{-1, 0, -1, 1}, {0, -1, -1, 1}, {-1, -1, -1, 1}, {1, -1, -1, 1}};
 
-__host__ __device__ inline int n_offsets(int conn) { return conn == 6 ? 3 : (conn == 18 ? 9 : 13); }
+__device__ __forceinline__ const NbrRow *nbr_rows(int conn, int *n)
+{
+	if (conn == 6) {
+		*n = 2;
+		return c_nbr6;
+	}
+	*n = 4;
+	return conn == 18 ? c_nbr18 : c_nbr26;
+}
+
+__device__ __forceinline__ uint32_t run_starts(uint32_t w) { return w & ~(w << 1); }
+
+// first bit of the run of w (starts = run_starts(w)) that contains bit b
+__device__ __forceinline__ int run_of(uint32_t starts, int b)
+{
+	uint32_t m = b == 31 ? starts : (starts & ((2u << b) - 1u));
+	return 31 - __clz(m);
+}
+
+// length of the run of w starting at bit s
+__device__ __forceinline__ int run_len(uint32_t w, int s)
+{
+	uint32_t t = ~(w >> s);
+	return t ? __ffs(t) - 1 : 32 - s;
+}
+
+__device__ __forceinline__ uint32_t run_mask(uint32_t w, int s)
+{
+	int l = run_len(w, s);
+	return (l == 32 ? 0xffffffffu : ((1u << l) - 1u)) << s;
+}
 
 struct Geo {
 	int nx, ny, nz, W;
 	__device__ uint32_t word(const uint32_t *bits, int wx, int y, int z, int invert) const
 	{
+		if (wx < 0 || wx >= W || y < 0 || y >= ny || z < 0 || z >= nz)
+			return 0u;
 		uint32_t w = bits[(size_t)wx + (size_t)W * ((size_t)y + (size_t)ny * z)];
 		if (invert) {
 			int rem = nx - wx * 32;
@@ -47,12 +89,6 @@ struct Geo {
 			w = ~w & valid;
 		}
 		return w;
-	}
-	__device__ bool fg(const uint32_t *bits, int x, int y, int z, int invert) const
-	{
-		if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz)
-			return false;
-		return (word(bits, x >> 5, y, z, invert) >> (x & 31)) & 1u;
 	}
 	__device__ uint32_t gidx(int x, int y, int z) const
 	{
@@ -89,135 +125,215 @@ __device__ __forceinline__ void lds_union(uint32_t *lab, uint32_t a, uint32_t b)
 	}
 }
 
+// Visit every (run of w, run of u) pair connected by the x shift dx
+// (u is the neighbour row word at the same word index; up / un are the
+// neighbour row words at word index -1 / +1, used for the bit that crosses
+// the word boundary).  f(bit_in_w, word_offset_of_u, bit_in_u).
+template <typename F>
+__device__ __forceinline__ void for_links(uint32_t w, uint32_t u, uint32_t up, uint32_t un, int dx, F f)
+{
+	uint32_t m;
+	if (dx == 0)
+		m = w & u;
+	else if (dx < 0)
+		m = w & (u << 1); // bit b of w touches bit b-1 of u
+	else
+		m = w & (u >> 1); // bit b of w touches bit b+1 of u
+	for (uint32_t sg = m & ~(m << 1); sg; sg &= sg - 1) {
+		int b = __ffs(sg) - 1;
+		f(b, 0, b + dx);
+	}
+	if (dx < 0 && (w & 1u) && (up >> 31))
+		f(0, -1, 31);
+	if (dx > 0 && (w >> 31) && (un & 1u))
+		f(31, 1, 0);
+}
+
+// Pass 1: label runs inside a tile of 16 words x 4 rows x 4 slices in LDS.
+// L[first voxel of run] = first voxel of the tile-local root run;
+// AUX[root] = 0 for every tile-local root.
 __global__ void __launch_bounds__(CC_BS)
 k_ccl_local(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint32_t *__restrict__ L,
             uint32_t *__restrict__ AUX)
 {
-	__shared__ uint32_t lab[CC_N];
-	__shared__ uint32_t rowbits[CC_TY * CC_TZ];
-	const int wx = blockIdx.x, ty0 = blockIdx.y * CC_TY, tz0 = blockIdx.z * CC_TZ;
-	const int lx = threadIdx.x % CC_TX, ly = threadIdx.x / CC_TX; // ly 0..7
-	if (threadIdx.x < CC_TY * CC_TZ) {
-		int ry = threadIdx.x % CC_TY, rz = threadIdx.x / CC_TY;
-		int y = ty0 + ry, z = tz0 + rz;
-		rowbits[threadIdx.x] = (y < g.ny && z < g.nz) ? g.word(bits, wx, y, z, invert) : 0u;
+	__shared__ uint32_t lab[CC_NODES];
+	__shared__ uint32_t s_w[CC_TY * CC_TZ][CC_KW];
+	const int k = threadIdx.x % CC_KW, r = threadIdx.x / CC_KW; // word in tile, row in tile
+	const int ry = r % CC_TY, rz = r / CC_TY;
+	const int wx = blockIdx.x * CC_KW + k, y = blockIdx.y * CC_TY + ry, z = blockIdx.z * CC_TZ + rz;
+	const uint32_t w = g.word(bits, wx, y, z, invert);
+	s_w[r][k] = w;
+	const uint32_t st = run_starts(w);
+	const int base = k * 32 + CC_TXV * r;
+	for (uint32_t t = st; t; t &= t - 1) {
+		int b = __ffs(t) - 1;
+		lab[base + b] = base + b;
 	}
 	__syncthreads();
-	for (int lz = 0; lz < CC_TZ; lz++) {
-		uint32_t li = lx + CC_TX * (ly + CC_TY * lz);
-		lab[li] = ((rowbits[ly + CC_TY * lz] >> lx) & 1u) ? li : NONE;
+	// x-1 across the word boundary inside the tile
+	if (k > 0 && (w & 1u) && (s_w[r][k - 1] >> 31)) {
+		uint32_t sp = run_starts(s_w[r][k - 1]);
+		lds_union(lab, base, base - 32 + run_of(sp, 31));
+	}
+	int nn;
+	const NbrRow *nb = nbr_rows(conn, &nn);
+	for (int i = 0; i < nn; i++) {
+		const int ny2 = ry + nb[i].dy, nz2 = rz + nb[i].dz;
+		if (ny2 < 0 || ny2 >= CC_TY || nz2 < 0 || nz2 >= CC_TZ)
+			continue; // another tile: merge pass
+		const int r2 = ny2 + CC_TY * nz2;
+		const uint32_t u = s_w[r2][k];
+		const uint32_t up = k > 0 ? s_w[r2][k - 1] : 0u, un = k + 1 < CC_KW ? s_w[r2][k + 1] : 0u;
+		for (int dx = nb[i].dxlo; dx <= nb[i].dxhi; dx++)
+			for_links(w, u, up, un, dx, [&](int bw, int wo, int bu) {
+				uint32_t uw = wo == 0 ? u : (wo < 0 ? up : un);
+				lds_union(lab, base + run_of(st, bw), (k + wo) * 32 + CC_TXV * r2 + run_of(run_starts(uw), bu));
+			});
 	}
 	__syncthreads();
-	const int no = n_offsets(conn);
-	for (int lz = 0; lz < CC_TZ; lz++) {
-		uint32_t li = lx + CC_TX * (ly + CC_TY * lz);
-		if (lab[li] == NONE)
-			continue;
-		for (int k = 0; k < no; k++) {
-			int nxl = lx + c_off[k][0], nyl = ly + c_off[k][1], nzl = lz + c_off[k][2];
-			if (nxl < 0 || nxl >= CC_TX || nyl < 0 || nyl >= CC_TY || nzl < 0 || nzl >= CC_TZ)
-				continue;
-			if (!((rowbits[nyl + CC_TY * nzl] >> nxl) & 1u))
-				continue;
-			lds_union(lab, li, (uint32_t)(nxl + CC_TX * (nyl + CC_TY * nzl)));
-		}
-	}
-	__syncthreads();
-	const int x = wx * 32 + lx;
-	for (int lz = 0; lz < CC_TZ; lz++) {
-		uint32_t li = lx + CC_TX * (ly + CC_TY * lz);
-		if (lab[li] == NONE)
-			continue;
-		uint32_t r = lds_find(lab, li);
-		int rx = wx * 32 + (int)(r % CC_TX), ry = ty0 + (int)((r / CC_TX) % CC_TY), rz = tz0 + (int)(r / (CC_TX * CC_TY));
-		uint32_t gi = g.gidx(x, ty0 + ly, tz0 + lz);
-		uint32_t gr = g.gidx(rx, ry, rz);
+	for (uint32_t t = st; t; t &= t - 1) {
+		int b = __ffs(t) - 1;
+		uint32_t root = lds_find(lab, base + b);
+		int rr = root / CC_TXV, rx = root % CC_TXV;
+		uint32_t gr = g.gidx(blockIdx.x * CC_TXV + rx, blockIdx.y * CC_TY + rr % CC_TY,
+		                     blockIdx.z * CC_TZ + rr / CC_TY);
+		uint32_t gi = g.gidx(wx * 32 + b, y, z);
 		L[gi] = gr;
 		if (gr == gi)
 			AUX[gi] = 0;
 	}
 }
 
+// Pass 2: links whose two runs lie in different tiles, through global
+// union-find on run names.
 __global__ void __launch_bounds__(CC_BS)
 k_ccl_merge(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint32_t *__restrict__ L)
 {
-	const int wx = blockIdx.x, ty0 = blockIdx.y * CC_TY, tz0 = blockIdx.z * CC_TZ;
-	const int lx = threadIdx.x % CC_TX, ly = threadIdx.x / CC_TX;
-	const int x = wx * 32 + lx, y = ty0 + ly;
-	const bool edge_xy = lx == 0 || lx == CC_TX - 1 || ly == 0 || ly == CC_TY - 1;
-	const int no = n_offsets(conn);
-	for (int lz = 0; lz < CC_TZ; lz++) {
-		if (!edge_xy && lz != 0)
+	const int k = threadIdx.x % CC_KW, r = threadIdx.x / CC_KW;
+	const int ry = r % CC_TY, rz = r / CC_TY;
+	const int wx = blockIdx.x * CC_KW + k, y = blockIdx.y * CC_TY + ry, z = blockIdx.z * CC_TZ + rz;
+	const uint32_t w = g.word(bits, wx, y, z, invert);
+	if (!w)
+		return;
+	const uint32_t st = run_starts(w);
+	// x-1 across a tile boundary
+	if (k == 0 && wx > 0 && (w & 1u)) {
+		uint32_t pw = g.word(bits, wx - 1, y, z, invert);
+		if (pw >> 31)
+			uf_union(L, g.gidx(wx * 32, y, z), g.gidx((wx - 1) * 32 + run_of(run_starts(pw), 31), y, z));
+	}
+	int nn;
+	const NbrRow *nb = nbr_rows(conn, &nn);
+	for (int i = 0; i < nn; i++) {
+		const int y2 = y + nb[i].dy, z2 = z + nb[i].dz;
+		const int ny2 = ry + nb[i].dy, nz2 = rz + nb[i].dz;
+		const bool other_tile_row = ny2 < 0 || ny2 >= CC_TY || nz2 < 0 || nz2 >= CC_TZ;
+		if (y2 < 0 || y2 >= g.ny || z2 < 0 || z2 >= g.nz)
 			continue;
-		const int z = tz0 + lz;
-		if (!g.fg(bits, x, y, z, invert))
-			continue;
-		uint32_t gi = g.gidx(x, y, z);
-		for (int k = 0; k < no; k++) {
-			int nxl = lx + c_off[k][0], nyl = ly + c_off[k][1], nzl = lz + c_off[k][2];
-			if (nxl >= 0 && nxl < CC_TX && nyl >= 0 && nyl < CC_TY && nzl >= 0 && nzl < CC_TZ)
-				continue; // same tile: done locally
-			int xx = x + c_off[k][0], yy = y + c_off[k][1], zz = z + c_off[k][2];
-			if (!g.fg(bits, xx, yy, zz, invert))
-				continue;
-			uf_union(L, gi, g.gidx(xx, yy, zz));
-		}
+		const uint32_t u = g.word(bits, wx, y2, z2, invert);
+		const uint32_t up = g.word(bits, wx - 1, y2, z2, invert), un = g.word(bits, wx + 1, y2, z2, invert);
+		for (int dx = nb[i].dxlo; dx <= nb[i].dxhi; dx++)
+			for_links(w, u, up, un, dx, [&](int bw, int wo, int bu) {
+				// same tile unless the row or the word lies outside it
+				bool other = other_tile_row || (wo < 0 && k == 0) || (wo > 0 && k == CC_KW - 1);
+				if (!other)
+					return;
+				uint32_t uw = wo == 0 ? u : (wo < 0 ? up : un);
+				uf_union(L, g.gidx(wx * 32 + run_of(st, bw), y, z),
+				         g.gidx((wx + wo) * 32 + run_of(run_starts(uw), bu), y2, z2));
+			});
 	}
 }
 
-// mode 0: count voxels per root into AUX[root], append roots to rootlist.
-// mode 1: AUX[root] = 1 for roots with a voxel on a volume face; count roots.
+// Pass 3, per run: L[run] = root.  mode 0: voxel count per root into
+// AUX[root] and the root list; mode 1: AUX[root] = 1 when the run touches a
+// volume face, and the number of roots.  Counts are summed per thread while
+// consecutive runs share a root, then per wave, then per workgroup, so the
+// giant component's root sees one atomic per workgroup.
 __global__ void __launch_bounds__(CC_BS)
 k_ccl_flatten(const uint32_t *__restrict__ bits, Geo g, int invert, uint32_t *__restrict__ L,
               uint32_t *__restrict__ AUX, int mode, uint32_t *__restrict__ nroots,
               uint32_t *__restrict__ rootlist)
 {
-	const int wx = blockIdx.x, ty0 = blockIdx.y * CC_TY, tz0 = blockIdx.z * CC_TZ;
-	const int lx = threadIdx.x % CC_TX, ly = threadIdx.x / CC_TX;
-	const int x = wx * 32 + lx, y = ty0 + ly;
-	for (int lz = 0; lz < CC_TZ; lz++) {
-		const int z = tz0 + lz;
-		const bool on = g.fg(bits, x, y, z, invert);
-		uint32_t gi = 0, r = NONE;
-		if (on) {
-			gi = g.gidx(x, y, z);
-			r = uf_find(L, gi);
-			L[gi] = r;
-		}
-		bool isroot = on && r == gi;
-		unsigned long long rm = __ballot(isroot);
+	__shared__ uint32_t s_root[CC_BS / N2M_WAVE], s_cnt[CC_BS / N2M_WAVE];
+	const int k = threadIdx.x % CC_KW, r = threadIdx.x / CC_KW;
+	const int wx = blockIdx.x * CC_KW + k, y = blockIdx.y * CC_TY + r % CC_TY, z = blockIdx.z * CC_TZ + r / CC_TY;
+	const uint32_t w = g.word(bits, wx, y, z, invert);
+	uint32_t nroot_local = 0, cur = NONE, cnt = 0;
+	const bool face_row = y == 0 || z == 0 || y == g.ny - 1 || z == g.nz - 1;
+	for (uint32_t t = run_starts(w); t; t &= t - 1) {
+		int b = __ffs(t) - 1;
+		uint32_t gi = g.gidx(wx * 32 + b, y, z);
+		uint32_t root = uf_find(L, gi);
+		L[gi] = root;
 		if (mode == 0) {
-			// wave-aggregated per-root counts
-			unsigned long long act = __ballot(on);
-			while (act) {
-				int leader = __ffsll((long long)act) - 1;
-				uint32_t lr = __shfl(r, leader, N2M_WAVE);
-				unsigned long long same = __ballot(on && r == lr);
-				if (lane_id() == leader)
-					atomicAdd(&AUX[lr], (uint32_t)__popcll(same));
-				act &= ~same;
+			if (root != cur) {
+				if (cnt)
+					atomicAdd(&AUX[cur], cnt);
+				cur = root;
+				cnt = 0;
 			}
-			if (rm) {
-				uint32_t base = 0;
-				int leader = __ffsll((long long)rm) - 1;
-				if (lane_id() == leader)
-					base = atomicAdd(nroots, (uint32_t)__popcll(rm));
-				base = __shfl(base, leader, N2M_WAVE);
-				if (isroot)
-					rootlist[base + __popcll(rm & ((1ull << lane_id()) - 1ull))] = gi;
-			}
+			cnt += (uint32_t)run_len(w, b);
+			if (root == gi)
+				rootlist[atomicAdd(nroots, 1u)] = gi;
 		} else {
-			if (on && (x == 0 || y == 0 || z == 0 || x == g.nx - 1 || y == g.ny - 1 || z == g.nz - 1))
-				AUX[r] = 1;
-			if (rm && lane_id() == __ffsll((long long)rm) - 1)
-				atomicAdd(nroots, (uint32_t)__popcll(rm));
+			int x0 = wx * 32 + b, x1 = x0 + run_len(w, b) - 1;
+			if ((face_row || x0 == 0 || x1 == g.nx - 1) && AUX[root] == 0)
+				AUX[root] = 1;
+			nroot_local += root == gi;
+		}
+	}
+	if (mode == 1) {
+		nroot_local = wave_sum(nroot_local);
+		if (lane_id() == 0 && nroot_local)
+			atomicAdd(nroots, nroot_local);
+		return;
+	}
+	// wave: lanes holding the same root add their counts
+	bool pend = cnt != 0;
+	bool first = true;
+	unsigned long long act = __ballot(pend);
+	while (act) {
+		int leader = __ffsll((long long)act) - 1;
+		uint32_t lr = __shfl(cur, leader, N2M_WAVE);
+		bool mine = pend && cur == lr;
+		uint32_t sum = wave_sum(mine ? cnt : 0u);
+		if (first) { // the wave's first root goes through LDS
+			if (lane_id() == 0) {
+				s_root[wave_id()] = lr;
+				s_cnt[wave_id()] = sum;
+			}
+			first = false;
+		} else if (lane_id() == leader) {
+			atomicAdd(&AUX[lr], sum);
+		}
+		if (mine)
+			pend = false;
+		act = __ballot(pend);
+	}
+	if (first && lane_id() == 0) {
+		s_root[wave_id()] = NONE;
+		s_cnt[wave_id()] = 0;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int i = 0; i < CC_BS / N2M_WAVE; i++) {
+			if (s_root[i] == NONE)
+				continue;
+			uint32_t sum = s_cnt[i];
+			for (int j = i + 1; j < CC_BS / N2M_WAVE; j++)
+				if (s_root[j] == s_root[i]) {
+					sum += s_cnt[j];
+					s_root[j] = NONE;
+				}
+			atomicAdd(&AUX[s_root[i]], sum);
 		}
 	}
 }
 
 static dim3 cc_grid(const Geo &g)
 {
-	return dim3(g.W, (g.ny + CC_TY - 1) / CC_TY, (g.nz + CC_TZ - 1) / CC_TZ);
+	return dim3((g.W + CC_KW - 1) / CC_KW, (g.ny + CC_TY - 1) / CC_TY, (g.nz + CC_TZ - 1) / CC_TZ);
 }
 
 void launch_ccl(const uint32_t *bits, int nx, int ny, int nz, int conn, int invert, uint32_t *L,
@@ -226,16 +342,15 @@ void launch_ccl(const uint32_t *bits, int nx, int ny, int nz, int conn, int inve
 {
 	Geo g{nx, ny, nz, (nx + 31) / 32};
 	dim3 grid = cc_grid(g);
-	const double nv = (double)nx * ny * nz;
-	// algorithmic bytes: mask bits in, labels out (local); mask bits
-	// (merge); labels in and out (flatten)
-	if (timer) timer->begin("ccl_local", nv / 8 + 4 * nv);
+	const double nb = (double)nx * ny * nz / 8;
+	// algorithmic bytes: mask bits in per pass; run labels are not counted
+	if (timer) timer->begin("ccl_local", nb);
 	hipLaunchKernelGGL(k_ccl_local, grid, dim3(CC_BS), 0, s, bits, g, conn, invert, L, AUX);
 	if (timer) timer->end();
-	if (timer) timer->begin("ccl_merge", nv / 8);
+	if (timer) timer->begin("ccl_merge", nb);
 	hipLaunchKernelGGL(k_ccl_merge, grid, dim3(CC_BS), 0, s, bits, g, conn, invert, L);
 	if (timer) timer->end();
-	if (timer) timer->begin("ccl_flatten", 8 * nv);
+	if (timer) timer->begin("ccl_flatten", nb);
 	hipLaunchKernelGGL(k_ccl_flatten, grid, dim3(CC_BS), 0, s, bits, g, invert, L, AUX, mode, nroots,
 	                   rootlist);
 	if (timer) timer->end();
@@ -249,8 +364,8 @@ __global__ void k_ccl_argmax(const uint32_t *__restrict__ rootlist, const uint32
 	unsigned long long b = 0;
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
 		uint32_t r = rootlist[i];
-		unsigned long long k = ((unsigned long long)AUX[r] << 32) | (unsigned long long)(0xffffffffu - r);
-		b = b > k ? b : k;
+		unsigned long long key = ((unsigned long long)AUX[r] << 32) | (unsigned long long)(0xffffffffu - r);
+		b = b > key ? b : key;
 	}
 #pragma unroll
 	for (int d = N2M_WAVE / 2; d > 0; d >>= 1) {
@@ -267,73 +382,67 @@ void launch_ccl_argmax(const uint32_t *rootlist, const uint32_t *nroots, const u
 	hipLaunchKernelGGL(k_ccl_argmax, dim3(256), dim3(256), 0, s, rootlist, nroots, AUX, best);
 }
 
-// out bit = voxel in `bits` whose root is `best` (largest cluster mask).
+// out = runs of `bits` whose root is `best` (the largest cluster), per word
 __global__ void __launch_bounds__(256)
-k_select_root(const uint32_t *__restrict__ bits, Geo g, int xpad, const uint32_t *__restrict__ L,
+k_select_root(const uint32_t *__restrict__ bits, Geo g, const uint32_t *__restrict__ L,
               const unsigned long long *__restrict__ bestp, uint32_t *__restrict__ out)
 {
 	size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-	size_t rows = (size_t)g.ny * g.nz;
-	size_t row = t / xpad;
-	int x = (int)(t % xpad);
-	if (row >= rows)
+	size_t nw = (size_t)g.W * g.ny * g.nz;
+	if (t >= nw)
 		return;
 	unsigned long long bk = *bestp;
 	uint32_t best = bk ? 0xffffffffu - (uint32_t)(bk & 0xffffffffu) : NONE;
-	int y = (int)(row % g.ny), z = (int)(row / g.ny);
-	bool p = x < g.nx && g.fg(bits, x, y, z, 0) && L[row * g.nx + x] == best;
-	unsigned long long m = __ballot(p);
-	int wx = (x & ~63) / 32;
-	size_t wi = (size_t)wx + (size_t)g.W * row;
-	if ((x & 63) == 0)
-		out[wi] = (uint32_t)m;
-	else if ((x & 63) == 32 && wx + 1 < g.W)
-		out[wi + 1] = (uint32_t)(m >> 32);
+	const int wx = (int)(t % g.W);
+	const size_t row = t / g.W;
+	const uint32_t w = bits[t];
+	uint32_t o = 0;
+	for (uint32_t s = run_starts(w); s; s &= s - 1) {
+		int b = __ffs(s) - 1;
+		if (L[row * g.nx + wx * 32 + b] == best)
+			o |= run_mask(w, b);
+	}
+	out[t] = o;
 }
 
 void launch_select_root(const uint32_t *bits, int nx, int ny, int nz, const uint32_t *L,
                         const unsigned long long *best, uint32_t *out, hipStream_t s)
 {
 	Geo g{nx, ny, nz, (nx + 31) / 32};
-	int xpad = (nx + 63) / 64 * 64;
-	size_t total = (size_t)xpad * ny * nz;
-	hipLaunchKernelGGL(k_select_root, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, bits, g, xpad,
-	                   L, best, out);
+	size_t nw = (size_t)g.W * ny * nz;
+	hipLaunchKernelGGL(k_select_root, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, bits, g, L, best, out);
 }
 
-// bubble fill (src/bwlabel.c:530-538): background voxels whose background
-// component touches no face are set.  `bits` (foreground) is updated in place.
+// bubble fill (src/bwlabel.c:530-538): background runs whose component
+// touches no face are set in the foreground `bits` (in place, per word)
 __global__ void __launch_bounds__(256)
-k_fill_bubbles(uint32_t *__restrict__ bits, Geo g, int xpad, const uint32_t *__restrict__ L,
-               const uint32_t *__restrict__ AUX)
+k_fill_bubbles(uint32_t *__restrict__ bits, Geo g, const uint32_t *__restrict__ L, const uint32_t *__restrict__ AUX)
 {
 	size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-	size_t rows = (size_t)g.ny * g.nz;
-	size_t row = t / xpad;
-	int x = (int)(t % xpad);
-	if (row >= rows)
+	size_t nw = (size_t)g.W * g.ny * g.nz;
+	if (t >= nw)
 		return;
-	int y = (int)(row % g.ny), z = (int)(row / g.ny);
-	int wx = (x & ~63) / 32;
-	size_t wi = (size_t)wx + (size_t)g.W * row;
-	bool set = false;
-	if (x < g.nx && !g.fg(bits, x, y, z, 0))
-		set = AUX[L[row * g.nx + x]] == 0;
-	unsigned long long m = __ballot(set);
-	if ((x & 63) == 0 && (uint32_t)m)
-		bits[wi] |= (uint32_t)m;
-	else if ((x & 63) == 32 && wx + 1 < g.W && (uint32_t)(m >> 32))
-		bits[wi + 1] |= (uint32_t)(m >> 32);
+	const int wx = (int)(t % g.W);
+	const size_t row = t / g.W;
+	const uint32_t fg = bits[t];
+	const int rem = g.nx - wx * 32;
+	const uint32_t bg = ~fg & (rem >= 32 ? 0xffffffffu : ((1u << rem) - 1u));
+	uint32_t add = 0;
+	for (uint32_t s = run_starts(bg); s; s &= s - 1) {
+		int b = __ffs(s) - 1;
+		if (AUX[L[row * g.nx + wx * 32 + b]] == 0)
+			add |= run_mask(bg, b);
+	}
+	if (add)
+		bits[t] = fg | add;
 }
 
 void launch_fill_bubbles(uint32_t *bits, int nx, int ny, int nz, const uint32_t *L, const uint32_t *AUX,
                          hipStream_t s)
 {
 	Geo g{nx, ny, nz, (nx + 31) / 32};
-	int xpad = (nx + 63) / 64 * 64;
-	size_t total = (size_t)xpad * ny * nz;
-	hipLaunchKernelGGL(k_fill_bubbles, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, bits, g,
-	                   xpad, L, AUX);
+	size_t nw = (size_t)g.W * ny * nz;
+	hipLaunchKernelGGL(k_fill_bubbles, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, bits, g, L, AUX);
 }
 
 // Full labelling output for bwlabel() without onlyLargest: label = rank of
@@ -349,8 +458,12 @@ k_write_labels(const uint32_t *__restrict__ bits, Geo g, const uint32_t *__restr
 		return;
 	int x = (int)(v % g.nx);
 	size_t row = v / g.nx;
-	int y = (int)(row % g.ny), z = (int)(row / g.ny);
-	img[v] = g.fg(bits, x, y, z, 0) ? (float)rank[L[v]] : 0.0f;
+	uint32_t w = bits[(size_t)(x >> 5) + (size_t)g.W * row];
+	int b = x & 31;
+	float out = 0.0f;
+	if ((w >> b) & 1u)
+		out = (float)rank[L[row * g.nx + (x & ~31) + run_of(run_starts(w), b)]];
+	img[v] = out;
 }
 
 void launch_write_labels(const uint32_t *bits, int nx, int ny, int nz, const uint32_t *L,

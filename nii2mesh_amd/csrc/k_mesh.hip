@@ -4,6 +4,7 @@
 // border detection (src/quadric.c:188-215) and apply_sform
 // (src/meshify.c:1021-1045).
 #include <float.h>
+#include <stdlib.h>
 
 #include "n2m_device.cuh"
 #include "n2m_internal.h"
@@ -487,33 +488,84 @@ struct d3 {
 	double x, y, z;
 };
 
+// Neighbour gathers of the smoothers.  Vertices are in Morton order (see
+// the relabelling below), so a workgroup of B consecutive vertices finds
+// nearly all of its neighbours among the B + 2D vertices around it: that
+// window is staged in LDS with coalesced loads, the rest come from global
+// memory.  (B, D) is chosen at run time from a few compiled shapes.
+template <int B, int D> struct Win {
+	static constexpr int W = B + 2 * D;
+	const double *__restrict__ lds; // W x 3 doubles
+	const d3 *__restrict__ g;
+	int w0;
+	__device__ __forceinline__ d3 get(int i) const
+	{
+		unsigned r = (unsigned)(i - w0);
+		if (r < (unsigned)W) {
+			d3 v;
+			v.x = lds[3 * r];
+			v.y = lds[3 * r + 1];
+			v.z = lds[3 * r + 2];
+			return v;
+		}
+		return g[i];
+	}
+};
+
+// stage p[w0, w0+W) (clamped to [0, nv)) into LDS; has a barrier
+template <int B, int D>
+__device__ __forceinline__ Win<B, D> stage_window(double *lds, const d3 *__restrict__ p, uint32_t nv)
+{
+	const int w0 = (int)(blockIdx.x * B) - D;
+	const double *src = reinterpret_cast<const double *>(p);
+	const long long lo = 3ll * w0, hi = 3ll * nv;
+	for (int i = threadIdx.x; i < 3 * Win<B, D>::W; i += B) {
+		long long e = lo + i;
+		lds[i] = (e >= 0 && e < hi) ? src[e] : 0.0;
+	}
+	__syncthreads();
+	return Win<B, D>{lds, p, w0};
+}
+
+// Laplacian term of one vertex, src/quadric.c:321-338: the sum over its
+// references (ascending triangle order) of (p[a] + p[b]), divided by twice
+// the number of references; unchanged when it has none.
+template <class WIN>
+__device__ __forceinline__ d3 laplace_at(uint32_t b, uint32_t e, const int2 *__restrict__ pairs, const WIN &w,
+                                         const d3 self)
+{
+	if (e <= b)
+		return self;
+	double sx = 0.0, sy = 0.0, sz = 0.0;
+	for (uint32_t i = b; i < e; i++) {
+		int2 q = pairs[i];
+		d3 A = w.get(q.x), B = w.get(q.y);
+		sx = sx + (A.x + B.x);
+		sy = sy + (A.y + B.y);
+		sz = sz + (A.z + B.z);
+	}
+	const double num = (double)(2 * (e - b));
+	d3 L;
+	L.x = sx / num;
+	L.y = sy / num;
+	L.z = sz / num;
+	return L;
+}
+
 // p' = L(p); b = p' - (o*alpha + p*alpha1)
-__global__ void __launch_bounds__(256)
+template <int B, int D>
+__global__ void __launch_bounds__(B)
 k_hc_a(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const d3 *__restrict__ o,
        const d3 *__restrict__ p, d3 *__restrict__ pn, d3 *__restrict__ bq, uint32_t nv, double alpha, double alpha1)
 {
-	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ double lds[3 * Win<B, D>::W];
+	const Win<B, D> w = stage_window<B, D>(lds, p, nv);
+	const uint32_t v = blockIdx.x * B + threadIdx.x;
 	if (v >= nv)
 		return;
-	uint32_t b = start[v], e = start[v + 1];
-	d3 pv = p[v], L;
-	if (e > b) {
-		double sx = 0.0, sy = 0.0, sz = 0.0;
-		for (uint32_t i = b; i < e; i++) {
-			int2 q = pairs[i];
-			d3 A = p[q.x], B = p[q.y];
-			sx = sx + (A.x + B.x);
-			sy = sy + (A.y + B.y);
-			sz = sz + (A.z + B.z);
-		}
-		double num = (double)(2 * (e - b));
-		L.x = sx / num;
-		L.y = sy / num;
-		L.z = sz / num;
-	} else {
-		L = pv;
-	}
-	d3 ov = o[v], bb;
+	const d3 pv = w.get(v), ov = o[v];
+	const d3 L = laplace_at(start[v], start[v + 1], pairs, w, pv);
+	d3 bb;
 	bb.x = L.x - (ov.x * alpha + pv.x * alpha1);
 	bb.y = L.y - (ov.y * alpha + pv.y * alpha1);
 	bb.z = L.z - (ov.z * alpha + pv.z * alpha1);
@@ -522,32 +574,19 @@ k_hc_a(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const
 }
 
 // p = p' - (b*beta + L(b)*beta1)
-__global__ void __launch_bounds__(256)
+template <int B, int D>
+__global__ void __launch_bounds__(B)
 k_hc_b(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const d3 *__restrict__ pn,
        const d3 *__restrict__ bq, d3 *__restrict__ pout, uint32_t nv, double beta, double beta1)
 {
-	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ double lds[3 * Win<B, D>::W];
+	const Win<B, D> w = stage_window<B, D>(lds, bq, nv);
+	const uint32_t v = blockIdx.x * B + threadIdx.x;
 	if (v >= nv)
 		return;
-	uint32_t b = start[v], e = start[v + 1];
-	d3 bv = bq[v], L;
-	if (e > b) {
-		double sx = 0.0, sy = 0.0, sz = 0.0;
-		for (uint32_t i = b; i < e; i++) {
-			int2 q = pairs[i];
-			d3 A = bq[q.x], B = bq[q.y];
-			sx = sx + (A.x + B.x);
-			sy = sy + (A.y + B.y);
-			sz = sz + (A.z + B.z);
-		}
-		double num = (double)(2 * (e - b));
-		L.x = sx / num;
-		L.y = sy / num;
-		L.z = sz / num;
-	} else {
-		L = bv;
-	}
-	d3 pv = pn[v], r;
+	const d3 bv = w.get(v), pv = pn[v];
+	const d3 L = laplace_at(start[v], start[v + 1], pairs, w, bv);
+	d3 r;
 	r.x = pv.x - (bv.x * beta + L.x * beta1);
 	r.y = pv.y - (bv.y * beta + L.y * beta1);
 	r.z = pv.z - (bv.z * beta + L.z * beta1);
@@ -555,39 +594,35 @@ k_hc_b(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const
 }
 
 // plain Laplacian step (src/quadric.c:316-341): pout = L(p)
-__global__ void __launch_bounds__(256)
+template <int B, int D>
+__global__ void __launch_bounds__(B)
 k_laplace(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const d3 *__restrict__ p,
           d3 *__restrict__ pout, uint32_t nv)
 {
-	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ double lds[3 * Win<B, D>::W];
+	const Win<B, D> w = stage_window<B, D>(lds, p, nv);
+	const uint32_t v = blockIdx.x * B + threadIdx.x;
 	if (v >= nv)
 		return;
-	uint32_t b = start[v], e = start[v + 1];
-	d3 L = p[v];
-	if (e > b) {
-		double sx = 0.0, sy = 0.0, sz = 0.0;
-		for (uint32_t i = b; i < e; i++) {
-			int2 q = pairs[i];
-			d3 A = p[q.x], B = p[q.y];
-			sx = sx + (A.x + B.x);
-			sy = sy + (A.y + B.y);
-			sz = sz + (A.z + B.z);
-		}
-		double num = (double)(2 * (e - b));
-		L.x = sx / num;
-		L.y = sy / num;
-		L.z = sz / num;
-	}
-	pout[v] = L;
+	pout[v] = laplace_at(start[v], start[v + 1], pairs, w, w.get(v));
 }
 
-__global__ void k_hc_final(const d3 *__restrict__ o, const d3 *__restrict__ p, const uint8_t *__restrict__ border,
-                           uint32_t nv, d3 *__restrict__ out)
+// gather through a permutation: dst[i] = src[perm ? perm[i] : i]
+__global__ void k_to4(const d3 *__restrict__ src, const uint32_t *__restrict__ perm, uint32_t n, d3 *__restrict__ dst)
 {
-	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-	if (v >= nv)
-		return;
-	out[v] = (border && border[v]) ? o[v] : p[v];
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		dst[i] = src[perm ? perm[i] : i];
+}
+
+// final HC positions (border vertices keep the original), scattered back
+// through the permutation: dst[perm ? perm[i] : i] = ...
+__global__ void k_from4(const d3 *__restrict__ p, const d3 *__restrict__ o, const uint8_t *__restrict__ border,
+                        const uint32_t *__restrict__ perm, uint32_t n, d3 *__restrict__ dst)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		dst[perm ? perm[i] : i] = (border && border[i]) ? o[i] : p[i];
 }
 
 void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, hipStream_t s)
@@ -609,28 +644,70 @@ void launch_border(const int *tris, uint32_t nv, const uint32_t *start, const ui
 {
 	hipLaunchKernelGGL(k_border, dim3(g1(nv)), dim3(256), 0, s, tris, nv, start, ref, border);
 }
+// window shapes (B vertices per workgroup, D on each side); N2M_HC_SHAPE
+// picks one (0..3), default 0
+static int hc_shape()
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("N2M_HC_SHAPE");
+		v = e ? atoi(e) : 0;
+		if (v < 0 || v > 3)
+			v = 0;
+	}
+	return v;
+}
+
+#define N2M_HC_SHAPES(X) X(0, 256, 256) X(1, 256, 128) X(2, 512, 256) X(3, 128, 128)
+
 void launch_hc_a(const uint32_t *start, const int2 *pairs, const double *o, const double *p, double *pn,
                  double *bq, uint32_t nv, double alpha, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_hc_a, dim3(g1(nv)), dim3(256), 0, s, start, pairs, (const d3 *)o, (const d3 *)p,
-	                   (d3 *)pn, (d3 *)bq, nv, alpha, 1.0 - alpha);
+	switch (hc_shape()) {
+#define X(k, B, D)                                                                                               \
+	case k:                                                                                                      \
+		hipLaunchKernelGGL((k_hc_a<B, D>), dim3((nv + B - 1) / B), dim3(B), 0, s, start, pairs, (const d3 *)o,      \
+		                   (const d3 *)p, (d3 *)pn, (d3 *)bq, nv, alpha, 1.0 - alpha);                             \
+		break;
+		N2M_HC_SHAPES(X)
+#undef X
+	}
 }
 void launch_hc_b(const uint32_t *start, const int2 *pairs, const double *pn, const double *bq, double *pout,
                  uint32_t nv, double beta, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_hc_b, dim3(g1(nv)), dim3(256), 0, s, start, pairs, (const d3 *)pn, (const d3 *)bq,
-	                   (d3 *)pout, nv, beta, 1.0 - beta);
+	switch (hc_shape()) {
+#define X(k, B, D)                                                                                               \
+	case k:                                                                                                      \
+		hipLaunchKernelGGL((k_hc_b<B, D>), dim3((nv + B - 1) / B), dim3(B), 0, s, start, pairs, (const d3 *)pn,     \
+		                   (const d3 *)bq, (d3 *)pout, nv, beta, 1.0 - beta);                                      \
+		break;
+		N2M_HC_SHAPES(X)
+#undef X
+	}
 }
 void launch_laplace(const uint32_t *start, const int2 *pairs, const double *p, double *pout, uint32_t nv,
                     hipStream_t s)
 {
-	hipLaunchKernelGGL(k_laplace, dim3(g1(nv)), dim3(256), 0, s, start, pairs, (const d3 *)p, (d3 *)pout, nv);
+	switch (hc_shape()) {
+#define X(k, B, D)                                                                                               \
+	case k:                                                                                                      \
+		hipLaunchKernelGGL((k_laplace<B, D>), dim3((nv + B - 1) / B), dim3(B), 0, s, start, pairs, (const d3 *)p,   \
+		                   (d3 *)pout, nv);                                                                        \
+		break;
+		N2M_HC_SHAPES(X)
+#undef X
+	}
 }
-void launch_hc_final(const double *o, const double *p, const uint8_t *border, uint32_t nv, double *out,
-                     hipStream_t s)
+void launch_to4(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_hc_final, dim3(g1(nv)), dim3(256), 0, s, (const d3 *)o, (const d3 *)p, border, nv,
-	                   (d3 *)out);
+	hipLaunchKernelGGL(k_to4, dim3(g1(n)), dim3(256), 0, s, (const d3 *)src, perm, n, (d3 *)dst);
+}
+void launch_from4(const double *p, const double *o, const uint8_t *border, const uint32_t *perm, uint32_t n,
+                  double *dst, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_from4, dim3(g1(n)), dim3(256), 0, s, (const d3 *)p, (const d3 *)o, border, perm, n,
+	                   (d3 *)dst);
 }
 
 // ---------------------------------------------------------------------------

@@ -100,8 +100,10 @@ void launch_hc_b(const uint32_t *start, const int2 *pairs, const double *pn, con
                  uint32_t nv, double beta, hipStream_t s);
 void launch_laplace(const uint32_t *start, const int2 *pairs, const double *p, double *pout, uint32_t nv,
                     hipStream_t s);
-void launch_hc_final(const double *o, const double *p, const uint8_t *border, uint32_t nv, double *out,
-                     hipStream_t s);
+// smoothing state uses 4 doubles per vertex (x, y, z, pad)
+void launch_to4(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s);
+void launch_from4(const double *p, const double *o, const uint8_t *border, const uint32_t *perm, uint32_t n,
+                  double *dst, hipStream_t s);
 void launch_pts_bbox(const double *pts, uint32_t n, uint32_t *bb, hipStream_t s);
 void launch_morton(const double *pts, uint32_t n, const uint32_t *bb, uint32_t *keys, uint32_t *vals,
                    hipStream_t s);

@@ -219,3 +219,57 @@ def test_hc_arbitrary_triangles(gpu, oracle, nv, nt):
     gpu.lib().laplacian_smoothHC(a, tris, 0.1, 0.5, 6, True)
     oracle.laplacian_smoothHC(b, tris, 0.1, 0.5, 6, True)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize('shape', [(9, 11, 1100), (20, 24, 600), (6, 5, 1025)])
+@pytest.mark.parametrize('p', [0.35, 0.6])
+@pytest.mark.parametrize('conn', [6, 18, 26])
+def test_bwlabel_wide_rows(gpu, oracle, shape, p, conn):
+    """Rows longer than one labelling tile (512 voxels): runs that cross
+    tile and word boundaries in x."""
+    m = support.random_mask(shape, conn + int(10 * p), p)
+    for largest, fill in [(False, False), (True, True)]:
+        a, b = m.copy(), m.copy()
+        assert gpu.lib().bwlabel(a, conn, largest, fill) == oracle.bwlabel(b, conn, largest, fill)
+        assert np.array_equal(a, b)
+
+
+def test_bwlabel_long_runs(gpu, oracle):
+    """Solid slabs and lines: runs spanning whole rows, giant components."""
+    m = np.zeros((17, 19, 700), np.float32)
+    m[2:15, 3:16, 5:690] = 1
+    m[8, :, :] = 1
+    m[:, 9, 100:600] = 0
+    for conn in (6, 18, 26):
+        for largest, fill in [(False, False), (True, False), (True, True)]:
+            a, b = m.copy(), m.copy()
+            assert gpu.lib().bwlabel(a, conn, largest, fill) == oracle.bwlabel(b, conn, largest, fill)
+            assert np.array_equal(a, b)
+
+
+def test_golden_fixtures_on_gpu(gpu):
+    """The reference's own outputs (tests/golden, written from the
+    reference build) reproduced by the HIP library."""
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+    files = sorted(f for f in os.listdir(here) if f.endswith('.npz'))
+    assert files
+    for f in files:
+        g = np.load(os.path.join(here, f))
+        kind = str(g['kind'])
+        if kind == 'meshify':
+            img = g['img'].copy()
+            rc, t, p = gpu.lib(int(g['cubes'])).meshify(img, float(g['iso']), *g['flags'].tolist(),
+                                                       originalMC=int(g['o1']))
+            assert rc == int(g['rc']) and _eq(t, g['tris']) and _eq(p, g['pts']), f
+            assert np.array_equal(img, g['img_after']), f
+            q = p.copy()
+            gpu.lib().laplacian_smoothHC(q, t, 0.1, 0.5, int(g['hc_iters']), True)
+            assert np.array_equal(q, g['pts_hc']), f
+        elif kind == 'bwlabel':
+            img = g['img'].copy()
+            n = gpu.lib().bwlabel(img, int(g['conn']), bool(g['largest']), bool(g['fill']))
+            assert n == int(g['n']) and np.array_equal(img, g['out']), f
+        elif kind == 'threshold':
+            for mode, val in zip(g['modes'], g['values']):
+                assert np.float32(gpu.lib().setThreshold(g['img'], int(mode))) == np.float32(val), f
