@@ -658,7 +658,7 @@ static int hc_shape()
 	return v;
 }
 
-#define N2M_HC_SHAPES(X) X(0, 256, 256) X(1, 256, 128) X(2, 512, 256) X(3, 128, 128)
+#define N2M_HC_SHAPES(X) X(0, 1024, 256) X(1, 512, 256) X(2, 1024, 512) X(3, 512, 128)
 
 void launch_hc_a(const uint32_t *start, const int2 *pairs, const double *o, const double *p, double *pn,
                  double *bq, uint32_t nv, double alpha, hipStream_t s)
