@@ -165,6 +165,59 @@ int n2m_ctx_download(n2m_ctx *ctx, int mesh, vec3i *t, vec3d *p);
 /* Device pointers of mesh `mesh` (valid until the next run on the context). */
 int n2m_ctx_device_mesh(n2m_ctx *ctx, int mesh, const vec3i **d_t, const vec3d **d_p);
 
+/* ---------------------------------------------------------------------------
+ * Atlas (src/nii2mesh.c:492-583, the -a path): every integer label i of an
+ * indexed volume is binarised as (i-0.5 < v < i+0.5) and meshed at isolevel
+ * prm->isolevel (the CLI passes 0.5) with onlyLargest forced off, then, like
+ * nii2() (src/nii2mesh.c:321-333), mapped through the sform (srow = 12
+ * floats: srow_x, srow_y, srow_z, or NULL for none) and HC-smoothed
+ * postSmooth times.  Labels run 1..trunc(max); a label with no voxels is
+ * skipped.  Each label's mesh equals the reference's for that label.
+ * ------------------------------------------------------------------------- */
+#define N2M_ATLAS_OK 0     /* meshed */
+#define N2M_ATLAS_EMPTY 1  /* no voxel carries the label (skipped) */
+#define N2M_ATLAS_FAILED 2 /* meshify() returned EXIT_FAILURE for it */
+
+typedef struct {
+	int label;
+	int status;      /* N2M_ATLAS_* */
+	int nt, np;      /* triangle / vertex counts */
+	int tri_offset;  /* first triangle in the atlas union mesh */
+	int vert_offset; /* first vertex in the atlas union mesh */
+} n2m_atlas_entry;
+
+/* Batched atlas: one pass over the label volume finds every label's voxel
+ * count and box; all labels are then smoothed, classified, meshed, welded
+ * and HC-smoothed together on per-label crops (box + 4 voxels).  With
+ * fillBubbles the per-label loop below is used.  Returns EXIT_SUCCESS if at
+ * least one label was meshed (the CLI's nOK > 0). */
+int n2m_ctx_atlas(n2m_ctx *ctx, const float *d_labels, const int dim[3], const n2m_params *prm,
+                  int postSmooth, const float *srow);
+/* The reference's loop run on the GPU: per label, binarise the whole volume
+ * and run meshify() + sform + HC on it.  Same results as n2m_ctx_atlas. */
+int n2m_ctx_atlas_loop(n2m_ctx *ctx, const float *d_labels, const int dim[3], const n2m_params *prm,
+                       int postSmooth, const float *srow);
+/* Number of label entries of the last atlas run (labels 1..n). */
+int n2m_ctx_atlas_count(n2m_ctx *ctx);
+int n2m_ctx_atlas_entry(n2m_ctx *ctx, int idx, n2m_atlas_entry *e);
+/* Copy one label's mesh (vertex ids local to the label) to host arrays. */
+int n2m_ctx_atlas_download(n2m_ctx *ctx, int idx, vec3i *t, vec3d *p);
+/* Device pointers of the union mesh (vertex ids global). */
+int n2m_ctx_atlas_device_mesh(n2m_ctx *ctx, const vec3i **d_t, const vec3d **d_p, int *nt, int *np);
+
+/* Host-memory form of the atlas loop for the CLI: meshes[k] for label k+1,
+ * arrays malloc'd (free with n2m_atlas_free).  Returns as n2m_ctx_atlas. */
+typedef struct {
+	int label;
+	int status;
+	int nt, np;
+	vec3i *t;
+	vec3d *p;
+} n2m_atlas_mesh;
+int n2m_atlas(const float *img, short dim[3], int originalMC, bool preSmooth, bool fillBubbles, int postSmooth,
+              const float *srow, bool verbose, n2m_atlas_mesh **meshes, int *nmesh);
+void n2m_atlas_free(n2m_atlas_mesh *meshes, int nmesh);
+
 /* Per-kernel timing with HIP events on the context stream. */
 void n2m_ctx_enable_timing(n2m_ctx *ctx, int on);
 void n2m_ctx_reset_timing(n2m_ctx *ctx);

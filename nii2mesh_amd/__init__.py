@@ -31,6 +31,13 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError('nii2mesh_amd: %s is missing; run python -m nii2mesh_amd.build' % LIB_PATH)
+        # torch ships its own libamdhip64.so.7 and HSA runtime.  Loaded first,
+        # it also serves this library (same soname); loaded after a ROCm copy,
+        # torch would bring a second runtime that cannot open the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = C.CDLL(LIB_PATH)
         _bind_ext(_lib)
     return _lib
@@ -40,6 +47,14 @@ class _Params(C.Structure):
     _fields_ = [('cubes', C.c_int), ('originalMC', C.c_int), ('isolevel', C.c_float),
                 ('preSmooth', C.c_int), ('onlyLargest', C.c_int), ('fillBubbles', C.c_int),
                 ('verbose', C.c_int)]
+
+
+class AtlasEntry(C.Structure):
+    _fields_ = [('label', C.c_int), ('status', C.c_int), ('nt', C.c_int), ('np', C.c_int),
+                ('tri_offset', C.c_int), ('vert_offset', C.c_int)]
+
+
+ATLAS_OK, ATLAS_EMPTY, ATLAS_FAILED = 0, 1, 2
 
 
 def _bind_ext(L):
@@ -76,6 +91,11 @@ def _bind_ext(L):
                                   np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
                                   f32, f32, f32]
     L.n2m_apply_sform.restype = None
+    for fn in (L.n2m_ctx_atlas, L.n2m_ctx_atlas_loop):
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(_Params), C.c_int, f32]
+    L.n2m_ctx_atlas_count.argtypes = [C.c_void_p]
+    L.n2m_ctx_atlas_entry.argtypes = [C.c_void_p, C.c_int, C.POINTER(AtlasEntry)]
+    L.n2m_ctx_atlas_download.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
 
 
 def lib(cubes=None):
@@ -133,6 +153,33 @@ class DeviceContext:
         iso = (C.c_float * len(isolevels))(*isolevels)
         return self.L.n2m_ctx_pipeline(self.h, C.c_void_p(d_ptr), d, C.byref(prm), len(isolevels), iso,
                                        float(alpha), float(beta), int(hc_iters))
+
+    def atlas(self, d_ptr, dims, prm, post_smooth=0, srow=None, loop=False):
+        """Mesh every label of a device label volume (float32, x fastest).
+        loop=True runs the per-label loop instead of the batched path."""
+        d = (C.c_int * 3)(*dims)
+        sr = None if srow is None else (C.c_float * 12)(*[float(v) for v in np.asarray(srow).ravel()])
+        fn = self.L.n2m_ctx_atlas_loop if loop else self.L.n2m_ctx_atlas
+        return fn(self.h, C.c_void_p(d_ptr), d, C.byref(prm), int(post_smooth), sr)
+
+    def atlas_entries(self):
+        out = []
+        for k in range(self.L.n2m_ctx_atlas_count(self.h)):
+            e = AtlasEntry()
+            self.L.n2m_ctx_atlas_entry(self.h, k, C.byref(e))
+            out.append(e)
+        return out
+
+    def atlas_mesh(self, k):
+        """(tris, pts) of atlas entry k (label k+1), label-local vertex ids."""
+        e = AtlasEntry()
+        if self.L.n2m_ctx_atlas_entry(self.h, k, C.byref(e)) != 0:
+            raise IndexError(k)
+        t = np.empty((e.nt, 3), np.int32)
+        p = np.empty((e.np, 3), np.float64)
+        if self.L.n2m_ctx_atlas_download(self.h, k, t.ctypes.data, p.ctypes.data) != 0:
+            raise RuntimeError(last_error())
+        return t, p
 
     def set_threshold(self, d_ptr, nvox, mode):
         return float(self.L.n2m_ctx_set_threshold(self.h, C.c_void_p(d_ptr), int(nvox), int(mode)))

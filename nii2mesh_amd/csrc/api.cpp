@@ -117,6 +117,22 @@ enum Slot {
 	S_HC_TRIS,
 	S_HC_INV,
 	S_HC_OUT,
+	S_AT_STATS,
+	S_AT_TASKS,
+	S_AT_BMAP,
+	S_AT_CROPS,
+	S_AT_RED,
+	S_AT_VBASE,
+	S_AT_VTASK,
+	S_AT_K2,
+	S_AT_V2,
+	S_AT_K3,
+	S_AT_V3,
+	S_AT_CBASE,
+	S_AT_IDENT,
+	S_AT_IDX,
+	S_AT_OUT,
+	S_AT_LABELS,
 	S_COUNT
 };
 
@@ -163,6 +179,8 @@ struct n2m_ctx {
 	hipStream_t stream = nullptr;
 	DevBuf bufs[S_COUNT];
 	std::vector<Mesh> meshes;
+	Mesh atlas_mesh;                    // union of the label meshes of the last atlas run
+	std::vector<n2m_atlas_entry> atlas; // one entry per label 1..n
 	uint32_t *d_small = nullptr;
 	uint32_t *h_small = nullptr;
 	bool timing = false;
@@ -304,6 +322,7 @@ void ctx_free(n2m_ctx *c)
 	for (auto &b : c->bufs)
 		if (b.p)
 			(void)hipFree(b.p);
+	c->meshes.push_back(c->atlas_mesh);
 	for (auto &m : c->meshes) {
 		if (m.tris.p)
 			(void)hipFree(m.tris.p);
@@ -489,8 +508,11 @@ struct McOut {
 };
 
 bool marching_cubes(n2m_ctx *C, const float *vol, const int dim[3], const int lo[3], const int hi[3], float iso,
-                    int cubes, int originalMC, float vmin, McOut &o)
+                    int cubes, int originalMC, float vmin, McOut &o, const int *origin = nullptr)
 {
+	static const int zero[3] = {0, 0, 0};
+	if (!origin)
+		origin = zero;
 	const int NX = dim[0], NY = dim[1], NZ = dim[2];
 	hipStream_t s = C->stream;
 	if (cubes == N2M_CUBES_CLASSIC) { // src/oldcubes.c:465-522
@@ -515,7 +537,7 @@ bool marching_cubes(n2m_ctx *C, const float *vol, const int dim[3], const int lo
 		double *pts = C->buf<double>(S_PTS0, (size_t)9 * T);
 		C->tbegin("mc_classic_emit", 1.0 * ncube + 72.0 * T);
 		launch_classic_emit(vol, NX, NY, NZ, lo, hi, iso, C->buf<uint8_t>(S_CFG, ncube),
-		                    C->buf<uint32_t>(S_BLOCKOFF, nb), pts, s);
+		                    C->buf<uint32_t>(S_BLOCKOFF, nb), pts, origin, s);
 		C->tend();
 		o.npt = 3 * T;
 		o.ntri = T;
@@ -543,7 +565,7 @@ bool marching_cubes(n2m_ctx *C, const float *vol, const int dim[3], const int lo
 	double *pts = C->buf<double>(S_PTS0, (size_t)3 * V);
 	int *tris = C->buf<int>(S_TRIS0, (size_t)3 * nT);
 	C->tbegin("mc_lewiner_emit", 3.0 * npts + 24.0 * V + 12.0 * nT);
-	launch_lw_emit(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, pf, cr, base, nE, pts, tris, s);
+	launch_lw_emit(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, pf, cr, base, nE, pts, tris, origin, s);
 	C->tend();
 	o.npt = V;
 	o.ntri = nT;
@@ -881,6 +903,582 @@ void thresholds_from(const ThrHist &H, int n, const int *modes, float *out);
 } // namespace
 
 // ===========================================================================
+// Atlas (src/nii2mesh.c:492-583)
+// ===========================================================================
+namespace {
+
+template <typename T> void upload(n2m_ctx *C, T *d, const std::vector<T> &h)
+{
+	if (h.empty())
+		return;
+	HIPX(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, C->stream));
+	C->sync(); // h may change right after
+}
+
+template <typename T> std::vector<T> download(n2m_ctx *C, const T *d, size_t n)
+{
+	std::vector<T> h(n);
+	if (n) {
+		HIPX(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, C->stream));
+		C->sync();
+	}
+	return h;
+}
+
+// d[idx[i]] for each i
+std::vector<uint32_t> values_at(n2m_ctx *C, const uint32_t *d, const std::vector<uint32_t> &idx)
+{
+	uint32_t *di = C->buf<uint32_t>(S_AT_IDX, idx.size());
+	upload(C, di, idx);
+	uint32_t *dout = C->buf<uint32_t>(S_AT_OUT, idx.size());
+	launch_gather_u32(d, di, (uint32_t)idx.size(), dout, C->stream);
+	C->check();
+	return download(C, dout, idx.size());
+}
+
+// grow b to `bytes`, keeping its first `keep` bytes
+void grow_keep(n2m_ctx *C, DevBuf &b, size_t bytes, size_t keep)
+{
+	if (b.cap >= bytes)
+		return;
+	size_t want = bytes > 2 * b.cap ? bytes : 2 * b.cap;
+	void *p = nullptr;
+	HIPX(hipMalloc(&p, want));
+	if (keep)
+		HIPX(hipMemcpyAsync(p, b.p, keep, hipMemcpyDeviceToDevice, C->stream));
+	C->sync();
+	if (b.p)
+		HIPX(hipFree(b.p));
+	b.p = p;
+	b.cap = want;
+}
+
+// U += (pts, tris): vertex ids of tris shifted by U's vertex count
+void append_mesh(n2m_ctx *C, Mesh &U, const double *pts, uint32_t np, const int *tris, uint32_t nt)
+{
+	grow_keep(C, U.pts, (size_t)24 * ((size_t)U.np + np), (size_t)24 * U.np);
+	grow_keep(C, U.tris, (size_t)12 * ((size_t)U.nt + nt), (size_t)12 * U.nt);
+	if (np)
+		HIPX(hipMemcpyAsync((double *)U.pts.p + (size_t)3 * U.np, pts, (size_t)24 * np, hipMemcpyDeviceToDevice,
+		                    C->stream));
+	launch_add_i32(tris, (size_t)3 * nt, U.np, (int *)U.tris.p + (size_t)3 * U.nt, C->stream);
+	C->check();
+	if ((size_t)U.np + np > (size_t)INT32_MAX || (size_t)U.nt + nt > (size_t)INT32_MAX)
+		fail("atlas mesh exceeds 2^31 vertices or triangles");
+	U.np += (int)np;
+	U.nt += (int)nt;
+}
+
+// label count as the CLI finds it: trunc(max), src/nii2mesh.c:494-501
+int atlas_label_count(n2m_ctx *C, const float *labels, size_t nvox)
+{
+	uint32_t init[2] = {0xffffffffu, 0u};
+	HIPX(hipMemcpyAsync(C->d_small + SM_RANGE, init, sizeof init, hipMemcpyHostToDevice, C->stream));
+	launch_minmax(labels, nvox, C->d_small + SM_RANGE, C->stream);
+	C->check();
+	uint32_t *r = C->fetch(SM_RANGE, 2);
+	float mn = ord2f_host(r[0]), mx = ord2f_host(r[1]);
+	if (!(mx >= 1.0f)) {
+		printf("intensity range not consistent with an indexed atlas %g..%g\n", mn, mx);
+		fail("no label >= 1 in the atlas volume");
+	}
+	if (mx >= 16777216.0f)
+		fail("atlas label %g is beyond the supported 2^24", mx);
+	return (int)truncf(mx);
+}
+
+void sform_and_smooth(n2m_ctx *C, Mesh &m, int postSmooth, const float *srow)
+{
+	if (srow && m.np) { // src/meshify.c:1025-1045
+		launch_sform((double *)m.pts.p, m.np, srow, srow + 4, srow + 8, C->stream);
+		double px = srow[0] + srow[1] + srow[2];
+		double py = srow[4] + srow[5] + srow[6];
+		double pz = srow[8] + srow[9] + srow[10];
+		float det = px * py * pz;
+		if (det < 0.0)
+			launch_swap_winding((int *)m.tris.p, m.nt, C->stream);
+		C->check();
+	}
+	if (postSmooth > 0 && m.np) // src/nii2mesh.c:327-331
+		smooth_hc_device(C, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, 0.1, 0.5, postSmooth, true);
+}
+
+// Tasks (labels) whose crops share one pass.  entry[k] = index of task k's
+// label in C->atlas.  Meshes of the tasks are appended to U in task order.
+void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_params &P, std::vector<AtlasTask> T,
+                 std::vector<int> entry, Mesh &U)
+{
+	hipStream_t s = C->stream;
+	const int NX = dim[0], NY = dim[1], NZ = dim[2];
+	const bool smooth = P.preSmooth && NX >= 5 && NY >= 5 && NZ >= 5; // src/meshify.c:171
+	auto failed = [&](int k) { C->atlas[entry[k]].status = N2M_ATLAS_FAILED; };
+	size_t nfl = 0;
+	for (auto &t : T) {
+		t.off = (long long)nfl;
+		nfl += (size_t)t.nx * t.ny * t.nz;
+	}
+	float *crops = C->buf<float>(S_AT_CROPS, nfl);
+	std::vector<uint32_t> bmap;
+	auto plan = [&](auto blocks_of) {
+		bmap.clear();
+		for (size_t k = 0; k < T.size(); k++) {
+			T[k].first_block = (int)bmap.size();
+			bmap.insert(bmap.end(), (size_t)blocks_of(T[k]), (uint32_t)k);
+		}
+		if (bmap.size() >= ((size_t)1 << 31))
+			fail("atlas launch of %zu workgroups", bmap.size());
+		upload(C, C->buf<AtlasTask>(S_AT_TASKS, T.size()), T);
+		upload(C, C->buf<uint32_t>(S_AT_BMAP, bmap.size()), bmap);
+		return (unsigned)bmap.size();
+	};
+	auto dT = [&] { return C->buf<AtlasTask>(S_AT_TASKS, T.size()); };
+	auto dmap = [&] { return C->buf<uint32_t>(S_AT_BMAP, bmap.size()); };
+
+	// 1. binarise + pre-smooth every crop, per-task range (src/meshify.c:299-311)
+	unsigned nb = plan([&](const AtlasTask &t) { return atlas_smooth_blocks(t.nx, t.ny, t.nz, smooth); });
+	uint32_t *red = C->buf<uint32_t>(S_AT_RED, (size_t)9 * T.size());
+	{
+		std::vector<uint32_t> init(2 * T.size());
+		for (size_t k = 0; k < T.size(); k++) {
+			init[2 * k] = 0xffffffffu;
+			init[2 * k + 1] = 0u;
+		}
+		upload(C, red, init);
+	}
+	C->tbegin("atlas_smooth", 8.0 * nfl);
+	launch_atlas_smooth(labels, NX, NY, NZ, dT(), dmap(), nb, smooth ? 1 : 0, crops, red, s);
+	C->tend();
+	{
+		std::vector<uint32_t> rr = download(C, red, 2 * T.size());
+		std::vector<AtlasTask> L;
+		std::vector<int> LE;
+		for (size_t k = 0; k < T.size(); k++) {
+			AtlasTask t = T[k];
+			float mn = ord2f_host(rr[2 * k]), mx = ord2f_host(rr[2 * k + 1]);
+			if (mn == mx) { // src/meshify.c:312-315
+				printf("Error: No variability in image intensity.\n");
+				failed((int)k);
+				continue;
+			}
+			float isolevel = P.isolevel;
+			if ((isolevel <= mn) || (isolevel > mx)) { // src/meshify.c:316-319
+				isolevel = 0.5 * (mn + mx);
+				printf("Suggested isolevel out of range. Intensity range %g..%g, setting isolevel to %g\n", mn, mx,
+				       isolevel);
+			}
+			if (P.verbose)
+				printf("intensity range %g..%g, isolevel %g\n", mn, mx, isolevel);
+			t.iso = isolevel;
+			t.mn = mn;
+			t.edgeMax = 0.75 * (mn + isolevel);
+			L.push_back(t);
+			LE.push_back(entry[k]);
+		}
+		T.swap(L);
+		entry.swap(LE);
+	}
+	if (T.empty())
+		return;
+	const size_t nt_ = T.size();
+
+	// 2. edge darkening, box of >= iso, minimum (src/meshify.c:347-371)
+	nb = plan([&](const AtlasTask &t) { return atlas_apply_blocks(t.nx, t.ny, t.nz); });
+	{
+		std::vector<uint32_t> init(7 * nt_);
+		for (size_t k = 0; k < nt_; k++) {
+			for (int j = 0; j < 3; j++) {
+				init[6 * k + j] = 0x7fffffffu;
+				init[6 * k + 3 + j] = 0xffffffffu; // -1
+			}
+			init[6 * nt_ + k] = 0xffffffffu;
+		}
+		upload(C, red, init);
+	}
+	C->tbegin("atlas_apply", 8.0 * nfl);
+	launch_atlas_apply(crops, NX, NY, NZ, dT(), dmap(), nb, (int *)red, red + 6 * nt_, s);
+	C->tend();
+	{
+		std::vector<uint32_t> bb = download(C, red, 7 * nt_);
+		for (size_t k = 0; k < nt_; k++) {
+			AtlasTask &t = T[k];
+			const int org[3] = {t.ox, t.oy, t.oz}, csz[3] = {t.nx, t.ny, t.nz};
+			t.mnv = ord2f_host(bb[6 * nt_ + k]) - t.iso;
+			if ((int)bb[6 * k + 3] < 0) {
+				// no voxel >= iso: cannot happen (the maximum is >= iso); the
+				// reference's marching cubes would find nothing
+				printf("marching cubes failed to identify triangles with an isolevel of %g\n", t.iso);
+				failed((int)k);
+				for (int i = 0; i < 3; i++)
+					t.lo3[i] = t.hi3[i] = 0;
+				continue;
+			}
+			for (int i = 0; i < 3; i++) {
+				int glo = (int)bb[6 * k + i] + org[i], ghi = (int)bb[6 * k + 3 + i] + org[i];
+				glo = glo - 1 > 0 ? glo - 1 : 0;
+				ghi = ghi + 2 < dim[i] ? ghi + 2 : dim[i];
+				t.lo3[i] = glo - org[i];
+				t.hi3[i] = ghi - org[i];
+				// the crop holds the label's box + 4 (clipped), the box of >= iso
+				// lies within the label's box + 2, so lo3 >= 0 and hi3 <= csz
+				if (t.lo3[i] < 0 || t.hi3[i] > csz[i])
+					fail("atlas label %d: box outside its crop", t.label);
+			}
+		}
+	}
+
+	// 3. marching cubes
+	uint32_t Vtot = 0, Ttot = 0;
+	bool implicit = P.cubes == N2M_CUBES_CLASSIC;
+	std::vector<uint32_t> vbase(nt_ + 1), tbase(nt_ + 1);
+	if (implicit) { // src/oldcubes.c:465-522
+		size_t cubes = 0;
+		nb = plan([&](AtlasTask &t) {
+			int cx = t.hi3[0] - 1 - t.lo3[0], cy = t.hi3[1] - 1 - t.lo3[1], cz = t.hi3[2] - 1 - t.lo3[2];
+			size_t nc = (cx > 0 && cy > 0 && cz > 0) ? (size_t)cx * cy * cz : 0;
+			t.cube_off = (long long)cubes;
+			cubes += nc;
+			return nc ? (unsigned)classic_blocks(cx, cy, cz) : 0u;
+		});
+		uint8_t *cfg = C->buf<uint8_t>(S_CFG, cubes);
+		uint32_t *bs = C->buf<uint32_t>(S_BLOCK, (size_t)nb + 1), *bo = C->buf<uint32_t>(S_BLOCKOFF, (size_t)nb + 1);
+		HIPX(hipMemsetAsync(bs + nb, 0, 4, s));
+		C->tbegin("atlas_mc_classic_count", 4.0 * cubes + cubes);
+		launch_atlas_classic_count(crops, dT(), dmap(), nb, cfg, bs, s);
+		C->tend();
+		scan_excl_u32(bs, bo, (size_t)nb + 1, nullptr, scan_scratch(C, (size_t)nb + 1), s);
+		C->check();
+		std::vector<uint32_t> idx(nt_ + 1);
+		for (size_t k = 0; k < nt_; k++)
+			idx[k] = (uint32_t)T[k].first_block;
+		idx[nt_] = nb;
+		tbase = values_at(C, bo, idx);
+		Ttot = tbase[nt_];
+		for (size_t k = 0; k <= nt_; k++)
+			vbase[k] = 3 * tbase[k];
+		Vtot = 3 * Ttot;
+		for (size_t k = 0; k < nt_; k++)
+			if (tbase[k + 1] == tbase[k]) {
+				printf("marching cubes failed to identify triangles with an isolevel of %g\n", T[k].iso);
+				failed((int)k);
+			}
+		if (Ttot) {
+			double *pts = C->buf<double>(S_PTS0, (size_t)9 * Ttot);
+			C->tbegin("atlas_mc_classic_emit", 1.0 * cubes + 72.0 * Ttot);
+			launch_atlas_classic_emit(crops, dT(), dmap(), nb, cfg, bo, pts, s);
+			C->tend();
+		}
+	} else { // src/MarchingCubes.c:1086-1143
+		size_t npts = 0;
+		const int seg = lw_seg_len();
+		nb = plan([&](AtlasTask &t) {
+			int sx = t.hi3[0] - t.lo3[0] + 1, sy = t.hi3[1] - t.lo3[1] + 1, sz = t.hi3[2] - t.lo3[2] + 1;
+			t.nseg = (sx + seg - 1) / seg;
+			t.cube_off = (long long)npts;
+			npts += (size_t)sx * sy * sz;
+			return (unsigned)((size_t)t.nseg * sy * sz);
+		});
+		const size_t stride = (size_t)nb + 1;
+		uint8_t *pf = C->buf<uint8_t>(S_PFLAGS, npts);
+		uint16_t *cr = C->buf<uint16_t>(S_CROW, npts);
+		uint32_t *cnt = C->buf<uint32_t>(S_SEGCNT, 3 * stride), *base = C->buf<uint32_t>(S_SEGBASE, 3 * stride);
+		for (int j = 0; j < 3; j++)
+			HIPX(hipMemsetAsync(cnt + j * stride + nb, 0, 4, s));
+		C->tbegin("atlas_mc_lewiner_classify", 7.0 * npts);
+		launch_atlas_lw_classify(crops, dT(), dmap(), nb, P.originalMC, pf, cr, cnt, stride, s);
+		C->tend();
+		for (int j = 0; j < 3; j++)
+			scan_excl_u32(cnt + j * stride, base + j * stride, stride, nullptr, scan_scratch(C, stride), s);
+		C->check();
+		std::vector<uint32_t> idx(3 * (nt_ + 1));
+		for (int j = 0; j < 3; j++) {
+			for (size_t k = 0; k < nt_; k++)
+				idx[j * (nt_ + 1) + k] = (uint32_t)(j * stride + T[k].first_block);
+			idx[j * (nt_ + 1) + nt_] = (uint32_t)(j * stride + nb);
+		}
+		std::vector<uint32_t> v = values_at(C, base, idx);
+		const uint32_t *E = v.data(), *Tb = v.data() + (nt_ + 1), *Cb = v.data() + 2 * (nt_ + 1);
+		for (size_t k = 0; k <= nt_; k++) {
+			vbase[k] = E[k] + Cb[k];
+			tbase[k] = Tb[k];
+		}
+		Vtot = vbase[nt_];
+		Ttot = tbase[nt_];
+		for (size_t k = 0; k < nt_; k++)
+			if (vbase[k + 1] - vbase[k] < 3 || tbase[k + 1] == tbase[k])
+				failed((int)k);
+		if (Vtot) {
+			double *pts = C->buf<double>(S_PTS0, (size_t)3 * Vtot);
+			int *tris = C->buf<int>(S_TRIS0, (size_t)3 * (Ttot ? Ttot : 1));
+			C->tbegin("atlas_mc_lewiner_emit", 3.0 * npts + 24.0 * Vtot + 12.0 * Ttot);
+			launch_atlas_lw_emit(crops, dT(), (int)nt_, dmap(), nb, P.originalMC, pf, cr, base, stride, E[nt_], pts,
+			                     tris, s);
+			C->tend();
+		}
+	}
+	if (!Vtot)
+		return;
+
+	// 4. weld each task's mesh on its own (src/meshify.c:45-106)
+	const double *pts0 = C->buf<double>(S_PTS0, (size_t)3 * Vtot);
+	uint32_t *dvb = C->buf<uint32_t>(S_AT_VBASE, nt_ + 1);
+	upload(C, dvb, vbase);
+	uint32_t *k0 = C->buf<uint32_t>(S_K0, Vtot), *v0 = C->buf<uint32_t>(S_V0, Vtot);
+	uint32_t *k1 = C->buf<uint32_t>(S_K1, Vtot), *v1 = C->buf<uint32_t>(S_V1, Vtot);
+	uint32_t *vtask = C->buf<uint32_t>(S_AT_VTASK, Vtot);
+	uint32_t *rc = C->buf<uint32_t>(S_RCOUNT, radix_counts_words(Vtot));
+	uint32_t *scr = scan_scratch(C, radix_counts_words(Vtot));
+	C->tbegin("atlas_weld_sort", 24.0 * Vtot + 12.0 * Vtot + 4 * 16.0 * Vtot);
+	launch_atlas_weld_keys(pts0, Vtot, dvb, (int)nt_, k0, v0, vtask, s);
+	bool in1 = radix_sort_u32(k0, v0, k1, v1, Vtot, 32, rc, scr, s);
+	uint32_t *K = in1 ? k1 : k0, *V = in1 ? v1 : v0;
+	uint32_t *tk = in1 ? k0 : k1, *pos = in1 ? v0 : v1;
+	uint32_t *tk2 = C->buf<uint32_t>(S_AT_K2, Vtot), *pos2 = C->buf<uint32_t>(S_AT_V2, Vtot);
+	launch_atlas_task_keys(V, vtask, Vtot, tk, pos, s);
+	const int tbits = nt_ <= 256 ? 8 : (nt_ <= 65536 ? 16 : 24);
+	bool in2 = radix_sort_u32(tk, pos, tk2, pos2, Vtot, tbits, rc, scr, s);
+	uint32_t *TK = in2 ? tk2 : tk, *PS = in2 ? pos2 : pos;
+	uint32_t *keys = in2 ? tk : tk2, *vals = in2 ? pos : pos2; // the free pair
+	launch_atlas_weld_gather(K, V, PS, Vtot, keys, vals, s);
+	C->tend();
+	uint32_t *flags = C->buf<uint32_t>(S_FLAGS, Vtot), *segid = C->buf<uint32_t>(S_SEGID, Vtot);
+	C->tbegin("atlas_weld_segments", 12.0 * Vtot);
+	launch_atlas_weld_segflags(keys, TK, Vtot, flags, s);
+	C->tend();
+	uint32_t nseg = scan(C, flags, segid, Vtot, true);
+	uint32_t *starts = C->buf<uint32_t>(S_STARTS, nseg);
+	int *lcls = C->buf<int>(S_LCLS, Vtot);
+	uint32_t *seeds = C->buf<uint32_t>(S_SEEDS, nseg), *seedbase = C->buf<uint32_t>(S_SEEDBASE, nseg);
+	C->tbegin("atlas_weld_greedy", 4.0 * Vtot + 4.0 * Vtot + 24.0 * Vtot + 4.0 * Vtot);
+	launch_weld_segstarts(flags, segid, Vtot, starts, s);
+	launch_weld_greedy(starts, nseg, Vtot, keys, vals, pts0, lcls, seeds, s);
+	C->tend();
+	uint32_t nnew = scan(C, seeds, seedbase, nseg, true);
+	uint32_t *dcb = C->buf<uint32_t>(S_AT_CBASE, nt_);
+	int *cls = C->buf<int>(S_CLS, Vtot), *rep = C->buf<int>(S_REP, nnew);
+	HIPX(hipMemsetAsync(rep, 0xff, (size_t)4 * nnew, s));
+	C->tbegin("atlas_weld_assign", 4.0 * Vtot * 4 + 8.0 * Vtot);
+	launch_atlas_class_base(TK, segid, seedbase, Vtot, dcb, s);
+	launch_weld_assign(vals, lcls, flags, segid, seedbase, Vtot, cls, rep, s);
+	C->tend();
+	std::vector<uint32_t> cb = download(C, dcb, nt_);
+	std::vector<uint32_t> ncls(nt_, 0);
+	std::vector<uint8_t> ident(nt_, 0);
+	bool any_ident = false;
+	{
+		uint32_t next = nnew;
+		for (size_t k = nt_; k-- > 0;) {
+			if (vbase[k + 1] == vbase[k]) {
+				cb[k] = next;
+				continue;
+			}
+			ncls[k] = next - cb[k];
+			next = cb[k];
+			if (ncls[k] == vbase[k + 1] - vbase[k]) {
+				ident[k] = 1;
+				any_ident = true;
+				if (P.verbose)
+					printf("Unify vertices found no shared vertices\n");
+			}
+			if (ncls[k] < 3) // src/meshify.c:384
+				failed((int)k);
+		}
+	}
+	if (any_ident) {
+		uint8_t *di = C->buf<uint8_t>(S_AT_IDENT, nt_);
+		upload(C, di, ident);
+		launch_atlas_weld_identity(Vtot, vtask, dvb, dcb, di, cls, rep, s);
+		C->check();
+	}
+	double *pts1 = C->buf<double>(S_PTS1, (size_t)3 * nnew);
+	int *tris1 = C->buf<int>(S_TRIS1, (size_t)3 * (Ttot ? Ttot : 1));
+	C->tbegin("atlas_weld_gather", 28.0 * nnew + 24.0 * nnew + 24.0 * Ttot);
+	launch_gather_pts(pts0, rep, nnew, pts1, s);
+	launch_remap_tris(implicit ? nullptr : C->buf<int>(S_TRIS0, (size_t)3 * (Ttot ? Ttot : 1)), cls, Ttot, Vtot,
+	                  tris1, s);
+	C->tend();
+
+	// 5. degenerate triangles (src/meshify.c:113-168)
+	uint32_t *keep = C->buf<uint32_t>(S_KEEP, (size_t)Ttot + 1), *kpos = C->buf<uint32_t>(S_KEEPPOS, (size_t)Ttot + 1);
+	HIPX(hipMemsetAsync(keep + Ttot, 0, 4, s));
+	C->tbegin("atlas_degenerate", 12.0 * Ttot + 72.0 * Ttot + 4.0 * Ttot);
+	if (Ttot)
+		launch_degen_flags(pts1, tris1, Ttot, nnew, keep, s);
+	C->tend();
+	scan_excl_u32(keep, kpos, (size_t)Ttot + 1, nullptr, scan_scratch(C, (size_t)Ttot + 1), s);
+	C->check();
+	std::vector<uint32_t> kb = values_at(C, kpos, tbase);
+	const uint32_t kept = kb[nt_];
+	int *tris2 = C->buf<int>(S_TRIS0, (size_t)3 * (kept ? kept : 1));
+	if (kept && Ttot)
+		launch_compact_tris(tris1, keep, kpos, Ttot, tris2, s);
+	C->check();
+
+	// 6. append to the union mesh
+	const uint32_t v0u = (uint32_t)U.np, t0u = (uint32_t)U.nt;
+	append_mesh(C, U, pts1, nnew, tris2, kept);
+	for (size_t k = 0; k < nt_; k++) {
+		n2m_atlas_entry &e = C->atlas[entry[k]];
+		e.tri_offset = (int)(t0u + kb[k]);
+		e.nt = (int)(kb[k + 1] - kb[k]);
+		e.vert_offset = (int)(v0u + cb[k]);
+		e.np = (int)ncls[k];
+		if (e.status != N2M_ATLAS_FAILED)
+			e.status = N2M_ATLAS_OK;
+	}
+}
+
+// chunk budget in crop voxels (env N2M_ATLAS_CHUNK overrides)
+size_t atlas_chunk_voxels()
+{
+	const char *e = getenv("N2M_ATLAS_CHUNK");
+	if (e && atoll(e) > 0)
+		return (size_t)atoll(e);
+	return (size_t)1 << 29; // 2 GiB of crops
+}
+
+int atlas_batched(n2m_ctx *C, const float *labels, const int dim[3], const n2m_params &prm, int postSmooth,
+                  const float *srow)
+{
+	const int NX = dim[0], NY = dim[1], NZ = dim[2];
+	const size_t nvox = (size_t)NX * NY * NZ;
+	hipStream_t s = C->stream;
+	n2m_params P = prm;
+	P.onlyLargest = 0; // src/nii2mesh.c:493
+	const int nL = atlas_label_count(C, labels, nvox);
+	C->atlas.assign(nL, n2m_atlas_entry{0, N2M_ATLAS_EMPTY, 0, 0, 0, 0});
+	Mesh &U = C->atlas_mesh;
+	U.nt = U.np = 0;
+	// one pass: voxel count and box of every label
+	uint32_t *st = C->buf<uint32_t>(S_AT_STATS, (size_t)7 * (nL + 1));
+	{
+		std::vector<uint32_t> init((size_t)7 * (nL + 1));
+		for (int i = 0; i <= nL; i++) {
+			init[7 * i] = 0;
+			init[7 * i + 1] = init[7 * i + 2] = init[7 * i + 3] = 0xffffffffu;
+			init[7 * i + 4] = init[7 * i + 5] = init[7 * i + 6] = 0u;
+		}
+		upload(C, st, init);
+	}
+	C->tbegin("atlas_stats", 4.0 * nvox);
+	launch_atlas_stats(labels, NX, NY, NZ, nL, st, s);
+	C->tend();
+	std::vector<uint32_t> hs = download(C, st, (size_t)7 * (nL + 1));
+	std::vector<AtlasTask> tasks;
+	std::vector<int> entry;
+	for (int i = 1; i <= nL; i++) {
+		n2m_atlas_entry &e = C->atlas[i - 1];
+		e.label = i;
+		const uint32_t *q = &hs[(size_t)7 * i];
+		if (!q[0]) { // src/nii2mesh.c:564-567
+			if (P.verbose)
+				printf("Skipping %d: no voxels with this intensity\n", i);
+			continue;
+		}
+		AtlasTask t;
+		memset(&t, 0, sizeof t);
+		t.label = i;
+		t.lo = (float)((double)i - 0.5); // src/nii2mesh.c:555-556
+		t.hi = (float)((double)i + 0.5);
+		const int lo[3] = {(int)q[1], (int)q[2], (int)q[3]}, hi[3] = {(int)q[4], (int)q[5], (int)q[6]};
+		int o[3], n[3];
+		for (int a = 0; a < 3; a++) {
+			o[a] = lo[a] - 4 > 0 ? lo[a] - 4 : 0;
+			int e2 = hi[a] + 4 < dim[a] - 1 ? hi[a] + 4 : dim[a] - 1;
+			n[a] = e2 - o[a] + 1;
+		}
+		t.ox = o[0];
+		t.oy = o[1];
+		t.oz = o[2];
+		t.nx = n[0];
+		t.ny = n[1];
+		t.nz = n[2];
+		tasks.push_back(t);
+		entry.push_back(i - 1);
+	}
+	// chunks of tasks whose crops fit the budget
+	const size_t budget = atlas_chunk_voxels();
+	size_t b = 0;
+	while (b < tasks.size()) {
+		size_t e = b, vox = 0;
+		while (e < tasks.size()) {
+			size_t v = (size_t)tasks[e].nx * tasks[e].ny * tasks[e].nz;
+			if (e > b && vox + v > budget)
+				break;
+			vox += v;
+			e++;
+		}
+		atlas_chunk(C, labels, dim, P, std::vector<AtlasTask>(tasks.begin() + b, tasks.begin() + e),
+		            std::vector<int>(entry.begin() + b, entry.begin() + e), U);
+		b = e;
+	}
+	sform_and_smooth(C, U, postSmooth, srow);
+	C->sync();
+	int ok = 0;
+	for (auto &e : C->atlas)
+		ok += e.status == N2M_ATLAS_OK;
+	return ok ? EXIT_SUCCESS : EXIT_FAILURE;
+}
+
+// the reference's per-label loop on the GPU (src/nii2mesh.c:551-573)
+int atlas_loop(n2m_ctx *C, const float *labels, const int dim[3], const n2m_params &prm, int postSmooth,
+               const float *srow)
+{
+	const size_t nvox = (size_t)dim[0] * dim[1] * dim[2];
+	hipStream_t s = C->stream;
+	n2m_params P = prm;
+	P.onlyLargest = 0;
+	const int nL = atlas_label_count(C, labels, nvox);
+	C->atlas.assign(nL, n2m_atlas_entry{0, N2M_ATLAS_EMPTY, 0, 0, 0, 0});
+	Mesh &U = C->atlas_mesh;
+	U.nt = U.np = 0;
+	float *bin = C->buf<float>(S_VOL, nvox);
+	Mesh &m = mesh_slot(C, 0);
+	int ok = 0;
+	for (int i = 1; i <= nL; i++) {
+		n2m_atlas_entry &e = C->atlas[i - 1];
+		e.label = i;
+		HIPX(hipMemsetAsync(C->d_small + SM_FLAG, 0, 4, s));
+		C->tbegin("atlas_binarize", 8.0 * nvox);
+		launch_binarize(labels, nvox, (float)((double)i - 0.5), (float)((double)i + 0.5), bin, C->d_small + SM_FLAG,
+		                s);
+		C->tend();
+		if (C->fetch(SM_FLAG, 1)[0] == 0) {
+			if (P.verbose)
+				printf("Skipping %d: no voxels with this intensity\n", i);
+			continue;
+		}
+		m.nt = m.np = 0;
+		if (!meshify_device(C, bin, bin, dim, P, m)) {
+			e.status = N2M_ATLAS_FAILED;
+			continue;
+		}
+		sform_and_smooth(C, m, postSmooth, srow);
+		e.status = N2M_ATLAS_OK;
+		e.tri_offset = U.nt;
+		e.vert_offset = U.np;
+		e.nt = m.nt;
+		e.np = m.np;
+		append_mesh(C, U, (const double *)m.pts.p, m.np, (const int *)m.tris.p, m.nt);
+		ok++;
+	}
+	C->sync();
+	return ok ? EXIT_SUCCESS : EXIT_FAILURE;
+}
+
+int atlas_run(n2m_ctx *C, const float *labels, const int dim[3], const n2m_params &prm, int postSmooth,
+              const float *srow, bool loop)
+{
+	if (dim[0] < 1 || dim[1] < 1 || dim[2] < 1)
+		fail("bad atlas dimensions %d x %d x %d", dim[0], dim[1], dim[2]);
+	// bubble filling needs the whole-volume label pass of meshify()
+	if (loop || prm.fillBubbles)
+		return atlas_loop(C, labels, dim, prm, postSmooth, srow);
+	return atlas_batched(C, labels, dim, prm, postSmooth, srow);
+}
+
+} // namespace
+
+// ===========================================================================
 // C ABI
 // ===========================================================================
 extern "C" {
@@ -1047,6 +1645,115 @@ int n2m_ctx_device_mesh(n2m_ctx *C, int mesh, const vec3i **d_t, const vec3d **d
 	*d_t = (const vec3i *)C->meshes[mesh].tris.p;
 	*d_p = (const vec3d *)C->meshes[mesh].pts.p;
 	return EXIT_SUCCESS;
+}
+
+int n2m_ctx_atlas(n2m_ctx *C, const float *d_labels, const int dim[3], const n2m_params *prm, int postSmooth,
+                  const float *srow)
+{
+	return guarded([&] { return atlas_run(C, d_labels, dim, *prm, postSmooth, srow, false); });
+}
+
+int n2m_ctx_atlas_loop(n2m_ctx *C, const float *d_labels, const int dim[3], const n2m_params *prm, int postSmooth,
+                       const float *srow)
+{
+	return guarded([&] { return atlas_run(C, d_labels, dim, *prm, postSmooth, srow, true); });
+}
+
+int n2m_ctx_atlas_count(n2m_ctx *C) { return (int)C->atlas.size(); }
+
+int n2m_ctx_atlas_entry(n2m_ctx *C, int idx, n2m_atlas_entry *e)
+{
+	if (idx < 0 || idx >= (int)C->atlas.size())
+		return EXIT_FAILURE;
+	*e = C->atlas[idx];
+	return EXIT_SUCCESS;
+}
+
+int n2m_ctx_atlas_download(n2m_ctx *C, int idx, vec3i *t, vec3d *p)
+{
+	return guarded([&] {
+		if (idx < 0 || idx >= (int)C->atlas.size())
+			fail("no atlas entry %d", idx);
+		const n2m_atlas_entry &e = C->atlas[idx];
+		if (e.status != N2M_ATLAS_OK)
+			fail("atlas label %d has no mesh", e.label);
+		const Mesh &U = C->atlas_mesh;
+		if (e.nt)
+			HIPX(hipMemcpyAsync(t, (const int *)U.tris.p + (size_t)3 * e.tri_offset, (size_t)12 * e.nt,
+			                    hipMemcpyDeviceToHost, C->stream));
+		if (e.np)
+			HIPX(hipMemcpyAsync(p, (const double *)U.pts.p + (size_t)3 * e.vert_offset, (size_t)24 * e.np,
+			                    hipMemcpyDeviceToHost, C->stream));
+		C->sync();
+		for (int i = 0; i < e.nt; i++) {
+			t[i].x -= e.vert_offset;
+			t[i].y -= e.vert_offset;
+			t[i].z -= e.vert_offset;
+		}
+		return EXIT_SUCCESS;
+	});
+}
+
+int n2m_ctx_atlas_device_mesh(n2m_ctx *C, const vec3i **d_t, const vec3d **d_p, int *nt, int *np)
+{
+	*d_t = (const vec3i *)C->atlas_mesh.tris.p;
+	*d_p = (const vec3d *)C->atlas_mesh.pts.p;
+	*nt = C->atlas_mesh.nt;
+	*np = C->atlas_mesh.np;
+	return EXIT_SUCCESS;
+}
+
+int n2m_atlas(const float *img, short dim[3], int originalMC, bool preSmooth, bool fillBubbles, int postSmooth,
+              const float *srow, bool verbose, n2m_atlas_mesh **meshes, int *nmesh)
+{
+	*meshes = nullptr;
+	*nmesh = 0;
+	return guarded([&] {
+		n2m_ctx *C = tl_ctx.get();
+		const int d[3] = {dim[0], dim[1], dim[2]};
+		const size_t nvox = (size_t)d[0] * d[1] * d[2];
+		float *dl = C->buf<float>(S_AT_LABELS, nvox);
+		HIPX(hipMemcpyAsync(dl, img, 4 * nvox, hipMemcpyHostToDevice, C->stream));
+		n2m_params P{cubes_flavour(), originalMC, 0.5f, preSmooth, 0, fillBubbles, verbose};
+		int rc = atlas_run(C, dl, d, P, postSmooth, srow, false);
+		const int n = (int)C->atlas.size();
+		n2m_atlas_mesh *out = (n2m_atlas_mesh *)calloc(n ? n : 1, sizeof(n2m_atlas_mesh));
+		if (!out)
+			fail("out of host memory");
+		for (int k = 0; k < n; k++) {
+			const n2m_atlas_entry &e = C->atlas[k];
+			out[k].label = e.label;
+			out[k].status = e.status;
+			if (e.status != N2M_ATLAS_OK)
+				continue;
+			out[k].nt = e.nt;
+			out[k].np = e.np;
+			out[k].t = (vec3i *)malloc((size_t)12 * (e.nt ? e.nt : 1));
+			out[k].p = (vec3d *)malloc((size_t)24 * (e.np ? e.np : 1));
+			if (!out[k].t || !out[k].p) {
+				n2m_atlas_free(out, n);
+				fail("out of host memory");
+			}
+			if (n2m_ctx_atlas_download(C, k, out[k].t, out[k].p) != EXIT_SUCCESS) {
+				n2m_atlas_free(out, n);
+				fail("%s", tl_err.c_str());
+			}
+		}
+		*meshes = out;
+		*nmesh = n;
+		return rc;
+	});
+}
+
+void n2m_atlas_free(n2m_atlas_mesh *meshes, int nmesh)
+{
+	if (!meshes)
+		return;
+	for (int k = 0; k < nmesh; k++) {
+		free(meshes[k].t);
+		free(meshes[k].p);
+	}
+	free(meshes);
 }
 
 void n2m_ctx_enable_timing(n2m_ctx *C, int on) { C->timing = on != 0; }

@@ -49,17 +49,19 @@ struct ClsGeo {
 	int lo0, lo1, lo2;
 	int cx, cy, cz; // cube counts
 	float iso;
+	int ox, oy, oz; // origin of this volume in output coordinates
 };
 
 constexpr int MC_BS = 256;
 
-__global__ void __launch_bounds__(MC_BS)
-k_classic_count(const float *__restrict__ img, ClsGeo g, uint8_t *__restrict__ cfgs,
-                uint32_t *__restrict__ blockSums)
+// count the triangles of cube block `blk` (MC_BS cubes); returns the block
+// total (valid in every thread)
+__device__ __forceinline__ uint32_t classic_count_block(const float *__restrict__ img, const ClsGeo &g,
+                                                        uint8_t *__restrict__ cfgs, size_t blk)
 {
 	__shared__ uint32_t scratch[MC_BS / N2M_WAVE + 1];
 	size_t ncube = (size_t)g.cx * g.cy * g.cz;
-	size_t c = blockIdx.x * (size_t)MC_BS + threadIdx.x;
+	size_t c = blk * MC_BS + threadIdx.x;
 	uint32_t nt = 0;
 	if (c < ncube) {
 		int x = g.lo0 + (int)(c % g.cx);
@@ -77,22 +79,21 @@ k_classic_count(const float *__restrict__ img, ClsGeo g, uint8_t *__restrict__ c
 		cfgs[c] = (uint8_t)cfg;
 		nt = classic_ntri(n2m_classic_tri[cfg]);
 	}
-	uint32_t s = block_sum_u32<MC_BS>(nt, scratch);
-	if (threadIdx.x == 0)
-		blockSums[blockIdx.x] = s;
+	return block_sum_u32<MC_BS>(nt, scratch);
 }
 
-__global__ void __launch_bounds__(MC_BS)
-k_classic_emit(const float *__restrict__ img, ClsGeo g, const uint8_t *__restrict__ cfgs,
-               const uint32_t *__restrict__ blockOffs, double *__restrict__ pts)
+// emit the triangle corners of cube block `blk`, first triangle at `off`
+__device__ __forceinline__ void classic_emit_block(const float *__restrict__ img, const ClsGeo &g,
+                                                   const uint8_t *__restrict__ cfgs, size_t blk, uint32_t off0,
+                                                   double *__restrict__ pts)
 {
 	__shared__ uint32_t scratch[MC_BS / N2M_WAVE + 1];
 	size_t ncube = (size_t)g.cx * g.cy * g.cz;
-	size_t c = blockIdx.x * (size_t)MC_BS + threadIdx.x;
+	size_t c = blk * MC_BS + threadIdx.x;
 	int cfg = c < ncube ? cfgs[c] : 0;
 	uint64_t w = n2m_classic_tri[cfg];
 	uint32_t nt = classic_ntri(w), tot;
-	uint32_t off = block_excl_scan<MC_BS>(nt, scratch, &tot) + blockOffs[blockIdx.x];
+	uint32_t off = block_excl_scan<MC_BS>(nt, scratch, &tot) + off0;
 	if (!nt)
 		return;
 	int x = g.lo0 + (int)(c % g.cx);
@@ -110,12 +111,28 @@ k_classic_emit(const float *__restrict__ img, ClsGeo g, const uint8_t *__restric
 		int e = (int)((w >> (4 * k)) & 0xF);
 		int a = c_ea[e], bb = c_eb[e];
 		double mu = (iso - val[a]) / (val[bb] - val[a]);
-		double ax = x + c_cx[a], ay = y + c_cy[a], az = z + c_cz[a];
-		double bx = x + c_cx[bb], by = y + c_cy[bb], bz = z + c_cz[bb];
+		double ax = x + g.ox + c_cx[a], ay = y + g.oy + c_cy[a], az = z + g.oz + c_cz[a];
+		double bx = x + g.ox + c_cx[bb], by = y + g.oy + c_cy[bb], bz = z + g.oz + c_cz[bb];
 		o[3 * k + 0] = ax + mu * (bx - ax);
 		o[3 * k + 1] = ay + mu * (by - ay);
 		o[3 * k + 2] = az + mu * (bz - az);
 	}
+}
+
+__global__ void __launch_bounds__(MC_BS)
+k_classic_count(const float *__restrict__ img, ClsGeo g, uint8_t *__restrict__ cfgs,
+                uint32_t *__restrict__ blockSums)
+{
+	uint32_t s = classic_count_block(img, g, cfgs, blockIdx.x);
+	if (threadIdx.x == 0)
+		blockSums[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(MC_BS)
+k_classic_emit(const float *__restrict__ img, ClsGeo g, const uint8_t *__restrict__ cfgs,
+               const uint32_t *__restrict__ blockOffs, double *__restrict__ pts)
+{
+	classic_emit_block(img, g, cfgs, blockIdx.x, blockOffs[blockIdx.x], pts);
 }
 
 size_t classic_blocks(int cx, int cy, int cz) { return ((size_t)cx * cy * cz + MC_BS - 1) / MC_BS; }
@@ -123,16 +140,17 @@ size_t classic_blocks(int cx, int cy, int cz) { return ((size_t)cx * cy * cz + M
 void launch_classic_count(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3],
                           float iso, uint8_t *cfgs, uint32_t *blockSums, hipStream_t s)
 {
-	ClsGeo g{nx, ny, nz, lo[0], lo[1], lo[2], hi[0] - 1 - lo[0], hi[1] - 1 - lo[1], hi[2] - 1 - lo[2], iso};
+	ClsGeo g{nx, ny, nz, lo[0], lo[1], lo[2], hi[0] - 1 - lo[0], hi[1] - 1 - lo[1], hi[2] - 1 - lo[2], iso, 0, 0, 0};
 	size_t nb = classic_blocks(g.cx, g.cy, g.cz);
 	hipLaunchKernelGGL(k_classic_count, dim3((unsigned)nb), dim3(MC_BS), 0, s, img, g, cfgs, blockSums);
 }
 
 void launch_classic_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3],
                          float iso, const uint8_t *cfgs, const uint32_t *blockOffs, double *pts,
-                         hipStream_t s)
+                         const int origin[3], hipStream_t s)
 {
-	ClsGeo g{nx, ny, nz, lo[0], lo[1], lo[2], hi[0] - 1 - lo[0], hi[1] - 1 - lo[1], hi[2] - 1 - lo[2], iso};
+	ClsGeo g{nx, ny, nz, lo[0], lo[1], lo[2], hi[0] - 1 - lo[0], hi[1] - 1 - lo[1], hi[2] - 1 - lo[2], iso,
+	         origin[0], origin[1], origin[2]};
 	size_t nb = classic_blocks(g.cx, g.cy, g.cz);
 	hipLaunchKernelGGL(k_classic_emit, dim3((unsigned)nb), dim3(MC_BS), 0, s, img, g, cfgs, blockOffs, pts);
 }
@@ -148,6 +166,7 @@ struct LwGeo {
 	float iso, mnv; // mnv = min(img) - iso
 	int originalMC;
 	int nseg;       // segments per row
+	int out0, out1, out2; // lo + origin: added (as float) to sub-box coordinates
 };
 
 constexpr int LW_SEG = 1024, LW_BS = 256;
@@ -351,14 +370,14 @@ __device__ __forceinline__ uint32_t lw_point_flags(const float *__restrict__ img
 	return f;
 }
 
-// Pass A: per point flags, per cube tiling row; per segment counts of edge
-// vertices, triangles and centre vertices.
-__global__ void __launch_bounds__(LW_BS)
-k_lw_classify(const float *__restrict__ img, LwGeo g, uint8_t *__restrict__ pflags,
-              uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt /* 3 arrays of nseg*sy*sz */)
+// Pass A for one row segment (seg, y, z): per point flags, per cube tiling
+// row; the segment's counts of edge vertices, triangles and centre vertices
+// go to cnt[0], cnt[cstride], cnt[2*cstride].
+__device__ __forceinline__ void lw_classify_row(const float *__restrict__ img, const LwGeo &g, int seg, int y, int z,
+                                                uint8_t *__restrict__ pflags, uint16_t *__restrict__ crow,
+                                                uint32_t *__restrict__ cnt, size_t cstride)
 {
 	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
-	const int seg = blockIdx.x, y = blockIdx.y, z = blockIdx.z;
 	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
 	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
 	uint32_t nv = 0, nt = 0, nc = 0;
@@ -391,12 +410,20 @@ k_lw_classify(const float *__restrict__ img, LwGeo g, uint8_t *__restrict__ pfla
 	nt = block_sum_u32<LW_BS>(nt, scratch);
 	nc = block_sum_u32<LW_BS>(nc, scratch);
 	if (threadIdx.x == 0) {
-		size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
-		size_t ns = (size_t)g.nseg * g.sy * g.sz;
-		segcnt[si] = nv;
-		segcnt[ns + si] = nt;
-		segcnt[2 * ns + si] = nc;
+		cnt[0] = nv;
+		cnt[cstride] = nt;
+		cnt[2 * cstride] = nc;
 	}
+}
+
+__global__ void __launch_bounds__(LW_BS)
+k_lw_classify(const float *__restrict__ img, LwGeo g, uint8_t *__restrict__ pflags,
+              uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt /* 3 arrays of nseg*sy*sz */)
+{
+	const int seg = blockIdx.x, y = blockIdx.y, z = blockIdx.z;
+	size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
+	size_t ns = (size_t)g.nseg * g.sy * g.sz;
+	lw_classify_row(img, g, seg, y, z, pflags, crow, segcnt + si, ns);
 }
 
 // edge vertex position in sub-box coordinates, src/MarchingCubes.c:935-1026
@@ -418,28 +445,28 @@ __constant__ int8_t c_code[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {
                                     {0, 0, 1, 0}, {1, 0, 1, 1}, {0, 1, 1, 0}, {0, 0, 1, 1},
                                     {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
 
-// Pass B: write edge vertices of this segment, and the triangles and centre
-// vertices of its cubes.  bases: 3 arrays (vertex, triangle, centre) of
-// exclusive segment offsets; nedge = total edge vertices.
-__global__ void __launch_bounds__(LW_BS)
-k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
-          const uint16_t *__restrict__ crow, const uint32_t *__restrict__ bases, uint32_t nedge,
-          double *__restrict__ pts, int *__restrict__ tris)
+// Pass B for one row segment: write the edge vertices of the segment's
+// points, and the triangles and centre vertices of its cubes.  bases holds
+// three arrays (stride bstride) of exclusive segment offsets for the volume
+// (segment index seg + nseg*(y + sy*z)); edge_add / centre_add are added to
+// edge-vertex / centre-vertex numbers.
+__device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const LwGeo &g, int seg, int y, int z,
+                                            const uint8_t *__restrict__ pflags, const uint16_t *__restrict__ crow,
+                                            const uint32_t *__restrict__ bases, size_t bstride, uint32_t edge_add,
+                                            uint32_t centre_add, double *__restrict__ pts, int *__restrict__ tris)
 {
 	__shared__ uint32_t s_pre[4][LW_SEG + 1]; // popcount prefix for the 4 point rows
 	__shared__ uint32_t s_base[4];
 	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
-	const int seg = blockIdx.x, y = blockIdx.y, z = blockIdx.z;
 	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
 	const int len = xe - xs; // points owned; prefix covers [xs, xe]
-	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
 	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
 	const int nrows = cuberow ? 4 : 1;
 	// row r: (y + (r&1), z + (r>>1))
 	if (threadIdx.x < nrows) {
 		int r = threadIdx.x;
 		int yy = y + (r & 1), zz = z + (r >> 1);
-		s_base[r] = bases[(size_t)seg + (size_t)g.nseg * ((size_t)yy + (size_t)g.sy * zz)];
+		s_base[r] = edge_add + bases[(size_t)seg + (size_t)g.nseg * ((size_t)yy + (size_t)g.sy * zz)];
 	}
 	for (int r = 0; r < nrows; r++) {
 		int yy = y + (r & 1), zz = z + (r >> 1);
@@ -469,17 +496,18 @@ k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pf
 				float p[3];
 				lw_edge_pos(img, g, x, y, z, a, p);
 				double *o = pts + (size_t)id * 3;
-				o[0] = (double)(p[0] + (float)g.lo0);
-				o[1] = (double)(p[1] + (float)g.lo1);
-				o[2] = (double)(p[2] + (float)g.lo2);
+				o[0] = (double)(p[0] + (float)g.out0);
+				o[1] = (double)(p[1] + (float)g.out1);
+				o[2] = (double)(p[2] + (float)g.out2);
 				id++;
 			}
 		}
 	}
 	if (!cuberow)
 		return;
-	const uint32_t tbase = bases[ns + (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z)];
-	const uint32_t cbase = nedge + bases[2 * ns + (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z)];
+	const size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
+	const uint32_t tbase = bases[bstride + si];
+	const uint32_t cbase = centre_add + bases[2 * bstride + si];
 	const size_t rb0 = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
 	uint32_t tcarry = 0, ccarry = 0;
 	const int ncube = min(xe, g.sx - 1) - xs;
@@ -526,9 +554,9 @@ k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pf
 				}
 				cid = cbase + ccarry + coff;
 				double *o = pts + (size_t)cid * 3;
-				o[0] = (double)(sxv + (float)g.lo0);
-				o[1] = (double)(syv + (float)g.lo1);
-				o[2] = (double)(szv + (float)g.lo2);
+				o[0] = (double)(sxv + (float)g.out0);
+				o[1] = (double)(syv + (float)g.out1);
+				o[2] = (double)(szv + (float)g.out2);
 			}
 			const uint8_t *codes = &n2m_lw_edges[n2m_lw_row_off[row]];
 			int *t = tris + (size_t)(tbase + tcarry + toff) * 3;
@@ -548,6 +576,15 @@ k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pf
 	}
 }
 
+__global__ void __launch_bounds__(LW_BS)
+k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
+          const uint16_t *__restrict__ crow, const uint32_t *__restrict__ bases, uint32_t nedge,
+          double *__restrict__ pts, int *__restrict__ tris)
+{
+	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
+	lw_emit_row(img, g, blockIdx.x, blockIdx.y, blockIdx.z, pflags, crow, bases, ns, 0u, nedge, pts, tris);
+}
+
 void lw_geo(LwGeo &g, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso, float mnv,
             int originalMC)
 {
@@ -564,6 +601,9 @@ void lw_geo(LwGeo &g, int nx, int ny, int nz, const int lo[3], const int hi[3], 
 	g.mnv = mnv;
 	g.originalMC = originalMC;
 	g.nseg = (g.sx + LW_SEG - 1) / LW_SEG;
+	g.out0 = lo[0];
+	g.out1 = lo[1];
+	g.out2 = lo[2];
 }
 
 size_t lw_nseg_total(int nx, int ny, int nz, const int lo[3], const int hi[3])
@@ -584,12 +624,139 @@ void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3
 
 void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                     float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow,
-                    const uint32_t *bases, uint32_t nedge, double *pts, int *tris, hipStream_t s)
+                    const uint32_t *bases, uint32_t nedge, double *pts, int *tris, const int origin[3],
+                    hipStream_t s)
 {
 	LwGeo g;
 	lw_geo(g, nx, ny, nz, lo, hi, iso, mnv, originalMC);
+	g.out0 = lo[0] + origin[0];
+	g.out1 = lo[1] + origin[1];
+	g.out2 = lo[2] + origin[2];
 	hipLaunchKernelGGL(k_lw_emit, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags, crow, bases,
 	                   nedge, pts, tris);
 }
+
+// ---------------------------------------------------------------------------
+// Batched atlas: every workgroup serves one task (label crop).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ ClsGeo cls_geo_task(const AtlasTask &t)
+{
+	return ClsGeo{t.nx, t.ny, t.nz, t.lo3[0], t.lo3[1], t.lo3[2], t.hi3[0] - 1 - t.lo3[0], t.hi3[1] - 1 - t.lo3[1],
+	              t.hi3[2] - 1 - t.lo3[2], t.iso, t.ox, t.oy, t.oz};
+}
+
+__global__ void __launch_bounds__(MC_BS)
+k_atlas_classic_count(const float *__restrict__ crops, const AtlasTask *__restrict__ tasks,
+                      const uint32_t *__restrict__ blockTask, uint8_t *__restrict__ cfgs,
+                      uint32_t *__restrict__ blockSums)
+{
+	const AtlasTask t = tasks[blockTask[blockIdx.x]];
+	const ClsGeo g = cls_geo_task(t);
+	uint32_t s = classic_count_block(crops + t.off, g, cfgs + t.cube_off, blockIdx.x - t.first_block);
+	if (threadIdx.x == 0)
+		blockSums[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(MC_BS)
+k_atlas_classic_emit(const float *__restrict__ crops, const AtlasTask *__restrict__ tasks,
+                     const uint32_t *__restrict__ blockTask, const uint8_t *__restrict__ cfgs,
+                     const uint32_t *__restrict__ blockOffs, double *__restrict__ pts)
+{
+	const AtlasTask t = tasks[blockTask[blockIdx.x]];
+	const ClsGeo g = cls_geo_task(t);
+	classic_emit_block(crops + t.off, g, cfgs + t.cube_off, blockIdx.x - t.first_block, blockOffs[blockIdx.x], pts);
+}
+
+__device__ __forceinline__ LwGeo lw_geo_task(const AtlasTask &t, int originalMC)
+{
+	LwGeo g;
+	g.nx = t.nx;
+	g.nxy = (long long)t.nx * t.ny;
+	g.nxyz = g.nxy * t.nz;
+	g.lo0 = t.lo3[0];
+	g.lo1 = t.lo3[1];
+	g.lo2 = t.lo3[2];
+	g.sx = t.hi3[0] - t.lo3[0] + 1;
+	g.sy = t.hi3[1] - t.lo3[1] + 1;
+	g.sz = t.hi3[2] - t.lo3[2] + 1;
+	g.iso = t.iso;
+	g.mnv = t.mnv;
+	g.originalMC = originalMC;
+	g.nseg = t.nseg;
+	g.out0 = t.lo3[0] + t.ox;
+	g.out1 = t.lo3[1] + t.oy;
+	g.out2 = t.lo3[2] + t.oz;
+	return g;
+}
+
+// segcnt: 3 arrays `stride` entries apart, one entry per workgroup (= segment)
+__global__ void __launch_bounds__(LW_BS)
+k_atlas_lw_classify(const float *__restrict__ crops, const AtlasTask *__restrict__ tasks,
+                    const uint32_t *__restrict__ blockTask, int originalMC, uint8_t *__restrict__ pflags,
+                    uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt, size_t stride)
+{
+	const AtlasTask t = tasks[blockTask[blockIdx.x]];
+	const LwGeo g = lw_geo_task(t, originalMC);
+	const int local = blockIdx.x - t.first_block;
+	const int seg = local % g.nseg, y = (local / g.nseg) % g.sy, z = local / (g.nseg * g.sy);
+	lw_classify_row(crops + t.off, g, seg, y, z, pflags + t.cube_off, crow + t.cube_off, segcnt + blockIdx.x,
+	                stride);
+}
+
+// Vertex numbers follow the reference within each task: the task's edge
+// vertices, then its centre vertices.  With E and C the exclusive scans of
+// the per-segment edge / centre counts over all tasks, task t (first segment
+// f, next task's first segment f') starts at E(f) + C(f); its edge vertex
+// of segment s is E(s) + C(f) and its centre vertex is E(f') + C(s).
+__global__ void __launch_bounds__(LW_BS)
+k_atlas_lw_emit(const float *__restrict__ crops, const AtlasTask *__restrict__ tasks, int ntasks,
+                const uint32_t *__restrict__ blockTask, int originalMC, const uint8_t *__restrict__ pflags,
+                const uint16_t *__restrict__ crow, const uint32_t *__restrict__ bases, size_t stride,
+                uint32_t totalE, double *__restrict__ pts, int *__restrict__ tris)
+{
+	const int ti = blockTask[blockIdx.x];
+	const AtlasTask t = tasks[ti];
+	const LwGeo g = lw_geo_task(t, originalMC);
+	const int local = blockIdx.x - t.first_block;
+	const int seg = local % g.nseg, y = (local / g.nseg) % g.sy, z = local / (g.nseg * g.sy);
+	const uint32_t cbase = bases[2 * stride + t.first_block];
+	const uint32_t enext = ti + 1 < ntasks ? bases[tasks[ti + 1].first_block] : totalE;
+	lw_emit_row(crops + t.off, g, seg, y, z, pflags + t.cube_off, crow + t.cube_off, bases + t.first_block, stride,
+	            cbase, enext, pts, tris);
+}
+
+void launch_atlas_classic_count(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
+                                unsigned nblocks, uint8_t *cfgs, uint32_t *blockSums, hipStream_t s)
+{
+	if (nblocks)
+		hipLaunchKernelGGL(k_atlas_classic_count, dim3(nblocks), dim3(MC_BS), 0, s, crops, tasks, blockTask, cfgs,
+		                   blockSums);
+}
+void launch_atlas_classic_emit(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
+                               unsigned nblocks, const uint8_t *cfgs, const uint32_t *blockOffs, double *pts,
+                               hipStream_t s)
+{
+	if (nblocks)
+		hipLaunchKernelGGL(k_atlas_classic_emit, dim3(nblocks), dim3(MC_BS), 0, s, crops, tasks, blockTask, cfgs,
+		                   blockOffs, pts);
+}
+void launch_atlas_lw_classify(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
+                              unsigned nblocks, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
+                              size_t stride, hipStream_t s)
+{
+	if (nblocks)
+		hipLaunchKernelGGL(k_atlas_lw_classify, dim3(nblocks), dim3(LW_BS), 0, s, crops, tasks, blockTask,
+		                   originalMC, pflags, crow, segcnt, stride);
+}
+void launch_atlas_lw_emit(const float *crops, const AtlasTask *tasks, int ntasks, const uint32_t *blockTask,
+                          unsigned nblocks, int originalMC, const uint8_t *pflags, const uint16_t *crow,
+                          const uint32_t *bases, size_t stride, uint32_t totalE, double *pts, int *tris,
+                          hipStream_t s)
+{
+	if (nblocks)
+		hipLaunchKernelGGL(k_atlas_lw_emit, dim3(nblocks), dim3(LW_BS), 0, s, crops, tasks, ntasks, blockTask,
+		                   originalMC, pflags, crow, bases, stride, totalE, pts, tris);
+}
+int lw_seg_len() { return LW_SEG; }
 
 } // namespace n2m

@@ -914,4 +914,162 @@ void launch_swap_winding(int *t, uint32_t nt, hipStream_t s)
 	hipLaunchKernelGGL(k_swap_winding, dim3(g1(nt)), dim3(256), 0, s, t, nt);
 }
 
+// ---------------------------------------------------------------------------
+// Batched atlas weld: every label mesh is welded on its own, as
+// unify_vertices() does per label (src/meshify.c:45-106), in one segmented
+// pass over the union of the label meshes.  Vertices of task t occupy
+// [vbase[t], vbase[t+1]); keys are distances to the task's own vertex 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int task_of(const uint32_t *__restrict__ vbase, int ntasks, uint32_t i)
+{
+	int lo = 0, hi = ntasks - 1; // largest t with vbase[t] <= i
+	while (lo < hi) {
+		int mid = (lo + hi + 1) >> 1;
+		if (vbase[mid] <= i)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return lo;
+}
+
+__global__ void k_atlas_weld_keys(const double *__restrict__ pts, uint32_t n, const uint32_t *__restrict__ vbase,
+                                  int ntasks, uint32_t *keys, uint32_t *vals, uint32_t *vtask)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	int t = task_of(vbase, ntasks, i);
+	float d = (float)dist3(pts + (size_t)vbase[t] * 3, pts + (size_t)i * 3);
+	keys[i] = __float_as_uint(d);
+	vals[i] = i;
+	vtask[i] = (uint32_t)t;
+}
+
+// second sort key: the task of each key-sorted vertex; value = its position
+__global__ void k_atlas_task_keys(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ vtask, uint32_t n,
+                                  uint32_t *tk, uint32_t *pos)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	tk[i] = vtask[vals[i]];
+	pos[i] = i;
+}
+
+__global__ void k_atlas_weld_gather(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+                                    const uint32_t *__restrict__ pos, uint32_t n, uint32_t *keys2, uint32_t *vals2)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	uint32_t p = pos[i];
+	keys2[i] = keys[p];
+	vals2[i] = vals[p];
+}
+
+__global__ void k_atlas_weld_segflags(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ tk, uint32_t n,
+                                      uint32_t *flags)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const float tol = 0.00001f;
+	uint32_t f = 1;
+	if (i > 0 && tk[i] == tk[i - 1])
+		f = (__uint_as_float(keys[i]) - __uint_as_float(keys[i - 1])) >= tol;
+	flags[i] = f;
+}
+
+// first class of each task (the seed at its first sorted position)
+__global__ void k_atlas_class_base(const uint32_t *__restrict__ tk, const uint32_t *__restrict__ segid,
+                                   const uint32_t *__restrict__ seedbase, uint32_t n, uint32_t *cbase)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	if (i == 0 || tk[i] != tk[i - 1])
+		cbase[tk[i]] = seedbase[segid[i]];
+}
+
+// a task whose weld merged nothing keeps its vertex order (src/meshify.c:82-87)
+__global__ void k_atlas_weld_identity(uint32_t n, const uint32_t *__restrict__ vtask,
+                                      const uint32_t *__restrict__ vbase, const uint32_t *__restrict__ cbase,
+                                      const uint8_t *__restrict__ ident, int *__restrict__ cls, int *__restrict__ rep)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= n)
+		return;
+	uint32_t t = vtask[v];
+	if (!ident[t])
+		return;
+	int c = (int)(cbase[t] + (v - vbase[t]));
+	cls[v] = c;
+	rep[c] = (int)v;
+}
+
+__global__ void k_gather_u32(const uint32_t *__restrict__ src, const uint32_t *__restrict__ idx, uint32_t n,
+                             uint32_t *__restrict__ dst)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		dst[i] = src[idx[i]];
+}
+
+__global__ void k_add_i32(const int *__restrict__ src, size_t n, int add, int *__restrict__ dst)
+{
+	size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+	if (i < n)
+		dst[i] = src[i] + add;
+}
+
+void launch_atlas_weld_keys(const double *pts, uint32_t n, const uint32_t *vbase, int ntasks, uint32_t *keys,
+                            uint32_t *vals, uint32_t *vtask, hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_atlas_weld_keys, dim3(g1(n)), dim3(256), 0, s, pts, n, vbase, ntasks, keys, vals,
+		                   vtask);
+}
+void launch_atlas_task_keys(const uint32_t *vals, const uint32_t *vtask, uint32_t n, uint32_t *tk, uint32_t *pos,
+                            hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_atlas_task_keys, dim3(g1(n)), dim3(256), 0, s, vals, vtask, n, tk, pos);
+}
+void launch_atlas_weld_gather(const uint32_t *keys, const uint32_t *vals, const uint32_t *pos, uint32_t n,
+                              uint32_t *keys2, uint32_t *vals2, hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_atlas_weld_gather, dim3(g1(n)), dim3(256), 0, s, keys, vals, pos, n, keys2, vals2);
+}
+void launch_atlas_weld_segflags(const uint32_t *keys, const uint32_t *tk, uint32_t n, uint32_t *flags,
+                                hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_atlas_weld_segflags, dim3(g1(n)), dim3(256), 0, s, keys, tk, n, flags);
+}
+void launch_atlas_class_base(const uint32_t *tk, const uint32_t *segid, const uint32_t *seedbase, uint32_t n,
+                             uint32_t *cbase, hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_atlas_class_base, dim3(g1(n)), dim3(256), 0, s, tk, segid, seedbase, n, cbase);
+}
+void launch_atlas_weld_identity(uint32_t n, const uint32_t *vtask, const uint32_t *vbase, const uint32_t *cbase,
+                                const uint8_t *ident, int *cls, int *rep, hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_atlas_weld_identity, dim3(g1(n)), dim3(256), 0, s, n, vtask, vbase, cbase, ident, cls,
+		                   rep);
+}
+void launch_gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t n, uint32_t *dst, hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_gather_u32, dim3(g1(n)), dim3(256), 0, s, src, idx, n, dst);
+}
+void launch_add_i32(const int *src, size_t n, int add, int *dst, hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_add_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, n, add, dst);
+}
+
 } // namespace n2m

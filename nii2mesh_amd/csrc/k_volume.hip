@@ -36,16 +36,17 @@ __device__ __forceinline__ float tap5(float a, float b, float c, float d, float 
 	return (float)acc;
 }
 
-__global__ void __launch_bounds__(SM_BS)
-k_smooth(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, int nz,
-         uint32_t *__restrict__ range /* [0]=ord min, [1]=ord max */, float iso,
-         uint32_t *__restrict__ bits, int W)
+// Smooth the tile (x0, y0, [z0, z1)) of an nx*ny*nz volume read through
+// `src(x, y, z)` (only called inside the volume) into out; returns the
+// tile's min/max through lmin/lmax (per thread) and writes speculative mask
+// bits (out >= iso) when bits != null.
+template <class Src>
+__device__ __forceinline__ void smooth_tile(const Src &src, float *__restrict__ out, int nx, int ny, int nz,
+                                            int x0, int y0, int z0, int z1, float iso, uint32_t *__restrict__ bits,
+                                            int W, float &lmin, float &lmax)
 {
 	__shared__ float s_in[SM_TY + 4][SM_TX + 4];
 	__shared__ float s_x[SM_TY + 4][SM_TX];
-	__shared__ uint32_t s_red[2][SM_BS / N2M_WAVE];
-	const int x0 = blockIdx.x * SM_TX, y0 = blockIdx.y * SM_TY;
-	const int z0 = blockIdx.z * SM_ZC, z1 = min(z0 + SM_ZC, nz);
 	const int tx = threadIdx.x % SM_TX, ty = threadIdx.x / SM_TX; // ty 0..3
 	const size_t nxy = (size_t)nx * ny;
 	float ring[SM_CPT][5];
@@ -54,18 +55,16 @@ k_smooth(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, 
 #pragma unroll
 		for (int k = 0; k < 5; k++)
 			ring[c][k] = 0.f;
-	float lmin = __builtin_inff(), lmax = -__builtin_inff();
 	const int x = x0 + tx;
 	for (int zz = z0 - 2; zz <= z1 + 1; zz++) {
 		const bool have = zz >= 0 && zz < nz;
 		if (have) {
-			const float *plane = in + (size_t)zz * nxy;
 			for (int i = threadIdx.x; i < (SM_TY + 4) * (SM_TX + 4); i += SM_BS) {
 				int ly = i / (SM_TX + 4), lx = i % (SM_TX + 4);
 				int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
 				float v = 0.f;
 				if (gx >= 0 && gx < nx && gy >= 0 && gy < ny)
-					v = plane[(size_t)gy * nx + gx];
+					v = src(gx, gy, zz);
 				s_in[ly][lx] = v;
 			}
 			__syncthreads();
@@ -132,6 +131,12 @@ k_smooth(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, 
 		if (have)
 			__syncthreads();
 	}
+}
+
+// block-wide min/max into range[0] (ord min) and range[1] (ord max)
+__device__ __forceinline__ void reduce_range(float lmin, float lmax, uint32_t *range)
+{
+	__shared__ uint32_t s_red[2][SM_BS / N2M_WAVE];
 	uint32_t omin = wave_min_u32(f2ord(lmin)), omax = wave_max_u32(f2ord(lmax));
 	if (lane_id() == 0) {
 		s_red[0][wave_id()] = omin;
@@ -143,9 +148,229 @@ k_smooth(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, 
 			omin = min(omin, s_red[0][w]);
 			omax = max(omax, s_red[1][w]);
 		}
-		atomicMin(&range[0], omin);
-		atomicMax(&range[1], omax);
+		if (omin < ld_agent(&range[0]))
+			atomicMin(&range[0], omin);
+		if (omax > ld_agent(&range[1]))
+			atomicMax(&range[1], omax);
 	}
+}
+
+struct ImgSrc {
+	const float *__restrict__ img;
+	int nx;
+	size_t nxy;
+	__device__ float operator()(int x, int y, int z) const { return img[(size_t)z * nxy + (size_t)y * nx + x]; }
+};
+
+__global__ void __launch_bounds__(SM_BS)
+k_smooth(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, int nz,
+         uint32_t *__restrict__ range /* [0]=ord min, [1]=ord max */, float iso,
+         uint32_t *__restrict__ bits, int W)
+{
+	const int x0 = blockIdx.x * SM_TX, y0 = blockIdx.y * SM_TY;
+	const int z0 = blockIdx.z * SM_ZC, z1 = min(z0 + SM_ZC, nz);
+	float lmin = __builtin_inff(), lmax = -__builtin_inff();
+	ImgSrc src{in, nx, (size_t)nx * ny};
+	smooth_tile(src, out, nx, ny, nz, x0, y0, z0, z1, iso, bits, W, lmin, lmax);
+	reduce_range(lmin, lmax, range);
+}
+
+// ---------------------------------------------------------------------------
+// Batched atlas (src/nii2mesh.c:492-583).  Each label is processed on its
+// own crop (label box grown by 4 voxels, clipped to the volume): beyond 2
+// voxels from the label its binarised and smoothed volume is 0, so the crop
+// reproduces the full-volume smoothing, range, darkening and bounding box.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int atlas_label_of(float v, int nlabel)
+{
+	float r = rintf(v); // the only label whose open interval (i-0.5, i+0.5) can hold v
+	if (!(r >= 1.0f && r <= (float)nlabel))
+		return 0;
+	int i = (int)r;
+	float lo = (float)((double)i - 0.5), hi = (float)((double)i + 0.5); // src/nii2mesh.c:555-556
+	return (v > lo && v < hi) ? i : 0;
+}
+
+// per label: voxel count and box (stats[7*i] = count, min xyz, max xyz)
+__global__ void __launch_bounds__(256)
+k_atlas_stats(const float *__restrict__ labels, int nx, int ny, int nz, int nlabel, uint32_t *__restrict__ stats)
+{
+	extern __shared__ uint32_t s_st[]; // 7 * (nlabel + 1) when it fits
+	const bool priv = nlabel <= 1023;
+	uint32_t *st = priv ? s_st : stats;
+	if (priv) {
+		for (int i = threadIdx.x; i <= nlabel; i += blockDim.x) {
+			st[7 * i] = 0;
+			st[7 * i + 1] = st[7 * i + 2] = st[7 * i + 3] = 0xffffffffu;
+			st[7 * i + 4] = st[7 * i + 5] = st[7 * i + 6] = 0u;
+		}
+		__syncthreads();
+	}
+	const size_t n = (size_t)nx * ny * nz;
+	for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+		int i = atlas_label_of(labels[v], nlabel);
+		if (!i)
+			continue;
+		uint32_t x = (uint32_t)(v % nx), y = (uint32_t)((v / nx) % ny), z = (uint32_t)(v / ((size_t)nx * ny));
+		atomicAdd(&st[7 * i], 1u);
+		atomicMin(&st[7 * i + 1], x);
+		atomicMin(&st[7 * i + 2], y);
+		atomicMin(&st[7 * i + 3], z);
+		atomicMax(&st[7 * i + 4], x);
+		atomicMax(&st[7 * i + 5], y);
+		atomicMax(&st[7 * i + 6], z);
+	}
+	if (!priv)
+		return;
+	__syncthreads();
+	for (int i = threadIdx.x; i <= nlabel; i += blockDim.x) {
+		if (!st[7 * i])
+			continue;
+		atomicAdd(&stats[7 * i], st[7 * i]);
+		for (int k = 1; k <= 3; k++)
+			atomicMin(&stats[7 * i + k], st[7 * i + k]);
+		for (int k = 4; k <= 6; k++)
+			atomicMax(&stats[7 * i + k], st[7 * i + k]);
+	}
+}
+
+void launch_atlas_stats(const float *labels, int nx, int ny, int nz, int nlabel, uint32_t *stats, hipStream_t s)
+{
+	size_t n = (size_t)nx * ny * nz;
+	size_t b = (n + 255) / 256;
+	b = b > 2048 ? 2048 : (b < 1 ? 1 : b);
+	size_t lds = nlabel <= 1023 ? (size_t)7 * (nlabel + 1) * 4 : 0;
+	hipLaunchKernelGGL(k_atlas_stats, dim3((unsigned)b), dim3(256), lds, s, labels, nx, ny, nz, nlabel, stats);
+}
+
+// the binarised label (src/nii2mesh.c:557-563) read inside a task's crop
+struct LabelSrc {
+	const float *__restrict__ labels;
+	int gnx;
+	size_t gnxy;
+	int ox, oy, oz;
+	float lo, hi;
+	__device__ float operator()(int x, int y, int z) const
+	{
+		float v = labels[(size_t)(oz + z) * gnxy + (size_t)(oy + y) * gnx + (ox + x)];
+		return (v > lo && v < hi) ? 1.0f : 0.0f;
+	}
+};
+
+// smooth (or just binarise) every task's crop; ranges[2*t] = ord min/max
+__global__ void __launch_bounds__(SM_BS)
+k_atlas_smooth(const float *__restrict__ labels, int gnx, int gny, const AtlasTask *__restrict__ tasks,
+               const uint32_t *__restrict__ blockTask, int smooth, float *__restrict__ crops,
+               uint32_t *__restrict__ ranges)
+{
+	const AtlasTask t = tasks[blockTask[blockIdx.x]];
+	const int local = blockIdx.x - t.first_block;
+	LabelSrc src{labels, gnx, (size_t)gnx * gny, t.ox, t.oy, t.oz, t.lo, t.hi};
+	float *out = crops + t.off;
+	float lmin = __builtin_inff(), lmax = -__builtin_inff();
+	if (smooth) {
+		const int tx = (t.nx + SM_TX - 1) / SM_TX, ty = (t.ny + SM_TY - 1) / SM_TY;
+		const int bx = local % tx, by = (local / tx) % ty, bz = local / (tx * ty);
+		const int z0 = bz * SM_ZC, z1 = min(z0 + SM_ZC, t.nz);
+		smooth_tile(src, out, t.nx, t.ny, t.nz, bx * SM_TX, by * SM_TY, z0, z1, 0.f, nullptr, 0, lmin, lmax);
+	} else {
+		const size_t n = (size_t)t.nx * t.ny * t.nz;
+		for (size_t v = (size_t)local * SM_BS + threadIdx.x; v < n; v += (size_t)SM_BS * (size_t)(1 << 30)) {
+			int x = (int)(v % t.nx), y = (int)((v / t.nx) % t.ny), z = (int)(v / ((size_t)t.nx * t.ny));
+			float o = src(x, y, z);
+			out[v] = o;
+			lmin = fminf(lmin, o);
+			lmax = fmaxf(lmax, o);
+		}
+	}
+	reduce_range(lmin, lmax, &ranges[2 * blockTask[blockIdx.x]]);
+}
+
+void launch_atlas_smooth(const float *labels, int gnx, int gny, int gnz, const AtlasTask *tasks,
+                         const uint32_t *blockTask, unsigned nblocks, int smooth, float *crops, uint32_t *ranges,
+                         hipStream_t s)
+{
+	if (nblocks)
+		hipLaunchKernelGGL(k_atlas_smooth, dim3(nblocks), dim3(SM_BS), 0, s, labels, gnx, gny, tasks, blockTask,
+		                   smooth, crops, ranges);
+}
+
+// darken the voxels on the volume's faces, bounding box of >= iso and the
+// minimum of the result, per task (src/meshify.c:347-371)
+__global__ void __launch_bounds__(256)
+k_atlas_apply(float *__restrict__ crops, int gnx, int gny, int gnz, const AtlasTask *__restrict__ tasks,
+              const uint32_t *__restrict__ blockTask, int *__restrict__ bbox, uint32_t *__restrict__ vmin)
+{
+	__shared__ int s_red[6][4];
+	__shared__ uint32_t s_min[4];
+	const int ti = blockTask[blockIdx.x];
+	const AtlasTask t = tasks[ti];
+	const int local = blockIdx.x - t.first_block;
+	const size_t n = (size_t)t.nx * t.ny * t.nz;
+	float lmin = __builtin_inff();
+	int bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, bmax[3] = {-1, -1, -1};
+	float *img = crops + t.off;
+	for (int k = 0; k < 4; k++) {
+		size_t v = ((size_t)local * 4 + k) * 256 + threadIdx.x;
+		if (v >= n)
+			break;
+		const int x = (int)(v % t.nx), y = (int)((v / t.nx) % t.ny), z = (int)(v / ((size_t)t.nx * t.ny));
+		float val = img[v];
+		if (val >= t.iso) {
+			bmin[0] = min(bmin[0], x);
+			bmax[0] = max(bmax[0], x);
+			bmin[1] = min(bmin[1], y);
+			bmax[1] = max(bmax[1], y);
+			bmin[2] = min(bmin[2], z);
+			bmax[2] = max(bmax[2], z);
+		}
+		const int gx = t.ox + x, gy = t.oy + y, gz = t.oz + z;
+		if (gx == 0 || gy == 0 || gz == 0 || gx == gnx - 1 || gy == gny - 1 || gz == gnz - 1) {
+			val = fminf(t.edgeMax, val);
+			img[v] = val;
+		}
+		lmin = fminf(lmin, val);
+	}
+	for (int k = 0; k < 3; k++) {
+		bmin[k] = wave_min_i32(bmin[k]);
+		bmax[k] = wave_max_i32(bmax[k]);
+	}
+	uint32_t om = wave_min_u32(f2ord(lmin));
+	if (lane_id() == 0) {
+		for (int k = 0; k < 3; k++) {
+			s_red[k][wave_id()] = bmin[k];
+			s_red[3 + k][wave_id()] = bmax[k];
+		}
+		s_min[wave_id()] = om;
+	}
+	__syncthreads();
+	int *bb = bbox + 6 * ti;
+	if (threadIdx.x < 6) {
+		int k = threadIdx.x;
+		int r = s_red[k][0];
+		for (int w = 1; w < 4; w++)
+			r = k < 3 ? min(r, s_red[k][w]) : max(r, s_red[k][w]);
+		const int cur = (int)ld_agent((const uint32_t *)&bb[k]);
+		if (k < 3) {
+			if (r != 0x7fffffff && r < cur)
+				atomicMin(&bb[k], r);
+		} else if (r >= 0 && r > cur) {
+			atomicMax(&bb[k], r);
+		}
+	}
+	if (threadIdx.x == 6) {
+		uint32_t m = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+		if (m < ld_agent(&vmin[ti]))
+			atomicMin(&vmin[ti], m);
+	}
+}
+
+void launch_atlas_apply(float *crops, int gnx, int gny, int gnz, const AtlasTask *tasks, const uint32_t *blockTask,
+                        unsigned nblocks, int *bbox, uint32_t *vmin, hipStream_t s)
+{
+	if (nblocks)
+		hipLaunchKernelGGL(k_atlas_apply, dim3(nblocks), dim3(256), 0, s, crops, gnx, gny, gnz, tasks, blockTask,
+		                   bbox, vmin);
 }
 
 void launch_smooth(const float *in, float *out, int nx, int ny, int nz, uint32_t *range, float iso,
@@ -463,6 +688,46 @@ void launch_hist_bins(const float *img, size_t n, float mn, float scl, int nbins
 {
 	hipLaunchKernelGGL(k_hist_bins, dim3(hist_blocks(n)), dim3(256), nbins * sizeof(uint32_t), s, img, n,
 	                   mn, scl, nbins, hist);
+}
+
+unsigned atlas_smooth_blocks(int nx, int ny, int nz, int smooth)
+{
+	if (smooth)
+		return (unsigned)(((nx + SM_TX - 1) / SM_TX) * ((ny + SM_TY - 1) / SM_TY) * ((nz + SM_ZC - 1) / SM_ZC));
+	return (unsigned)(((size_t)nx * ny * nz + SM_BS - 1) / SM_BS);
+}
+unsigned atlas_apply_blocks(int nx, int ny, int nz) { return (unsigned)(((size_t)nx * ny * nz + 1023) / 1024); }
+
+// one label of the per-label loop: out = (lo < v < hi), count of ones
+// (src/nii2mesh.c:553-563)
+__global__ void __launch_bounds__(256)
+k_binarize(const float *__restrict__ labels, size_t n, float lo, float hi, float *__restrict__ out,
+           uint32_t *__restrict__ count)
+{
+	uint32_t c = 0;
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		float v = labels[i];
+		bool in = v > lo && v < hi;
+		out[i] = in ? 1.0f : 0.0f;
+		c += in;
+	}
+	__shared__ uint32_t s_c[4];
+	c = wave_sum(c);
+	if (lane_id() == 0)
+		s_c[wave_id()] = c;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		c = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+		if (c)
+			atomicAdd(count, c);
+	}
+}
+
+void launch_binarize(const float *labels, size_t n, float lo, float hi, float *out, uint32_t *count, hipStream_t s)
+{
+	size_t b = (n + 255) / 256;
+	b = b > 4096 ? 4096 : (b < 1 ? 1 : b);
+	hipLaunchKernelGGL(k_binarize, dim3((unsigned)b), dim3(256), 0, s, labels, n, lo, hi, out, count);
 }
 
 } // namespace n2m

@@ -18,6 +18,23 @@ inline float ord2f_host(uint32_t o)
 	return f;
 }
 
+// One label of the batched atlas (src/nii2mesh.c:551-573): its padded crop
+// of the volume and the per-label values meshify() derives.
+struct AtlasTask {
+	int label;
+	float lo, hi;       // voxel v belongs to the label when lo < v < hi
+	int ox, oy, oz;     // crop origin in the volume
+	int nx, ny, nz;     // crop size
+	long long off;      // crop offset in the crop buffer (floats)
+	float iso, mn, edgeMax;
+	int lo3[3], hi3[3]; // marching-cubes box in crop coordinates
+	float mnv;          // Lewiner padding value (min - iso)
+	long long cube_off; // offset of the task's per-cube / per-point arrays
+	int first_block;    // first workgroup of the task in the current launch
+	int nseg;           // Lewiner segments per row
+	int pad;
+};
+
 struct CclTimer {
 	virtual void begin(const char *name, double bytes) = 0;
 	virtual void end() = 0;
@@ -35,6 +52,17 @@ void launch_apply(float *img, int nx, int ny, int nz, float iso, float mn, float
                   const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s);
 void launch_hist_range(const float *img, size_t n, uint32_t *stats, hipStream_t s);
 void launch_hist_bins(const float *img, size_t n, float mn, float scl, int nbins, uint32_t *hist, hipStream_t s);
+
+// atlas volume stages (k_volume.hip)
+void launch_atlas_stats(const float *labels, int nx, int ny, int nz, int nlabel, uint32_t *stats, hipStream_t s);
+void launch_atlas_smooth(const float *labels, int gnx, int gny, int gnz, const AtlasTask *tasks,
+                         const uint32_t *blockTask, unsigned nblocks, int smooth, float *crops, uint32_t *ranges,
+                         hipStream_t s);
+void launch_atlas_apply(float *crops, int gnx, int gny, int gnz, const AtlasTask *tasks, const uint32_t *blockTask,
+                        unsigned nblocks, int *bbox, uint32_t *vmin, hipStream_t s);
+unsigned atlas_smooth_blocks(int nx, int ny, int nz, int smooth);
+unsigned atlas_apply_blocks(int nx, int ny, int nz);
+void launch_binarize(const float *labels, size_t n, float lo, float hi, float *out, uint32_t *count, hipStream_t s);
 
 // k_ccl.hip
 void launch_ccl(const uint32_t *bits, int nx, int ny, int nz, int conn, int invert, uint32_t *L, uint32_t *AUX,
@@ -55,14 +83,30 @@ size_t classic_blocks(int cx, int cy, int cz);
 void launch_classic_count(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                           uint8_t *cfgs, uint32_t *blockSums, hipStream_t s);
 void launch_classic_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
-                         const uint8_t *cfgs, const uint32_t *blockOffs, double *pts, hipStream_t s);
+                         const uint8_t *cfgs, const uint32_t *blockOffs, double *pts, const int origin[3],
+                         hipStream_t s);
 size_t lw_nseg_total(int nx, int ny, int nz, const int lo[3], const int hi[3]);
 void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                         float mnv, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
                         hipStream_t s);
 void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                     float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow, const uint32_t *bases,
-                    uint32_t nedge, double *pts, int *tris, hipStream_t s);
+                    uint32_t nedge, double *pts, int *tris, const int origin[3], hipStream_t s);
+
+// atlas marching cubes (k_mc.hip): one workgroup per block of a task
+void launch_atlas_classic_count(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
+                                unsigned nblocks, uint8_t *cfgs, uint32_t *blockSums, hipStream_t s);
+void launch_atlas_classic_emit(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
+                               unsigned nblocks, const uint8_t *cfgs, const uint32_t *blockOffs, double *pts,
+                               hipStream_t s);
+void launch_atlas_lw_classify(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
+                              unsigned nblocks, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
+                              size_t stride, hipStream_t s);
+void launch_atlas_lw_emit(const float *crops, const AtlasTask *tasks, int ntasks, const uint32_t *blockTask,
+                          unsigned nblocks, int originalMC, const uint8_t *pflags, const uint16_t *crow,
+                          const uint32_t *bases, size_t stride, uint32_t totalE, double *pts, int *tris,
+                          hipStream_t s);
+int lw_seg_len();
 
 // k_mesh.hip
 size_t scan_scratch_words(size_t n);
@@ -113,6 +157,21 @@ void launch_scatter_rows(const double *src, const uint32_t *perm, uint32_t n, do
 void launch_relabel_tris(const int *tris, const uint32_t *inv, uint32_t nt, int *out, hipStream_t s);
 void launch_tris_repeat(const int *tris, uint32_t nt, uint32_t *flag, hipStream_t s);
 void launch_border_pairs(const uint32_t *start, const int2 *pairs, uint32_t nv, uint8_t *border, hipStream_t s);
+// batched atlas weld (k_mesh.hip)
+void launch_atlas_weld_keys(const double *pts, uint32_t n, const uint32_t *vbase, int ntasks, uint32_t *keys,
+                            uint32_t *vals, uint32_t *vtask, hipStream_t s);
+void launch_atlas_task_keys(const uint32_t *vals, const uint32_t *vtask, uint32_t n, uint32_t *tk, uint32_t *pos,
+                            hipStream_t s);
+void launch_atlas_weld_gather(const uint32_t *keys, const uint32_t *vals, const uint32_t *pos, uint32_t n,
+                              uint32_t *keys2, uint32_t *vals2, hipStream_t s);
+void launch_atlas_weld_segflags(const uint32_t *keys, const uint32_t *tk, uint32_t n, uint32_t *flags,
+                                hipStream_t s);
+void launch_atlas_class_base(const uint32_t *tk, const uint32_t *segid, const uint32_t *seedbase, uint32_t n,
+                             uint32_t *cbase, hipStream_t s);
+void launch_atlas_weld_identity(uint32_t n, const uint32_t *vtask, const uint32_t *vbase, const uint32_t *cbase,
+                                const uint8_t *ident, int *cls, int *rep, hipStream_t s);
+void launch_gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t n, uint32_t *dst, hipStream_t s);
+void launch_add_i32(const int *src, size_t n, int add, int *dst, hipStream_t s);
 void launch_sform(double *p, uint32_t np, const float *sx, const float *sy, const float *sz, hipStream_t s);
 void launch_swap_winding(int *t, uint32_t nt, hipStream_t s);
 
