@@ -218,6 +218,25 @@ int n2m_atlas(const float *img, short dim[3], int originalMC, bool preSmooth, bo
               const float *srow, bool verbose, n2m_atlas_mesh **meshes, int *nmesh);
 void n2m_atlas_free(n2m_atlas_mesh *meshes, int nmesh);
 
+/* ---------------------------------------------------------------------------
+ * Differentiable path.  Forward: Lewiner marchingCubes over the whole volume
+ * (lo = 0, hi = dim-1; src/MarchingCubes.c:1086-1143) at `isolevel`,
+ * unify_vertices + remove_degenerate_triangles (src/meshify.c:45-168) and
+ * lapIter laplacian_smooth iterations (src/quadric.c:316-341); the mesh is
+ * mesh 0 of the context.  exact = 0 keeps the reference's FP32 vertex
+ * positions; exact = 1 evaluates them in FP64 (the smooth map the adjoint
+ * differentiates).  d_vol must stay unchanged until the backward.
+ * Backward: dL/dvolume (nx*ny*nz doubles, x fastest) from dL/dp (np x 3
+ * doubles) of the final vertices: adjoint of the edge interpolation
+ * u = c0/(c0-c1), of the centre-vertex mean, of the weld and of the
+ * transposed Laplacian.  No counterpart in the reference.
+ * ------------------------------------------------------------------------- */
+int n2m_ctx_diff_forward(n2m_ctx *ctx, const float *d_vol, const int dim[3], float isolevel, int originalMC,
+                         int lapIter, int exact);
+int n2m_ctx_diff_backward(n2m_ctx *ctx, const double *d_dLdp, double *d_dLdvol);
+/* Transpose of n2m_laplacian_smooth (iter times) on host arrays. */
+void n2m_laplacian_smooth_T(vec3d *g, vec3i *tris, int nvert, int ntri, int iter);
+
 /* Per-kernel timing with HIP events on the context stream. */
 void n2m_ctx_enable_timing(n2m_ctx *ctx, int on);
 void n2m_ctx_reset_timing(n2m_ctx *ctx);

@@ -93,6 +93,13 @@ def _bind_ext(L):
     L.n2m_apply_sform.restype = None
     for fn in (L.n2m_ctx_atlas, L.n2m_ctx_atlas_loop):
         fn.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(_Params), C.c_int, f32]
+    L.n2m_ctx_diff_forward.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.c_float, C.c_int, C.c_int,
+                                       C.c_int]
+    L.n2m_ctx_diff_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.n2m_laplacian_smooth_T.argtypes = [np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'),
+                                         np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                         C.c_int]
+    L.n2m_laplacian_smooth_T.restype = None
     L.n2m_ctx_atlas_count.argtypes = [C.c_void_p]
     L.n2m_ctx_atlas_entry.argtypes = [C.c_void_p, C.c_int, C.POINTER(AtlasEntry)]
     L.n2m_ctx_atlas_download.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
@@ -104,6 +111,14 @@ def lib(cubes=None):
     if cubes is not None:
         L.n2m_set_cubes(int(cubes))
     return abi.MeshLib(L)
+
+
+def laplacian_smooth_T(g, tris, iters=1):
+    """Transpose of laplacian_smooth (src/quadric.c:316-341), iters times,
+    in place on a float64 (np, 3) array."""
+    L = load()
+    L.n2m_laplacian_smooth_T(g, np.ascontiguousarray(tris, np.int32), len(g), len(tris), int(iters))
+    return g
 
 
 def last_error():
@@ -180,6 +195,16 @@ class DeviceContext:
         if self.L.n2m_ctx_atlas_download(self.h, k, t.ctypes.data, p.ctypes.data) != 0:
             raise RuntimeError(last_error())
         return t, p
+
+    def diff_forward(self, d_ptr, dims, isolevel=0.0, originalMC=0, lap_iter=20, exact=False):
+        """Differentiable forward on a device volume; the mesh is mesh 0."""
+        d = (C.c_int * 3)(*dims)
+        return self.L.n2m_ctx_diff_forward(self.h, C.c_void_p(d_ptr), d, float(isolevel), int(originalMC),
+                                           int(lap_iter), int(exact))
+
+    def diff_backward(self, d_dLdp, d_dLdvol):
+        """dL/dvolume (float64 device array) from dL/dp (float64 device np x 3)."""
+        return self.L.n2m_ctx_diff_backward(self.h, C.c_void_p(d_dLdp), C.c_void_p(d_dLdvol))
 
     def set_threshold(self, d_ptr, nvox, mode):
         return float(self.L.n2m_ctx_set_threshold(self.h, C.c_void_p(d_ptr), int(nvox), int(mode)))

@@ -133,6 +133,13 @@ enum Slot {
 	S_AT_IDX,
 	S_AT_OUT,
 	S_AT_LABELS,
+	S_DF_VREC,
+	S_DF_CREC,
+	S_DF_CLS,
+	S_DF_REP,
+	S_DF_G,
+	S_DF_H,
+	S_DF_GM,
 	S_COUNT
 };
 
@@ -181,6 +188,15 @@ struct n2m_ctx {
 	std::vector<Mesh> meshes;
 	Mesh atlas_mesh;                    // union of the label meshes of the last atlas run
 	std::vector<n2m_atlas_entry> atlas; // one entry per label 1..n
+	struct {                            // last differentiable forward
+		bool valid = false;
+		const float *vol = nullptr;
+		int dim[3] = {0, 0, 0};
+		float iso = 0.f;
+		int lapIter = 0;
+		uint32_t nmc = 0; // marching-cubes vertices before the weld
+		bool welded = false;
+	} diff;
 	uint32_t *d_small = nullptr;
 	uint32_t *h_small = nullptr;
 	bool timing = false;
@@ -1478,6 +1494,121 @@ int atlas_run(n2m_ctx *C, const float *labels, const int dim[3], const n2m_param
 
 } // namespace
 
+
+// ===========================================================================
+// Differentiable path: forward and adjoint (kernels in k_adjoint.hip)
+// ===========================================================================
+namespace {
+
+// Lewiner over the whole volume (lo = 0, hi = dim-1), weld, degenerate
+// removal, lapIter Laplacian iterations; the mesh goes to mesh slot 0.
+int diff_forward(n2m_ctx *C, const float *vol, const int dim[3], float iso, int originalMC, int lapIter, bool exact)
+{
+	hipStream_t s = C->stream;
+	C->diff.valid = false;
+	const int NX = dim[0], NY = dim[1], NZ = dim[2];
+	if (NX < 2 || NY < 2 || NZ < 2)
+		fail("differentiable path needs at least 2 voxels per axis");
+	const int lo[3] = {0, 0, 0}, hi[3] = {NX - 1, NY - 1, NZ - 1};
+	const size_t npts = (size_t)NX * NY * NZ;
+	const size_t nseg = lw_nseg_total(NX, NY, NZ, lo, hi);
+	uint8_t *pf = C->buf<uint8_t>(S_PFLAGS, npts);
+	uint16_t *cr = C->buf<uint16_t>(S_CROW, npts);
+	uint32_t *cnt = C->buf<uint32_t>(S_SEGCNT, 3 * nseg);
+	uint32_t *base = C->buf<uint32_t>(S_SEGBASE, 3 * nseg);
+	// the sub-box is the volume: no padding value is ever read
+	C->tbegin("mc_lewiner_classify", 7.0 * npts);
+	launch_lw_classify(vol, NX, NY, NZ, lo, hi, iso, 0.f, originalMC, pf, cr, cnt, s);
+	C->tend();
+	uint32_t nE = scan(C, cnt, base, nseg, true);
+	uint32_t nT = scan(C, cnt + nseg, base + nseg, nseg, true);
+	uint32_t nC = scan(C, cnt + 2 * nseg, base + 2 * nseg, nseg, true);
+	uint32_t V = nE + nC;
+	if (V < 3 || nT < 1)
+		return EXIT_FAILURE;
+	double *pts0 = C->buf<double>(S_PTS0, (size_t)3 * V);
+	int *tris0 = C->buf<int>(S_TRIS0, (size_t)3 * nT);
+	unsigned long long *vrec = C->buf<unsigned long long>(S_DF_VREC, V);
+	uint32_t *crec = C->buf<uint32_t>(S_DF_CREC, (size_t)12 * (nC ? nC : 1));
+	C->tbegin("mc_lewiner_emit", 3.0 * npts + 32.0 * V + 12.0 * nT + 48.0 * nC);
+	launch_lw_emit_rec(vol, NX, NY, NZ, lo, hi, iso, 0.f, originalMC, pf, cr, base, nE, pts0, tris0, exact ? 1 : 0,
+	                   vrec, crec, s);
+	C->tend();
+	double *pts = nullptr;
+	int *tris = nullptr;
+	uint32_t np = weld(C, pts0, V, tris0, nT, pts, tris, false);
+	bool welded = np != V;
+	if (welded) { // keep the class map for the backward
+		HIPX(hipMemcpyAsync(C->buf<int>(S_DF_CLS, V), C->buf<int>(S_CLS, V), (size_t)4 * V, hipMemcpyDeviceToDevice,
+		                    s));
+		HIPX(hipMemcpyAsync(C->buf<int>(S_DF_REP, np), C->buf<int>(S_REP, np), (size_t)4 * np,
+		                    hipMemcpyDeviceToDevice, s));
+	}
+	if (np < 3)
+		return EXIT_FAILURE;
+	uint32_t nt = remove_degenerate(C, pts, np, tris, nT, false);
+	C->meshes.resize(1);
+	Mesh &m = C->meshes[0];
+	int *mt = (int *)n2m_ctx::grow(m.tris, (size_t)12 * (nt ? nt : 1));
+	double *mp = (double *)n2m_ctx::grow(m.pts, (size_t)24 * np);
+	HIPX(hipMemcpyAsync(mt, tris, (size_t)12 * nt, hipMemcpyDeviceToDevice, s));
+	HIPX(hipMemcpyAsync(mp, pts, (size_t)24 * np, hipMemcpyDeviceToDevice, s));
+	m.nt = (int)nt;
+	m.np = (int)np;
+	laplace_device(C, mp, mt, np, nt, lapIter);
+	C->sync();
+	C->diff.valid = true;
+	C->diff.vol = vol;
+	for (int i = 0; i < 3; i++)
+		C->diff.dim[i] = dim[i];
+	C->diff.iso = iso;
+	C->diff.lapIter = lapIter;
+	C->diff.nmc = V;
+	C->diff.welded = welded;
+	return EXIT_SUCCESS;
+}
+
+int diff_backward(n2m_ctx *C, const double *dLdp, double *dLdvol)
+{
+	if (!C->diff.valid || C->meshes.empty())
+		fail("no differentiable forward on this context");
+	hipStream_t s = C->stream;
+	const Mesh &m = C->meshes[0];
+	const uint32_t np = (uint32_t)m.np, nt = (uint32_t)m.nt, nmc = C->diff.nmc;
+	const int *D = C->diff.dim;
+	const size_t nvox = (size_t)D[0] * D[1] * D[2];
+	double *g = C->buf<double>(S_DF_G, (size_t)3 * np), *h = C->buf<double>(S_DF_H, (size_t)3 * np);
+	HIPX(hipMemcpyAsync(g, dLdp, (size_t)24 * np, hipMemcpyDeviceToDevice, s));
+	if (C->diff.lapIter > 0) {
+		Csr c = build_csr(C, (const int *)m.tris.p, np, nt);
+		C->tbegin("adj_laplace", 4.0 * np + 8.0 * 3 * nt + 72.0 * np);
+		for (int it = 0; it < C->diff.lapIter; it++)
+			launch_adj_laplace_T(c.start, c.pairs, g, h, np, s);
+		C->tend();
+	}
+	double *gm = C->buf<double>(S_DF_GM, (size_t)3 * nmc);
+	AdjGeo ag;
+	ag.nx = D[0];
+	ag.nxy = (long long)D[0] * D[1];
+	ag.nxyz = (long long)nvox;
+	ag.lo0 = ag.lo1 = ag.lo2 = 0;
+	ag.sx = D[0];
+	ag.sy = D[1];
+	ag.iso = C->diff.iso;
+	ag.mnv = 0.f;
+	HIPX(hipMemsetAsync(dLdvol, 0, 8 * nvox, s));
+	C->tbegin("adj_mc", 24.0 * nmc * 2 + 8.0 * nmc + 16.0 * nmc);
+	launch_adj_unweld(C->diff.welded ? C->buf<int>(S_DF_CLS, nmc) : nullptr,
+	                  C->diff.welded ? C->buf<int>(S_DF_REP, np) : nullptr, g, nmc, gm, s);
+	launch_adj_centre(C->buf<unsigned long long>(S_DF_VREC, nmc), C->buf<uint32_t>(S_DF_CREC, 12), nmc, gm, s);
+	launch_adj_edge(C->buf<unsigned long long>(S_DF_VREC, nmc), nmc, gm, C->diff.vol, ag, dLdvol, s);
+	C->tend();
+	C->sync();
+	return EXIT_SUCCESS;
+}
+
+} // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1754,6 +1885,38 @@ void n2m_atlas_free(n2m_atlas_mesh *meshes, int nmesh)
 		free(meshes[k].p);
 	}
 	free(meshes);
+}
+
+int n2m_ctx_diff_forward(n2m_ctx *C, const float *d_vol, const int dim[3], float isolevel, int originalMC,
+                         int lapIter, int exact)
+{
+	return guarded([&] { return diff_forward(C, d_vol, dim, isolevel, originalMC, lapIter, exact != 0); });
+}
+
+int n2m_ctx_diff_backward(n2m_ctx *C, const double *d_dLdp, double *d_dLdvol)
+{
+	return guarded([&] { return diff_backward(C, d_dLdp, d_dLdvol); });
+}
+
+void n2m_laplacian_smooth_T(vec3d *g, vec3i *tris, int nvert, int ntri, int iter)
+{
+	guarded([&] {
+		if (nvert <= 0 || iter <= 0)
+			return 0;
+		check_tris((const int *)tris, ntri, nvert);
+		n2m_ctx *C = tl_ctx.get();
+		double *dg = C->buf<double>(S_DF_G, (size_t)3 * nvert), *dh = C->buf<double>(S_DF_H, (size_t)3 * nvert);
+		int *dt = C->buf<int>(S_TRIS1, (size_t)3 * (ntri ? ntri : 1));
+		HIPX(hipMemcpyAsync(dg, g, (size_t)24 * nvert, hipMemcpyHostToDevice, C->stream));
+		HIPX(hipMemcpyAsync(dt, tris, (size_t)12 * ntri, hipMemcpyHostToDevice, C->stream));
+		Csr c = build_csr(C, dt, nvert, ntri);
+		for (int it = 0; it < iter; it++)
+			launch_adj_laplace_T(c.start, c.pairs, dg, dh, nvert, C->stream);
+		C->check();
+		HIPX(hipMemcpyAsync(g, dg, (size_t)24 * nvert, hipMemcpyDeviceToHost, C->stream));
+		C->sync();
+		return 0;
+	});
 }
 
 void n2m_ctx_enable_timing(n2m_ctx *C, int on) { C->timing = on != 0; }

@@ -440,6 +440,27 @@ __device__ __forceinline__ void lw_edge_pos(const float *__restrict__ img, const
 	p[a] = p[a] + u;
 }
 
+// the same position in double (differentiable path, exact mode): x + u with
+// u = c0 / (c0 - c1) evaluated in double from the float corner values
+__device__ __forceinline__ void lw_edge_pos_exact(const float *__restrict__ img, const LwGeo &g, int x, int y, int z,
+                                                  int a, double *p)
+{
+	float c0 = lw_data(img, g, x, y, z);
+	float c1 = lw_data(img, g, x + (a == 0), y + (a == 1), z + (a == 2));
+	double dd = (double)c0 - (double)c1;
+	double u = dd != 0.0 ? (double)c0 / dd : 0.5;
+	p[0] = (double)x;
+	p[1] = (double)y;
+	p[2] = (double)z;
+	p[a] = p[a] + u;
+}
+
+// Emit modes: positions as the reference computes them, or (differentiable
+// path) with each vertex's provenance recorded -- an edge vertex stores
+// point*4 + axis, a centre vertex 2^63 + its centre index with its cube's
+// 12 edge-vertex ids in crec -- and optionally FP64 positions.
+enum { LW_PLAIN = 0, LW_RECORD = 1, LW_RECORD_EXACT = 2 };
+
 // cube edge code -> (point dx, dy, dz, axis), src/MarchingCubes.c:811-824
 __constant__ int8_t c_code[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1},
                                     {0, 0, 1, 0}, {1, 0, 1, 1}, {0, 1, 1, 0}, {0, 0, 1, 1},
@@ -450,10 +471,13 @@ __constant__ int8_t c_code[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {
 // three arrays (stride bstride) of exclusive segment offsets for the volume
 // (segment index seg + nseg*(y + sy*z)); edge_add / centre_add are added to
 // edge-vertex / centre-vertex numbers.
+template <int MODE = LW_PLAIN>
 __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const LwGeo &g, int seg, int y, int z,
                                             const uint8_t *__restrict__ pflags, const uint16_t *__restrict__ crow,
                                             const uint32_t *__restrict__ bases, size_t bstride, uint32_t edge_add,
-                                            uint32_t centre_add, double *__restrict__ pts, int *__restrict__ tris)
+                                            uint32_t centre_add, double *__restrict__ pts, int *__restrict__ tris,
+                                            unsigned long long *__restrict__ vrec = nullptr,
+                                            uint32_t *__restrict__ crec = nullptr)
 {
 	__shared__ uint32_t s_pre[4][LW_SEG + 1]; // popcount prefix for the 4 point rows
 	__shared__ uint32_t s_base[4];
@@ -493,12 +517,22 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 			for (int a = 0; a < 3; a++) {
 				if (!((f >> a) & 1u))
 					continue;
-				float p[3];
-				lw_edge_pos(img, g, x, y, z, a, p);
 				double *o = pts + (size_t)id * 3;
-				o[0] = (double)(p[0] + (float)g.out0);
-				o[1] = (double)(p[1] + (float)g.out1);
-				o[2] = (double)(p[2] + (float)g.out2);
+				if (MODE == LW_RECORD_EXACT) {
+					double q[3];
+					lw_edge_pos_exact(img, g, x, y, z, a, q);
+					o[0] = q[0] + (double)g.out0;
+					o[1] = q[1] + (double)g.out1;
+					o[2] = q[2] + (double)g.out2;
+				} else {
+					float p[3];
+					lw_edge_pos(img, g, x, y, z, a, p);
+					o[0] = (double)(p[0] + (float)g.out0);
+					o[1] = (double)(p[1] + (float)g.out1);
+					o[2] = (double)(p[2] + (float)g.out2);
+				}
+				if (MODE != LW_PLAIN)
+					vrec[id] = ((unsigned long long)(rb + x) << 2) | (unsigned)a;
 				id++;
 			}
 		}
@@ -557,6 +591,34 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 				o[0] = (double)(sxv + (float)g.out0);
 				o[1] = (double)(syv + (float)g.out1);
 				o[2] = (double)(szv + (float)g.out2);
+				if (MODE != LW_PLAIN) {
+					const uint32_t ci = cid - centre_add;
+					vrec[cid] = (1ull << 63) | ci;
+					for (int e = 0; e < 12; e++)
+						crec[(size_t)ci * 12 + e] = ids[e];
+				}
+				if (MODE == LW_RECORD_EXACT) { // mean of the FP64 edge positions
+					double ex = 0.0, ey = 0.0, ez = 0.0, dc = 0.0;
+					for (int e = 0; e < 12; e++) {
+						if (ids[e] == 0xffffffffu)
+							continue;
+						double q[3];
+						lw_edge_pos_exact(img, g, x + c_code[e][0], y + c_code[e][1], z + c_code[e][2],
+						                  c_code[e][3], q);
+						dc += 1.0;
+						ex += q[0];
+						ey += q[1];
+						ez += q[2];
+					}
+					if (dc > 0.0) {
+						ex /= dc;
+						ey /= dc;
+						ez /= dc;
+					}
+					o[0] = ex + (double)g.out0;
+					o[1] = ey + (double)g.out1;
+					o[2] = ez + (double)g.out2;
+				}
 			}
 			const uint8_t *codes = &n2m_lw_edges[n2m_lw_row_off[row]];
 			int *t = tris + (size_t)(tbase + tcarry + toff) * 3;
@@ -583,6 +645,18 @@ k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pf
 {
 	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
 	lw_emit_row(img, g, blockIdx.x, blockIdx.y, blockIdx.z, pflags, crow, bases, ns, 0u, nedge, pts, tris);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(LW_BS)
+k_lw_emit_rec(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
+              const uint16_t *__restrict__ crow, const uint32_t *__restrict__ bases, uint32_t nedge,
+              double *__restrict__ pts, int *__restrict__ tris, unsigned long long *__restrict__ vrec,
+              uint32_t *__restrict__ crec)
+{
+	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
+	lw_emit_row<MODE>(img, g, blockIdx.x, blockIdx.y, blockIdx.z, pflags, crow, bases, ns, 0u, nedge, pts, tris,
+	                  vrec, crec);
 }
 
 void lw_geo(LwGeo &g, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso, float mnv,
@@ -634,6 +708,24 @@ void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], c
 	g.out2 = lo[2] + origin[2];
 	hipLaunchKernelGGL(k_lw_emit, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags, crow, bases,
 	                   nedge, pts, tris);
+}
+
+void launch_lw_emit_rec(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
+                        float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow,
+                        const uint32_t *bases, uint32_t nedge, double *pts, int *tris, int exact,
+                        unsigned long long *vrec, uint32_t *crec, hipStream_t s)
+{
+	LwGeo g;
+	lw_geo(g, nx, ny, nz, lo, hi, iso, mnv, originalMC);
+	g.out0 = lo[0];
+	g.out1 = lo[1];
+	g.out2 = lo[2];
+	if (exact)
+		hipLaunchKernelGGL(k_lw_emit_rec<LW_RECORD_EXACT>, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g,
+		                   pflags, crow, bases, nedge, pts, tris, vrec, crec);
+	else
+		hipLaunchKernelGGL(k_lw_emit_rec<LW_RECORD>, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags,
+		                   crow, bases, nedge, pts, tris, vrec, crec);
 }
 
 // ---------------------------------------------------------------------------

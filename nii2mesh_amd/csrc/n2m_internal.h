@@ -35,6 +35,15 @@ struct AtlasTask {
 	int pad;
 };
 
+// sub-box geometry of the differentiable path's marching cubes
+struct AdjGeo {
+	int nx;
+	long long nxy, nxyz;
+	int lo0, lo1, lo2;
+	int sx, sy;
+	float iso, mnv;
+};
+
 struct CclTimer {
 	virtual void begin(const char *name, double bytes) = 0;
 	virtual void end() = 0;
@@ -92,6 +101,20 @@ void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3
 void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                     float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow, const uint32_t *bases,
                     uint32_t nedge, double *pts, int *tris, const int origin[3], hipStream_t s);
+
+void launch_lw_emit_rec(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
+                        float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow,
+                        const uint32_t *bases, uint32_t nedge, double *pts, int *tris, int exact,
+                        unsigned long long *vrec, uint32_t *crec, hipStream_t s);
+
+// adjoint (k_adjoint.hip)
+void launch_adj_laplace_T(const uint32_t *start, const int2 *pairs, double *g, double *h, uint32_t n,
+                          hipStream_t s);
+void launch_adj_unweld(const int *cls, const int *rep, const double *g, uint32_t nmc, double *gm, hipStream_t s);
+void launch_adj_centre(const unsigned long long *vrec, const uint32_t *crec, uint32_t nmc, double *gm,
+                       hipStream_t s);
+void launch_adj_edge(const unsigned long long *vrec, uint32_t nmc, const double *gm, const float *img,
+                     const AdjGeo &g, double *dvol, hipStream_t s);
 
 // atlas marching cubes (k_mc.hip): one workgroup per block of a task
 void launch_atlas_classic_count(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,

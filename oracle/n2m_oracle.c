@@ -510,8 +510,15 @@ static float lw_clamp(float v)
 	return fabs(v) < FLT_EPSILON ? FLT_EPSILON : v; /* src/MarchingCubes.c:132,252-255 */
 }
 
-int orc_mc_lewiner(const float *img, const short dim[3], const int lo[3], const int hi[3],
-                   int originalMC, float isolevel, vec3d **vs, vec3i **ts, int *nv, int *nt)
+/* Provenance of a Lewiner vertex (differentiable path): an edge vertex
+ * records its sub-box point and axis as point*4+axis; a centre vertex
+ * records -(1 + k) with its cube's 12 edge-vertex ids (-1 = none) in
+ * cedges[12k..12k+11].  exact != 0 computes positions in double (x + u, u =
+ * c0/(c0-c1) from the float corner values) instead of the reference's float
+ * sums. */
+static int lewiner_core(const float *img, const short dim[3], const int lo[3], const int hi[3], int originalMC,
+                        float isolevel, int exact, vec3d **vs, vec3i **ts, int *nv, int *nt, long long **prov,
+                        int **cedges, int *ncentre)
 {
 	/* sub-box copy as img-iso, pre-filled with min(img)-iso; the index only
 	 * guards the end of the volume, so an extra column/row wraps (:1086-1115) */
@@ -535,8 +542,12 @@ int orc_mc_lewiner(const float *img, const short dim[3], const int lo[3], const 
 #define D(x, y, z) d[(size_t)(x) + (size_t)SX * ((size_t)(y) + (size_t)SY * (size_t)(z))]
 	/* edge vertices in point raster order, x then y then z edge (:235-270) */
 	int *vid = (int *)malloc(3 * ns * sizeof(int));
-	size_t vcap = 1 << 16, nverts = 0;
+	size_t vcap = 1 << 16, nverts = 0, ncen = 0;
 	float *vp = (float *)malloc(vcap * 3 * sizeof(float));
+	double *vx = (double *)malloc(vcap * 3 * sizeof(double)); /* exact positions */
+	long long *pv = (long long *)malloc(vcap * sizeof(long long));
+	size_t ccap = 1 << 12;
+	int *ce = (int *)malloc(ccap * 12 * sizeof(int));
 	i = 0;
 	for (int z = 0; z < SZ; z++)
 		for (int y = 0; y < SY; y++)
@@ -556,10 +567,20 @@ int orc_mc_lewiner(const float *img, const short dim[3], const int lo[3], const 
 					if (nverts + 1 > vcap) {
 						vcap *= 2;
 						vp = (float *)realloc(vp, vcap * 3 * sizeof(float));
+						vx = (double *)realloc(vx, vcap * 3 * sizeof(double));
+						pv = (long long *)realloc(pv, vcap * sizeof(long long));
 					}
 					vp[3 * nverts + 0] = a == 0 ? (float)x + u : (float)x;
 					vp[3 * nverts + 1] = a == 1 ? (float)y + u : (float)y;
 					vp[3 * nverts + 2] = a == 2 ? (float)z + u : (float)z;
+					{
+						double dd = (double)c0 - (double)cn[a];
+						double ux = dd != 0.0 ? (double)c0 / dd : 0.5;
+						vx[3 * nverts + 0] = a == 0 ? (double)x + ux : (double)x;
+						vx[3 * nverts + 1] = a == 1 ? (double)y + ux : (double)y;
+						vx[3 * nverts + 2] = a == 2 ? (double)z + ux : (double)z;
+					}
+					pv[nverts] = (long long)i * 4 + a;
 					vid[3 * i + a] = (int)nverts++;
 				}
 			}
@@ -585,29 +606,48 @@ int orc_mc_lewiner(const float *img, const short dim[3], const int lo[3], const 
 				int cid = -1;
 				if (!originalMC && n2m_lw_row_centre[row]) { /* add_c_vertex :1029-1082 */
 					float sx = 0, sy = 0, sz = 0, cnt = 0;
+					double ex = 0, ey = 0, ez = 0;
+					if (ncen + 1 > ccap) {
+						ccap *= 2;
+						ce = (int *)realloc(ce, ccap * 12 * sizeof(int));
+					}
 					for (int e = 0; e < 12; e++) {
 						const int *q = kCodePt[e];
 						size_t pi = (size_t)(x + q[0]) + (size_t)SX * ((size_t)(y + q[1]) + (size_t)SY * (size_t)(z + q[2]));
 						int id = vid[3 * pi + q[3]];
+						ce[12 * ncen + e] = id;
 						if (id == -1)
 							continue;
 						++cnt;
 						sx += vp[3 * id];
 						sy += vp[3 * id + 1];
 						sz += vp[3 * id + 2];
+						ex += vx[3 * id];
+						ey += vx[3 * id + 1];
+						ez += vx[3 * id + 2];
 					}
 					if (cnt > 0) {
 						sx /= cnt;
 						sy /= cnt;
 						sz /= cnt;
+						ex /= cnt;
+						ey /= cnt;
+						ez /= cnt;
 					}
 					if (nverts + 1 > vcap) {
 						vcap *= 2;
 						vp = (float *)realloc(vp, vcap * 3 * sizeof(float));
+						vx = (double *)realloc(vx, vcap * 3 * sizeof(double));
+						pv = (long long *)realloc(pv, vcap * sizeof(long long));
 					}
 					vp[3 * nverts] = sx;
 					vp[3 * nverts + 1] = sy;
 					vp[3 * nverts + 2] = sz;
+					vx[3 * nverts] = ex;
+					vx[3 * nverts + 1] = ey;
+					vx[3 * nverts + 2] = ez;
+					pv[nverts] = -(long long)(1 + ncen);
+					ncen++;
 					cid = (int)nverts++;
 				}
 				int ntri = n2m_lw_row_ntri[row];
@@ -637,15 +677,33 @@ int orc_mc_lewiner(const float *img, const short dim[3], const int lo[3], const 
 	free(vid);
 	if (nverts < 3 || ntr < 1) {
 		free(vp);
+		free(vx);
+		free(pv);
+		free(ce);
 		free(tr);
 		return EXIT_FAILURE;
 	}
 	*vs = (vec3d *)malloc(nverts * sizeof(vec3d));
-	for (size_t k = 0; k < nverts; k++) { /* float + (float)int, :1126-1130 */
-		(*vs)[k].x = vp[3 * k] + (float)lo[0];
-		(*vs)[k].y = vp[3 * k + 1] + (float)lo[1];
-		(*vs)[k].z = vp[3 * k + 2] + (float)lo[2];
+	for (size_t k = 0; k < nverts; k++) {
+		if (exact) {
+			(*vs)[k].x = vx[3 * k] + (double)lo[0];
+			(*vs)[k].y = vx[3 * k + 1] + (double)lo[1];
+			(*vs)[k].z = vx[3 * k + 2] + (double)lo[2];
+		} else { /* float + (float)int, :1126-1130 */
+			(*vs)[k].x = vp[3 * k] + (float)lo[0];
+			(*vs)[k].y = vp[3 * k + 1] + (float)lo[1];
+			(*vs)[k].z = vp[3 * k + 2] + (float)lo[2];
+		}
 	}
+	if (prov) {
+		*prov = pv;
+		*cedges = ce;
+		*ncentre = (int)ncen;
+	} else {
+		free(pv);
+		free(ce);
+	}
+	free(vx);
 	*ts = (vec3i *)malloc(ntr * sizeof(vec3i));
 	for (size_t k = 0; k < ntr; k++) { /* winding v3,v2,v1, :1133-1137 */
 		(*ts)[k].x = tr[3 * k + 2];
@@ -657,6 +715,12 @@ int orc_mc_lewiner(const float *img, const short dim[3], const int lo[3], const 
 	*nv = (int)nverts;
 	*nt = (int)ntr;
 	return EXIT_SUCCESS;
+}
+
+int orc_mc_lewiner(const float *img, const short dim[3], const int lo[3], const int hi[3],
+                   int originalMC, float isolevel, vec3d **vs, vec3i **ts, int *nv, int *nt)
+{
+	return lewiner_core(img, dim, lo, hi, originalMC, isolevel, 0, vs, ts, nv, nt, NULL, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -684,7 +748,9 @@ static int keyidx_cmp(const void *pa, const void *pb)
 	return a->idx < b->idx ? -1 : (a->idx > b->idx);
 }
 
-int orc_unify_vertices(vec3d **inpt, vec3i *tris, int npt, int ntri)
+/* Weld; cls_out (npt entries, may be NULL) receives each vertex's class
+ * (identity when nothing merged). */
+static int weld_core(vec3d **inpt, vec3i *tris, int npt, int ntri, int *cls_out)
 {
 	vec3d *pts = *inpt;
 	/* stable sort by FP32 distance to vertex 0: ties keep index order */
@@ -710,6 +776,9 @@ int orc_unify_vertices(vec3d **inpt, vec3i *tris, int npt, int ntri)
 	}
 	free(s);
 	if (nnew == npt) { /* nothing merged: unchanged, :82-87 */
+		if (cls_out)
+			for (int i = 0; i < npt; i++)
+				cls_out[i] = i;
 		free(cls);
 		return npt;
 	}
@@ -723,8 +792,15 @@ int orc_unify_vertices(vec3d **inpt, vec3i *tris, int npt, int ntri)
 		out[cls[i]] = pts[i];
 	free(pts);
 	*inpt = out;
+	if (cls_out)
+		memcpy(cls_out, cls, (size_t)npt * sizeof(int));
 	free(cls);
 	return nnew;
+}
+
+int orc_unify_vertices(vec3d **inpt, vec3i *tris, int npt, int ntri)
+{
+	return weld_core(inpt, tris, npt, ntri, NULL);
 }
 
 /* remove_degenerate_triangles(): src/meshify.c:113-168 */
@@ -1176,4 +1252,169 @@ void orc_apply_sform(vec3i *t, vec3d *p, int nt, int np, const float srow_x[4],
 		t[i].x = t[i].y;
 		t[i].y = a;
 	}
+}
+
+/* ------------------------------------------------------------------------ */
+/* Differentiable path (forward: Lewiner MC over the whole volume at the    */
+/* isolevel, weld, degenerate removal, lapIter Laplacian iterations) and    */
+/* its adjoint in FP64.  The forward is the reference's functions chained   */
+/* as src/meshify.c:377-385 and src/quadric.c:316-341 chain them; the      */
+/* adjoint is written here from the derivative of each step.               */
+/* ------------------------------------------------------------------------ */
+
+/* Transposed laplacian_smooth() (src/quadric.c:316-341): the forward maps
+ * p_i -> sum_i / num_i (num_i > 0), with sum_i gathering the other two
+ * corners of every triangle slot of i; so g -> A^T g scatters g_i / num_i
+ * to those corners.  A vertex with num = 0 keeps its value. */
+void orc_laplacian_smooth_T(vec3d *g, const vec3i *tris, int nvert, int ntri)
+{
+	vec3d *out = (vec3d *)calloc((size_t)nvert, sizeof(vec3d));
+	int *num = (int *)calloc((size_t)nvert, sizeof(int));
+	for (int i = 0; i < ntri; i++) {
+		num[tris[i].x] += 2;
+		num[tris[i].y] += 2;
+		num[tris[i].z] += 2;
+	}
+	for (int i = 0; i < ntri; i++) {
+		int c[3] = {tris[i].x, tris[i].y, tris[i].z};
+		for (int k = 0; k < 3; k++) {
+			int a = c[k == 0 ? 1 : 0], b = c[k == 2 ? 1 : 2];
+			double w = 1.0 / num[c[k]];
+			out[a].x += g[c[k]].x * w;
+			out[a].y += g[c[k]].y * w;
+			out[a].z += g[c[k]].z * w;
+			out[b].x += g[c[k]].x * w;
+			out[b].y += g[c[k]].y * w;
+			out[b].z += g[c[k]].z * w;
+		}
+	}
+	for (int i = 0; i < nvert; i++)
+		g[i] = num[i] > 0 ? out[i] : g[i];
+	free(out);
+	free(num);
+}
+
+typedef struct {
+	vec3d *p;
+	vec3i *t;
+	int np, nt;
+	int nmc;         /* marching-cubes vertices before the weld */
+	int *cls;        /* class of each marching-cubes vertex */
+	long long *prov; /* provenance of each marching-cubes vertex */
+	int *cedges;
+	int ncentre;
+} diff_state;
+
+static void diff_free(diff_state *d)
+{
+	free(d->p);
+	free(d->t);
+	free(d->cls);
+	free(d->prov);
+	free(d->cedges);
+}
+
+static int diff_run(const float *img, const short dim[3], float isolevel, int originalMC, int lapIter, int exact,
+                    diff_state *d)
+{
+	memset(d, 0, sizeof *d);
+	const int lo[3] = {0, 0, 0}, hi[3] = {dim[0] - 1, dim[1] - 1, dim[2] - 1};
+	if (lewiner_core(img, dim, lo, hi, originalMC, isolevel, exact, &d->p, &d->t, &d->nmc, &d->nt, &d->prov,
+	                 &d->cedges, &d->ncentre) != EXIT_SUCCESS)
+		return EXIT_FAILURE;
+	d->cls = (int *)malloc((size_t)d->nmc * sizeof(int));
+	d->np = weld_core(&d->p, d->t, d->nmc, d->nt, d->cls);
+	if (d->np < 3) {
+		diff_free(d);
+		return EXIT_FAILURE;
+	}
+	d->nt = orc_remove_degenerate_triangles(d->p, &d->t, d->nt);
+	for (int k = 0; k < lapIter; k++)
+		orc_laplacian_smooth(d->p, d->t, d->np, d->nt);
+	return EXIT_SUCCESS;
+}
+
+int orc_diff_forward(const float *img, const short dim[3], float isolevel, int originalMC, int lapIter, int exact,
+                     vec3i **t, vec3d **p, int *nt, int *np)
+{
+	diff_state d;
+	if (diff_run(img, dim, isolevel, originalMC, lapIter, exact, &d) != EXIT_SUCCESS)
+		return EXIT_FAILURE;
+	*t = d.t;
+	*p = d.p;
+	*nt = d.nt;
+	*np = d.np;
+	d.t = NULL;
+	d.p = NULL;
+	diff_free(&d);
+	return EXIT_SUCCESS;
+}
+
+static float diff_clamp(float v) { return fabs(v) < FLT_EPSILON ? FLT_EPSILON : v; }
+
+/* dL/dvolume (nx*ny*nz doubles, x fastest) from dL/dp of the forward's
+ * final vertices (np x 3 doubles). */
+int orc_diff_gradient(const float *img, const short dim[3], float isolevel, int originalMC, int lapIter,
+                      const double *dLdp, double *dLdvol)
+{
+	diff_state d;
+	if (diff_run(img, dim, isolevel, originalMC, lapIter, 1, &d) != EXIT_SUCCESS)
+		return EXIT_FAILURE;
+	const size_t nx = (size_t)dim[0], nxy = nx * (size_t)dim[1], nvox = nxy * (size_t)dim[2];
+	vec3d *g = (vec3d *)malloc((size_t)d.np * sizeof(vec3d));
+	memcpy(g, dLdp, (size_t)d.np * sizeof(vec3d));
+	for (int k = 0; k < lapIter; k++)
+		orc_laplacian_smooth_T(g, d.t, d.np, d.nt);
+	/* weld: the class position is its last member's (src/meshify.c:99-100) */
+	int *rep = (int *)malloc((size_t)d.np * sizeof(int));
+	for (int c = 0; c < d.np; c++)
+		rep[c] = -1;
+	for (int m = 0; m < d.nmc; m++)
+		rep[d.cls[m]] = m;
+	vec3d *gm = (vec3d *)calloc((size_t)d.nmc, sizeof(vec3d));
+	for (int m = 0; m < d.nmc; m++)
+		if (rep[d.cls[m]] == m)
+			gm[m] = g[d.cls[m]];
+	/* centre vertex = mean of its cube's edge vertices */
+	for (int m = 0; m < d.nmc; m++) {
+		if (d.prov[m] >= 0)
+			continue;
+		const int *ce = &d.cedges[12 * (size_t)(-d.prov[m] - 1)];
+		int cnt = 0;
+		for (int e = 0; e < 12; e++)
+			cnt += ce[e] >= 0;
+		if (!cnt)
+			continue;
+		for (int e = 0; e < 12; e++) {
+			if (ce[e] < 0)
+				continue;
+			gm[ce[e]].x += gm[m].x / cnt;
+			gm[ce[e]].y += gm[m].y / cnt;
+			gm[ce[e]].z += gm[m].z / cnt;
+		}
+	}
+	/* edge vertex x + u, u = c0 / (c0 - c1), c = img - iso (clamped) */
+	memset(dLdvol, 0, nvox * sizeof(double));
+	for (int m = 0; m < d.nmc; m++) {
+		if (d.prov[m] < 0)
+			continue;
+		const size_t ia = (size_t)(d.prov[m] >> 2); /* the sub-box is the volume */
+		const int a = (int)(d.prov[m] & 3);
+		const size_t ib = ia + (a == 0 ? 1 : (a == 1 ? nx : nxy));
+		const float r0 = img[ia] - isolevel, r1 = img[ib] - isolevel;
+		const float c0 = diff_clamp(r0), c1 = diff_clamp(r1);
+		const double dd = (double)c0 - (double)c1;
+		if (dd == 0.0 || (float)(c0 - c1) == 0.0f)
+			continue;
+		const double gu = a == 0 ? gm[m].x : (a == 1 ? gm[m].y : gm[m].z);
+		if (c0 == r0) /* not clamped */
+			dLdvol[ia] += gu * (-(double)c1 / (dd * dd));
+		if (c1 == r1)
+			dLdvol[ib] += gu * ((double)c0 / (dd * dd));
+	}
+	free(g);
+	free(rep);
+	free(gm);
+	diff_free(&d);
+	return EXIT_SUCCESS;
 }

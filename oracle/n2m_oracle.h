@@ -37,6 +37,12 @@ int orc_otsu(const int *H, int nBin, int mode, int *dark, int *mid, int *bright)
 void orc_apply_sform(vec3i *t, vec3d *p, int nt, int np, const float srow_x[4],
                      const float srow_y[4], const float srow_z[4]);
 /* counts of 6/18/26-connected components and their voxel totals (tests) */
+/* differentiable path: forward (exact != 0: FP64 positions) and adjoint */
+void orc_laplacian_smooth_T(vec3d *g, const vec3i *tris, int nvert, int ntri);
+int orc_diff_forward(const float *img, const short dim[3], float isolevel, int originalMC, int lapIter, int exact,
+                     vec3i **t, vec3d **p, int *nt, int *np);
+int orc_diff_gradient(const float *img, const short dim[3], float isolevel, int originalMC, int lapIter,
+                      const double *dLdp, double *dLdvol);
 int orc_label_components(const uint8_t *bw, const size_t dim[3], int conn, uint32_t *labels);
 
 #endif

@@ -38,6 +38,17 @@ class Oracle(abi.MeshLib):
                                       np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
                                       f32, f32, f32]
         h.orc_apply_sform.restype = None
+        h.orc_laplacian_smooth_T.argtypes = [np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'),
+                                             np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int]
+        h.orc_laplacian_smooth_T.restype = None
+        h.orc_diff_forward.argtypes = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'),
+                                       C.POINTER(C.c_short), C.c_float, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]
+        h.orc_diff_gradient.argtypes = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'),
+                                        C.POINTER(C.c_short), C.c_float, C.c_int, C.c_int,
+                                        np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'),
+                                        np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS')]
         self.mc_classic = _bind_mc(h, 'orc_mc_classic', classic=True)
         self.mc_lewiner = _bind_mc(h, 'orc_mc_lewiner', classic=False)
         self.unify = abi.bind_unify(h, 'orc_unify_vertices')
@@ -71,6 +82,34 @@ class Oracle(abi.MeshLib):
     def apply_sform(self, tris, pts, sx, sy, sz):
         f = C.c_float * 4
         self.lib.orc_apply_sform(tris, pts, len(tris), len(pts), f(*sx), f(*sy), f(*sz))
+
+    def laplacian_smooth_T(self, g, tris, iters=1):
+        for _ in range(iters):
+            self.lib.orc_laplacian_smooth_T(g, np.ascontiguousarray(tris, np.int32), len(g), len(tris))
+        return g
+
+    def diff_forward(self, img, iso, lap_iter=20, originalMC=0, exact=False):
+        """Lewiner over the whole volume + weld + degenerate + Laplacian."""
+        nz, ny, nx = img.shape
+        t, p = C.c_void_p(), C.c_void_p()
+        nt, npt = C.c_int(0), C.c_int(0)
+        rc = self.lib.orc_diff_forward(np.ascontiguousarray(img, np.float32), (C.c_short * 3)(nx, ny, nz),
+                                       float(iso), int(originalMC), int(lap_iter), int(exact), C.byref(t),
+                                       C.byref(p), C.byref(nt), C.byref(npt))
+        if rc != 0:
+            return rc, np.zeros((0, 3), np.int32), np.zeros((0, 3), np.float64)
+        return rc, abi._take(t.value, nt.value, np.int32), abi._take(p.value, npt.value, np.float64)
+
+    def diff_gradient(self, img, iso, dLdp, lap_iter=20, originalMC=0):
+        """FP64 dL/dvolume (shape of img) from dL/dp of diff_forward's vertices."""
+        nz, ny, nx = img.shape
+        out = np.zeros(img.shape, np.float64)
+        rc = self.lib.orc_diff_gradient(np.ascontiguousarray(img, np.float32), (C.c_short * 3)(nx, ny, nz),
+                                        float(iso), int(originalMC), int(lap_iter),
+                                        np.ascontiguousarray(dLdp, np.float64), out)
+        if rc != 0:
+            raise RuntimeError('orc_diff_gradient failed')
+        return out
 
 
 def _bind_mc(h, name, classic):
@@ -170,6 +209,36 @@ def blobs(shape, seed, nblobs=6, noise=0.0, dtype=np.float32):
     if noise:
         v += rng.normal(0, noise, shape)
     return np.ascontiguousarray(v.astype(dtype))
+
+
+def sphere_loss(p, centre, r0):
+    """Mean squared radial distance of the vertices to a sphere and its
+    gradient with respect to the vertices."""
+    d = p - np.asarray(centre, np.float64)
+    r = np.sqrt((d * d).sum(1))
+    e = r - r0
+    L = float((e * e).mean())
+    g = (2.0 / len(p)) * (e / r)[:, None] * d
+    return L, g
+
+
+def sphere_field(n, radius, seed, nharm=4, amp=1.5, noise=0.0):
+    """Small perturbed-sphere signed field on an n^3 grid (float32)."""
+    rng = np.random.default_rng(seed)
+    c = (n - 1) / 2.0
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64),) * 3, indexing='ij')
+    dx, dy, dz = x - c, y - c, z - c
+    r = np.sqrt(dx * dx + dy * dy + dz * dz) + 1e-9
+    pert = np.zeros_like(r)
+    for _ in range(nharm):
+        k = rng.normal(size=3)
+        k /= np.linalg.norm(k)
+        pert += amp * np.cos(rng.integers(1, 4) * np.arccos(np.clip((dx * k[0] + dy * k[1] + dz * k[2]) / r, -1, 1))
+                             + rng.uniform(0, 2 * np.pi))
+    f = r - (radius + pert / nharm)
+    if noise:
+        f += rng.normal(0, noise, f.shape)
+    return f.astype(np.float32)
 
 
 def random_mask(shape, seed, p=0.5):

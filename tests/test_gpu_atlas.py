@@ -64,6 +64,7 @@ def run_gpu(gpu, vol, cubes, originalMC, preSmooth, fillBubbles, post, srow, loo
     prm = ctx.params(0.5, cubes=cubes, originalMC=originalMC, preSmooth=preSmooth, onlyLargest=False,
                      fillBubbles=fillBubbles)
     nz, ny, nx = vol.shape
+    torch.cuda.synchronize()  # the library runs on its own stream
     rc = ctx.atlas(d.data_ptr(), (nx, ny, nz), prm, post, srow, loop=loop)
     res = []
     for k, e in enumerate(ctx.atlas_entries()):
@@ -142,8 +143,10 @@ def test_atlas_edge_cases(gpu, oracle):
     import torch
     ctx = gpu.DeviceContext()
     z = torch.zeros((8, 8, 8), dtype=torch.float32, device='cuda')
+    torch.cuda.synchronize()
     assert ctx.atlas(z.data_ptr(), (8, 8, 8), ctx.params(0.5, onlyLargest=False)) != 0
     assert 'label' in gpu.last_error()
+    ctx.close()
     # labels on a 1-voxel-thin volume, and a tiny label that takes the
     # isolevel fallback
     vol = np.zeros((1, 9, 11), np.float32)
