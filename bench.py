@@ -95,8 +95,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--no-atlas', action='store_true', help='skip the batched-atlas leg')
     ap.add_argument('--calib-copy', action='store_true',
-                    help='also run one device copy of the 512 MiB volume (a known-size kernel that '
-                         'calibrates FETCH_SIZE / WRITE_SIZE counters)')
+                    help='also copy the 512 MiB volume once with a blit and once with an elementwise kernel '
+                         '(known-size kernels that calibrate the FETCH_SIZE / WRITE_SIZE counters)')
+    ap.add_argument('--atlas-only', action='store_true', help='run only the batched-atlas leg')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build measures one GPU; --gpus %d is not supported (the path would '
@@ -107,6 +108,9 @@ def main():
     import nii2mesh_amd as n2m
     from nii2mesh_amd import synth
 
+    if args.atlas_only:
+        print(json.dumps({'atlas': atlas_leg(torch, n2m, synth)}), flush=True)
+        return
     t = time.perf_counter()
     vol = synth.large_f32()
     log('generated 512^3 volume in %.1f s' % (time.perf_counter() - t))
@@ -115,6 +119,7 @@ def main():
     if args.calib_copy:
         e = torch.empty_like(d)
         e.copy_(d)
+        torch.mul(d, 1.0, out=e)
         torch.cuda.synchronize()
         del e
     nvox = vol.size
@@ -135,11 +140,28 @@ def main():
             raise RuntimeError('pipeline failed: ' + n2m.last_error())
         return isos
 
+    # warmup; the last warmup step times every stage (the per-kernel table)
+    stats = {}
     for i in range(args.warmup):
+        last = i == args.warmup - 1
+        if last:
+            torch.cuda.synchronize()
+            ctx.timing(True)
+            ctx.reset_timing()
         isos = step()
+        torch.cuda.synchronize()
+        if last:
+            stats = ctx.kernel_stats()
         log('warmup %d done, isolevels %s' % (i, isos))
-    torch.cuda.synchronize()
+    tot_ms = sum(v[0] for v in stats.values())
+    tot_bytes = sum(v[2] for v in stats.values())
+    for k, (ms, n, b) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
+        log('  %-22s %9.3f ms/step %6d launches  %8.1f GB/s algorithmic' %
+            (k, ms, n, (b / (ms * 1e-3) / 1e9) if ms > 0 else 0))
+    dom = max(stats.items(), key=lambda kv: kv[1][0])[0] if stats else 'hc_a'
+    # timed region: only the dominant kernel carries HIP events
     ctx.timing(True)
+    ctx.timing_only(dom)
     ctx.reset_timing()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -148,17 +170,12 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ctx.timing(False)
-    stats = ctx.kernel_stats()
+    ctx.timing_only(None)
+    dms, dn, dbytes = ctx.kernel_stats()[dom]
     ms_step = (t1 - t0) * 1000.0 / args.steps
     sizes = [ctx.mesh_size(k) for k in range(3)]
     ntri = sum(s[0] for s in sizes)
     log('step %.2f ms; meshes (tris, verts): %s' % (ms_step, sizes))
-    tot_ms = sum(v[0] for v in stats.values())
-    tot_bytes = sum(v[2] for v in stats.values())
-    for k, (ms, n, b) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
-        log('  %-22s %9.3f ms/step %6d launches  %8.1f GB/s algorithmic' %
-            (k, ms / args.steps, n, (b / (ms * 1e-3) / 1e9) if ms > 0 else 0))
-    dom, (dms, dn, dbytes) = max(stats.items(), key=lambda kv: kv[1][0])
     achieved = (dbytes / dn) / ((dms / dn) * 1e-3) / 1e9 if dn and dms else 0.0
     roof = {'kernel': dom, 'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
             'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
@@ -181,8 +198,8 @@ def main():
                                'bubble fill, 100 HC iterations each', 'shape': [N, N, N],
                    'isolevels': [round(v, 4) for v in isos], 'triangles_per_step': ntri},
         'triangles_per_s': round(ntri / (ms_step * 1e-3), 1),
-        'pipeline_hbm_frac': round((tot_bytes / args.steps) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        'kernel_ms_per_step': round(tot_ms / args.steps, 3),
+        'pipeline_hbm_frac': round(tot_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if stats else None,
+        'kernel_ms_per_step': round(tot_ms, 3) if stats else None,
         'roofline': roof,
     }
     if not args.no_atlas:
@@ -219,6 +236,12 @@ def atlas_leg(torch, n2m, synth, reps=5):
     nok = sum(e.status == n2m.ATLAS_OK for e in ent)
     loop = min(run(True) for _ in range(2))
     log('atlas: batched %.2f ms, per-label loop %.2f ms (%d labels, %d triangles)' % (batched, loop, nok, ntri))
+    ctx.timing(True)
+    ctx.reset_timing()
+    run(False)
+    ctx.timing(False)
+    for k, (ms, n, b) in sorted(ctx.kernel_stats().items(), key=lambda kv: -kv[1][0]):
+        log('  atlas %-28s %8.3f ms %5d launches' % (k, ms, n))
     ctx.close()
     return {'workload': '256^3 Voronoi atlas, 370 labels, binarise 0.5 + pre-smooth + classic marching cubes',
             'labels_meshed': nok, 'triangles': ntri, 'batched_ms': round(batched, 3),

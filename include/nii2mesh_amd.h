@@ -239,6 +239,8 @@ void n2m_laplacian_smooth_T(vec3d *g, vec3i *tris, int nvert, int ntri, int iter
 
 /* Per-kernel timing with HIP events on the context stream. */
 void n2m_ctx_enable_timing(n2m_ctx *ctx, int on);
+/* Time only the stage called `name` (NULL: all stages). */
+void n2m_ctx_timing_only(n2m_ctx *ctx, const char *name);
 void n2m_ctx_reset_timing(n2m_ctx *ctx);
 /* Returns the number of timed kernels; fills entry idx if idx < count. */
 int n2m_ctx_kernel_stat(n2m_ctx *ctx, int idx, const char **name, double *ms_total, int *launches,

@@ -78,6 +78,7 @@ def _bind_ext(L):
     L.n2m_ctx_download.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     L.n2m_ctx_enable_timing.argtypes = [C.c_void_p, C.c_int]
     L.n2m_ctx_reset_timing.argtypes = [C.c_void_p]
+    L.n2m_ctx_timing_only.argtypes = [C.c_void_p, C.c_char_p]
     L.n2m_ctx_kernel_stat.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                                       C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.n2m_quick_smooth.argtypes = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
@@ -229,6 +230,9 @@ class DeviceContext:
 
     def timing(self, on=True):
         self.L.n2m_ctx_enable_timing(self.h, int(on))
+
+    def timing_only(self, name=None):
+        self.L.n2m_ctx_timing_only(self.h, None if name is None else name.encode())
 
     def reset_timing(self):
         self.L.n2m_ctx_reset_timing(self.h)

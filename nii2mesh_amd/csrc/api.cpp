@@ -200,6 +200,7 @@ struct n2m_ctx {
 	uint32_t *d_small = nullptr;
 	uint32_t *h_small = nullptr;
 	bool timing = false;
+	std::string timing_only; // when set, only this stage is timed
 	std::vector<Stat> stats;
 	std::vector<Pending> pending;
 	std::vector<hipEvent_t> evpool;
@@ -264,7 +265,8 @@ struct n2m_ctx {
 	void tbegin(const char *name, double bytes = 0)
 	{
 		check();
-		if (!timing)
+		cur_stat = -1;
+		if (!timing || (!timing_only.empty() && timing_only != name))
 			return;
 		int k = -1;
 		for (size_t i = 0; i < stats.size(); i++)
@@ -1019,6 +1021,51 @@ void sform_and_smooth(n2m_ctx *C, Mesh &m, int postSmooth, const float *srow)
 		smooth_hc_device(C, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, 0.1, 0.5, postSmooth, true);
 }
 
+// Crop tile of the atlas smoothing: N2M_ATLAS_TILE forces one; otherwise the
+// first smoothed atlas run on a device times every shape on its own crops
+// and keeps the fastest for the process.
+int g_atlas_tile[64];
+bool g_atlas_tile_init = false;
+
+template <typename Run> int atlas_tile_for(n2m_ctx *C, Run run)
+{
+	const char *e = getenv("N2M_ATLAS_TILE");
+	if (e && *e) {
+		int t = atoi(e);
+		if (t < 0 || t >= atlas_tile_count())
+			fail("N2M_ATLAS_TILE=%s: expected 0..%d", e, atlas_tile_count() - 1);
+		return t;
+	}
+	if (!g_atlas_tile_init) {
+		for (int &t : g_atlas_tile)
+			t = -1;
+		g_atlas_tile_init = true;
+	}
+	const int dev = C->device >= 0 && C->device < 64 ? C->device : 0;
+	if (g_atlas_tile[dev] >= 0)
+		return g_atlas_tile[dev];
+	hipEvent_t a = C->ev(), b = C->ev();
+	float best = 0.f;
+	int pick = 0;
+	for (int t = 0; t < atlas_tile_count(); t++) {
+		run(t); // warm the shape's code object
+		HIPX(hipEventRecord(a, C->stream));
+		run(t);
+		HIPX(hipEventRecord(b, C->stream));
+		HIPX(hipEventSynchronize(b));
+		float ms = 0.f;
+		HIPX(hipEventElapsedTime(&ms, a, b));
+		if (t == 0 || ms < best) {
+			best = ms;
+			pick = t;
+		}
+	}
+	C->evpool.push_back(a);
+	C->evpool.push_back(b);
+	g_atlas_tile[dev] = pick;
+	return pick;
+}
+
 // Tasks (labels) whose crops share one pass.  entry[k] = index of task k's
 // label in C->atlas.  Meshes of the tasks are appended to U in task order.
 void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_params &P, std::vector<AtlasTask> T,
@@ -1051,18 +1098,21 @@ void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_pa
 	auto dmap = [&] { return C->buf<uint32_t>(S_AT_BMAP, bmap.size()); };
 
 	// 1. binarise + pre-smooth every crop, per-task range (src/meshify.c:299-311)
-	unsigned nb = plan([&](const AtlasTask &t) { return atlas_smooth_blocks(t.nx, t.ny, t.nz, smooth); });
 	uint32_t *red = C->buf<uint32_t>(S_AT_RED, (size_t)9 * T.size());
-	{
-		std::vector<uint32_t> init(2 * T.size());
-		for (size_t k = 0; k < T.size(); k++) {
-			init[2 * k] = 0xffffffffu;
-			init[2 * k + 1] = 0u;
-		}
-		upload(C, red, init);
+	std::vector<uint32_t> rinit(2 * T.size());
+	for (size_t k = 0; k < T.size(); k++) {
+		rinit[2 * k] = 0xffffffffu;
+		rinit[2 * k + 1] = 0u;
 	}
+	auto smooth_with = [&](int tile) {
+		unsigned nb = plan([&](const AtlasTask &t) { return atlas_smooth_blocks(t.nx, t.ny, t.nz, smooth, tile); });
+		upload(C, red, rinit);
+		launch_atlas_smooth(labels, NX, NY, NZ, dT(), dmap(), nb, smooth ? 1 : 0, tile, crops, red, s);
+		C->check();
+	};
+	const int tile = smooth ? atlas_tile_for(C, smooth_with) : 0;
 	C->tbegin("atlas_smooth", 8.0 * nfl);
-	launch_atlas_smooth(labels, NX, NY, NZ, dT(), dmap(), nb, smooth ? 1 : 0, crops, red, s);
+	smooth_with(tile);
 	C->tend();
 	{
 		std::vector<uint32_t> rr = download(C, red, 2 * T.size());
@@ -1098,7 +1148,7 @@ void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_pa
 	const size_t nt_ = T.size();
 
 	// 2. edge darkening, box of >= iso, minimum (src/meshify.c:347-371)
-	nb = plan([&](const AtlasTask &t) { return atlas_apply_blocks(t.nx, t.ny, t.nz); });
+	unsigned nb = plan([&](const AtlasTask &t) { return atlas_apply_blocks(t.nx, t.ny, t.nz); });
 	{
 		std::vector<uint32_t> init(7 * nt_);
 		for (size_t k = 0; k < nt_; k++) {
@@ -1920,6 +1970,8 @@ void n2m_laplacian_smooth_T(vec3d *g, vec3i *tris, int nvert, int ntri, int iter
 }
 
 void n2m_ctx_enable_timing(n2m_ctx *C, int on) { C->timing = on != 0; }
+
+void n2m_ctx_timing_only(n2m_ctx *C, const char *name) { C->timing_only = name ? name : ""; }
 
 void n2m_ctx_reset_timing(n2m_ctx *C)
 {

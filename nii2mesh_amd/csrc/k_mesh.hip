@@ -493,9 +493,14 @@ struct d3 {
 // nearly all of its neighbours among the B + 2D vertices around it: that
 // window is staged in LDS with coalesced loads, the rest come from global
 // memory.  (B, D) is chosen at run time from a few compiled shapes.
+// LDS pointer type: keeps ds_read instructions for window hits (a generic
+// pointer would let the compiler merge the LDS and global paths into one
+// flat load).
+typedef __attribute__((address_space(3))) double lds_double;
+
 template <int B, int D> struct Win {
 	static constexpr int W = B + 2 * D;
-	const double *__restrict__ lds; // W x 3 doubles
+	const lds_double *lds; // W x 3 doubles
 	const d3 *__restrict__ g;
 	int w0;
 	__device__ __forceinline__ d3 get(int i) const
@@ -512,11 +517,12 @@ template <int B, int D> struct Win {
 	}
 };
 
-// stage p[w0, w0+W) (clamped to [0, nv)) into LDS; has a barrier
+// stage p[w0, w0+W) (clamped to [0, nv)) into LDS for vertex block blk;
+// has a barrier
 template <int B, int D>
-__device__ __forceinline__ Win<B, D> stage_window(double *lds, const d3 *__restrict__ p, uint32_t nv)
+__device__ __forceinline__ Win<B, D> stage_window(double *lds, const d3 *__restrict__ p, uint32_t nv, uint32_t blk)
 {
-	const int w0 = (int)(blockIdx.x * B) - D;
+	const int w0 = (int)(blk * B) - D;
 	const double *src = reinterpret_cast<const double *>(p);
 	const long long lo = 3ll * w0, hi = 3ll * nv;
 	for (int i = threadIdx.x; i < 3 * Win<B, D>::W; i += B) {
@@ -524,27 +530,48 @@ __device__ __forceinline__ Win<B, D> stage_window(double *lds, const d3 *__restr
 		lds[i] = (e >= 0 && e < hi) ? src[e] : 0.0;
 	}
 	__syncthreads();
-	return Win<B, D>{lds, p, w0};
+	return Win<B, D>{(const lds_double *)lds, p, w0};
 }
 
 // Laplacian term of one vertex, src/quadric.c:321-338: the sum over its
 // references (ascending triangle order) of (p[a] + p[b]), divided by twice
-// the number of references; unchanged when it has none.
-template <class WIN>
+// the number of references; unchanged when it has none.  The first 8 pairs
+// are loaded before any is summed, so their global loads overlap; the sum
+// keeps the reference's order.
+// Laplacian term of one vertex, src/quadric.c:321-338: the sum over its
+// references (ascending triangle order) of (p[a] + p[b]), divided by twice
+// the number of references; unchanged when it has none.  The first K pairs
+// are loaded before any is summed, so their global loads overlap; the sum
+// keeps the reference's order.
+template <int K, class WIN>
 __device__ __forceinline__ d3 laplace_at(uint32_t b, uint32_t e, const int2 *__restrict__ pairs, const WIN &w,
                                          const d3 self)
 {
 	if (e <= b)
 		return self;
 	double sx = 0.0, sy = 0.0, sz = 0.0;
-	for (uint32_t i = b; i < e; i++) {
-		int2 q = pairs[i];
-		d3 A = w.get(q.x), B = w.get(q.y);
-		sx = sx + (A.x + B.x);
-		sy = sy + (A.y + B.y);
-		sz = sz + (A.z + B.z);
+	const uint32_t n = e - b;
+	int2 q[K > 0 ? K : 1];
+#pragma unroll
+	for (int k = 0; k < K; k++)
+		q[k] = (uint32_t)k < n ? pairs[b + k] : make_int2(0, 0);
+#pragma unroll
+	for (int k = 0; k < K; k++) {
+		if ((uint32_t)k < n) {
+			d3 P = w.get(q[k].x), Q = w.get(q[k].y);
+			sx = sx + (P.x + Q.x);
+			sy = sy + (P.y + Q.y);
+			sz = sz + (P.z + Q.z);
+		}
 	}
-	const double num = (double)(2 * (e - b));
+	for (uint32_t i = b + K; i < e; i++) {
+		int2 qq = pairs[i];
+		d3 P = w.get(qq.x), Q = w.get(qq.y);
+		sx = sx + (P.x + Q.x);
+		sy = sy + (P.y + Q.y);
+		sz = sz + (P.z + Q.z);
+	}
+	const double num = (double)(2 * n);
 	d3 L;
 	L.x = sx / num;
 	L.y = sy / num;
@@ -553,18 +580,19 @@ __device__ __forceinline__ d3 laplace_at(uint32_t b, uint32_t e, const int2 *__r
 }
 
 // p' = L(p); b = p' - (o*alpha + p*alpha1)
-template <int B, int D>
+template <int B, int D, int K>
 __global__ void __launch_bounds__(B)
 k_hc_a(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const d3 *__restrict__ o,
        const d3 *__restrict__ p, d3 *__restrict__ pn, d3 *__restrict__ bq, uint32_t nv, double alpha, double alpha1)
 {
 	__shared__ double lds[3 * Win<B, D>::W];
-	const Win<B, D> w = stage_window<B, D>(lds, p, nv);
-	const uint32_t v = blockIdx.x * B + threadIdx.x;
+	const uint32_t blk = blockIdx.x;
+	const Win<B, D> w = stage_window<B, D>(lds, p, nv, blk);
+	const uint32_t v = blk * B + threadIdx.x;
 	if (v >= nv)
 		return;
 	const d3 pv = w.get(v), ov = o[v];
-	const d3 L = laplace_at(start[v], start[v + 1], pairs, w, pv);
+	const d3 L = laplace_at<K>(start[v], start[v + 1], pairs, w, pv);
 	d3 bb;
 	bb.x = L.x - (ov.x * alpha + pv.x * alpha1);
 	bb.y = L.y - (ov.y * alpha + pv.y * alpha1);
@@ -574,18 +602,19 @@ k_hc_a(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const
 }
 
 // p = p' - (b*beta + L(b)*beta1)
-template <int B, int D>
+template <int B, int D, int K>
 __global__ void __launch_bounds__(B)
 k_hc_b(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const d3 *__restrict__ pn,
        const d3 *__restrict__ bq, d3 *__restrict__ pout, uint32_t nv, double beta, double beta1)
 {
 	__shared__ double lds[3 * Win<B, D>::W];
-	const Win<B, D> w = stage_window<B, D>(lds, bq, nv);
-	const uint32_t v = blockIdx.x * B + threadIdx.x;
+	const uint32_t blk = blockIdx.x;
+	const Win<B, D> w = stage_window<B, D>(lds, bq, nv, blk);
+	const uint32_t v = blk * B + threadIdx.x;
 	if (v >= nv)
 		return;
 	const d3 bv = w.get(v), pv = pn[v];
-	const d3 L = laplace_at(start[v], start[v + 1], pairs, w, bv);
+	const d3 L = laplace_at<K>(start[v], start[v + 1], pairs, w, bv);
 	d3 r;
 	r.x = pv.x - (bv.x * beta + L.x * beta1);
 	r.y = pv.y - (bv.y * beta + L.y * beta1);
@@ -594,17 +623,18 @@ k_hc_b(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const
 }
 
 // plain Laplacian step (src/quadric.c:316-341): pout = L(p)
-template <int B, int D>
+template <int B, int D, int K>
 __global__ void __launch_bounds__(B)
 k_laplace(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, const d3 *__restrict__ p,
           d3 *__restrict__ pout, uint32_t nv)
 {
 	__shared__ double lds[3 * Win<B, D>::W];
-	const Win<B, D> w = stage_window<B, D>(lds, p, nv);
-	const uint32_t v = blockIdx.x * B + threadIdx.x;
+	const uint32_t blk = blockIdx.x;
+	const Win<B, D> w = stage_window<B, D>(lds, p, nv, blk);
+	const uint32_t v = blk * B + threadIdx.x;
 	if (v >= nv)
 		return;
-	pout[v] = laplace_at(start[v], start[v + 1], pairs, w, w.get(v));
+	pout[v] = laplace_at<K>(start[v], start[v + 1], pairs, w, w.get(v));
 }
 
 // gather through a permutation: dst[i] = src[perm ? perm[i] : i]
@@ -644,8 +674,10 @@ void launch_border(const int *tris, uint32_t nv, const uint32_t *start, const ui
 {
 	hipLaunchKernelGGL(k_border, dim3(g1(nv)), dim3(256), 0, s, tris, nv, start, ref, border);
 }
-// window shapes (B vertices per workgroup, D on each side); N2M_HC_SHAPE
-// picks one (0..3), default 0
+// Window shapes (B vertices per workgroup, D on each side, K pairs
+// prefetched per vertex); N2M_HC_SHAPE picks one (default 0).
+#define N2M_HC_SHAPES(X) X(0, 1024, 256, 8) X(1, 512, 256, 8) X(2, 1024, 128, 8) X(3, 512, 128, 8)
+
 static int hc_shape()
 {
 	static int v = -1;
@@ -658,15 +690,13 @@ static int hc_shape()
 	return v;
 }
 
-#define N2M_HC_SHAPES(X) X(0, 1024, 256) X(1, 512, 256) X(2, 1024, 512) X(3, 512, 128)
-
 void launch_hc_a(const uint32_t *start, const int2 *pairs, const double *o, const double *p, double *pn,
                  double *bq, uint32_t nv, double alpha, hipStream_t s)
 {
 	switch (hc_shape()) {
-#define X(k, B, D)                                                                                               \
+#define X(k, B, D, K)                                                                                              \
 	case k:                                                                                                      \
-		hipLaunchKernelGGL((k_hc_a<B, D>), dim3((nv + B - 1) / B), dim3(B), 0, s, start, pairs, (const d3 *)o,      \
+		hipLaunchKernelGGL((k_hc_a<B, D, K>), dim3((nv + B - 1) / B), dim3(B), 0, s, start, pairs, (const d3 *)o,   \
 		                   (const d3 *)p, (d3 *)pn, (d3 *)bq, nv, alpha, 1.0 - alpha);                             \
 		break;
 		N2M_HC_SHAPES(X)
@@ -677,9 +707,9 @@ void launch_hc_b(const uint32_t *start, const int2 *pairs, const double *pn, con
                  uint32_t nv, double beta, hipStream_t s)
 {
 	switch (hc_shape()) {
-#define X(k, B, D)                                                                                               \
+#define X(k, B, D, K)                                                                                              \
 	case k:                                                                                                      \
-		hipLaunchKernelGGL((k_hc_b<B, D>), dim3((nv + B - 1) / B), dim3(B), 0, s, start, pairs, (const d3 *)pn,     \
+		hipLaunchKernelGGL((k_hc_b<B, D, K>), dim3((nv + B - 1) / B), dim3(B), 0, s, start, pairs, (const d3 *)pn,  \
 		                   (const d3 *)bq, (d3 *)pout, nv, beta, 1.0 - beta);                                      \
 		break;
 		N2M_HC_SHAPES(X)
@@ -690,10 +720,10 @@ void launch_laplace(const uint32_t *start, const int2 *pairs, const double *p, d
                     hipStream_t s)
 {
 	switch (hc_shape()) {
-#define X(k, B, D)                                                                                               \
+#define X(k, B, D, K)                                                                                              \
 	case k:                                                                                                      \
-		hipLaunchKernelGGL((k_laplace<B, D>), dim3((nv + B - 1) / B), dim3(B), 0, s, start, pairs, (const d3 *)p,   \
-		                   (d3 *)pout, nv);                                                                        \
+		hipLaunchKernelGGL((k_laplace<B, D, K>), dim3((nv + B - 1) / B), dim3(B), 0, s, start, pairs,              \
+		                   (const d3 *)p, (d3 *)pout, nv);                                                         \
 		break;
 		N2M_HC_SHAPES(X)
 #undef X

@@ -36,22 +36,25 @@ __device__ __forceinline__ float tap5(float a, float b, float c, float d, float 
 	return (float)acc;
 }
 
-// Smooth the tile (x0, y0, [z0, z1)) of an nx*ny*nz volume read through
-// `src(x, y, z)` (only called inside the volume) into out; returns the
-// tile's min/max through lmin/lmax (per thread) and writes speculative mask
-// bits (out >= iso) when bits != null.
-template <class Src>
+// Smooth the TX x TY tile (x0, y0, [z0, z1)) of an nx*ny*nz volume read
+// through `src(x, y, z)` (only called inside the volume) into out; returns
+// the tile's min/max through lmin/lmax (per thread) and, for TX = 64 (one
+// wave per tile row), writes speculative mask bits (out >= iso) when
+// bits != null.
+template <int TX, int TY, class Src>
 __device__ __forceinline__ void smooth_tile(const Src &src, float *__restrict__ out, int nx, int ny, int nz,
                                             int x0, int y0, int z0, int z1, float iso, uint32_t *__restrict__ bits,
                                             int W, float &lmin, float &lmax)
 {
-	__shared__ float s_in[SM_TY + 4][SM_TX + 4];
-	__shared__ float s_x[SM_TY + 4][SM_TX];
-	const int tx = threadIdx.x % SM_TX, ty = threadIdx.x / SM_TX; // ty 0..3
+	constexpr int CPT = TX * TY / SM_BS, RPP = SM_BS / TX; // columns per thread, rows per pass
+	static_assert(CPT * SM_BS == TX * TY && RPP * TX == SM_BS, "tile must tile the workgroup");
+	__shared__ float s_in[TY + 4][TX + 4];
+	__shared__ float s_x[TY + 4][TX];
+	const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
 	const size_t nxy = (size_t)nx * ny;
-	float ring[SM_CPT][5];
+	float ring[CPT][5];
 #pragma unroll
-	for (int c = 0; c < SM_CPT; c++)
+	for (int c = 0; c < CPT; c++)
 #pragma unroll
 		for (int k = 0; k < 5; k++)
 			ring[c][k] = 0.f;
@@ -59,8 +62,8 @@ __device__ __forceinline__ void smooth_tile(const Src &src, float *__restrict__ 
 	for (int zz = z0 - 2; zz <= z1 + 1; zz++) {
 		const bool have = zz >= 0 && zz < nz;
 		if (have) {
-			for (int i = threadIdx.x; i < (SM_TY + 4) * (SM_TX + 4); i += SM_BS) {
-				int ly = i / (SM_TX + 4), lx = i % (SM_TX + 4);
+			for (int i = threadIdx.x; i < (TY + 4) * (TX + 4); i += SM_BS) {
+				int ly = i / (TX + 4), lx = i % (TX + 4);
 				int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
 				float v = 0.f;
 				if (gx >= 0 && gx < nx && gy >= 0 && gy < ny)
@@ -68,8 +71,8 @@ __device__ __forceinline__ void smooth_tile(const Src &src, float *__restrict__ 
 				s_in[ly][lx] = v;
 			}
 			__syncthreads();
-			for (int i = threadIdx.x; i < (SM_TY + 4) * SM_TX; i += SM_BS) {
-				int ly = i / SM_TX, lx = i % SM_TX;
+			for (int i = threadIdx.x; i < (TY + 4) * TX; i += SM_BS) {
+				int ly = i / TX, lx = i % TX;
 				int gx = x0 + lx;
 				float v;
 				if (gx >= 2 && gx < nx - 2)
@@ -82,8 +85,8 @@ __device__ __forceinline__ void smooth_tile(const Src &src, float *__restrict__ 
 			__syncthreads();
 		}
 #pragma unroll
-		for (int c = 0; c < SM_CPT; c++) {
-			const int ly = ty + c * (SM_BS / SM_TX);
+		for (int c = 0; c < CPT; c++) {
+			const int ly = ty + c * RPP;
 			const int y = y0 + ly;
 			float v = 0.f;
 			if (have) {
@@ -102,8 +105,8 @@ __device__ __forceinline__ void smooth_tile(const Src &src, float *__restrict__ 
 		const int zo = zz - 2;
 		if (zo >= z0 && zo < z1) {
 #pragma unroll
-			for (int c = 0; c < SM_CPT; c++) {
-				const int y = y0 + ty + c * (SM_BS / SM_TX);
+			for (int c = 0; c < CPT; c++) {
+				const int y = y0 + ty + c * RPP;
 				float o;
 				if (zo >= 2 && zo < nz - 2)
 					o = tap5(ring[c][0], ring[c][1], ring[c][2], ring[c][3], ring[c][4]);
@@ -115,7 +118,7 @@ __device__ __forceinline__ void smooth_tile(const Src &src, float *__restrict__ 
 					lmin = fminf(lmin, o);
 					lmax = fmaxf(lmax, o);
 				}
-				if (bits) { // speculative threshold mask (the common iso)
+				if (TX == 64 && bits) { // speculative threshold mask (the common iso)
 					unsigned long long m = __ballot(valid && o >= iso);
 					if (y < ny) {
 						const int wx = x0 / 32;
@@ -171,7 +174,7 @@ k_smooth(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, 
 	const int z0 = blockIdx.z * SM_ZC, z1 = min(z0 + SM_ZC, nz);
 	float lmin = __builtin_inff(), lmax = -__builtin_inff();
 	ImgSrc src{in, nx, (size_t)nx * ny};
-	smooth_tile(src, out, nx, ny, nz, x0, y0, z0, z1, iso, bits, W, lmin, lmax);
+	smooth_tile<SM_TX, SM_TY>(src, out, nx, ny, nz, x0, y0, z0, z1, iso, bits, W, lmin, lmax);
 	reduce_range(lmin, lmax, range);
 }
 
@@ -191,18 +194,18 @@ __device__ __forceinline__ int atlas_label_of(float v, int nlabel)
 	return (v > lo && v < hi) ? i : 0;
 }
 
-// per label: voxel count and box (stats[7*i] = count, min xyz, max xyz)
+// per label: voxel count and box (stats[7*i] = count, min xyz, max xyz);
+// PRIV: accumulate in LDS (7 * (nlabel + 1) words) and flush once per block
+template <bool PRIV>
 __global__ void __launch_bounds__(256)
 k_atlas_stats(const float *__restrict__ labels, int nx, int ny, int nz, int nlabel, uint32_t *__restrict__ stats)
 {
-	extern __shared__ uint32_t s_st[]; // 7 * (nlabel + 1) when it fits
-	const bool priv = nlabel <= 1023;
-	uint32_t *st = priv ? s_st : stats;
-	if (priv) {
+	extern __shared__ uint32_t s_st[];
+	if (PRIV) {
 		for (int i = threadIdx.x; i <= nlabel; i += blockDim.x) {
-			st[7 * i] = 0;
-			st[7 * i + 1] = st[7 * i + 2] = st[7 * i + 3] = 0xffffffffu;
-			st[7 * i + 4] = st[7 * i + 5] = st[7 * i + 6] = 0u;
+			s_st[7 * i] = 0;
+			s_st[7 * i + 1] = s_st[7 * i + 2] = s_st[7 * i + 3] = 0xffffffffu;
+			s_st[7 * i + 4] = s_st[7 * i + 5] = s_st[7 * i + 6] = 0u;
 		}
 		__syncthreads();
 	}
@@ -212,25 +215,26 @@ k_atlas_stats(const float *__restrict__ labels, int nx, int ny, int nz, int nlab
 		if (!i)
 			continue;
 		uint32_t x = (uint32_t)(v % nx), y = (uint32_t)((v / nx) % ny), z = (uint32_t)(v / ((size_t)nx * ny));
-		atomicAdd(&st[7 * i], 1u);
-		atomicMin(&st[7 * i + 1], x);
-		atomicMin(&st[7 * i + 2], y);
-		atomicMin(&st[7 * i + 3], z);
-		atomicMax(&st[7 * i + 4], x);
-		atomicMax(&st[7 * i + 5], y);
-		atomicMax(&st[7 * i + 6], z);
+		uint32_t *st = PRIV ? s_st + 7 * i : stats + 7 * i;
+		atomicAdd(&st[0], 1u);
+		atomicMin(&st[1], x);
+		atomicMin(&st[2], y);
+		atomicMin(&st[3], z);
+		atomicMax(&st[4], x);
+		atomicMax(&st[5], y);
+		atomicMax(&st[6], z);
 	}
-	if (!priv)
+	if (!PRIV)
 		return;
 	__syncthreads();
 	for (int i = threadIdx.x; i <= nlabel; i += blockDim.x) {
-		if (!st[7 * i])
+		if (!s_st[7 * i])
 			continue;
-		atomicAdd(&stats[7 * i], st[7 * i]);
+		atomicAdd(&stats[7 * i], s_st[7 * i]);
 		for (int k = 1; k <= 3; k++)
-			atomicMin(&stats[7 * i + k], st[7 * i + k]);
+			atomicMin(&stats[7 * i + k], s_st[7 * i + k]);
 		for (int k = 4; k <= 6; k++)
-			atomicMax(&stats[7 * i + k], st[7 * i + k]);
+			atomicMax(&stats[7 * i + k], s_st[7 * i + k]);
 	}
 }
 
@@ -239,8 +243,12 @@ void launch_atlas_stats(const float *labels, int nx, int ny, int nz, int nlabel,
 	size_t n = (size_t)nx * ny * nz;
 	size_t b = (n + 255) / 256;
 	b = b > 2048 ? 2048 : (b < 1 ? 1 : b);
-	size_t lds = nlabel <= 1023 ? (size_t)7 * (nlabel + 1) * 4 : 0;
-	hipLaunchKernelGGL(k_atlas_stats, dim3((unsigned)b), dim3(256), lds, s, labels, nx, ny, nz, nlabel, stats);
+	if (nlabel <= 1023)
+		hipLaunchKernelGGL(k_atlas_stats<true>, dim3((unsigned)b), dim3(256), (size_t)7 * (nlabel + 1) * 4, s, labels,
+		                   nx, ny, nz, nlabel, stats);
+	else
+		hipLaunchKernelGGL(k_atlas_stats<false>, dim3((unsigned)b), dim3(256), 0, s, labels, nx, ny, nz, nlabel,
+		                   stats);
 }
 
 // the binarised label (src/nii2mesh.c:557-563) read inside a task's crop
@@ -257,7 +265,11 @@ struct LabelSrc {
 	}
 };
 
+// Crop tile shapes the atlas autotuner chooses from (x, y, z-chunk).
+#define N2M_ATLAS_TILES(X) X(0, 64, 8, 32) X(1, 32, 8, 16) X(2, 16, 16, 16) X(3, 32, 8, 32)
+
 // smooth (or just binarise) every task's crop; ranges[2*t] = ord min/max
+template <int TX, int TY, int ZC>
 __global__ void __launch_bounds__(SM_BS)
 k_atlas_smooth(const float *__restrict__ labels, int gnx, int gny, const AtlasTask *__restrict__ tasks,
                const uint32_t *__restrict__ blockTask, int smooth, float *__restrict__ crops,
@@ -269,30 +281,61 @@ k_atlas_smooth(const float *__restrict__ labels, int gnx, int gny, const AtlasTa
 	float *out = crops + t.off;
 	float lmin = __builtin_inff(), lmax = -__builtin_inff();
 	if (smooth) {
-		const int tx = (t.nx + SM_TX - 1) / SM_TX, ty = (t.ny + SM_TY - 1) / SM_TY;
+		const int tx = (t.nx + TX - 1) / TX, ty = (t.ny + TY - 1) / TY;
 		const int bx = local % tx, by = (local / tx) % ty, bz = local / (tx * ty);
-		const int z0 = bz * SM_ZC, z1 = min(z0 + SM_ZC, t.nz);
-		smooth_tile(src, out, t.nx, t.ny, t.nz, bx * SM_TX, by * SM_TY, z0, z1, 0.f, nullptr, 0, lmin, lmax);
+		const int z0 = bz * ZC, z1 = min(z0 + ZC, t.nz);
+		smooth_tile<TX, TY>(src, out, t.nx, t.ny, t.nz, bx * TX, by * TY, z0, z1, 0.f, nullptr, 0, lmin, lmax);
 	} else {
 		const size_t n = (size_t)t.nx * t.ny * t.nz;
-		for (size_t v = (size_t)local * SM_BS + threadIdx.x; v < n; v += (size_t)SM_BS * (size_t)(1 << 30)) {
+		const size_t v = (size_t)local * SM_BS + threadIdx.x;
+		if (v < n) {
 			int x = (int)(v % t.nx), y = (int)((v / t.nx) % t.ny), z = (int)(v / ((size_t)t.nx * t.ny));
 			float o = src(x, y, z);
 			out[v] = o;
-			lmin = fminf(lmin, o);
-			lmax = fmaxf(lmax, o);
+			lmin = o;
+			lmax = o;
 		}
 	}
 	reduce_range(lmin, lmax, &ranges[2 * blockTask[blockIdx.x]]);
 }
 
-void launch_atlas_smooth(const float *labels, int gnx, int gny, int gnz, const AtlasTask *tasks,
-                         const uint32_t *blockTask, unsigned nblocks, int smooth, float *crops, uint32_t *ranges,
-                         hipStream_t s)
+int atlas_tile_count()
 {
-	if (nblocks)
-		hipLaunchKernelGGL(k_atlas_smooth, dim3(nblocks), dim3(SM_BS), 0, s, labels, gnx, gny, tasks, blockTask,
-		                   smooth, crops, ranges);
+#define N2M_COUNT_TILE(i, tx, ty, zc) +1
+	return 0 N2M_ATLAS_TILES(N2M_COUNT_TILE);
+#undef N2M_COUNT_TILE
+}
+
+void atlas_tile_shape(int tile, int shape[3])
+{
+#define N2M_TILE_SHAPE(i, tx, ty, zc)                                                                              \
+	if (tile == i) {                                                                                              \
+		shape[0] = tx;                                                                                             \
+		shape[1] = ty;                                                                                             \
+		shape[2] = zc;                                                                                             \
+		return;                                                                                                    \
+	}
+	N2M_ATLAS_TILES(N2M_TILE_SHAPE)
+#undef N2M_TILE_SHAPE
+	shape[0] = SM_TX;
+	shape[1] = SM_TY;
+	shape[2] = SM_ZC;
+}
+
+void launch_atlas_smooth(const float *labels, int gnx, int gny, int gnz, const AtlasTask *tasks,
+                         const uint32_t *blockTask, unsigned nblocks, int smooth, int tile, float *crops,
+                         uint32_t *ranges, hipStream_t s)
+{
+	if (!nblocks)
+		return;
+#define N2M_LAUNCH_TILE(i, tx, ty, zc)                                                                             \
+	if (tile == i) {                                                                                              \
+		hipLaunchKernelGGL((k_atlas_smooth<tx, ty, zc>), dim3(nblocks), dim3(SM_BS), 0, s, labels, gnx, gny, tasks, \
+		                   blockTask, smooth, crops, ranges);                                                      \
+		return;                                                                                                    \
+	}
+	N2M_ATLAS_TILES(N2M_LAUNCH_TILE)
+#undef N2M_LAUNCH_TILE
 }
 
 // darken the voxels on the volume's faces, bounding box of >= iso and the
@@ -690,10 +733,13 @@ void launch_hist_bins(const float *img, size_t n, float mn, float scl, int nbins
 	                   mn, scl, nbins, hist);
 }
 
-unsigned atlas_smooth_blocks(int nx, int ny, int nz, int smooth)
+unsigned atlas_smooth_blocks(int nx, int ny, int nz, int smooth, int tile)
 {
-	if (smooth)
-		return (unsigned)(((nx + SM_TX - 1) / SM_TX) * ((ny + SM_TY - 1) / SM_TY) * ((nz + SM_ZC - 1) / SM_ZC));
+	if (smooth) {
+		int sh[3];
+		atlas_tile_shape(tile, sh);
+		return (unsigned)(((nx + sh[0] - 1) / sh[0]) * ((ny + sh[1] - 1) / sh[1]) * ((nz + sh[2] - 1) / sh[2]));
+	}
 	return (unsigned)(((size_t)nx * ny * nz + SM_BS - 1) / SM_BS);
 }
 unsigned atlas_apply_blocks(int nx, int ny, int nz) { return (unsigned)(((size_t)nx * ny * nz + 1023) / 1024); }

@@ -65,11 +65,13 @@ void launch_hist_bins(const float *img, size_t n, float mn, float scl, int nbins
 // atlas volume stages (k_volume.hip)
 void launch_atlas_stats(const float *labels, int nx, int ny, int nz, int nlabel, uint32_t *stats, hipStream_t s);
 void launch_atlas_smooth(const float *labels, int gnx, int gny, int gnz, const AtlasTask *tasks,
-                         const uint32_t *blockTask, unsigned nblocks, int smooth, float *crops, uint32_t *ranges,
-                         hipStream_t s);
+                         const uint32_t *blockTask, unsigned nblocks, int smooth, int tile, float *crops,
+                         uint32_t *ranges, hipStream_t s);
+int atlas_tile_count();
+void atlas_tile_shape(int tile, int shape[3]);
 void launch_atlas_apply(float *crops, int gnx, int gny, int gnz, const AtlasTask *tasks, const uint32_t *blockTask,
                         unsigned nblocks, int *bbox, uint32_t *vmin, hipStream_t s);
-unsigned atlas_smooth_blocks(int nx, int ny, int nz, int smooth);
+unsigned atlas_smooth_blocks(int nx, int ny, int nz, int smooth, int tile);
 unsigned atlas_apply_blocks(int nx, int ny, int nz);
 void launch_binarize(const float *labels, size_t n, float lo, float hi, float *out, uint32_t *count, hipStream_t s);
 

@@ -184,3 +184,14 @@ def test_atlas_256_batched_vs_loop(gpu, oracle):
         assert rc == 0
         oracle.laplacian_smoothHC(p, t, 0.1, 0.5, 10, True)
         assert np.array_equal(got_b[k][1], t) and np.array_equal(got_b[k][2], p), 'label %d' % i
+
+
+@pytest.mark.parametrize('tile', [0, 1, 2, 3])
+@pytest.mark.parametrize('cubes', [0, 1])
+def test_atlas_every_tile(gpu, oracle, cubes, tile, monkeypatch):
+    """Each crop tile the autotuner may pick gives the oracle's meshes."""
+    monkeypatch.setenv('N2M_ATLAS_TILE', str(tile))
+    vol = small_atlas((33, 35, 31), 11, 8)
+    want = expect(oracle, vol, cubes, 0, True, False, 0, None)
+    rc, got = run_gpu(gpu, vol, cubes, 0, True, False, 0, None, False)
+    compare(got, want)
