@@ -109,6 +109,11 @@ int n2m_remove_degenerate_triangles(vec3d *pts, vec3i **tris, int ntri);
 /* src/quadric.c:316-341 laplacian_smooth(), applied iter times. */
 void n2m_laplacian_smooth(vec3d *verts, vec3i *tris, int nvert, int ntri, int iter);
 
+/* src/isolevel.c:141-243 nii_otsu() (file-static in the reference): returns
+ * the threshold bin and fills dark/mid/bright; modes 1 and 5 (the
+ * three-threshold search) run on the GPU.  nBin must be 256 for them. */
+int n2m_otsu(const int *H, int nBin, int mode, int *dark, int *mid, int *bright);
+
 /* src/meshify.c:1025-1045 apply_sform() on the GPU (FP64). */
 void n2m_apply_sform(vec3i *t, vec3d *p, int nt, int np, float srow_x[4], float srow_y[4],
                      float srow_z[4]);

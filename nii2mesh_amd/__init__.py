@@ -101,6 +101,8 @@ def _bind_ext(L):
                                          np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
                                          C.c_int]
     L.n2m_laplacian_smooth_T.restype = None
+    L.n2m_otsu.argtypes = [np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                           C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.n2m_ctx_atlas_count.argtypes = [C.c_void_p]
     L.n2m_ctx_atlas_entry.argtypes = [C.c_void_p, C.c_int, C.POINTER(AtlasEntry)]
     L.n2m_ctx_atlas_download.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
@@ -120,6 +122,14 @@ def laplacian_smooth_T(g, tris, iters=1):
     L = load()
     L.n2m_laplacian_smooth_T(g, np.ascontiguousarray(tris, np.int32), len(g), len(tris), int(iters))
     return g
+
+
+def otsu(hist, mode):
+    """nii_otsu (src/isolevel.c:141-243): (threshold, dark, mid, bright)."""
+    L = load()
+    d, m, b = C.c_int(), C.c_int(), C.c_int()
+    t = L.n2m_otsu(np.ascontiguousarray(hist, np.int32), len(hist), int(mode), C.byref(d), C.byref(m), C.byref(b))
+    return t, d.value, m.value, b.value
 
 
 def last_error():

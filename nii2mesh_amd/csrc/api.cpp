@@ -140,6 +140,7 @@ enum Slot {
 	S_DF_G,
 	S_DF_H,
 	S_DF_GM,
+	S_OTSU,
 	S_COUNT
 };
 
@@ -849,7 +850,7 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 	const size_t n3 = 3 * (size_t)nv;
 	double *o = C->buf<double>(S_HC_O, n3), *p = C->buf<double>(S_HC_P, n3);
 	double *pn = C->buf<double>(S_HC_PN, n3), *bq = C->buf<double>(S_HC_B, n3);
-	launch_to4(pts, R.on ? R.perm : nullptr, nv, o, s);
+	launch_hc_load(pts, R.on ? R.perm : nullptr, nv, o, s);
 	HIPX(hipMemcpyAsync(p, o, 8 * n3, hipMemcpyDeviceToDevice, s));
 	const double pair_bytes = 8.0 * 3 * nt;
 	for (int it = 0; it < iter; it++) {
@@ -861,7 +862,7 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 		C->tend();
 	}
 	uint8_t *border = lockEdges ? border_flags(C, c, T, nv, nt) : nullptr;
-	launch_from4(p, o, border, R.on ? R.perm : nullptr, nv, pts, s);
+	launch_hc_store(p, o, border, R.on ? R.perm : nullptr, nv, pts, s);
 	C->check();
 }
 
@@ -874,7 +875,7 @@ void laplace_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint3
 	Csr c = build_csr(C, R.on ? R.tris : tris, nv, nt);
 	const size_t n3 = 3 * (size_t)nv;
 	double *p = C->buf<double>(S_HC_P, n3), *q = C->buf<double>(S_HC_PN, n3);
-	launch_to4(pts, R.on ? R.perm : nullptr, nv, q, s);
+	launch_hc_load(pts, R.on ? R.perm : nullptr, nv, q, s);
 	double *cur = q, *nxt = p;
 	for (int it = 0; it < iter; it++) {
 		C->tbegin("laplace", 4.0 * nv + 8.0 * 3 * nt + 48.0 * nv);
@@ -884,7 +885,7 @@ void laplace_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint3
 		cur = nxt;
 		nxt = t;
 	}
-	launch_from4(cur, cur, nullptr, R.on ? R.perm : nullptr, nv, pts, s);
+	launch_hc_store(cur, cur, nullptr, R.on ? R.perm : nullptr, nv, pts, s);
 	C->check();
 }
 
@@ -911,12 +912,13 @@ template <typename T> T *host_copy_out(n2m_ctx *C, const void *d, size_t count)
 }
 
 int set_threshold_device(n2m_ctx *C, const float *d_img, int64_t nvox, int mode, float *result);
+int otsu(n2m_ctx *C, const int *H, int nBin, int mode, int *dark, int *mid, int *bright);
 struct ThrHist {
 	float mn, scl;
 	std::vector<int> h;
 };
 bool threshold_hist(n2m_ctx *C, const float *d_img, int64_t nvox, ThrHist &out);
-void thresholds_from(const ThrHist &H, int n, const int *modes, float *out);
+void thresholds_from(n2m_ctx *C, const ThrHist &H, int n, const int *modes, float *out);
 
 } // namespace
 
@@ -1775,7 +1777,7 @@ int n2m_ctx_set_thresholds(n2m_ctx *C, const float *d_img, int64_t nvox, int n, 
 				out[i] = 1.0f;
 			return EXIT_SUCCESS;
 		}
-		thresholds_from(H, n, modes, out);
+		thresholds_from(C, H, n, modes, out);
 		return EXIT_SUCCESS;
 	});
 }
@@ -1935,6 +1937,16 @@ void n2m_atlas_free(n2m_atlas_mesh *meshes, int nmesh)
 		free(meshes[k].p);
 	}
 	free(meshes);
+}
+
+int n2m_otsu(const int *H, int nBin, int mode, int *dark, int *mid, int *bright)
+{
+	int r = 0;
+	guarded([&] {
+		r = otsu(tl_ctx.get(), H, nBin, mode, dark, mid, bright);
+		return 0;
+	});
+	return r;
 }
 
 int n2m_ctx_diff_forward(n2m_ctx *C, const float *d_vol, const int dim[3], float isolevel, int originalMC,
@@ -2289,8 +2301,10 @@ float setThreshold(float *img, int nvox, int darkMediumBright123)
 
 namespace {
 
-// nii_otsu(), src/isolevel.c:141-243: stays on the host (256 bins).
-int otsu(const int *H, int nBin, int mode, int *dark, int *mid, int *bright)
+// nii_otsu(), src/isolevel.c:141-243.  The Liao table is built on the host
+// in the reference's order; the O(nBin^3) three-threshold search of modes
+// 1 and 5 runs on the GPU (k_otsu3), the other modes on the host.
+int otsu(n2m_ctx *C, const int *H, int nBin, int mode, int *dark, int *mid, int *bright)
 {
 	*dark = *mid = *bright = 0;
 	double Sum = 0.0;
@@ -2319,22 +2333,31 @@ int otsu(const int *H, int nBin, int mode, int *dark, int *mid, int *bright)
 		}
 	int thresh = 0;
 	if (mode == 1 || mode == 5) {
+		if (nBin != 256)
+			fail("three-threshold Otsu search needs 256 bins");
 		int lo = (int)(0.25 * nBin), mi = (int)(0.50 * nBin), hi = (int)(0.75 * nBin);
 		double best = P[lo] + P[(size_t)(lo + 1) * nBin + mi] + P[(size_t)(mi + 1) * nBin + hi] +
 		              P[(size_t)(hi + 1) * nBin + 255];
-		for (int l = 0; l < nBin - 3; l++)
-			for (int m = l + 1; m < nBin - 2; m++) {
-				const double a = P[l] + P[(size_t)(l + 1) * nBin + m];
-				const double *row = &P[(size_t)(m + 1) * nBin];
-				for (int h = m + 1; h < nBin - 1; h++) {
-					double v = a + row[h] + P[(size_t)(h + 1) * nBin + 255];
-					if (v > best) {
-						lo = l;
-						mi = m;
-						hi = h;
-						best = v;
-					}
-				}
+		const int nl = nBin - 3;
+		char *buf = C->buf<char>(S_OTSU, P.size() * 8 + (size_t)nl * 16 + 64);
+		double *dP = (double *)buf, *bv = dP + P.size();
+		int *bm = (int *)(bv + nl), *bh = bm + nl;
+		HIPX(hipMemcpyAsync(dP, P.data(), P.size() * 8, hipMemcpyHostToDevice, C->stream));
+		C->tbegin("otsu_search", 8.0 * P.size());
+		launch_otsu3(dP, nBin, bv, bm, bh, C->stream);
+		C->tend();
+		std::vector<double> hv(nl);
+		std::vector<int> hm(nl), hh(nl);
+		HIPX(hipMemcpyAsync(hv.data(), bv, (size_t)nl * 8, hipMemcpyDeviceToHost, C->stream));
+		HIPX(hipMemcpyAsync(hm.data(), bm, (size_t)nl * 4, hipMemcpyDeviceToHost, C->stream));
+		HIPX(hipMemcpyAsync(hh.data(), bh, (size_t)nl * 4, hipMemcpyDeviceToHost, C->stream));
+		C->sync();
+		for (int l = 0; l < nl; l++) // first strict maximum in l order
+			if (hm[l] >= 0 && hv[l] > best) {
+				lo = l;
+				mi = hm[l];
+				hi = hh[l];
+				best = hv[l];
 			}
 		thresh = mode == 1 ? hi : lo;
 		*dark = lo;
@@ -2466,20 +2489,20 @@ bool threshold_hist(n2m_ctx *C, const float *d_img, int64_t nvox, ThrHist &out)
 
 // isolevels for several darkMediumBright123 modes from one set of histograms
 // (src/isolevel.c:264-276: modes 1 and 3 share the 3-threshold search)
-void thresholds_from(const ThrHist &H, int n, const int *modes, float *out)
+void thresholds_from(n2m_ctx *C, const ThrHist &H, int n, const int *modes, float *out)
 {
 	int d5 = -1, b5 = -1, m3 = -1, dark, mid, bright;
 	for (int i = 0; i < n; i++) {
 		if (modes[i] == 1 || modes[i] == 3) {
 			if (d5 < 0) {
-				otsu(H.h.data(), 256, 5, &dark, &mid, &bright);
+				otsu(C, H.h.data(), 256, 5, &dark, &mid, &bright);
 				d5 = dark;
 				b5 = bright;
 			}
 			out[i] = modes[i] == 1 ? (d5 / H.scl) + H.mn : (b5 / H.scl) + H.mn;
 		} else {
 			if (m3 < 0) {
-				otsu(H.h.data(), 256, 3, &dark, &mid, &bright);
+				otsu(C, H.h.data(), 256, 3, &dark, &mid, &bright);
 				m3 = mid;
 			}
 			out[i] = (m3 / H.scl) + H.mn;
@@ -2493,7 +2516,7 @@ int set_threshold_device(n2m_ctx *C, const float *d_img, int64_t nvox, int mode,
 	*result = 1.0f;
 	if (!threshold_hist(C, d_img, nvox, H))
 		return 0;
-	thresholds_from(H, 1, &mode, result);
+	thresholds_from(C, H, 1, &mode, result);
 	return 0;
 }
 

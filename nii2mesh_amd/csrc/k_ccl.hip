@@ -230,6 +230,10 @@ k_ccl_merge(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint
 		const bool other_tile_row = ny2 < 0 || ny2 >= CC_TY || nz2 < 0 || nz2 >= CC_TZ;
 		if (y2 < 0 || y2 >= g.ny || z2 < 0 || z2 >= g.nz)
 			continue;
+		// a row inside the tile links across the tile only through the
+		// neighbouring words of the tile's first and last word
+		if (!other_tile_row && k != 0 && k != CC_KW - 1)
+			continue;
 		const uint32_t u = g.word(bits, wx, y2, z2, invert);
 		const uint32_t up = g.word(bits, wx - 1, y2, z2, invert), un = g.word(bits, wx + 1, y2, z2, invert);
 		for (int dx = nb[i].dxlo; dx <= nb[i].dxhi; dx++)

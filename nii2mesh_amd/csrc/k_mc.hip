@@ -352,39 +352,46 @@ __device__ int lw_select(const float c[8], int lut, int originalMC)
 	return -1;
 }
 
-// edge-vertex flags of a point: bit a set when the a-edge (x,y,z) carries a
-// vertex, src/MarchingCubes.c:238-269
-__device__ __forceinline__ uint32_t lw_point_flags(const float *__restrict__ img, const LwGeo &g, int x, int y,
-                                                   int z, float c0)
-{
-	float cn[3];
-	cn[0] = x < g.sx - 1 ? lw_data(img, g, x + 1, y, z) : c0;
-	cn[1] = y < g.sy - 1 ? lw_data(img, g, x, y + 1, z) : c0;
-	cn[2] = z < g.sz - 1 ? lw_data(img, g, x, y, z + 1) : c0;
-	uint32_t f = 0;
-#pragma unroll
-	for (int a = 0; a < 3; a++) {
-		bool cross = c0 < 0.f ? cn[a] > 0.f : cn[a] < 0.f;
-		f |= (uint32_t)cross << a;
-	}
-	return f;
-}
-
 // Pass A for one row segment (seg, y, z): per point flags, per cube tiling
-// row; the segment's counts of edge vertices, triangles and centre vertices
-// go to cnt[0], cnt[cstride], cnt[2*cstride].
+// row (a point's bit a is set when its a-edge carries a vertex,
+// src/MarchingCubes.c:238-269); the segment's counts of edge vertices,
+// triangles and centre vertices go to cnt[0], cnt[cstride], cnt[2*cstride].
+// The four point rows the
+// segment reads -- (y,z), (y+1,z), (y,z+1), (y+1,z+1) -- are staged in LDS
+// first (sub-box values, clamped as lw_data returns them).
 __device__ __forceinline__ void lw_classify_row(const float *__restrict__ img, const LwGeo &g, int seg, int y, int z,
                                                 uint8_t *__restrict__ pflags, uint16_t *__restrict__ crow,
                                                 uint32_t *__restrict__ cnt, size_t cstride)
 {
 	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
+	__shared__ float s_row[4][LW_SEG + 1];
 	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
+	const int xl = min(xe + 1, g.sx) - xs; // staged points per row: [xs, xs + xl)
 	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
+#pragma unroll
+	for (int r = 0; r < 4; r++) {
+		const int yy = y + (r & 1), zz = z + (r >> 1);
+		if (yy >= g.sy || zz >= g.sz)
+			continue;
+		for (int i = threadIdx.x; i < xl; i += LW_BS)
+			s_row[r][i] = lw_data(img, g, xs + i, yy, zz);
+	}
+	__syncthreads();
 	uint32_t nv = 0, nt = 0, nc = 0;
 	const size_t rowbase = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
 	for (int x = xs + threadIdx.x; x < xe; x += LW_BS) {
-		float c0 = lw_data(img, g, x, y, z);
-		uint32_t f = lw_point_flags(img, g, x, y, z, c0);
+		const int i = x - xs;
+		const float c0 = s_row[0][i];
+		float cn[3];
+		cn[0] = x < g.sx - 1 ? s_row[0][i + 1] : c0;
+		cn[1] = y < g.sy - 1 ? s_row[1][i] : c0;
+		cn[2] = z < g.sz - 1 ? s_row[2][i] : c0;
+		uint32_t f = 0;
+#pragma unroll
+		for (int a = 0; a < 3; a++) {
+			bool cross = c0 < 0.f ? cn[a] > 0.f : cn[a] < 0.f;
+			f |= (uint32_t)cross << a;
+		}
 		pflags[rowbase + x] = (uint8_t)f;
 		nv += __popc(f);
 		uint16_t cr = 0;
@@ -393,7 +400,7 @@ __device__ __forceinline__ void lw_classify_row(const float *__restrict__ img, c
 			int lut = 0;
 #pragma unroll
 			for (int p = 0; p < 8; p++) {
-				c[p] = lw_data(img, g, x + c_cx[p], y + c_cy[p], z + c_cz[p]);
+				c[p] = s_row[c_cy[p] + 2 * c_cz[p]][i + c_cx[p]];
 				if (c[p] > 0.f)
 					lut += 1 << p;
 			}

@@ -638,7 +638,7 @@ k_laplace(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, co
 }
 
 // gather through a permutation: dst[i] = src[perm ? perm[i] : i]
-__global__ void k_to4(const d3 *__restrict__ src, const uint32_t *__restrict__ perm, uint32_t n, d3 *__restrict__ dst)
+__global__ void k_hc_load(const d3 *__restrict__ src, const uint32_t *__restrict__ perm, uint32_t n, d3 *__restrict__ dst)
 {
 	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n)
@@ -647,7 +647,7 @@ __global__ void k_to4(const d3 *__restrict__ src, const uint32_t *__restrict__ p
 
 // final HC positions (border vertices keep the original), scattered back
 // through the permutation: dst[perm ? perm[i] : i] = ...
-__global__ void k_from4(const d3 *__restrict__ p, const d3 *__restrict__ o, const uint8_t *__restrict__ border,
+__global__ void k_hc_store(const d3 *__restrict__ p, const d3 *__restrict__ o, const uint8_t *__restrict__ border,
                         const uint32_t *__restrict__ perm, uint32_t n, d3 *__restrict__ dst)
 {
 	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -729,14 +729,14 @@ void launch_laplace(const uint32_t *start, const int2 *pairs, const double *p, d
 #undef X
 	}
 }
-void launch_to4(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s)
+void launch_hc_load(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_to4, dim3(g1(n)), dim3(256), 0, s, (const d3 *)src, perm, n, (d3 *)dst);
+	hipLaunchKernelGGL(k_hc_load, dim3(g1(n)), dim3(256), 0, s, (const d3 *)src, perm, n, (d3 *)dst);
 }
-void launch_from4(const double *p, const double *o, const uint8_t *border, const uint32_t *perm, uint32_t n,
+void launch_hc_store(const double *p, const double *o, const uint8_t *border, const uint32_t *perm, uint32_t n,
                   double *dst, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_from4, dim3(g1(n)), dim3(256), 0, s, (const d3 *)p, (const d3 *)o, border, perm, n,
+	hipLaunchKernelGGL(k_hc_store, dim3(g1(n)), dim3(256), 0, s, (const d3 *)p, (const d3 *)o, border, perm, n,
 	                   (d3 *)dst);
 }
 

@@ -776,4 +776,57 @@ void launch_binarize(const float *labels, size_t n, float lo, float hi, float *o
 	hipLaunchKernelGGL(k_binarize, dim3((unsigned)b), dim3(256), 0, s, labels, n, lo, hi, out, count);
 }
 
+// ---------------------------------------------------------------------------
+// Three-threshold Otsu search of nii_otsu() mode 5 (src/isolevel.c:176-197)
+// over the Liao table P (nBin x nBin doubles, built on the host exactly as
+// the reference builds it).  Workgroup l scans every (m, h) for that l;
+// each lane keeps the first strict maximum in h order, the workgroup the
+// first in m order, and the host the first in l order -- the reference's
+// first occurrence of the maximum in (l, m, h) order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_otsu3(const double *__restrict__ P, int nBin, double *__restrict__ bv, int *__restrict__ bm, int *__restrict__ bh)
+{
+	__shared__ double s_v[256];
+	__shared__ int s_h[256];
+	const int l = blockIdx.x, m = l + 1 + threadIdx.x;
+	double best = 0.0;
+	int bestH = -1;
+	if (m < nBin - 2) {
+		const double a = P[l] + P[(size_t)(l + 1) * nBin + m];
+		const double *row = P + (size_t)(m + 1) * nBin;
+		for (int h = m + 1; h < nBin - 1; h++) {
+			const double v = a + row[h] + P[(size_t)(h + 1) * nBin + 255];
+			if (bestH < 0 || v > best) {
+				best = v;
+				bestH = h;
+			}
+		}
+	}
+	s_v[threadIdx.x] = best;
+	s_h[threadIdx.x] = bestH;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double b = 0.0;
+		int bt = -1;
+		for (int t = 0; t < 256; t++) {
+			if (s_h[t] < 0)
+				continue;
+			if (bt < 0 || s_v[t] > b) {
+				b = s_v[t];
+				bt = t;
+			}
+		}
+		bv[l] = b;
+		bm[l] = bt < 0 ? -1 : l + 1 + bt;
+		bh[l] = bt < 0 ? -1 : s_h[bt];
+	}
+}
+
+void launch_otsu3(const double *P, int nBin, double *bv, int *bm, int *bh, hipStream_t s)
+{
+	if (nBin > 3)
+		hipLaunchKernelGGL(k_otsu3, dim3(nBin - 3), dim3(256), 0, s, P, nBin, bv, bm, bh);
+}
+
 } // namespace n2m

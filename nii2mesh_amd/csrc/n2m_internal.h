@@ -60,6 +60,8 @@ void launch_dilate_bits(const uint32_t *in, uint32_t *out, int nx, int ny, int n
 void launch_apply(float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax, const uint32_t *keep,
                   const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s);
 void launch_hist_range(const float *img, size_t n, uint32_t *stats, hipStream_t s);
+// mode-5 Otsu search, one workgroup per first threshold l (nBin <= 258)
+void launch_otsu3(const double *P, int nBin, double *bv, int *bm, int *bh, hipStream_t s);
 void launch_hist_bins(const float *img, size_t n, float mn, float scl, int nbins, uint32_t *hist, hipStream_t s);
 
 // atlas volume stages (k_volume.hip)
@@ -169,9 +171,10 @@ void launch_hc_b(const uint32_t *start, const int2 *pairs, const double *pn, con
                  uint32_t nv, double beta, hipStream_t s);
 void launch_laplace(const uint32_t *start, const int2 *pairs, const double *p, double *pout, uint32_t nv,
                     hipStream_t s);
-// smoothing state uses 4 doubles per vertex (x, y, z, pad)
-void launch_to4(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s);
-void launch_from4(const double *p, const double *o, const uint8_t *border, const uint32_t *perm, uint32_t n,
+// smoothing state: 3 doubles per vertex, optionally gathered / scattered
+// through a vertex permutation
+void launch_hc_load(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s);
+void launch_hc_store(const double *p, const double *o, const uint8_t *border, const uint32_t *perm, uint32_t n,
                   double *dst, hipStream_t s);
 void launch_pts_bbox(const double *pts, uint32_t n, uint32_t *bb, hipStream_t s);
 void launch_morton(const double *pts, uint32_t n, const uint32_t *bb, uint32_t *keys, uint32_t *vals,
