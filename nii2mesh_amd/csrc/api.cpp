@@ -584,7 +584,7 @@ bool marching_cubes(n2m_ctx *C, const float *vol, const int dim[3], const int lo
 	double *pts = C->buf<double>(S_PTS0, (size_t)3 * V);
 	int *tris = C->buf<int>(S_TRIS0, (size_t)3 * nT);
 	C->tbegin("mc_lewiner_emit", 3.0 * npts + 24.0 * V + 12.0 * nT);
-	launch_lw_emit(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, pf, cr, base, nE, pts, tris, origin, s);
+	launch_lw_emit(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, pf, cr, cnt, base, nE, pts, tris, origin, s);
 	C->tend();
 	o.npt = V;
 	o.ntri = nT;
@@ -1278,8 +1278,8 @@ void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_pa
 			double *pts = C->buf<double>(S_PTS0, (size_t)3 * Vtot);
 			int *tris = C->buf<int>(S_TRIS0, (size_t)3 * (Ttot ? Ttot : 1));
 			C->tbegin("atlas_mc_lewiner_emit", 3.0 * npts + 24.0 * Vtot + 12.0 * Ttot);
-			launch_atlas_lw_emit(crops, dT(), (int)nt_, dmap(), nb, P.originalMC, pf, cr, base, stride, E[nt_], pts,
-			                     tris, s);
+			launch_atlas_lw_emit(crops, dT(), (int)nt_, dmap(), nb, P.originalMC, pf, cr, cnt, base, stride, E[nt_],
+			                     pts, tris, s);
 			C->tend();
 		}
 	}
@@ -1583,8 +1583,8 @@ int diff_forward(n2m_ctx *C, const float *vol, const int dim[3], float iso, int 
 	unsigned long long *vrec = C->buf<unsigned long long>(S_DF_VREC, V);
 	uint32_t *crec = C->buf<uint32_t>(S_DF_CREC, (size_t)12 * (nC ? nC : 1));
 	C->tbegin("mc_lewiner_emit", 3.0 * npts + 32.0 * V + 12.0 * nT + 48.0 * nC);
-	launch_lw_emit_rec(vol, NX, NY, NZ, lo, hi, iso, 0.f, originalMC, pf, cr, base, nE, pts0, tris0, exact ? 1 : 0,
-	                   vrec, crec, s);
+	launch_lw_emit_rec(vol, NX, NY, NZ, lo, hi, iso, 0.f, originalMC, pf, cr, cnt, base, nE, pts0, tris0,
+	                   exact ? 1 : 0, vrec, crec, s);
 	C->tend();
 	double *pts = nullptr;
 	int *tris = nullptr;

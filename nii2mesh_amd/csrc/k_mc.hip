@@ -483,34 +483,39 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
                                             const uint8_t *__restrict__ pflags, const uint16_t *__restrict__ crow,
                                             const uint32_t *__restrict__ bases, size_t bstride, uint32_t edge_add,
                                             uint32_t centre_add, double *__restrict__ pts, int *__restrict__ tris,
+                                            const uint32_t *__restrict__ cnt, size_t cstride,
                                             unsigned long long *__restrict__ vrec = nullptr,
                                             uint32_t *__restrict__ crec = nullptr)
 {
+	static_assert(LW_BS == 4 * N2M_WAVE, "one wave per point row");
 	__shared__ uint32_t s_pre[4][LW_SEG + 1]; // popcount prefix for the 4 point rows
 	__shared__ uint32_t s_base[4];
 	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
+	// a segment without edge vertices, triangles or centre vertices of its
+	// own writes nothing (its counts from pass A, the same for every lane)
+	if (cnt[0] == 0 && cnt[cstride] == 0 && cnt[2 * cstride] == 0)
+		return;
 	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
 	const int len = xe - xs; // points owned; prefix covers [xs, xe]
 	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
 	const int nrows = cuberow ? 4 : 1;
-	// row r: (y + (r&1), z + (r>>1))
-	if (threadIdx.x < nrows) {
-		int r = threadIdx.x;
-		int yy = y + (r & 1), zz = z + (r >> 1);
-		s_base[r] = edge_add + bases[(size_t)seg + (size_t)g.nseg * ((size_t)yy + (size_t)g.sy * zz)];
-	}
-	for (int r = 0; r < nrows; r++) {
-		int yy = y + (r & 1), zz = z + (r >> 1);
-		const size_t rb = (size_t)g.sx * ((size_t)yy + (size_t)g.sy * zz);
-		uint32_t carry = 0;
-		for (int x0 = 0; x0 <= len; x0 += LW_BS) {
-			int x = xs + x0 + threadIdx.x;
-			uint32_t f = (x0 + (int)threadIdx.x < len) ? __popc(pflags[rb + x]) : 0u;
-			uint32_t tot;
-			uint32_t ex = block_excl_scan<LW_BS>(f, scratch, &tot);
-			if (x0 + (int)threadIdx.x <= len)
-				s_pre[r][x0 + threadIdx.x] = carry + ex;
-			carry += tot;
+	// row r = wave r: (y + (r&1), z + (r>>1)); exclusive popcount prefix
+	{
+		const int r = wave_id(), lane = lane_id();
+		if (r < nrows) {
+			const int yy = y + (r & 1), zz = z + (r >> 1);
+			if (lane == 0)
+				s_base[r] = edge_add + bases[(size_t)seg + (size_t)g.nseg * ((size_t)yy + (size_t)g.sy * zz)];
+			const size_t rb = (size_t)g.sx * ((size_t)yy + (size_t)g.sy * zz);
+			uint32_t carry = 0;
+			for (int x0 = 0; x0 <= len; x0 += N2M_WAVE) {
+				const int i = x0 + lane;
+				uint32_t f = i < len ? __popc(pflags[rb + xs + i]) : 0u;
+				uint32_t incl = wave_incl_scan(f);
+				if (i <= len)
+					s_pre[r][i] = carry + incl - f;
+				carry += __shfl(incl, N2M_WAVE - 1, N2M_WAVE);
+			}
 		}
 	}
 	__syncthreads();
@@ -557,11 +562,16 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 		int row = -1;
 		if (i < ncube)
 			row = (int)crow[rb0 + xs + i] - 1;
+		if (!__syncthreads_or(row >= 0))
+			continue; // no cube of this chunk has triangles
 		uint32_t nt = row >= 0 ? n2m_lw_row_ntri[row] : 0u;
 		uint32_t hc = (row >= 0 && !g.originalMC && n2m_lw_row_centre[row]) ? 1u : 0u;
-		uint32_t ttot, ctot;
-		uint32_t toff = block_excl_scan<LW_BS>(nt, scratch, &ttot);
-		uint32_t coff = block_excl_scan<LW_BS>(hc, scratch, &ctot);
+		// one scan of (triangles | centres << 16): a chunk holds at most
+		// LW_BS * 12 triangles and LW_BS centres
+		uint32_t ptot;
+		const uint32_t poff = block_excl_scan<LW_BS>(nt | (hc << 16), scratch, &ptot);
+		const uint32_t toff = poff & 0xffffu, coff = poff >> 16;
+		const uint32_t ttot = ptot & 0xffffu, ctot = ptot >> 16;
 		if (row >= 0) {
 			const int x = xs + i;
 			// vertex id of cube edge code e
@@ -647,23 +657,26 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 
 __global__ void __launch_bounds__(LW_BS)
 k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
-          const uint16_t *__restrict__ crow, const uint32_t *__restrict__ bases, uint32_t nedge,
-          double *__restrict__ pts, int *__restrict__ tris)
+          const uint16_t *__restrict__ crow, const uint32_t *__restrict__ segcnt, const uint32_t *__restrict__ bases,
+          uint32_t nedge, double *__restrict__ pts, int *__restrict__ tris)
 {
 	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
-	lw_emit_row(img, g, blockIdx.x, blockIdx.y, blockIdx.z, pflags, crow, bases, ns, 0u, nedge, pts, tris);
+	const size_t si = (size_t)blockIdx.x + (size_t)g.nseg * ((size_t)blockIdx.y + (size_t)g.sy * blockIdx.z);
+	lw_emit_row(img, g, blockIdx.x, blockIdx.y, blockIdx.z, pflags, crow, bases, ns, 0u, nedge, pts, tris, segcnt + si,
+	            ns);
 }
 
 template <int MODE>
 __global__ void __launch_bounds__(LW_BS)
 k_lw_emit_rec(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
-              const uint16_t *__restrict__ crow, const uint32_t *__restrict__ bases, uint32_t nedge,
-              double *__restrict__ pts, int *__restrict__ tris, unsigned long long *__restrict__ vrec,
-              uint32_t *__restrict__ crec)
+              const uint16_t *__restrict__ crow, const uint32_t *__restrict__ segcnt,
+              const uint32_t *__restrict__ bases, uint32_t nedge, double *__restrict__ pts, int *__restrict__ tris,
+              unsigned long long *__restrict__ vrec, uint32_t *__restrict__ crec)
 {
 	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
+	const size_t si = (size_t)blockIdx.x + (size_t)g.nseg * ((size_t)blockIdx.y + (size_t)g.sy * blockIdx.z);
 	lw_emit_row<MODE>(img, g, blockIdx.x, blockIdx.y, blockIdx.z, pflags, crow, bases, ns, 0u, nedge, pts, tris,
-	                  vrec, crec);
+	                  segcnt + si, ns, vrec, crec);
 }
 
 void lw_geo(LwGeo &g, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso, float mnv,
@@ -704,7 +717,7 @@ void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3
 }
 
 void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
-                    float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow,
+                    float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow, const uint32_t *segcnt,
                     const uint32_t *bases, uint32_t nedge, double *pts, int *tris, const int origin[3],
                     hipStream_t s)
 {
@@ -713,13 +726,14 @@ void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], c
 	g.out0 = lo[0] + origin[0];
 	g.out1 = lo[1] + origin[1];
 	g.out2 = lo[2] + origin[2];
-	hipLaunchKernelGGL(k_lw_emit, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags, crow, bases,
+	hipLaunchKernelGGL(k_lw_emit, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags, crow, segcnt, bases,
 	                   nedge, pts, tris);
 }
 
 void launch_lw_emit_rec(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                         float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow,
-                        const uint32_t *bases, uint32_t nedge, double *pts, int *tris, int exact,
+                        const uint32_t *segcnt, const uint32_t *bases, uint32_t nedge, double *pts, int *tris,
+                        int exact,
                         unsigned long long *vrec, uint32_t *crec, hipStream_t s)
 {
 	LwGeo g;
@@ -729,10 +743,10 @@ void launch_lw_emit_rec(const float *img, int nx, int ny, int nz, const int lo[3
 	g.out2 = lo[2];
 	if (exact)
 		hipLaunchKernelGGL(k_lw_emit_rec<LW_RECORD_EXACT>, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g,
-		                   pflags, crow, bases, nedge, pts, tris, vrec, crec);
+		                   pflags, crow, segcnt, bases, nedge, pts, tris, vrec, crec);
 	else
 		hipLaunchKernelGGL(k_lw_emit_rec<LW_RECORD>, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags,
-		                   crow, bases, nedge, pts, tris, vrec, crec);
+		                   crow, segcnt, bases, nedge, pts, tris, vrec, crec);
 }
 
 // ---------------------------------------------------------------------------
@@ -810,8 +824,9 @@ k_atlas_lw_classify(const float *__restrict__ crops, const AtlasTask *__restrict
 __global__ void __launch_bounds__(LW_BS)
 k_atlas_lw_emit(const float *__restrict__ crops, const AtlasTask *__restrict__ tasks, int ntasks,
                 const uint32_t *__restrict__ blockTask, int originalMC, const uint8_t *__restrict__ pflags,
-                const uint16_t *__restrict__ crow, const uint32_t *__restrict__ bases, size_t stride,
-                uint32_t totalE, double *__restrict__ pts, int *__restrict__ tris)
+                const uint16_t *__restrict__ crow, const uint32_t *__restrict__ segcnt,
+                const uint32_t *__restrict__ bases, size_t stride, uint32_t totalE, double *__restrict__ pts,
+                int *__restrict__ tris)
 {
 	const int ti = blockTask[blockIdx.x];
 	const AtlasTask t = tasks[ti];
@@ -821,7 +836,7 @@ k_atlas_lw_emit(const float *__restrict__ crops, const AtlasTask *__restrict__ t
 	const uint32_t cbase = bases[2 * stride + t.first_block];
 	const uint32_t enext = ti + 1 < ntasks ? bases[tasks[ti + 1].first_block] : totalE;
 	lw_emit_row(crops + t.off, g, seg, y, z, pflags + t.cube_off, crow + t.cube_off, bases + t.first_block, stride,
-	            cbase, enext, pts, tris);
+	            cbase, enext, pts, tris, segcnt + blockIdx.x, stride);
 }
 
 void launch_atlas_classic_count(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
@@ -849,12 +864,13 @@ void launch_atlas_lw_classify(const float *crops, const AtlasTask *tasks, const 
 }
 void launch_atlas_lw_emit(const float *crops, const AtlasTask *tasks, int ntasks, const uint32_t *blockTask,
                           unsigned nblocks, int originalMC, const uint8_t *pflags, const uint16_t *crow,
-                          const uint32_t *bases, size_t stride, uint32_t totalE, double *pts, int *tris,
+                          const uint32_t *segcnt, const uint32_t *bases, size_t stride, uint32_t totalE,
+                          double *pts, int *tris,
                           hipStream_t s)
 {
 	if (nblocks)
 		hipLaunchKernelGGL(k_atlas_lw_emit, dim3(nblocks), dim3(LW_BS), 0, s, crops, tasks, ntasks, blockTask,
-		                   originalMC, pflags, crow, bases, stride, totalE, pts, tris);
+		                   originalMC, pflags, crow, segcnt, bases, stride, totalE, pts, tris);
 }
 int lw_seg_len() { return LW_SEG; }
 

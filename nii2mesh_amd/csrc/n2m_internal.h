@@ -103,13 +103,14 @@ void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3
                         float mnv, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
                         hipStream_t s);
 void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
-                    float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow, const uint32_t *bases,
-                    uint32_t nedge, double *pts, int *tris, const int origin[3], hipStream_t s);
+                    float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow, const uint32_t *segcnt,
+                    const uint32_t *bases, uint32_t nedge, double *pts, int *tris, const int origin[3],
+                    hipStream_t s);
 
 void launch_lw_emit_rec(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                         float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow,
-                        const uint32_t *bases, uint32_t nedge, double *pts, int *tris, int exact,
-                        unsigned long long *vrec, uint32_t *crec, hipStream_t s);
+                        const uint32_t *segcnt, const uint32_t *bases, uint32_t nedge, double *pts, int *tris,
+                        int exact, unsigned long long *vrec, uint32_t *crec, hipStream_t s);
 
 // adjoint (k_adjoint.hip)
 void launch_adj_laplace_T(const uint32_t *start, const int2 *pairs, double *g, double *h, uint32_t n,
@@ -131,8 +132,8 @@ void launch_atlas_lw_classify(const float *crops, const AtlasTask *tasks, const 
                               size_t stride, hipStream_t s);
 void launch_atlas_lw_emit(const float *crops, const AtlasTask *tasks, int ntasks, const uint32_t *blockTask,
                           unsigned nblocks, int originalMC, const uint8_t *pflags, const uint16_t *crow,
-                          const uint32_t *bases, size_t stride, uint32_t totalE, double *pts, int *tris,
-                          hipStream_t s);
+                          const uint32_t *segcnt, const uint32_t *bases, size_t stride, uint32_t totalE,
+                          double *pts, int *tris, hipStream_t s);
 int lw_seg_len();
 
 // k_mesh.hip
