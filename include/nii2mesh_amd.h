@@ -109,6 +109,13 @@ int n2m_remove_degenerate_triangles(vec3d *pts, vec3i **tris, int ntri);
 /* src/quadric.c:316-341 laplacian_smooth(), applied iter times. */
 void n2m_laplacian_smooth(vec3d *verts, vec3i *tris, int nvert, int ntri, int iter);
 
+/* src/nii2mesh.c:119,155-172 (the image loader): raw voxels to the FP32
+ * working volume, out = (raw * scl_slope) + scl_inter in float, slope 0
+ * read as 1.  datatype: 2 uint8, 4 int16, 512 uint16, 16 float32 (the
+ * types the CLI reads).  Ships 1/4 (uint8) or 1/2 (16-bit) of the float
+ * bytes over PCIe; n2m_ctx_decode does the same on device buffers. */
+int n2m_decode(const void *raw, int datatype, int64_t nvox, float scl_slope, float scl_inter, float *out);
+
 /* src/isolevel.c:141-243 nii_otsu() (file-static in the reference): returns
  * the threshold bin and fills dark/mid/bright; modes 1 and 5 (the
  * three-threshold search) run on the GPU.  nBin must be 256 for them. */
@@ -236,6 +243,10 @@ void n2m_atlas_free(n2m_atlas_mesh *meshes, int nmesh);
  * u = c0/(c0-c1), of the centre-vertex mean, of the weld and of the
  * transposed Laplacian.  No counterpart in the reference.
  * ------------------------------------------------------------------------- */
+/* Device form of n2m_decode (d_raw and d_out on the context's device). */
+int n2m_ctx_decode(n2m_ctx *ctx, const void *d_raw, int datatype, int64_t nvox, float scl_slope, float scl_inter,
+                   float *d_out);
+
 int n2m_ctx_diff_forward(n2m_ctx *ctx, const float *d_vol, const int dim[3], float isolevel, int originalMC,
                          int lapIter, int exact);
 int n2m_ctx_diff_backward(n2m_ctx *ctx, const double *d_dLdp, double *d_dLdvol);

@@ -101,6 +101,8 @@ def _bind_ext(L):
                                          np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
                                          C.c_int]
     L.n2m_laplacian_smooth_T.restype = None
+    L.n2m_decode.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_float,
+                             np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS')]
     L.n2m_otsu.argtypes = [np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
                            C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.n2m_ctx_atlas_count.argtypes = [C.c_void_p]
@@ -122,6 +124,19 @@ def laplacian_smooth_T(g, tris, iters=1):
     L = load()
     L.n2m_laplacian_smooth_T(g, np.ascontiguousarray(tris, np.int32), len(g), len(tris), int(iters))
     return g
+
+
+DT_UINT8, DT_INT16, DT_FLOAT32, DT_UINT16 = 2, 4, 16, 512
+
+
+def decode(raw, datatype, slope=1.0, inter=0.0):
+    """Raw NIfTI voxels -> float32 on the GPU (src/nii2mesh.c:155-172)."""
+    L = load()
+    raw = np.ascontiguousarray(raw)
+    out = np.empty(raw.size, np.float32)
+    if L.n2m_decode(raw.ctypes.data, int(datatype), raw.size, float(slope), float(inter), out) != 0:
+        raise RuntimeError(last_error())
+    return out.reshape(raw.shape)
 
 
 def otsu(hist, mode):

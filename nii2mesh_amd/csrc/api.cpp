@@ -1939,6 +1939,40 @@ void n2m_atlas_free(n2m_atlas_mesh *meshes, int nmesh)
 	free(meshes);
 }
 
+int n2m_ctx_decode(n2m_ctx *C, const void *d_raw, int datatype, int64_t nvox, float scl_slope, float scl_inter,
+                   float *d_out)
+{
+	return guarded([&] {
+		if (nvox < 0)
+			fail("negative voxel count");
+		if (scl_slope == 0.0f) // src/nii2mesh.c:119
+			scl_slope = 1.0f;
+		C->tbegin("decode", 4.0 * nvox + 4.0 * nvox);
+		if (launch_decode(d_raw, datatype, (size_t)nvox, scl_slope, scl_inter, d_out, C->stream) != 0)
+			fail("unsupported NIfTI datatype %d (uint8 2, int16 4, uint16 512, float32 16)", datatype);
+		C->tend();
+		C->sync();
+		return EXIT_SUCCESS;
+	});
+}
+
+int n2m_decode(const void *raw, int datatype, int64_t nvox, float scl_slope, float scl_inter, float *out)
+{
+	return guarded([&] {
+		n2m_ctx *C = tl_ctx.get();
+		const size_t bpp = datatype == 2 ? 1 : (datatype == 16 ? 4 : 2);
+		void *draw = C->buf<char>(S_AT_LABELS, (size_t)nvox * bpp);
+		float *dout = C->buf<float>(S_VOL, (size_t)nvox);
+		HIPX(hipMemcpyAsync(draw, raw, (size_t)nvox * bpp, hipMemcpyHostToDevice, C->stream));
+		int rc = n2m_ctx_decode(C, draw, datatype, nvox, scl_slope, scl_inter, dout);
+		if (rc != EXIT_SUCCESS)
+			fail("%s", tl_err.c_str());
+		HIPX(hipMemcpyAsync(out, dout, (size_t)nvox * 4, hipMemcpyDeviceToHost, C->stream));
+		C->sync();
+		return EXIT_SUCCESS;
+	});
+}
+
 int n2m_otsu(const int *H, int nBin, int mode, int *dark, int *mid, int *bright)
 {
 	int r = 0;

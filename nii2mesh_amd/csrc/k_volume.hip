@@ -829,4 +829,50 @@ void launch_otsu3(const double *P, int nBin, double *bv, int *bm, int *bh, hipSt
 		hipLaunchKernelGGL(k_otsu3, dim3(nBin - 3), dim3(256), 0, s, P, nBin, bv, bm, bh);
 }
 
+// ---------------------------------------------------------------------------
+// Raw NIfTI voxels to the FP32 working volume, src/nii2mesh.c:155-172:
+// out = (raw * slope) + inter in float (two roundings), for uint8, int16,
+// uint16 and float32 data.  Four voxels per thread.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_decode(const T *__restrict__ raw, size_t n, float slope, float inter, float *__restrict__ out)
+{
+	const size_t stride = (size_t)gridDim.x * blockDim.x * 4;
+	for (size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 4; i < n; i += stride) {
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			if (i + k < n) {
+				const float v = (float)raw[i + k] * slope;
+				out[i + k] = v + inter;
+			}
+		}
+	}
+}
+
+int launch_decode(const void *raw, int datatype, size_t n, float slope, float inter, float *out, hipStream_t s)
+{
+	size_t b = (n + 1023) / 1024;
+	b = b > 8192 ? 8192 : (b < 1 ? 1 : b);
+	switch (datatype) {
+	case 2: // DT_UINT8
+		hipLaunchKernelGGL(k_decode<uint8_t>, dim3((unsigned)b), dim3(256), 0, s, (const uint8_t *)raw, n, slope,
+		                   inter, out);
+		return 0;
+	case 4: // DT_INT16
+		hipLaunchKernelGGL(k_decode<int16_t>, dim3((unsigned)b), dim3(256), 0, s, (const int16_t *)raw, n, slope,
+		                   inter, out);
+		return 0;
+	case 512: // DT_UINT16
+		hipLaunchKernelGGL(k_decode<uint16_t>, dim3((unsigned)b), dim3(256), 0, s, (const uint16_t *)raw, n, slope,
+		                   inter, out);
+		return 0;
+	case 16: // DT_FLOAT32
+		hipLaunchKernelGGL(k_decode<float>, dim3((unsigned)b), dim3(256), 0, s, (const float *)raw, n, slope, inter,
+		                   out);
+		return 0;
+	}
+	return -1;
+}
+
 } // namespace n2m

@@ -126,12 +126,18 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p)
 {
 	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ uint32_t uf_find_live(const uint32_t *L, uint32_t x)
+// Find with path halving: every visited non-root is re-pointed to its
+// grandparent.  That is always an ancestor in the same tree with a smaller
+// index, and only roots are ever hooked, so the stores cannot lose a union.
+__device__ __forceinline__ uint32_t uf_find_live(uint32_t *L, uint32_t x)
 {
 	uint32_t p = ld_agent(&L[x]);
 	while (p != x) {
+		const uint32_t gp = ld_agent(&L[p]);
+		if (gp != p)
+			__hip_atomic_store(&L[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		x = p;
-		p = ld_agent(&L[x]);
+		p = gp;
 	}
 	return x;
 }

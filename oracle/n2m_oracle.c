@@ -1418,3 +1418,26 @@ int orc_diff_gradient(const float *img, const short dim[3], float isolevel, int 
 	diff_free(&d);
 	return EXIT_SUCCESS;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Image loader conversion, src/nii2mesh.c:119,155-172: img32 = raw * slope   */
+/* + inter in float for uint8 (2), int16 (4), uint16 (512), float32 (16).   */
+/* ------------------------------------------------------------------------ */
+int orc_decode(const void *raw, int datatype, int nvox, float slope, float inter, float *out)
+{
+	if (slope == 0.0f)
+		slope = 1.0f;
+	for (int i = 0; i < nvox; i++) {
+		float v;
+		switch (datatype) {
+		case 2: v = ((const uint8_t *)raw)[i]; break;
+		case 4: v = ((const int16_t *)raw)[i]; break;
+		case 512: v = ((const uint16_t *)raw)[i]; break;
+		case 16: v = ((const float *)raw)[i]; break;
+		default: return EXIT_FAILURE;
+		}
+		float m = v * slope;
+		out[i] = m + inter;
+	}
+	return EXIT_SUCCESS;
+}

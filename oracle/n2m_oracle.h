@@ -43,6 +43,7 @@ int orc_diff_forward(const float *img, const short dim[3], float isolevel, int o
                      vec3i **t, vec3d **p, int *nt, int *np);
 int orc_diff_gradient(const float *img, const short dim[3], float isolevel, int originalMC, int lapIter,
                       const double *dLdp, double *dLdvol);
+int orc_decode(const void *raw, int datatype, int nvox, float slope, float inter, float *out);
 int orc_label_components(const uint8_t *bw, const size_t dim[3], int conn, uint32_t *labels);
 
 #endif

@@ -49,6 +49,8 @@ class Oracle(abi.MeshLib):
                                         C.POINTER(C.c_short), C.c_float, C.c_int, C.c_int,
                                         np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'),
                                         np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS')]
+        h.orc_decode.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float,
+                                 np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS')]
         self.mc_classic = _bind_mc(h, 'orc_mc_classic', classic=True)
         self.mc_lewiner = _bind_mc(h, 'orc_mc_lewiner', classic=False)
         self.unify = abi.bind_unify(h, 'orc_unify_vertices')
@@ -82,6 +84,12 @@ class Oracle(abi.MeshLib):
     def apply_sform(self, tris, pts, sx, sy, sz):
         f = C.c_float * 4
         self.lib.orc_apply_sform(tris, pts, len(tris), len(pts), f(*sx), f(*sy), f(*sz))
+
+    def decode(self, raw, datatype, slope, inter):
+        raw = np.ascontiguousarray(raw)
+        out = np.empty(raw.size, np.float32)
+        rc = self.lib.orc_decode(raw.ctypes.data, int(datatype), raw.size, float(slope), float(inter), out)
+        return rc, out
 
     def laplacian_smooth_T(self, g, tris, iters=1):
         for _ in range(iters):
