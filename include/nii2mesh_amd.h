@@ -109,6 +109,12 @@ int n2m_remove_degenerate_triangles(vec3d *pts, vec3i **tris, int ntri);
 /* src/quadric.c:316-341 laplacian_smooth(), applied iter times. */
 void n2m_laplacian_smooth(vec3d *verts, vec3i *tris, int nvert, int ntri, int iter);
 
+/* save_mesh() (src/meshify.c:984-1019) for the binary formats .mz3, .gii and
+ * .ply, byte-identical files; the float32 vertex copy, the PLY face records
+ * and the GIfTI base64 run on the GPU, zlib / gzip (isGz) on the host.  Other
+ * extensions fail: the reference's text writers stay on the CPU. */
+int n2m_save_mesh(const char *fnm, vec3i *tris, vec3d *pts, int ntri, int npt, bool isGz);
+
 /* src/nii2mesh.c:119,155-172 (the image loader): raw voxels to the FP32
  * working volume, out = (raw * scl_slope) + scl_inter in float, slope 0
  * read as 1.  datatype: 2 uint8, 4 int16, 512 uint16, 16 float32 (the
@@ -243,6 +249,9 @@ void n2m_atlas_free(n2m_atlas_mesh *meshes, int nmesh);
  * u = c0/(c0-c1), of the centre-vertex mean, of the weld and of the
  * transposed Laplacian.  No counterpart in the reference.
  * ------------------------------------------------------------------------- */
+/* n2m_save_mesh for mesh `mesh` held by the context (no upload). */
+int n2m_ctx_save_mesh(n2m_ctx *ctx, int mesh, const char *fnm, int isGz);
+
 /* Device form of n2m_decode (d_raw and d_out on the context's device). */
 int n2m_ctx_decode(n2m_ctx *ctx, const void *d_raw, int datatype, int64_t nvox, float scl_slope, float scl_inter,
                    float *d_out);

@@ -103,6 +103,8 @@ def _bind_ext(L):
     L.n2m_laplacian_smooth_T.restype = None
     L.n2m_decode.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_float,
                              np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS')]
+    L.n2m_save_mesh.argtypes = [C.c_char_p, np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'),
+                                np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'), C.c_int, C.c_int, C.c_bool]
     L.n2m_otsu.argtypes = [np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
                            C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.n2m_ctx_atlas_count.argtypes = [C.c_void_p]
@@ -137,6 +139,14 @@ def decode(raw, datatype, slope=1.0, inter=0.0):
     if L.n2m_decode(raw.ctypes.data, int(datatype), raw.size, float(slope), float(inter), out) != 0:
         raise RuntimeError(last_error())
     return out.reshape(raw.shape)
+
+
+def save_mesh(fnm, tris, pts, gz=False):
+    """save_mesh() for .mz3 / .gii / .ply with the byte streams encoded on
+    the GPU (src/meshify.c:984-1019); returns 0 on success."""
+    L = load()
+    return L.n2m_save_mesh(str(fnm).encode(), np.ascontiguousarray(tris, np.int32),
+                           np.ascontiguousarray(pts, np.float64), len(tris), len(pts), bool(gz))
 
 
 def otsu(hist, mode):

@@ -53,7 +53,7 @@ def build(verbose=False):
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + '.tmp'
     cmd = [hipcc(), '-shared', '-fPIC', '--offload-arch=' + ARCH, '-o', tmp] + objs + [
-        '-Wl,--version-script=' + os.path.join(CSRC, 'exports.map'), '-Wl,-Bsymbolic']
+        '-Wl,--version-script=' + os.path.join(CSRC, 'exports.map'), '-Wl,-Bsymbolic', '-lz']
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError('link failed: %s\n%s\n%s' % (' '.join(cmd), r.stdout, r.stderr))

@@ -18,6 +18,8 @@
 #include <string>
 #include <vector>
 
+#include <zlib.h>
+
 #include "../../include/nii2mesh_amd.h"
 #include "n2m_internal.h"
 
@@ -141,6 +143,9 @@ enum Slot {
 	S_DF_H,
 	S_DF_GM,
 	S_OTSU,
+	S_ENC_A,
+	S_ENC_B,
+	S_ENC_C,
 	S_COUNT
 };
 
@@ -1661,6 +1666,223 @@ int diff_backward(n2m_ctx *C, const double *dLdp, double *dLdvol)
 
 } // namespace
 
+
+// ===========================================================================
+// Mesh writers for the binary formats (src/meshify.c:602-930): the byte
+// streams are encoded on the GPU (FP32 vertices, PLY face records, base64),
+// the headers and the optional zlib / gzip layer on the host.  The header
+// text must equal the reference's byte for byte, so it is written with the
+// same strings.
+// ===========================================================================
+namespace {
+
+std::vector<uint8_t> dl_bytes(n2m_ctx *C, const void *d, size_t n)
+{
+	std::vector<uint8_t> h(n);
+	if (n)
+		HIPX(hipMemcpyAsync(h.data(), d, n, hipMemcpyDeviceToHost, C->stream));
+	C->sync();
+	return h;
+}
+
+// float32 copy of the vertices (vec3d2vec4s, src/meshify.c:414-416)
+const float *pts_f32(n2m_ctx *C, const Mesh &m)
+{
+	float *d = C->buf<float>(S_ENC_A, (size_t)3 * (m.np ? m.np : 1));
+	C->tbegin("encode_f32", 24.0 * m.np + 12.0 * m.np);
+	launch_f64_to_f32((const double *)m.pts.p, (size_t)3 * m.np, d, C->stream);
+	C->tend();
+	return d;
+}
+
+// base64 of n host or device bytes (src/base64.c: no line breaks)
+std::vector<uint8_t> b64_of(n2m_ctx *C, const void *src, size_t n, bool src_on_host)
+{
+	const uint8_t *d = (const uint8_t *)src;
+	if (src_on_host) {
+		uint8_t *dd = C->buf<uint8_t>(S_ENC_B, n ? n : 1);
+		if (n)
+			HIPX(hipMemcpyAsync(dd, src, n, hipMemcpyHostToDevice, C->stream));
+		d = dd;
+	}
+	const size_t olen = 4 * ((n + 2) / 3);
+	uint8_t *o = C->buf<uint8_t>(S_ENC_C, olen ? olen : 1);
+	C->tbegin("encode_base64", 1.0 * n + 1.0 * olen);
+	launch_base64(d, n, o, C->stream);
+	C->tend();
+	return dl_bytes(C, o, olen);
+}
+
+// zlib compress() as the reference's GIfTI writer calls it (src/meshify.c:862-870)
+std::vector<uint8_t> zlib_of(const std::vector<uint8_t> &raw)
+{
+	uLongf dl = compressBound((uLong)raw.size());
+	std::vector<uint8_t> out(dl);
+	if (compress(out.data(), &dl, raw.data(), (uLong)raw.size()) != Z_OK)
+		printf("Compression error\n");
+	out.resize(dl);
+	return out;
+}
+
+bool write_ply(n2m_ctx *C, const Mesh &m, const char *fnm) // src/meshify.c:764-816
+{
+	FILE *fp = fopen(fnm, "wb");
+	if (!fp)
+		return false;
+	fputs("ply\n", fp);
+	fputs("format binary_little_endian 1.0\n", fp);
+	fputs("comment niimath\n", fp);
+	fprintf(fp, "element vertex %d\n", m.np);
+	fputs("property float x\n", fp);
+	fputs("property float y\n", fp);
+	fputs("property float z\n", fp);
+	fprintf(fp, "element face %d\n", m.nt);
+	fputs("property list uchar int vertex_indices\n", fp);
+	fputs("end_header\n", fp);
+	const float *p32 = pts_f32(C, m);
+	uint8_t *faces = C->buf<uint8_t>(S_ENC_B, (size_t)13 * (m.nt ? m.nt : 1));
+	C->tbegin("encode_ply", 12.0 * m.nt + 13.0 * m.nt);
+	launch_ply_faces((const int *)m.tris.p, (size_t)m.nt, faces, C->stream);
+	C->tend();
+	std::vector<uint8_t> a = dl_bytes(C, p32, (size_t)12 * m.np), b = dl_bytes(C, faces, (size_t)13 * m.nt);
+	fwrite(a.data(), 1, a.size(), fp);
+	fwrite(b.data(), 1, b.size(), fp);
+	fclose(fp);
+	return true;
+}
+
+void gii_array_head(FILE *fp, const char *type, int n, bool gz, const char *intent)
+{
+	fputs("   <DataArray  ArrayIndexingOrder=\"RowMajorOrder\"\n", fp);
+	fprintf(fp, "               DataType=\"%s\"\n", type);
+	fprintf(fp, "               Dim0=\"%d\"\n", n);
+	fputs("               Dim1=\"3\"\n", fp);
+	fputs("               Dimensionality=\"2\"\n", fp);
+	fputs(gz ? "               Encoding=\"GZipBase64Binary\"\n" : "               Encoding=\"Base64Binary\"\n", fp);
+	fputs("               Endian=\"LittleEndian\"\n", fp);
+	fputs("               ExternalFileName=\"\"\n", fp);
+	fputs("               ExternalFileOffset=\"\"\n", fp);
+	fprintf(fp, "               Intent=\"%s\">\n", intent);
+	fputs("      <MetaData>\n", fp);
+	fputs("      </MetaData>\n", fp);
+}
+
+bool write_gii(n2m_ctx *C, const Mesh &m, const char *fnm, bool gz) // src/meshify.c:819-930
+{
+	FILE *fp = fopen(fnm, "wb");
+	if (!fp)
+		return false;
+	fputs("<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n", fp);
+	fputs("<!DOCTYPE GIFTI SYSTEM \"http://www.nitrc.org/frs/download.php/115/gifti.dtd\">\n", fp);
+	fputs("<GIFTI Version=\"1.0\"  NumberOfDataArrays=\"2\">\n", fp);
+	fputs("   <MetaData>\n", fp);
+	fputs("      <MD>\n", fp);
+	fputs("          <Name><![CDATA[nii2mesh-version]]></Name>\n", fp);
+	fputs("          <Value><![CDATA[nii2mesh, 1 Jan 2022]]></Value>\n", fp);
+	fputs("      </MD>\n", fp);
+	fputs("   </MetaData>\n", fp);
+	fputs("   <LabelTable/>\n", fp);
+	gii_array_head(fp, "NIFTI_TYPE_INT32", m.nt, gz, "NIFTI_INTENT_TRIANGLE");
+	fputs("      <Data>", fp);
+	std::vector<uint8_t> e;
+	if (gz) {
+		std::vector<uint8_t> z = zlib_of(dl_bytes(C, m.tris.p, (size_t)12 * m.nt));
+		e = b64_of(C, z.data(), z.size(), true);
+	} else {
+		e = b64_of(C, m.tris.p, (size_t)12 * m.nt, false);
+	}
+	fwrite(e.data(), 1, e.size(), fp);
+	fputs("</Data>\n", fp);
+	fputs("   </DataArray>\n", fp);
+	gii_array_head(fp, "NIFTI_TYPE_FLOAT32", m.np, gz, "NIFTI_INTENT_POINTSET");
+	fputs("      <CoordinateSystemTransformMatrix>\n", fp);
+	fputs("         <DataSpace><![CDATA[NIFTI_XFORM_UNKNOWN]]></DataSpace>\n", fp);
+	fputs("         <TransformedSpace><![CDATA[NIFTI_XFORM_UNKNOWN]]></TransformedSpace>\n", fp);
+	fputs("         <MatrixData>1.000000 0.000000 0.000000 0.000000 0.000000 1.000000 0.000000 0.000000 0.000000 "
+	      "0.000000 1.000000 0.000000 0.000000 0.000000 0.000000 1.000000 </MatrixData>\n",
+	      fp);
+	fputs("      </CoordinateSystemTransformMatrix>\n", fp);
+	fputs("      <Data>", fp);
+	const float *p32 = pts_f32(C, m);
+	if (gz) {
+		std::vector<uint8_t> z = zlib_of(dl_bytes(C, p32, (size_t)12 * m.np));
+		e = b64_of(C, z.data(), z.size(), true);
+	} else {
+		e = b64_of(C, p32, (size_t)12 * m.np, false);
+	}
+	fwrite(e.data(), 1, e.size(), fp);
+	fputs("</Data>\n", fp);
+	fputs("   </DataArray>\n", fp);
+	fputs("</GIFTI>\n", fp);
+	fclose(fp);
+	return true;
+}
+
+bool write_mz3(n2m_ctx *C, const Mesh &m, const char *fnm, bool gz) // src/meshify.c:602-660
+{
+	uint8_t h[16];
+	const uint16_t sig = 0x5A4D, attr = 3;
+	const uint32_t nf = (uint32_t)m.nt, nv = (uint32_t)m.np, nskip = 0;
+	memcpy(h, &sig, 2);
+	memcpy(h + 2, &attr, 2);
+	memcpy(h + 4, &nf, 4);
+	memcpy(h + 8, &nv, 4);
+	memcpy(h + 12, &nskip, 4);
+	std::vector<uint8_t> t = dl_bytes(C, m.tris.p, (size_t)12 * m.nt);
+	std::vector<uint8_t> p = dl_bytes(C, pts_f32(C, m), (size_t)12 * m.np);
+	if (gz) {
+		gzFile f = gzopen(fnm, "w");
+		if (!f)
+			return false;
+		gzwrite(f, h, sizeof h);
+		if (!t.empty())
+			gzwrite(f, t.data(), (unsigned)t.size());
+		if (!p.empty())
+			gzwrite(f, p.data(), (unsigned)p.size());
+		gzclose(f);
+		return true;
+	}
+	FILE *fp = fopen(fnm, "wb");
+	if (!fp)
+		return false;
+	fwrite(h, 1, sizeof h, fp);
+	fwrite(t.data(), 1, t.size(), fp);
+	fwrite(p.data(), 1, p.size(), fp);
+	fclose(fp);
+	return true;
+}
+
+// save_mesh() dispatch (src/meshify.c:984-1019) for the binary formats
+int save_mesh_device(n2m_ctx *C, const Mesh &m, const char *fnm, bool gz)
+{
+	// extension = what strip_ext() (src/meshify.c:973-982) removes: the text
+	// from the last '.' after the last path separator, unless that '.' starts
+	// the name
+	std::string f(fnm), ext;
+	size_t e = f.size();
+	while (e > 0 && f[e] != '.' && f[e] != '\\' && f[e] != '/')
+		--e;
+	if (e > 0 && e < f.size() && f[e] == '.' && f[e - 1] != '\\' && f[e - 1] != '/')
+		ext = f.substr(e);
+	bool ok;
+	if (ext.find(".mz3") != std::string::npos)
+		ok = write_mz3(C, m, fnm, gz);
+	else if (ext.find(".gii") != std::string::npos)
+		ok = write_gii(C, m, fnm, gz);
+	else if (ext.find(".ply") != std::string::npos)
+		ok = write_ply(C, m, fnm);
+	else if (ext.find(".inflated") != std::string::npos || ext.find(".pial") != std::string::npos ||
+	         ext.find(".json") != std::string::npos || ext.find(".off") != std::string::npos ||
+	         ext.find(".obj") != std::string::npos || ext.find(".stl") != std::string::npos ||
+	         ext.find(".vtk") != std::string::npos)
+		fail("%s: the device writer handles .mz3, .gii and .ply; use the reference's save_mesh", fnm);
+	else
+		ok = write_mz3(C, m, (f + ".mz3").c_str(), gz);
+	return ok ? EXIT_SUCCESS : EXIT_FAILURE;
+}
+
+} // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1970,6 +2192,34 @@ int n2m_decode(const void *raw, int datatype, int64_t nvox, float scl_slope, flo
 		HIPX(hipMemcpyAsync(out, dout, (size_t)nvox * 4, hipMemcpyDeviceToHost, C->stream));
 		C->sync();
 		return EXIT_SUCCESS;
+	});
+}
+
+int n2m_ctx_save_mesh(n2m_ctx *C, int mesh, const char *fnm, int isGz)
+{
+	return guarded([&] {
+		if (mesh < 0 || mesh >= (int)C->meshes.size())
+			fail("no mesh %d", mesh);
+		return save_mesh_device(C, C->meshes[mesh], fnm, isGz != 0);
+	});
+}
+
+int n2m_save_mesh(const char *fnm, vec3i *tris, vec3d *pts, int ntri, int npt, bool isGz)
+{
+	return guarded([&] {
+		if (ntri < 0 || npt < 0)
+			fail("negative mesh size");
+		n2m_ctx *C = tl_ctx.get();
+		Mesh m;
+		m.nt = ntri;
+		m.np = npt;
+		m.tris.p = C->buf<int>(S_TRIS1, (size_t)3 * (ntri ? ntri : 1));
+		m.pts.p = C->buf<double>(S_PTS1, (size_t)3 * (npt ? npt : 1));
+		if (ntri)
+			HIPX(hipMemcpyAsync(m.tris.p, tris, (size_t)12 * ntri, hipMemcpyHostToDevice, C->stream));
+		if (npt)
+			HIPX(hipMemcpyAsync(m.pts.p, pts, (size_t)24 * npt, hipMemcpyHostToDevice, C->stream));
+		return save_mesh_device(C, m, fnm, isGz);
 	});
 }
 

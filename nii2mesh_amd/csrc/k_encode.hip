@@ -1,0 +1,73 @@
+// k_encode.hip -- mesh output encoding on gfx950 (the byte streams of the
+// reference's writers, src/meshify.c:602-930):
+//   * float64 -> float32 vertices, as vec3d2vec4s (src/meshify.c:414-416);
+//   * PLY face records: uint8 3 then three little-endian int32
+//     (src/meshify.c:806-813);
+//   * base64 without line breaks, '=' padded (src/base64.c, RFC 1341), for
+//     the GIfTI data arrays.
+#include "n2m_device.cuh"
+#include "n2m_internal.h"
+
+namespace n2m {
+
+static unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535); }
+
+__global__ void k_f64_to_f32(const double *__restrict__ in, size_t n, float *__restrict__ out)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		out[i] = (float)in[i];
+}
+
+__global__ void k_ply_faces(const int *__restrict__ tris, size_t nt, uint8_t *__restrict__ out)
+{
+	for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nt; t += (size_t)gridDim.x * blockDim.x) {
+		uint8_t *o = out + 13 * t;
+		o[0] = 3;
+		for (int k = 0; k < 3; k++) {
+			const uint32_t v = (uint32_t)tris[3 * t + k];
+			o[1 + 4 * k] = (uint8_t)v;
+			o[2 + 4 * k] = (uint8_t)(v >> 8);
+			o[3 + 4 * k] = (uint8_t)(v >> 16);
+			o[4 + 4 * k] = (uint8_t)(v >> 24);
+		}
+	}
+}
+
+__device__ __forceinline__ uint8_t b64(uint32_t v)
+{
+	// 'A'-'Z', 'a'-'z', '0'-'9', '+', '/'
+	return (uint8_t)(v < 26 ? 'A' + v : v < 52 ? 'a' + (v - 26) : v < 62 ? '0' + (v - 52) : v == 62 ? '+' : '/');
+}
+
+// one thread per 3-byte group; the last group may be short
+__global__ void k_base64(const uint8_t *__restrict__ src, size_t len, uint8_t *__restrict__ out)
+{
+	const size_t ng = (len + 2) / 3;
+	for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < ng; g += (size_t)gridDim.x * blockDim.x) {
+		const size_t i = 3 * g, left = len - i;
+		const uint32_t a = src[i], b = left > 1 ? src[i + 1] : 0u, c = left > 2 ? src[i + 2] : 0u;
+		uint8_t *o = out + 4 * g;
+		o[0] = b64(a >> 2);
+		o[1] = b64(((a & 3u) << 4) | (b >> 4));
+		o[2] = left > 1 ? b64(((b & 15u) << 2) | (c >> 6)) : (uint8_t)'=';
+		o[3] = left > 2 ? b64(c & 63u) : (uint8_t)'=';
+	}
+}
+
+void launch_f64_to_f32(const double *in, size_t n, float *out, hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(n)), dim3(256), 0, s, in, n, out);
+}
+void launch_ply_faces(const int *tris, size_t nt, uint8_t *out, hipStream_t s)
+{
+	if (nt)
+		hipLaunchKernelGGL(k_ply_faces, dim3(blocks_for(nt)), dim3(256), 0, s, tris, nt, out);
+}
+void launch_base64(const uint8_t *src, size_t len, uint8_t *out, hipStream_t s)
+{
+	if (len)
+		hipLaunchKernelGGL(k_base64, dim3(blocks_for((len + 2) / 3)), dim3(256), 0, s, src, len, out);
+}
+
+} // namespace n2m

@@ -60,6 +60,10 @@ void launch_dilate_bits(const uint32_t *in, uint32_t *out, int nx, int ny, int n
 void launch_apply(float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax, const uint32_t *keep,
                   const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s);
 void launch_hist_range(const float *img, size_t n, uint32_t *stats, hipStream_t s);
+// mesh output encoding (k_encode.hip)
+void launch_f64_to_f32(const double *in, size_t n, float *out, hipStream_t s);
+void launch_ply_faces(const int *tris, size_t nt, uint8_t *out, hipStream_t s);
+void launch_base64(const uint8_t *src, size_t len, uint8_t *out, hipStream_t s);
 // raw NIfTI voxels -> float (datatype 2 uint8, 4 int16, 512 uint16, 16 float32);
 // returns -1 for another datatype
 int launch_decode(const void *raw, int datatype, size_t n, float slope, float inter, float *out, hipStream_t s);

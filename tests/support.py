@@ -167,6 +167,10 @@ class Ref(abi.MeshLib):
                                   np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
                                   f32, f32, f32]
         h.apply_sform.restype = None
+        h.save_mesh.argtypes = [C.c_char_p, np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'),
+                                np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'), C.c_int, C.c_int, C.c_bool]
+        h.base64_encode.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        h.base64_encode.restype = C.c_void_p
 
     def quick_smooth(self, img):
         nz, ny, nx = img.shape
@@ -180,6 +184,17 @@ class Ref(abi.MeshLib):
     def dilate(self, img):
         nz, ny, nx = img.shape
         self.lib.ref_dilate(img, (C.c_size_t * 3)(nx, ny, nz))
+
+    def save_mesh(self, fnm, tris, pts, gz):
+        return self.lib.save_mesh(fnm.encode(), np.ascontiguousarray(tris, np.int32),
+                                  np.ascontiguousarray(pts, np.float64), len(tris), len(pts), bool(gz))
+
+    def base64(self, data):
+        n = C.c_size_t()
+        ptr = self.lib.base64_encode(data, len(data), C.byref(n))
+        out = C.string_at(ptr, n.value)
+        self.lib.ref_free(C.c_void_p(ptr))
+        return out
 
     def apply_sform(self, tris, pts, sx, sy, sz):
         f = C.c_float * 4
