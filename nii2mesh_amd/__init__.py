@@ -57,6 +57,11 @@ class AtlasEntry(C.Structure):
 ATLAS_OK, ATLAS_EMPTY, ATLAS_FAILED = 0, 1, 2
 
 
+class _AtlasMesh(C.Structure):
+    _fields_ = [('label', C.c_int), ('status', C.c_int), ('nt', C.c_int), ('np', C.c_int),
+                ('t', C.c_void_p), ('p', C.c_void_p)]
+
+
 def _bind_ext(L):
     L.n2m_set_cubes.argtypes = [C.c_int]
     L.n2m_set_cubes.restype = C.c_int
@@ -107,6 +112,11 @@ def _bind_ext(L):
                                 np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'), C.c_int, C.c_int, C.c_bool]
     L.n2m_otsu.argtypes = [np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
                            C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.n2m_atlas.argtypes = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'), C.POINTER(C.c_short), C.c_int,
+                            C.c_bool, C.c_bool, C.c_int, f32, C.c_bool, C.POINTER(C.POINTER(_AtlasMesh)),
+                            C.POINTER(C.c_int)]
+    L.n2m_atlas_free.argtypes = [C.POINTER(_AtlasMesh), C.c_int]
+    L.n2m_atlas_free.restype = None
     L.n2m_ctx_atlas_count.argtypes = [C.c_void_p]
     L.n2m_ctx_atlas_entry.argtypes = [C.c_void_p, C.c_int, C.POINTER(AtlasEntry)]
     L.n2m_ctx_atlas_download.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
@@ -147,6 +157,31 @@ def save_mesh(fnm, tris, pts, gz=False):
     L = load()
     return L.n2m_save_mesh(str(fnm).encode(), np.ascontiguousarray(tris, np.int32),
                            np.ascontiguousarray(pts, np.float64), len(tris), len(pts), bool(gz))
+
+
+def atlas(img, originalMC=0, preSmooth=True, fillBubbles=False, post_smooth=0, srow=None, verbose=False):
+    """Host-memory atlas entry (n2m_atlas): returns (rc, [(label, status,
+    tris, pts)] for labels 1..n) with copies of the library's arrays."""
+    L = load()
+    img = np.ascontiguousarray(img, np.float32)
+    nz, ny, nx = img.shape
+    sr = None if srow is None else (C.c_float * 12)(*[float(v) for v in np.asarray(srow).ravel()])
+    out = C.POINTER(_AtlasMesh)()
+    n = C.c_int(0)
+    rc = L.n2m_atlas(img, (C.c_short * 3)(nx, ny, nz), int(originalMC), bool(preSmooth), bool(fillBubbles),
+                     int(post_smooth), sr, bool(verbose), C.byref(out), C.byref(n))
+    res = []
+    for k in range(n.value):
+        m = out[k]
+        t = p = None
+        if m.status == ATLAS_OK:
+            t = np.ctypeslib.as_array((C.c_int32 * (3 * m.nt)).from_address(m.t)).reshape(-1, 3).copy() \
+                if m.nt else np.zeros((0, 3), np.int32)
+            p = np.ctypeslib.as_array((C.c_double * (3 * m.np)).from_address(m.p)).reshape(-1, 3).copy()
+        res.append((m.label, m.status, t, p))
+    if n.value:
+        L.n2m_atlas_free(out, n)
+    return rc, res
 
 
 def otsu(hist, mode):

@@ -13,6 +13,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <atomic>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -1031,8 +1032,7 @@ void sform_and_smooth(n2m_ctx *C, Mesh &m, int postSmooth, const float *srow)
 // Crop tile of the atlas smoothing: N2M_ATLAS_TILE forces one; otherwise the
 // first smoothed atlas run on a device times every shape on its own crops
 // and keeps the fastest for the process.
-int g_atlas_tile[64];
-bool g_atlas_tile_init = false;
+std::atomic<int> g_atlas_tile[64]; // per device: tuned tile + 1, 0 until tuned
 
 template <typename Run> int atlas_tile_for(n2m_ctx *C, Run run)
 {
@@ -1043,14 +1043,10 @@ template <typename Run> int atlas_tile_for(n2m_ctx *C, Run run)
 			fail("N2M_ATLAS_TILE=%s: expected 0..%d", e, atlas_tile_count() - 1);
 		return t;
 	}
-	if (!g_atlas_tile_init) {
-		for (int &t : g_atlas_tile)
-			t = -1;
-		g_atlas_tile_init = true;
-	}
 	const int dev = C->device >= 0 && C->device < 64 ? C->device : 0;
-	if (g_atlas_tile[dev] >= 0)
-		return g_atlas_tile[dev];
+	const int known = g_atlas_tile[dev].load(std::memory_order_relaxed);
+	if (known > 0)
+		return known - 1;
 	hipEvent_t a = C->ev(), b = C->ev();
 	float best = 0.f;
 	int pick = 0;
@@ -1069,7 +1065,7 @@ template <typename Run> int atlas_tile_for(n2m_ctx *C, Run run)
 	}
 	C->evpool.push_back(a);
 	C->evpool.push_back(b);
-	g_atlas_tile[dev] = pick;
+	g_atlas_tile[dev].store(pick + 1, std::memory_order_relaxed);
 	return pick;
 }
 

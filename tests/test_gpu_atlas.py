@@ -195,3 +195,19 @@ def test_atlas_every_tile(gpu, oracle, cubes, tile, monkeypatch):
     want = expect(oracle, vol, cubes, 0, True, False, 0, None)
     rc, got = run_gpu(gpu, vol, cubes, 0, True, False, 0, None, False)
     compare(got, want)
+
+
+@pytest.mark.parametrize('cubes', [0, 1])
+def test_atlas_host_entry(gpu, oracle, cubes):
+    """n2m_atlas (host arrays, the CLI's form) gives the oracle's meshes."""
+    vol = small_atlas((27, 29, 31), 9, 12)
+    want = expect(oracle, vol, cubes, 0, True, False, 5, SROW_FLIP)
+    L = gpu.load()
+    prev = L.n2m_set_cubes(cubes)
+    try:
+        rc, res = gpu.atlas(vol, 0, True, False, 5, SROW_FLIP)
+    finally:
+        L.n2m_set_cubes(prev)
+    assert rc == 0
+    compare([(s, t, p) for _, s, t, p in res], want)
+    assert [lab for lab, _, _, _ in res] == list(range(1, len(want) + 1))
