@@ -205,7 +205,7 @@ __global__ void k_weld_keys(const double *__restrict__ pts, uint32_t n, uint32_t
 	if (i >= n)
 		return;
 	float d = (float)dist3(pts, pts + (size_t)i * 3);
-	keys[i] = __float_as_uint(d); // d >= +0: bit order is value order
+	keys[i] = f2ord(d); // the radix order of src/radixsort.c:31-35 (NaN included)
 	vals[i] = i;
 }
 
@@ -217,7 +217,7 @@ __global__ void k_weld_segflags(const uint32_t *__restrict__ keys, uint32_t n, u
 	const float tol = 0.00001f;
 	uint32_t f = 1;
 	if (i > 0)
-		f = (__uint_as_float(keys[i]) - __uint_as_float(keys[i - 1])) >= tol;
+		f = (ord2f(keys[i]) - ord2f(keys[i - 1])) >= tol;
 	flags[i] = f;
 }
 
@@ -245,9 +245,14 @@ __global__ void k_weld_greedy(const uint32_t *__restrict__ starts, uint32_t nseg
 	for (uint32_t i = b; i < e; i++) {
 		if (lcls[i] >= 0)
 			continue;
-		const float k0 = __uint_as_float(keys[i]);
+		const float k0 = ord2f(keys[i]);
 		const double *seed = pts + (size_t)vals[i] * 3;
-		for (uint32_t j = i; j < e && (__uint_as_float(keys[j]) - k0) < tol; j++)
+		// the seed is always in its own class: for a finite position its
+		// distance to itself is 0 < tol anyway; a NaN position matches
+		// nothing, and the reference would then leave it unassigned
+		// (old2new -1, src/meshify.c:66-79)
+		lcls[i] = ns;
+		for (uint32_t j = i + 1; j < e && (ord2f(keys[j]) - k0) < tol; j++)
 			if (dist3(seed, pts + (size_t)vals[j] * 3) < (double)tol)
 				lcls[j] = ns;
 		ns++;
@@ -971,7 +976,7 @@ __global__ void k_atlas_weld_keys(const double *__restrict__ pts, uint32_t n, co
 		return;
 	int t = task_of(vbase, ntasks, i);
 	float d = (float)dist3(pts + (size_t)vbase[t] * 3, pts + (size_t)i * 3);
-	keys[i] = __float_as_uint(d);
+	keys[i] = f2ord(d);
 	vals[i] = i;
 	vtask[i] = (uint32_t)t;
 }
@@ -1007,7 +1012,7 @@ __global__ void k_atlas_weld_segflags(const uint32_t *__restrict__ keys, const u
 	const float tol = 0.00001f;
 	uint32_t f = 1;
 	if (i > 0 && tk[i] == tk[i - 1])
-		f = (__uint_as_float(keys[i]) - __uint_as_float(keys[i - 1])) >= tol;
+		f = (ord2f(keys[i]) - ord2f(keys[i - 1])) >= tol;
 	flags[i] = f;
 }
 

@@ -738,13 +738,22 @@ typedef struct {
 	uint32_t idx;
 } keyidx;
 
+/* The reference radix-sorts the float keys by their "flipped" bit pattern
+ * (src/radixsort.c:31-35, the usual sign-magnitude to unsigned map): a total
+ * order that also places NaN keys (negative NaN first, positive NaN last). */
+static uint32_t key_order(float f)
+{
+	uint32_t u;
+	memcpy(&u, &f, 4);
+	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
 static int keyidx_cmp(const void *pa, const void *pb)
 {
 	const keyidx *a = (const keyidx *)pa, *b = (const keyidx *)pb;
-	if (a->key < b->key)
-		return -1;
-	if (a->key > b->key)
-		return 1;
+	const uint32_t ka = key_order(a->key), kb = key_order(b->key);
+	if (ka != kb)
+		return ka < kb ? -1 : 1;
 	return a->idx < b->idx ? -1 : (a->idx > b->idx);
 }
 
@@ -769,7 +778,12 @@ static int weld_core(vec3d **inpt, vec3i *tris, int npt, int ntri, int *cls_out)
 		if (cls[s[i].idx] >= 0)
 			continue;
 		vec3d seed = pts[s[i].idx];
-		for (int j = i; j < npt && (s[j].key - s[i].key) < tol; j++)
+		/* The seed always joins its own class.  For a finite position this
+		 * is what :72-74 do (distance 0 < tol); a NaN position matches
+		 * nothing, itself included, and the reference then leaves
+		 * old2new at -1 and stores to pts[-1] at :99-100. */
+		cls[s[i].idx] = nnew;
+		for (int j = i + 1; j < npt && (s[j].key - s[i].key) < tol; j++)
 			if (dist3(seed, pts[s[j].idx]) < tol)
 				cls[s[j].idx] = nnew;
 		nnew++;
@@ -782,10 +796,13 @@ static int weld_core(vec3d **inpt, vec3i *tris, int npt, int ntri, int *cls_out)
 		free(cls);
 		return npt;
 	}
+	/* A corner without a vertex (-1: the Lewiner tables found no vertex on
+	 * the edge, src/MarchingCubes.c:829-833) stays -1; the reference reads
+	 * old2new[-1] there (src/meshify.c:88-92). */
 	for (int i = 0; i < ntri; i++) {
-		tris[i].x = cls[tris[i].x];
-		tris[i].y = cls[tris[i].y];
-		tris[i].z = cls[tris[i].z];
+		tris[i].x = (tris[i].x >= 0 && tris[i].x < npt) ? cls[tris[i].x] : -1;
+		tris[i].y = (tris[i].y >= 0 && tris[i].y < npt) ? cls[tris[i].y] : -1;
+		tris[i].z = (tris[i].z >= 0 && tris[i].z < npt) ? cls[tris[i].z] : -1;
 	}
 	vec3d *out = (vec3d *)malloc((size_t)nnew * sizeof(vec3d));
 	for (int i = 0; i < npt; i++) /* last member in index order wins, :99-100 */
@@ -810,6 +827,10 @@ int orc_remove_degenerate_triangles(vec3d *pts, vec3i **intris, int ntri)
 	int keep = 0;
 	uint8_t *bad = (uint8_t *)calloc((size_t)ntri + 1, 1);
 	for (int i = 0; i < ntri; i++) {
+		if (tris[i].x < 0 || tris[i].y < 0 || tris[i].z < 0) {
+			bad[i] = 1; /* a corner without a vertex (the reference reads pts[-1]) */
+			continue;
+		}
 		double l = dist3(pts[tris[i].x], pts[tris[i].y]);
 		double m = dist3(pts[tris[i].x], pts[tris[i].z]);
 		double n = dist3(pts[tris[i].y], pts[tris[i].z]);

@@ -267,3 +267,25 @@ def sphere_field(n, radius, seed, nharm=4, amp=1.5, noise=0.0):
 def random_mask(shape, seed, p=0.5):
     rng = np.random.Generator(np.random.PCG64(seed))
     return (rng.uniform(size=shape) < p).astype(np.float32)
+
+
+def nan_volume(shape, seed, frac=0.002):
+    """blobs() with a fraction of voxels set to NaN, as in masked float maps;
+    voxel 0 is NaN too (the range loops of the reference start from it)."""
+    img = blobs(shape, seed, noise=2.0)
+    rng = np.random.default_rng(seed)
+    img[rng.uniform(size=img.shape) < frac] = np.nan
+    img[0, 0, 0] = np.nan
+    return img
+
+
+def nan_eq(a, b):
+    """Bitwise equality of float arrays, except that any NaN equals any NaN
+    (the payload a NaN carries through arithmetic is not part of the
+    contract)."""
+    a, b = np.asarray(a), np.asarray(b)
+    na, nb = np.isnan(a), np.isnan(b)
+    if a.shape != b.shape or a.dtype != b.dtype or not np.array_equal(na, nb):
+        return False
+    u = np.uint64 if a.dtype.itemsize == 8 else np.uint32
+    return np.array_equal(np.where(na, 0, a).view(u), np.where(nb, 0, b).view(u))

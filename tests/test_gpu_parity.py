@@ -101,6 +101,32 @@ def test_unify_and_degenerate(gpu, oracle, shape, flavour):
     assert _eq(d1, d2)
 
 
+@pytest.mark.parametrize('first_nan', [False, True])
+def test_unify_nan_rows(gpu, oracle, first_nan):
+    """Weld with vertices at NaN positions: each is a class of its own (its
+    key sorts last); with vertex 0 at NaN every key is NaN and nothing
+    merges."""
+    import nii2mesh_amd.abi as abi
+    rc, tris, pts = _raw_mesh(oracle, (24, 31, 40), 5, 'classic')
+    assert rc == 0
+    rng = np.random.Generator(np.random.PCG64(3))
+    n, k = len(pts), 90
+    at = np.sort(rng.choice(np.arange(0 if first_nan else 1, n + k), size=k, replace=False))
+    if first_nan:
+        at[0] = 0
+    isnan = np.zeros(n + k, bool)
+    isnan[at] = True
+    newidx = np.flatnonzero(~isnan)
+    pts2 = np.full((n + k, 3), np.nan)
+    pts2[newidx] = pts
+    tris2 = np.concatenate([newidx[tris], at[:k // 3 * 3].reshape(-1, 3)]).astype(np.int32)
+    gu = abi.bind_unify(gpu.load(), 'n2m_unify_vertices')
+    p1, t1 = gu(pts2, tris2)
+    p2, t2 = oracle.unify(pts2, tris2)
+    assert support.nan_eq(p1, p2) and _eq(t1, t2)
+    assert (len(p1) == n + k) if first_nan else (len(p1) < n + k)
+
+
 @pytest.mark.parametrize('flavour', ['classic', 'lewiner'])
 @pytest.mark.parametrize('lock', [True, False])
 def test_hc_smooth(gpu, oracle, flavour, lock):
@@ -157,6 +183,25 @@ def test_meshify_blobs(gpu, oracle, shape, flags, flavour):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert _eq(ra[1], rb[1])
     assert _eq(ra[2], rb[2])
+
+
+@pytest.mark.parametrize('flags', [(True, True, True), (False, False, False), (False, True, False)])
+@pytest.mark.parametrize('flavour', ['classic', 'lewiner'])
+def test_meshify_nan_volume(gpu, oracle, flags, flavour):
+    """Volumes with NaN voxels: a vertex at a NaN position welds alone, and a
+    triangle whose corner the Lewiner tables left without a vertex (-1) is
+    dropped by the degenerate filter -- the rules of the oracle, where the
+    reference indexes old2new[-1] (src/meshify.c:66-79, :88-100)."""
+    img = support.nan_volume((26, 28, 30), 3)
+    iso = float(np.float32(np.nanpercentile(img, 60)))
+    cubes = 0 if flavour == 'classic' else 1
+    a, b = img.copy(), img.copy()
+    ra = gpu.lib(cubes).meshify(a, iso, *flags)
+    rb = oracle.meshify(b, iso, *flags, cubes=cubes)
+    assert ra[0] == rb[0] == 0
+    assert support.nan_eq(a, b)
+    assert _eq(ra[1], rb[1]) and support.nan_eq(ra[2], rb[2])
+    assert ra[1].min() >= 0 and ra[1].max() < len(ra[2])
 
 
 def test_meshify_iso_fallback_and_failures(gpu, oracle):
