@@ -199,13 +199,23 @@ __device__ __forceinline__ double dist3(const double *a, const double *b)
 	return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
+// Sort key of a weld distance: the radix order of src/radixsort.c:31-35.
+// A NaN distance is keyed without its sign (after every number, by
+// payload): the sign a NaN picks up in a difference is not portable (x86
+// returns the NaN operand, the GPU's subtract negates it), and a positive
+// NaN is what the reference gets from positive NaN voxels.
+__device__ __forceinline__ uint32_t weld_key(float d)
+{
+	return d != d ? (__float_as_uint(d) | 0x80000000u) : f2ord(d);
+}
+
 __global__ void k_weld_keys(const double *__restrict__ pts, uint32_t n, uint32_t *keys, uint32_t *vals)
 {
 	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
 	float d = (float)dist3(pts, pts + (size_t)i * 3);
-	keys[i] = f2ord(d); // the radix order of src/radixsort.c:31-35 (NaN included)
+	keys[i] = weld_key(d);
 	vals[i] = i;
 }
 
@@ -976,7 +986,7 @@ __global__ void k_atlas_weld_keys(const double *__restrict__ pts, uint32_t n, co
 		return;
 	int t = task_of(vbase, ntasks, i);
 	float d = (float)dist3(pts + (size_t)vbase[t] * 3, pts + (size_t)i * 3);
-	keys[i] = f2ord(d);
+	keys[i] = weld_key(d);
 	vals[i] = i;
 	vtask[i] = (uint32_t)t;
 }

@@ -739,12 +739,17 @@ typedef struct {
 } keyidx;
 
 /* The reference radix-sorts the float keys by their "flipped" bit pattern
- * (src/radixsort.c:31-35, the usual sign-magnitude to unsigned map): a total
- * order that also places NaN keys (negative NaN first, positive NaN last). */
+ * (src/radixsort.c:31-35, the usual sign-magnitude to unsigned map), a total
+ * order that also places NaN keys.  A NaN key is ordered without its sign
+ * (after every number, by payload): the sign a NaN takes through a
+ * difference is not portable, and positive NaN voxels give the reference
+ * positive NaN keys. */
 static uint32_t key_order(float f)
 {
 	uint32_t u;
 	memcpy(&u, &f, 4);
+	if (f != f)
+		return u | 0x80000000u;
 	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
