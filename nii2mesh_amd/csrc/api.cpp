@@ -120,6 +120,9 @@ enum Slot {
 	S_HC_TRIS,
 	S_HC_INV,
 	S_HC_OUT,
+	S_HC_SLOTS,
+	S_HC_HALO,
+	S_HC_HCNT,
 	S_AT_STATS,
 	S_AT_TASKS,
 	S_AT_BMAP,
@@ -171,6 +174,7 @@ enum Small {
 	SM_HSTAT = 24, // 3 words
 	SM_PBOX = 32,  // 6 words: float box of vertex positions
 	SM_FLAG = 40,  // 1 word
+	SM_HALO = 44,  // 1 word: a smoothing block overflowed its halo
 	SM_WORDS = 64
 };
 
@@ -843,6 +847,32 @@ uint8_t *border_flags(n2m_ctx *C, const Csr &c, const int *tris, uint32_t nv, ui
 	return border;
 }
 
+// Halo form of the smoothing CSR (k_halo_build).  on = false when some
+// vertex block has too many outside neighbours for its LDS halo (or
+// N2M_HC_HALO=0): the window kernels are used then.
+struct Halo {
+	bool on = false;
+	uint32_t *slots = nullptr, *halo = nullptr, *hcount = nullptr;
+};
+
+Halo build_halo(n2m_ctx *C, const Csr &c, uint32_t nv, uint32_t nt)
+{
+	Halo h;
+	const char *e = getenv("N2M_HC_HALO");
+	if ((e && *e == '0') || nv == 0)
+		return h;
+	hipStream_t s = C->stream;
+	h.slots = C->buf<uint32_t>(S_HC_SLOTS, (size_t)3 * nt + 1);
+	h.halo = C->buf<uint32_t>(S_HC_HALO, halo_words(nv));
+	h.hcount = C->buf<uint32_t>(S_HC_HCNT, halo_blocks(nv));
+	HIPX(hipMemsetAsync(C->d_small + SM_HALO, 0, 4, s));
+	C->tbegin("hc_halo", 8.0 * nv + 12.0 * 3 * nt);
+	launch_halo_build(c.start, c.pairs, nv, h.halo, h.hcount, h.slots, C->d_small + SM_HALO, s);
+	C->tend();
+	h.on = C->fetch(SM_HALO, 1)[0] == 0;
+	return h;
+}
+
 // laplacian_smoothHC on a device mesh in place, src/quadric.c:343-394
 void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint32_t nt, double alpha,
                       double beta, int iter, bool lockEdges)
@@ -853,18 +883,27 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 	Relabel R = morton_relabel(C, pts, tris, nv, nt);
 	const int *T = R.on ? R.tris : tris;
 	Csr c = build_csr(C, T, nv, nt);
+	const Halo H = iter > 0 ? build_halo(C, c, nv, nt) : Halo();
 	const size_t n3 = 3 * (size_t)nv;
 	double *o = C->buf<double>(S_HC_O, n3), *p = C->buf<double>(S_HC_P, n3);
 	double *pn = C->buf<double>(S_HC_PN, n3), *bq = C->buf<double>(S_HC_B, n3);
 	launch_hc_load(pts, R.on ? R.perm : nullptr, nv, o, s);
 	HIPX(hipMemcpyAsync(p, o, 8 * n3, hipMemcpyDeviceToDevice, s));
-	const double pair_bytes = 8.0 * 3 * nt;
+	// algorithmic bytes: CSR offsets, one pair word per reference (4 bytes
+	// as LDS slots, 8 as vertex ids), each position array once, the outputs
+	const double pair_bytes = (H.on ? 4.0 : 8.0) * 3 * nt;
 	for (int it = 0; it < iter; it++) {
 		C->tbegin("hc_a", 4.0 * nv + pair_bytes + 24.0 * nv * 2 + 48.0 * nv);
-		launch_hc_a(c.start, c.pairs, o, p, pn, bq, nv, alpha, s);
+		if (H.on)
+			launch_hc_a_halo(c.start, H.slots, H.halo, H.hcount, o, p, pn, bq, nv, alpha, s);
+		else
+			launch_hc_a(c.start, c.pairs, o, p, pn, bq, nv, alpha, s);
 		C->tend();
 		C->tbegin("hc_b", 4.0 * nv + pair_bytes + 24.0 * nv * 2 + 24.0 * nv);
-		launch_hc_b(c.start, c.pairs, pn, bq, p, nv, beta, s);
+		if (H.on)
+			launch_hc_b_halo(c.start, H.slots, H.halo, H.hcount, pn, bq, p, nv, beta, s);
+		else
+			launch_hc_b(c.start, c.pairs, pn, bq, p, nv, beta, s);
 		C->tend();
 	}
 	uint8_t *border = lockEdges ? border_flags(C, c, T, nv, nt) : nullptr;
@@ -879,13 +918,17 @@ void laplace_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint3
 	hipStream_t s = C->stream;
 	Relabel R = morton_relabel(C, pts, tris, nv, nt);
 	Csr c = build_csr(C, R.on ? R.tris : tris, nv, nt);
+	const Halo H = build_halo(C, c, nv, nt);
 	const size_t n3 = 3 * (size_t)nv;
 	double *p = C->buf<double>(S_HC_P, n3), *q = C->buf<double>(S_HC_PN, n3);
 	launch_hc_load(pts, R.on ? R.perm : nullptr, nv, q, s);
 	double *cur = q, *nxt = p;
 	for (int it = 0; it < iter; it++) {
-		C->tbegin("laplace", 4.0 * nv + 8.0 * 3 * nt + 48.0 * nv);
-		launch_laplace(c.start, c.pairs, cur, nxt, nv, s);
+		C->tbegin("laplace", 4.0 * nv + (H.on ? 4.0 : 8.0) * 3 * nt + 48.0 * nv);
+		if (H.on)
+			launch_laplace_halo(c.start, H.slots, H.halo, H.hcount, cur, nxt, nv, s);
+		else
+			launch_laplace(c.start, c.pairs, cur, nxt, nv, s);
 		C->tend();
 		double *t = cur;
 		cur = nxt;

@@ -652,6 +652,280 @@ k_laplace(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, co
 	pout[v] = laplace_at<K>(start[v], start[v + 1], pairs, w, w.get(v));
 }
 
+// ---------------------------------------------------------------------------
+// Halo form of the same gathers.  A workgroup owns HB consecutive vertices
+// (Morton order) and stages their positions plus its halo -- the sorted,
+// deduplicated neighbours outside the block -- in LDS.  Every gather is then
+// an LDS read, and a neighbour outside the block is fetched once per
+// workgroup instead of once per reference.  A pair is two 16-bit LDS slots
+// (slot < HB: vertex blk*HB + slot; otherwise halo entry slot - HB), 4 bytes
+// instead of 8.  Built once per mesh by k_halo_build; a block with more than
+// HRAW outside references or HCAP distinct outside neighbours raises
+// *overflow, and the caller keeps the window kernels above.
+// ---------------------------------------------------------------------------
+constexpr int HB = 1024, HCAP = 1024, HRAW = 4096;
+
+// slot of vertex x for block base `base` whose halo is the sorted h[0, H)
+__device__ __forceinline__ uint32_t halo_slot(const uint32_t *h, uint32_t H, uint32_t base, uint32_t x)
+{
+	if (x - base < (uint32_t)HB)
+		return x - base;
+	uint32_t lo = 0, hi = H; // first entry >= x (x is present)
+	while (lo < hi) {
+		uint32_t mid = (lo + hi) >> 1;
+		if (h[mid] < x)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return HB + lo;
+}
+
+__global__ void __launch_bounds__(HB)
+k_halo_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, uint32_t nv,
+             uint32_t *__restrict__ halo, uint32_t *__restrict__ hcount, uint32_t *__restrict__ slots,
+             uint32_t *__restrict__ overflow)
+{
+	__shared__ uint32_t raw[HRAW];
+	__shared__ uint32_t scratch[HB / N2M_WAVE + 1];
+	__shared__ uint32_t s_n;
+	const uint32_t blk = blockIdx.x, base = blk * HB, v = base + threadIdx.x;
+	if (threadIdx.x == 0)
+		s_n = 0;
+	__syncthreads();
+	uint32_t b = 0, e = 0;
+	if (v < nv) {
+		b = start[v];
+		e = start[v + 1];
+	}
+	// 1. every reference to a vertex outside the block
+	for (uint32_t k = b; k < e; k++) {
+		const int2 q = pairs[k];
+		const uint32_t x0 = (uint32_t)q.x, x1 = (uint32_t)q.y;
+		if (x0 - base >= (uint32_t)HB) {
+			uint32_t i = atomicAdd(&s_n, 1u);
+			if (i < HRAW)
+				raw[i] = x0;
+		}
+		if (x1 - base >= (uint32_t)HB) {
+			uint32_t i = atomicAdd(&s_n, 1u);
+			if (i < HRAW)
+				raw[i] = x1;
+		}
+	}
+	__syncthreads();
+	const uint32_t n = s_n;
+	if (n > HRAW) { // uniform: every thread read the same count
+		if (threadIdx.x == 0)
+			atomicOr(overflow, 1u);
+		return;
+	}
+	// 2. bitonic sort of raw[0, P), padded with ~0
+	uint32_t P = 1;
+	while (P < n)
+		P <<= 1;
+	for (uint32_t i = n + threadIdx.x; i < P; i += HB)
+		raw[i] = 0xffffffffu;
+	__syncthreads();
+	for (uint32_t size = 2; size <= P; size <<= 1)
+		for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+			for (uint32_t i = threadIdx.x; i < P; i += HB) {
+				const uint32_t j = i ^ stride;
+				if (j > i) {
+					const uint32_t a = raw[i], c = raw[j];
+					if ((a > c) == ((i & size) == 0)) {
+						raw[i] = c;
+						raw[j] = a;
+					}
+				}
+			}
+			__syncthreads();
+		}
+	// 3. keep the first of each run of equal ids (4 entries per thread;
+	// HRAW = 4 * HB); compacted in place after the scan's barriers
+	uint32_t val[4], cnt = 0;
+	bool keep[4];
+#pragma unroll
+	for (int q = 0; q < 4; q++) {
+		const uint32_t i = 4 * threadIdx.x + q;
+		val[q] = i < n ? raw[i] : 0u;
+		keep[q] = i < n && (i == 0 || raw[i] != raw[i - 1]);
+		cnt += keep[q];
+	}
+	uint32_t H;
+	uint32_t off = block_excl_scan<HB>(cnt, scratch, &H);
+#pragma unroll
+	for (int q = 0; q < 4; q++)
+		if (keep[q])
+			raw[off++] = val[q];
+	__syncthreads();
+	if (H > HCAP) { // uniform
+		if (threadIdx.x == 0)
+			atomicOr(overflow, 1u);
+		return;
+	}
+	for (uint32_t i = threadIdx.x; i < H; i += HB)
+		halo[(size_t)blk * HCAP + i] = raw[i];
+	if (threadIdx.x == 0)
+		hcount[blk] = H;
+	// 4. the pairs as LDS slots
+	for (uint32_t k = b; k < e; k++) {
+		const int2 q = pairs[k];
+		slots[k] = halo_slot(raw, H, base, (uint32_t)q.x) | (halo_slot(raw, H, base, (uint32_t)q.y) << 16);
+	}
+}
+
+// the first K slots of a vertex, loaded before the LDS staging so that their
+// latency overlaps it
+template <int K> struct SlotRefs {
+	uint32_t b, e, q[K > 0 ? K : 1];
+	__device__ __forceinline__ void load(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots,
+	                                     uint32_t v, bool live)
+	{
+		b = live ? start[v] : 0u;
+		e = live ? start[v + 1] : 0u;
+#pragma unroll
+		for (int k = 0; k < K; k++)
+			q[k] = b + k < e ? slots[b + k] : 0u;
+	}
+};
+
+__device__ __forceinline__ d3 lds_row(const lds_double *L, uint32_t s)
+{
+	d3 r;
+	r.x = L[3 * s];
+	r.y = L[3 * s + 1];
+	r.z = L[3 * s + 2];
+	return r;
+}
+
+// block rows and halo rows of p into LDS (has a barrier)
+__device__ __forceinline__ void stage_halo(double *lds, const d3 *__restrict__ p, uint32_t nv, uint32_t blk,
+                                           const uint32_t *__restrict__ halo, uint32_t H)
+{
+	const double *src = reinterpret_cast<const double *>(p);
+	const size_t lo = 3 * (size_t)blk * HB, hi = 3 * (size_t)nv;
+	for (int i = threadIdx.x; i < 3 * HB; i += HB) {
+		const size_t e = lo + i;
+		lds[i] = e < hi ? src[e] : 0.0;
+	}
+	const uint32_t *h = halo + (size_t)blk * HCAP;
+	for (uint32_t i = threadIdx.x; i < H; i += HB) {
+		const d3 q = p[h[i]];
+		lds[3 * (HB + i)] = q.x;
+		lds[3 * (HB + i) + 1] = q.y;
+		lds[3 * (HB + i) + 2] = q.z;
+	}
+	__syncthreads();
+}
+
+// laplace_at() over LDS slots: same terms, same order
+template <int K>
+__device__ __forceinline__ d3 laplace_slots(const SlotRefs<K> &r, const uint32_t *__restrict__ slots,
+                                            const lds_double *L, const d3 self)
+{
+	if (r.e <= r.b)
+		return self;
+	double sx = 0.0, sy = 0.0, sz = 0.0;
+	const uint32_t n = r.e - r.b;
+#pragma unroll
+	for (int k = 0; k < K; k++) {
+		if ((uint32_t)k < n) {
+			const d3 P = lds_row(L, r.q[k] & 0xffffu), Q = lds_row(L, r.q[k] >> 16);
+			sx = sx + (P.x + Q.x);
+			sy = sy + (P.y + Q.y);
+			sz = sz + (P.z + Q.z);
+		}
+	}
+	for (uint32_t i = r.b + K; i < r.e; i++) {
+		const uint32_t s = slots[i];
+		const d3 P = lds_row(L, s & 0xffffu), Q = lds_row(L, s >> 16);
+		sx = sx + (P.x + Q.x);
+		sy = sy + (P.y + Q.y);
+		sz = sz + (P.z + Q.z);
+	}
+	const double num = (double)(2 * n);
+	d3 out;
+	out.x = sx / num;
+	out.y = sy / num;
+	out.z = sz / num;
+	return out;
+}
+
+constexpr int HK = 8; // slots prefetched per vertex
+
+// k_hc_a in halo form
+__global__ void __launch_bounds__(HB)
+k_hc_a_h(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots, const uint32_t *__restrict__ halo,
+         const uint32_t *__restrict__ hcount, const d3 *__restrict__ o, const d3 *__restrict__ p,
+         d3 *__restrict__ pn, d3 *__restrict__ bq, uint32_t nv, double alpha, double alpha1)
+{
+	__shared__ double lds[3 * (HB + HCAP)];
+	const uint32_t blk = blockIdx.x, v = blk * HB + threadIdx.x;
+	const bool live = v < nv;
+	SlotRefs<HK> r;
+	r.load(start, slots, v, live);
+	d3 ov = {0.0, 0.0, 0.0};
+	if (live)
+		ov = o[v];
+	stage_halo(lds, p, nv, blk, halo, hcount[blk]);
+	if (!live)
+		return;
+	const lds_double *L = (const lds_double *)lds;
+	const d3 pv = lds_row(L, threadIdx.x);
+	const d3 Lp = laplace_slots<HK>(r, slots, L, pv);
+	d3 bb;
+	bb.x = Lp.x - (ov.x * alpha + pv.x * alpha1);
+	bb.y = Lp.y - (ov.y * alpha + pv.y * alpha1);
+	bb.z = Lp.z - (ov.z * alpha + pv.z * alpha1);
+	pn[v] = Lp;
+	bq[v] = bb;
+}
+
+// k_hc_b in halo form
+__global__ void __launch_bounds__(HB)
+k_hc_b_h(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots, const uint32_t *__restrict__ halo,
+         const uint32_t *__restrict__ hcount, const d3 *__restrict__ pn, const d3 *__restrict__ bq,
+         d3 *__restrict__ pout, uint32_t nv, double beta, double beta1)
+{
+	__shared__ double lds[3 * (HB + HCAP)];
+	const uint32_t blk = blockIdx.x, v = blk * HB + threadIdx.x;
+	const bool live = v < nv;
+	SlotRefs<HK> r;
+	r.load(start, slots, v, live);
+	d3 pv = {0.0, 0.0, 0.0};
+	if (live)
+		pv = pn[v];
+	stage_halo(lds, bq, nv, blk, halo, hcount[blk]);
+	if (!live)
+		return;
+	const lds_double *L = (const lds_double *)lds;
+	const d3 bv = lds_row(L, threadIdx.x);
+	const d3 Lb = laplace_slots<HK>(r, slots, L, bv);
+	d3 out;
+	out.x = pv.x - (bv.x * beta + Lb.x * beta1);
+	out.y = pv.y - (bv.y * beta + Lb.y * beta1);
+	out.z = pv.z - (bv.z * beta + Lb.z * beta1);
+	pout[v] = out;
+}
+
+// k_laplace in halo form
+__global__ void __launch_bounds__(HB)
+k_laplace_h(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots, const uint32_t *__restrict__ halo,
+            const uint32_t *__restrict__ hcount, const d3 *__restrict__ p, d3 *__restrict__ pout, uint32_t nv)
+{
+	__shared__ double lds[3 * (HB + HCAP)];
+	const uint32_t blk = blockIdx.x, v = blk * HB + threadIdx.x;
+	const bool live = v < nv;
+	SlotRefs<HK> r;
+	r.load(start, slots, v, live);
+	stage_halo(lds, p, nv, blk, halo, hcount[blk]);
+	if (!live)
+		return;
+	const lds_double *L = (const lds_double *)lds;
+	pout[v] = laplace_slots<HK>(r, slots, L, lds_row(L, threadIdx.x));
+}
+
 // gather through a permutation: dst[i] = src[perm ? perm[i] : i]
 __global__ void k_hc_load(const d3 *__restrict__ src, const uint32_t *__restrict__ perm, uint32_t n, d3 *__restrict__ dst)
 {
@@ -743,6 +1017,39 @@ void launch_laplace(const uint32_t *start, const int2 *pairs, const double *p, d
 		N2M_HC_SHAPES(X)
 #undef X
 	}
+}
+
+uint32_t halo_blocks(uint32_t nv) { return (nv + HB - 1) / HB; }
+size_t halo_words(uint32_t nv) { return (size_t)halo_blocks(nv) * HCAP; }
+
+void launch_halo_build(const uint32_t *start, const int2 *pairs, uint32_t nv, uint32_t *halo, uint32_t *hcount,
+                       uint32_t *slots, uint32_t *overflow, hipStream_t s)
+{
+	if (nv)
+		hipLaunchKernelGGL(k_halo_build, dim3(halo_blocks(nv)), dim3(HB), 0, s, start, pairs, nv, halo, hcount, slots,
+		                   overflow);
+}
+void launch_hc_a_halo(const uint32_t *start, const uint32_t *slots, const uint32_t *halo, const uint32_t *hcount,
+                      const double *o, const double *p, double *pn, double *bq, uint32_t nv, double alpha,
+                      hipStream_t s)
+{
+	if (nv)
+		hipLaunchKernelGGL(k_hc_a_h, dim3(halo_blocks(nv)), dim3(HB), 0, s, start, slots, halo, hcount,
+		                   (const d3 *)o, (const d3 *)p, (d3 *)pn, (d3 *)bq, nv, alpha, 1.0 - alpha);
+}
+void launch_hc_b_halo(const uint32_t *start, const uint32_t *slots, const uint32_t *halo, const uint32_t *hcount,
+                      const double *pn, const double *bq, double *pout, uint32_t nv, double beta, hipStream_t s)
+{
+	if (nv)
+		hipLaunchKernelGGL(k_hc_b_h, dim3(halo_blocks(nv)), dim3(HB), 0, s, start, slots, halo, hcount,
+		                   (const d3 *)pn, (const d3 *)bq, (d3 *)pout, nv, beta, 1.0 - beta);
+}
+void launch_laplace_halo(const uint32_t *start, const uint32_t *slots, const uint32_t *halo,
+                         const uint32_t *hcount, const double *p, double *pout, uint32_t nv, hipStream_t s)
+{
+	if (nv)
+		hipLaunchKernelGGL(k_laplace_h, dim3(halo_blocks(nv)), dim3(HB), 0, s, start, slots, halo, hcount,
+		                   (const d3 *)p, (d3 *)pout, nv);
 }
 void launch_hc_load(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s)
 {

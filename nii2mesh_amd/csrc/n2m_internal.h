@@ -179,6 +179,20 @@ void launch_hc_b(const uint32_t *start, const int2 *pairs, const double *pn, con
                  uint32_t nv, double beta, hipStream_t s);
 void launch_laplace(const uint32_t *start, const int2 *pairs, const double *p, double *pout, uint32_t nv,
                     hipStream_t s);
+// halo form of the smoothing gathers: per 1024-vertex block, the sorted
+// distinct neighbours outside the block (halo_words(nv) words, stride 1024)
+// and their count; the pairs as two 16-bit LDS slots (3*nt words)
+uint32_t halo_blocks(uint32_t nv);
+size_t halo_words(uint32_t nv);
+void launch_halo_build(const uint32_t *start, const int2 *pairs, uint32_t nv, uint32_t *halo, uint32_t *hcount,
+                       uint32_t *slots, uint32_t *overflow, hipStream_t s);
+void launch_hc_a_halo(const uint32_t *start, const uint32_t *slots, const uint32_t *halo, const uint32_t *hcount,
+                      const double *o, const double *p, double *pn, double *bq, uint32_t nv, double alpha,
+                      hipStream_t s);
+void launch_hc_b_halo(const uint32_t *start, const uint32_t *slots, const uint32_t *halo, const uint32_t *hcount,
+                      const double *pn, const double *bq, double *pout, uint32_t nv, double beta, hipStream_t s);
+void launch_laplace_halo(const uint32_t *start, const uint32_t *slots, const uint32_t *halo,
+                         const uint32_t *hcount, const double *p, double *pout, uint32_t nv, hipStream_t s);
 // smoothing state: 3 doubles per vertex, optionally gathered / scattered
 // through a vertex permutation
 void launch_hc_load(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s);
