@@ -146,6 +146,36 @@ def test_hc_smooth(gpu, oracle, flavour, lock):
     assert np.array_equal(c, d)
 
 
+@pytest.mark.parametrize('form', ['window', 'halo'])
+@pytest.mark.parametrize('iters', [1, 2, 12])
+@pytest.mark.parametrize('lock', [True, False])
+def test_hc_gather_forms(gpu, oracle, monkeypatch, form, iters, lock):
+    """Both forms of the HC gathers -- LDS window with global escapes, and
+    LDS block + halo with 16-bit slots -- give the oracle's
+    laplacian_smoothHC bit for bit on a Morton-relabelled mesh of several
+    blocks.  The stage statistics show which form ran."""
+    import torch
+    monkeypatch.setenv('N2M_HC_HALO', '0' if form == 'window' else '1')
+    # 34865 vertices: Morton-relabelled (>= 32768), 35 blocks of 1024
+    img = support.blobs((80, 84, 88), 4, noise=2.0)
+    iso = float(np.float32(np.percentile(img, 50)))
+    ctx = gpu.DeviceContext()
+    ctx.timing(True)
+    d = torch.from_numpy(img).cuda()
+    torch.cuda.synchronize()  # the library runs on its own stream
+    prm = ctx.params(iso, cubes=1, preSmooth=True, onlyLargest=False, fillBubbles=False)
+    assert ctx.meshify(d.data_ptr(), (88, 84, 80), prm) == 0
+    assert ctx.smooth_hc(0, 0.1, 0.5, iters, lock) == 0
+    t, p = ctx.mesh(0)
+    stats = ctx.kernel_stats()
+    ctx.close()
+    rc, to, po = oracle.meshify(img.copy(), iso, True, False, False, cubes=1)
+    assert rc == 0 and len(po) >= 32768
+    oracle.laplacian_smoothHC(po, to, 0.1, 0.5, iters, lock)
+    assert _eq(t, to) and np.array_equal(p, po)
+    assert ('hc_halo' in stats) == (form == 'halo')
+
+
 @pytest.mark.parametrize('seed', [11, 12, 13])
 @pytest.mark.parametrize('mode', [1, 2, 3])
 def test_set_threshold(gpu, oracle, seed, mode):
