@@ -888,16 +888,16 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 	double *o = C->buf<double>(S_HC_O, n3), *p = C->buf<double>(S_HC_P, n3);
 	double *pn = C->buf<double>(S_HC_PN, n3), *bq = C->buf<double>(S_HC_B, n3);
 	launch_hc_load(pts, R.on ? R.perm : nullptr, nv, o, s);
-	HIPX(hipMemcpyAsync(p, o, 8 * n3, hipMemcpyDeviceToDevice, s));
 	// algorithmic bytes: CSR offsets, one pair word per reference (4 bytes
 	// as LDS slots, 8 as vertex ids), each position array once, the outputs
 	const double pair_bytes = (H.on ? 4.0 : 8.0) * 3 * nt;
 	for (int it = 0; it < iter; it++) {
+		const double *cur = it == 0 ? o : p; // p starts as the original
 		C->tbegin("hc_a", 4.0 * nv + pair_bytes + 24.0 * nv * 2 + 48.0 * nv);
 		if (H.on)
-			launch_hc_a_halo(c.start, H.slots, H.halo, H.hcount, o, p, pn, bq, nv, alpha, s);
+			launch_hc_a_halo(c.start, H.slots, H.halo, H.hcount, o, cur, pn, bq, nv, alpha, s);
 		else
-			launch_hc_a(c.start, c.pairs, o, p, pn, bq, nv, alpha, s);
+			launch_hc_a(c.start, c.pairs, o, cur, pn, bq, nv, alpha, s);
 		C->tend();
 		C->tbegin("hc_b", 4.0 * nv + pair_bytes + 24.0 * nv * 2 + 24.0 * nv);
 		if (H.on)
@@ -907,7 +907,7 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 		C->tend();
 	}
 	uint8_t *border = lockEdges ? border_flags(C, c, T, nv, nt) : nullptr;
-	launch_hc_store(p, o, border, R.on ? R.perm : nullptr, nv, pts, s);
+	launch_hc_store(iter > 0 ? p : o, o, border, R.on ? R.perm : nullptr, nv, pts, s);
 	C->check();
 }
 

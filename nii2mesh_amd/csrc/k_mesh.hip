@@ -1180,6 +1180,26 @@ __global__ void k_border_pairs(const uint32_t *__restrict__ start, const int2 *_
 		return;
 	uint32_t b = start[v], e = start[v + 1];
 	uint8_t r = 0;
+	constexpr int RB = 16;
+	if (e - b <= RB) { // the common case: the pairs held in registers
+		int2 q[RB];
+#pragma unroll
+		for (int k = 0; k < RB; k++)
+			q[k] = b + k < e ? pairs[b + k] : make_int2(-1, -1); // ids are >= 0
+#pragma unroll
+		for (int i = 0; i < RB; i++)
+#pragma unroll
+			for (int s = 0; s < 2; s++) {
+				const int a = s ? q[i].y : q[i].x;
+				uint32_t cnt = 0;
+#pragma unroll
+				for (int k = 0; k < RB; k++)
+					cnt += (q[k].x == a) + (q[k].y == a);
+				r |= (a >= 0 && cnt == 1);
+			}
+		border[v] = r;
+		return;
+	}
 	for (uint32_t i = b; i < e && !r; i++) {
 		int2 q = pairs[i];
 		for (int s = 0; s < 2 && !r; s++) {

@@ -470,39 +470,44 @@ void launch_minmax(const float *img, size_t n, uint32_t *range, hipStream_t s)
 
 // ---------------------------------------------------------------------------
 // Threshold mask bits: bit = (img >= iso) (or img != 0 when `nonzero`).
-// One wave covers 64 consecutive x of one row; rows are padded to 64.
+// A workgroup walks whole rows (the row index is uniform, so no per-voxel
+// division); one wave covers 64 consecutive x of a row, padded to 64.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_mask_bits(const float *__restrict__ img, int nx, int ny, int nz, int W, int xpad, float iso,
             int nonzero, uint32_t *__restrict__ bits)
 {
-	size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-	size_t rows = (size_t)ny * nz;
-	size_t row = t / xpad;
-	int x = (int)(t % xpad);
-	if (row >= rows)
-		return; // whole waves exit together (xpad is a multiple of 64)
-	bool p = false;
-	if (x < nx) {
-		float v = img[row * nx + x];
-		p = nonzero ? (v != 0.0f) : (v >= iso);
+	const size_t rows = (size_t)ny * nz;
+	for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
+		const float *src = img + row * nx;
+		uint32_t *dst = bits + (size_t)W * row;
+		for (int x0 = 0; x0 < xpad; x0 += blockDim.x) {
+			const int x = x0 + threadIdx.x;
+			if (x - (int)lane_id() >= xpad)
+				break; // whole waves past the padded row (xpad is a multiple of 64)
+			bool p = false;
+			if (x < nx) {
+				const float v = src[x];
+				p = nonzero ? (v != 0.0f) : (v >= iso);
+			}
+			const unsigned long long m = __ballot(p);
+			const int wx = (x & ~63) / 32;
+			if ((x & 63) == 0)
+				dst[wx] = (uint32_t)m;
+			else if ((x & 63) == 32 && wx + 1 < W)
+				dst[wx + 1] = (uint32_t)(m >> 32);
+		}
 	}
-	unsigned long long m = __ballot(p);
-	int wx = (x & ~63) / 32;
-	size_t wi = (size_t)wx + (size_t)W * row;
-	if ((x & 63) == 0)
-		bits[wi] = (uint32_t)m;
-	else if ((x & 63) == 32 && wx + 1 < W)
-		bits[wi + 1] = (uint32_t)(m >> 32);
 }
+
+static unsigned row_blocks(size_t rows) { return (unsigned)(rows < 8192 ? (rows ? rows : 1) : 8192); }
 
 void launch_mask_bits(const float *img, int nx, int ny, int nz, float iso, int nonzero, uint32_t *bits,
                       hipStream_t s)
 {
 	int W = (nx + 31) / 32, xpad = (nx + 63) / 64 * 64;
-	size_t total = (size_t)xpad * ny * nz;
-	hipLaunchKernelGGL(k_mask_bits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, img, nx, ny,
-	                   nz, W, xpad, iso, nonzero, bits);
+	hipLaunchKernelGGL(k_mask_bits, dim3(row_blocks((size_t)ny * nz)), dim3(256), 0, s, img, nx, ny, nz, W, xpad,
+	                   iso, nonzero, bits);
 }
 
 // ---------------------------------------------------------------------------
@@ -580,32 +585,32 @@ k_apply(float *__restrict__ img, int nx, int ny, int nz, int W, int xpad, float 
 	float lmin = __builtin_inff();
 	int bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, bmax[3] = {-1, -1, -1};
 	const size_t rows = (size_t)ny * nz;
-	const size_t total = rows * xpad;
-	for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-		size_t row = t / xpad;
-		const int x = (int)(t % xpad);
-		if (x >= nx)
-			continue;
+	// whole rows per workgroup: y and z are uniform, no per-voxel division
+	for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
 		const int y = (int)(row % ny), z = (int)(row / ny);
-		size_t v = row * nx + x;
-		size_t wi = (size_t)(x >> 5) + (size_t)W * row;
-		float val = img[v];
-		if (fill && ((keep[wi] >> (x & 31)) & 1u))
-			val = (float)fmax((double)val, (double)iso);
-		if (dil && !((dil[wi] >> (x & 31)) & 1u))
-			val = mn;
-		if (val >= iso) {
-			bmin[0] = min(bmin[0], x);
-			bmax[0] = max(bmax[0], x);
-			bmin[1] = min(bmin[1], y);
-			bmax[1] = max(bmax[1], y);
-			bmin[2] = min(bmin[2], z);
-			bmax[2] = max(bmax[2], z);
+		const bool face_row = y == 0 || z == 0 || y == ny - 1 || z == nz - 1;
+		float *line = img + row * nx;
+		const uint32_t *krow = fill && keep ? keep + (size_t)W * row : nullptr;
+		const uint32_t *drow = dil ? dil + (size_t)W * row : nullptr;
+		for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+			float val = line[x];
+			if (krow && ((krow[x >> 5] >> (x & 31)) & 1u))
+				val = (float)fmax((double)val, (double)iso);
+			if (drow && !((drow[x >> 5] >> (x & 31)) & 1u))
+				val = mn;
+			if (val >= iso) {
+				bmin[0] = min(bmin[0], x);
+				bmax[0] = max(bmax[0], x);
+				bmin[1] = min(bmin[1], y);
+				bmax[1] = max(bmax[1], y);
+				bmin[2] = min(bmin[2], z);
+				bmax[2] = max(bmax[2], z);
+			}
+			if (face_row || x == 0 || x == nx - 1)
+				val = fminf(edgeMax, val);
+			line[x] = val;
+			lmin = fminf(lmin, val);
 		}
-		if (x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1)
-			val = fminf(edgeMax, val);
-		img[v] = val;
-		lmin = fminf(lmin, val);
 	}
 	for (int k = 0; k < 3; k++) {
 		bmin[k] = wave_min_i32(bmin[k]);
@@ -645,11 +650,8 @@ void launch_apply(float *img, int nx, int ny, int nz, float iso, float mn, float
                   const uint32_t *keep, const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s)
 {
 	int W = (nx + 31) / 32, xpad = (nx + 63) / 64 * 64;
-	size_t total = (size_t)xpad * ny * nz;
-	size_t blocks = (total + 255) / 256;
-	blocks = blocks > 4096 ? 4096 : (blocks < 1 ? 1 : blocks);
-	hipLaunchKernelGGL(k_apply, dim3((unsigned)blocks), dim3(256), 0, s, img, nx, ny, nz, W, xpad, iso, mn,
-	                   edgeMax, keep, dil, fill, bbox, vmin);
+	hipLaunchKernelGGL(k_apply, dim3(row_blocks((size_t)ny * nz)), dim3(256), 0, s, img, nx, ny, nz, W, xpad, iso,
+	                   mn, edgeMax, keep, dil, fill, bbox, vmin);
 }
 
 // ---------------------------------------------------------------------------
