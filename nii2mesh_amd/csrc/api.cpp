@@ -79,6 +79,7 @@ enum Slot {
 	S_L,
 	S_AUX,
 	S_ROOTS,
+	S_RBASE,
 	S_CFG,
 	S_CROW,
 	S_PFLAGS,
@@ -470,14 +471,16 @@ bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P
 		if (label_ok) {
 			uint32_t *L = C->buf<uint32_t>(S_L, nvox);
 			uint32_t *AUX = C->buf<uint32_t>(S_AUX, nvox);
+			uint32_t *rbase = C->buf<uint32_t>(S_RBASE, nwords + 1);
+			uint32_t *scr = scan_scratch(C, nwords + 1);
 			Timer tm(C);
 			if (P.fillBubbles) { // src/bwlabel.c:478-541
 				HIPX(hipMemsetAsync(C->d_small + SM_NROOTS, 0, 4, s));
-				launch_ccl(bits, NX, NY, NZ, 6, 1, L, AUX, 1, C->d_small + SM_NROOTS, nullptr, s, &tm);
+				launch_ccl(bits, NX, NY, NZ, 6, 1, L, AUX, 1, C->d_small + SM_NROOTS, nullptr, rbase, scr, s, &tm);
 				uint32_t ninv = C->fetch(SM_NROOTS, 1)[0];
 				if (ninv > 1) {
 					C->tbegin("fill_bubbles", 4.0 * nvox);
-					launch_fill_bubbles(bits, NX, NY, NZ, L, AUX, s);
+					launch_fill_bubbles(bits, NX, NY, NZ, L, AUX, rbase, s);
 					C->tend();
 				}
 			}
@@ -485,13 +488,13 @@ bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P
 				uint32_t *roots = C->buf<uint32_t>(S_ROOTS, nvox / 2 + 1024);
 				HIPX(hipMemsetAsync(C->d_small + SM_NROOTS, 0, 4, s));
 				HIPX(hipMemsetAsync(C->d_small + SM_BEST, 0, 8, s));
-				launch_ccl(bits, NX, NY, NZ, 18, 0, L, AUX, 0, C->d_small + SM_NROOTS, roots, s, &tm);
+				launch_ccl(bits, NX, NY, NZ, 18, 0, L, AUX, 0, C->d_small + SM_NROOTS, roots, rbase, scr, s, &tm);
 				C->tbegin("ccl_argmax");
 				launch_ccl_argmax(roots, C->d_small + SM_NROOTS, AUX, (unsigned long long *)(C->d_small + SM_BEST), s);
 				C->tend();
 				uint32_t *sel = C->buf<uint32_t>(S_BITS2, nwords);
 				C->tbegin("select_root", 4.0 * nvox);
-				launch_select_root(bits, NX, NY, NZ, L, (unsigned long long *)(C->d_small + SM_BEST), sel, s);
+				launch_select_root(bits, NX, NY, NZ, L, (unsigned long long *)(C->d_small + SM_BEST), sel, rbase, s);
 				C->tend();
 				keep = sel;
 			}
@@ -2574,20 +2577,23 @@ int bwlabel(float *img, int conn, size_t dim[3], bool onlyLargest, bool fillBubb
 		launch_mask_bits(vol, NX, NY, NZ, 0.f, 1, bits, s);
 		uint32_t *L = C->buf<uint32_t>(S_L, nvox), *AUX = C->buf<uint32_t>(S_AUX, nvox);
 		uint32_t *roots = C->buf<uint32_t>(S_ROOTS, nvox / 2 + 1024);
+		uint32_t *rbase = C->buf<uint32_t>(S_RBASE, nwords + 1);
 		if (fillBubbles) { // src/bwlabel.c:478-541
 			HIPX(hipMemsetAsync(C->d_small + SM_NROOTS, 0, 4, s));
-			launch_ccl(bits, NX, NY, NZ, 6, 1, L, AUX, 1, C->d_small + SM_NROOTS, nullptr, s, nullptr);
+			launch_ccl(bits, NX, NY, NZ, 6, 1, L, AUX, 1, C->d_small + SM_NROOTS, nullptr, rbase,
+			           scan_scratch(C, nwords + 1), s, nullptr);
 			if (C->fetch(SM_NROOTS, 1)[0] > 1)
-				launch_fill_bubbles(bits, NX, NY, NZ, L, AUX, s);
+				launch_fill_bubbles(bits, NX, NY, NZ, L, AUX, rbase, s);
 		}
 		HIPX(hipMemsetAsync(C->d_small + SM_NROOTS, 0, 4, s));
-		launch_ccl(bits, NX, NY, NZ, conn, 0, L, AUX, 0, C->d_small + SM_NROOTS, roots, s, nullptr);
+		launch_ccl(bits, NX, NY, NZ, conn, 0, L, AUX, 0, C->d_small + SM_NROOTS, roots, rbase,
+		           scan_scratch(C, nwords + 1), s, nullptr);
 		uint32_t nl = C->fetch(SM_NROOTS, 1)[0];
 		if (nl > 0 && onlyLargest) { // src/bwlabel.c:453-471
 			HIPX(hipMemsetAsync(C->d_small + SM_BEST, 0, 8, s));
 			launch_ccl_argmax(roots, C->d_small + SM_NROOTS, AUX, (unsigned long long *)(C->d_small + SM_BEST), s);
 			uint32_t *sel = C->buf<uint32_t>(S_BITS2, nwords);
-			launch_select_root(bits, NX, NY, NZ, L, (unsigned long long *)(C->d_small + SM_BEST), sel, s);
+			launch_select_root(bits, NX, NY, NZ, L, (unsigned long long *)(C->d_small + SM_BEST), sel, rbase, s);
 			launch_bits_to_float(sel, NX, NY, NZ, vol, s);
 			nl = 1;
 		} else {
@@ -2598,7 +2604,7 @@ int bwlabel(float *img, int conn, size_t dim[3], bool onlyLargest, bool fillBubb
 			uint32_t *scr = scan_scratch(C, radix_counts_words(nl + 1));
 			bool in1 = radix_sort_u32(roots, v0, k1, v1, nl, 32, rc, scr, s);
 			launch_scatter_rank(in1 ? k1 : roots, nl, AUX, s);
-			launch_write_labels(bits, NX, NY, NZ, L, AUX, vol, s);
+			launch_write_labels(bits, NX, NY, NZ, L, AUX, vol, rbase, s);
 		}
 		C->check();
 		HIPX(hipMemcpyAsync(img, vol, 4 * nvox, hipMemcpyDeviceToHost, s));

@@ -76,8 +76,23 @@ __device__ __forceinline__ uint32_t run_mask(uint32_t w, int s)
 	return (l == 32 ? 0xffffffffu : ((1u << l) - 1u)) << s;
 }
 
+// index, among the runs of a word, of the run that contains bit `bit`
+__device__ __forceinline__ uint32_t run_index(uint32_t starts, int bit)
+{
+	return __popc(bit == 31 ? starts : (starts & ((2u << bit) - 1u))) - 1u;
+}
+
+// Runs are named by dense ids in voxel order: rb[word] = number of runs in
+// the words before it (exclusive scan of the per-word counts), and a run's id
+// is rb[word] + its index in the word.  The smallest id of a component is its
+// first run in x-fastest order.
 struct Geo {
 	int nx, ny, nz, W;
+	const uint32_t *rb;
+	__device__ size_t widx(int wx, int y, int z) const
+	{
+		return (size_t)wx + (size_t)W * ((size_t)y + (size_t)ny * z);
+	}
 	__device__ uint32_t word(const uint32_t *bits, int wx, int y, int z, int invert) const
 	{
 		if (wx < 0 || wx >= W || y < 0 || y >= ny || z < 0 || z >= nz)
@@ -89,10 +104,6 @@ struct Geo {
 			w = ~w & valid;
 		}
 		return w;
-	}
-	__device__ uint32_t gidx(int x, int y, int z) const
-	{
-		return (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
 	}
 };
 
@@ -149,20 +160,40 @@ __device__ __forceinline__ void for_links(uint32_t w, uint32_t u, uint32_t up, u
 		f(31, 1, 0);
 }
 
-// Pass 1: label runs inside a tile of 16 words x 4 rows x 4 slices in LDS.
-// L[first voxel of run] = first voxel of the tile-local root run;
-// AUX[root] = 0 for every tile-local root.
+// per word: the number of runs (starts) of the predicate; cnt[nw] = 0, so
+// that the exclusive scan of cnt[0, nw] ends with the total
+__global__ void __launch_bounds__(256)
+k_run_count(const uint32_t *__restrict__ bits, Geo g, int invert, uint32_t *__restrict__ cnt)
+{
+	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t nw = (uint32_t)g.W * g.ny * g.nz;
+	if (t > nw)
+		return;
+	uint32_t c = 0;
+	if (t < nw) {
+		const uint32_t row = t / g.W;
+		c = __popc(run_starts(g.word(bits, (int)(t % g.W), (int)(row % g.ny), (int)(row / g.ny), invert)));
+	}
+	cnt[t] = c;
+}
+
+// Pass 1: label runs inside a tile of 16 words x 4 rows x 4 slices in LDS
+// (node = position of the run's first bit in the tile; the smallest node of
+// a tile component is its smallest run id).  L[run] = the tile-local root
+// run; AUX[root] = 0 for every tile-local root.
 __global__ void __launch_bounds__(CC_BS)
 k_ccl_local(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint32_t *__restrict__ L,
             uint32_t *__restrict__ AUX)
 {
 	__shared__ uint32_t lab[CC_NODES];
 	__shared__ uint32_t s_w[CC_TY * CC_TZ][CC_KW];
+	__shared__ uint32_t s_rb[CC_TY * CC_TZ][CC_KW];
 	const int k = threadIdx.x % CC_KW, r = threadIdx.x / CC_KW; // word in tile, row in tile
 	const int ry = r % CC_TY, rz = r / CC_TY;
 	const int wx = blockIdx.x * CC_KW + k, y = blockIdx.y * CC_TY + ry, z = blockIdx.z * CC_TZ + rz;
 	const uint32_t w = g.word(bits, wx, y, z, invert);
 	s_w[r][k] = w;
+	s_rb[r][k] = w ? g.rb[g.widx(wx, y, z)] : 0u;
 	const uint32_t st = run_starts(w);
 	const int base = k * 32 + CC_TXV * r;
 	for (uint32_t t = st; t; t &= t - 1) {
@@ -191,13 +222,13 @@ k_ccl_local(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint
 			});
 	}
 	__syncthreads();
-	for (uint32_t t = st; t; t &= t - 1) {
+	uint32_t gi = s_rb[r][k];
+	for (uint32_t t = st; t; t &= t - 1, gi++) {
 		int b = __ffs(t) - 1;
 		uint32_t root = lds_find(lab, base + b);
 		int rr = root / CC_TXV, rx = root % CC_TXV;
-		uint32_t gr = g.gidx(blockIdx.x * CC_TXV + rx, blockIdx.y * CC_TY + rr % CC_TY,
-		                     blockIdx.z * CC_TZ + rr / CC_TY);
-		uint32_t gi = g.gidx(wx * 32 + b, y, z);
+		const uint32_t rw = s_w[rr][rx >> 5];
+		const uint32_t gr = s_rb[rr][rx >> 5] + run_index(run_starts(rw), rx & 31);
 		L[gi] = gr;
 		if (gr == gi)
 			AUX[gi] = 0;
@@ -216,11 +247,13 @@ k_ccl_merge(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint
 	if (!w)
 		return;
 	const uint32_t st = run_starts(w);
+	const size_t wi = g.widx(wx, y, z);
+	const uint32_t rb0 = g.rb[wi];
 	// x-1 across a tile boundary
 	if (k == 0 && wx > 0 && (w & 1u)) {
 		uint32_t pw = g.word(bits, wx - 1, y, z, invert);
 		if (pw >> 31)
-			uf_union(L, g.gidx(wx * 32, y, z), g.gidx((wx - 1) * 32 + run_of(run_starts(pw), 31), y, z));
+			uf_union(L, rb0, g.rb[wi - 1] + run_index(run_starts(pw), 31));
 	}
 	int nn;
 	const NbrRow *nb = nbr_rows(conn, &nn);
@@ -243,8 +276,7 @@ k_ccl_merge(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint
 				if (!other)
 					return;
 				uint32_t uw = wo == 0 ? u : (wo < 0 ? up : un);
-				uf_union(L, g.gidx(wx * 32 + run_of(st, bw), y, z),
-				         g.gidx((wx + wo) * 32 + run_of(run_starts(uw), bu), y2, z2));
+				uf_union(L, rb0 + run_index(st, bw), g.rb[g.widx(wx + wo, y2, z2)] + run_index(run_starts(uw), bu));
 			});
 	}
 }
@@ -265,9 +297,9 @@ k_ccl_flatten(const uint32_t *__restrict__ bits, Geo g, int invert, uint32_t *__
 	const uint32_t w = g.word(bits, wx, y, z, invert);
 	uint32_t nroot_local = 0, cur = NONE, cnt = 0;
 	const bool face_row = y == 0 || z == 0 || y == g.ny - 1 || z == g.nz - 1;
-	for (uint32_t t = run_starts(w); t; t &= t - 1) {
+	uint32_t gi = w ? g.rb[g.widx(wx, y, z)] : 0u;
+	for (uint32_t t = run_starts(w); t; t &= t - 1, gi++) {
 		int b = __ffs(t) - 1;
-		uint32_t gi = g.gidx(wx * 32 + b, y, z);
 		uint32_t root = uf_find(L, gi);
 		L[gi] = root;
 		if (mode == 0) {
@@ -341,13 +373,18 @@ static dim3 cc_grid(const Geo &g)
 }
 
 void launch_ccl(const uint32_t *bits, int nx, int ny, int nz, int conn, int invert, uint32_t *L,
-                uint32_t *AUX, int mode, uint32_t *nroots, uint32_t *rootlist, hipStream_t s,
-                CclTimer *timer)
+                uint32_t *AUX, int mode, uint32_t *nroots, uint32_t *rootlist, uint32_t *rbase,
+                uint32_t *scan_scratch, hipStream_t s, CclTimer *timer)
 {
-	Geo g{nx, ny, nz, (nx + 31) / 32};
+	Geo g{nx, ny, nz, (nx + 31) / 32, rbase};
 	dim3 grid = cc_grid(g);
 	const double nb = (double)nx * ny * nz / 8;
+	const size_t nw = (size_t)g.W * ny * nz;
 	// algorithmic bytes: mask bits in per pass; run labels are not counted
+	if (timer) timer->begin("ccl_runs", nb + 8.0 * nw);
+	hipLaunchKernelGGL(k_run_count, dim3((unsigned)((nw + 1 + 255) / 256)), dim3(256), 0, s, bits, g, invert, rbase);
+	scan_excl_u32(rbase, rbase, nw + 1, nullptr, scan_scratch, s);
+	if (timer) timer->end();
 	if (timer) timer->begin("ccl_local", nb);
 	hipLaunchKernelGGL(k_ccl_local, grid, dim3(CC_BS), 0, s, bits, g, conn, invert, L, AUX);
 	if (timer) timer->end();
@@ -397,22 +434,21 @@ k_select_root(const uint32_t *__restrict__ bits, Geo g, const uint32_t *__restri
 		return;
 	unsigned long long bk = *bestp;
 	uint32_t best = bk ? 0xffffffffu - (uint32_t)(bk & 0xffffffffu) : NONE;
-	const int wx = (int)(t % g.W);
-	const size_t row = t / g.W;
 	const uint32_t w = bits[t];
 	uint32_t o = 0;
-	for (uint32_t s = run_starts(w); s; s &= s - 1) {
+	uint32_t id = w ? g.rb[t] : 0u;
+	for (uint32_t s = run_starts(w); s; s &= s - 1, id++) {
 		int b = __ffs(s) - 1;
-		if (L[row * g.nx + wx * 32 + b] == best)
+		if (L[id] == best)
 			o |= run_mask(w, b);
 	}
 	out[t] = o;
 }
 
 void launch_select_root(const uint32_t *bits, int nx, int ny, int nz, const uint32_t *L,
-                        const unsigned long long *best, uint32_t *out, hipStream_t s)
+                        const unsigned long long *best, uint32_t *out, const uint32_t *rbase, hipStream_t s)
 {
-	Geo g{nx, ny, nz, (nx + 31) / 32};
+	Geo g{nx, ny, nz, (nx + 31) / 32, rbase};
 	size_t nw = (size_t)g.W * ny * nz;
 	hipLaunchKernelGGL(k_select_root, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, bits, g, L, best, out);
 }
@@ -427,14 +463,14 @@ k_fill_bubbles(uint32_t *__restrict__ bits, Geo g, const uint32_t *__restrict__ 
 	if (t >= nw)
 		return;
 	const int wx = (int)(t % g.W);
-	const size_t row = t / g.W;
 	const uint32_t fg = bits[t];
 	const int rem = g.nx - wx * 32;
 	const uint32_t bg = ~fg & (rem >= 32 ? 0xffffffffu : ((1u << rem) - 1u));
 	uint32_t add = 0;
-	for (uint32_t s = run_starts(bg); s; s &= s - 1) {
+	uint32_t id = bg ? g.rb[t] : 0u; // run ids of the background labelling
+	for (uint32_t s = run_starts(bg); s; s &= s - 1, id++) {
 		int b = __ffs(s) - 1;
-		if (AUX[L[row * g.nx + wx * 32 + b]] == 0)
+		if (AUX[L[id]] == 0)
 			add |= run_mask(bg, b);
 	}
 	if (add)
@@ -442,9 +478,9 @@ k_fill_bubbles(uint32_t *__restrict__ bits, Geo g, const uint32_t *__restrict__ 
 }
 
 void launch_fill_bubbles(uint32_t *bits, int nx, int ny, int nz, const uint32_t *L, const uint32_t *AUX,
-                         hipStream_t s)
+                         const uint32_t *rbase, hipStream_t s)
 {
-	Geo g{nx, ny, nz, (nx + 31) / 32};
+	Geo g{nx, ny, nz, (nx + 31) / 32, rbase};
 	size_t nw = (size_t)g.W * ny * nz;
 	hipLaunchKernelGGL(k_fill_bubbles, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, bits, g, L, AUX);
 }
@@ -462,18 +498,19 @@ k_write_labels(const uint32_t *__restrict__ bits, Geo g, const uint32_t *__restr
 		return;
 	int x = (int)(v % g.nx);
 	size_t row = v / g.nx;
-	uint32_t w = bits[(size_t)(x >> 5) + (size_t)g.W * row];
+	const size_t wi = (size_t)(x >> 5) + (size_t)g.W * row;
+	uint32_t w = bits[wi];
 	int b = x & 31;
 	float out = 0.0f;
 	if ((w >> b) & 1u)
-		out = (float)rank[L[row * g.nx + (x & ~31) + run_of(run_starts(w), b)]];
+		out = (float)rank[L[g.rb[wi] + run_index(run_starts(w), b)]];
 	img[v] = out;
 }
 
 void launch_write_labels(const uint32_t *bits, int nx, int ny, int nz, const uint32_t *L,
-                         const uint32_t *rank, float *img, hipStream_t s)
+                         const uint32_t *rank, float *img, const uint32_t *rbase, hipStream_t s)
 {
-	Geo g{nx, ny, nz, (nx + 31) / 32};
+	Geo g{nx, ny, nz, (nx + 31) / 32, rbase};
 	size_t n = (size_t)nx * ny * nz;
 	hipLaunchKernelGGL(k_write_labels, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bits, g, L, rank,
 	                   img);

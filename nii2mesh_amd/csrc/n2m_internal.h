@@ -84,17 +84,21 @@ unsigned atlas_smooth_blocks(int nx, int ny, int nz, int smooth, int tile);
 unsigned atlas_apply_blocks(int nx, int ny, int nz);
 void launch_binarize(const float *labels, size_t n, float lo, float hi, float *out, uint32_t *count, hipStream_t s);
 
-// k_ccl.hip
+// k_ccl.hip.  Runs of set bits are named by dense ids in voxel order:
+// launch_ccl fills rbase[w] (nwords + 1 entries) with the number of runs
+// before word w, and the selection kernels take the same rbase.  L and AUX
+// are indexed by run id (nvox / 2 + nwords entries bound the run count).
 void launch_ccl(const uint32_t *bits, int nx, int ny, int nz, int conn, int invert, uint32_t *L, uint32_t *AUX,
-                int mode, uint32_t *nroots, uint32_t *rootlist, hipStream_t s, CclTimer *timer);
+                int mode, uint32_t *nroots, uint32_t *rootlist, uint32_t *rbase, uint32_t *scan_scratch,
+                hipStream_t s, CclTimer *timer);
 void launch_ccl_argmax(const uint32_t *rootlist, const uint32_t *nroots, const uint32_t *AUX,
                        unsigned long long *best, hipStream_t s);
 void launch_select_root(const uint32_t *bits, int nx, int ny, int nz, const uint32_t *L,
-                        const unsigned long long *best, uint32_t *out, hipStream_t s);
+                        const unsigned long long *best, uint32_t *out, const uint32_t *rbase, hipStream_t s);
 void launch_fill_bubbles(uint32_t *bits, int nx, int ny, int nz, const uint32_t *L, const uint32_t *AUX,
-                         hipStream_t s);
+                         const uint32_t *rbase, hipStream_t s);
 void launch_write_labels(const uint32_t *bits, int nx, int ny, int nz, const uint32_t *L, const uint32_t *rank,
-                         float *img, hipStream_t s);
+                         float *img, const uint32_t *rbase, hipStream_t s);
 void launch_scatter_rank(const uint32_t *sorted, uint32_t n, uint32_t *rank, hipStream_t s);
 void launch_bits_to_float(const uint32_t *bits, int nx, int ny, int nz, float *img, hipStream_t s);
 
