@@ -356,36 +356,28 @@ __device__ int lw_select(const float c[8], int lut, int originalMC)
 // row (a point's bit a is set when its a-edge carries a vertex,
 // src/MarchingCubes.c:238-269); the segment's counts of edge vertices,
 // triangles and centre vertices go to cnt[0], cnt[cstride], cnt[2*cstride].
-// The four point rows the
-// segment reads -- (y,z), (y+1,z), (y,z+1), (y+1,z+1) -- are staged in LDS
-// first (sub-box values, clamped as lw_data returns them).
-__device__ __forceinline__ void lw_classify_row(const float *__restrict__ img, const LwGeo &g, int seg, int y, int z,
-                                                uint8_t *__restrict__ pflags, uint16_t *__restrict__ crow,
-                                                uint32_t *__restrict__ cnt, size_t cstride)
+// The four point rows the segment reads -- r = dy + 2*dz for (y+dy, z+dz) --
+// are staged in LDS (sub-box values, clamped as lw_data returns them); row r
+// sits in slot (r & 1) + 2 * (((r >> 1) + par) & 1), so that a z walk can
+// keep the shared plane where it is.
+__device__ __forceinline__ void lw_classify_core(const float (*s_row)[LW_SEG + 1], int par, const LwGeo &g, int seg,
+                                                 int y, int z, uint8_t *__restrict__ pflags,
+                                                 uint16_t *__restrict__ crow, uint32_t *__restrict__ cnt,
+                                                 size_t cstride, uint32_t *scratch)
 {
-	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
-	__shared__ float s_row[4][LW_SEG + 1];
 	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
-	const int xl = min(xe + 1, g.sx) - xs; // staged points per row: [xs, xs + xl)
 	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
-#pragma unroll
-	for (int r = 0; r < 4; r++) {
-		const int yy = y + (r & 1), zz = z + (r >> 1);
-		if (yy >= g.sy || zz >= g.sz)
-			continue;
-		for (int i = threadIdx.x; i < xl; i += LW_BS)
-			s_row[r][i] = lw_data(img, g, xs + i, yy, zz);
-	}
-	__syncthreads();
+	const int sl0 = 2 * (par & 1), sl1 = 2 * ((1 + par) & 1); // slot of dz = 0, 1
+	const float *r00 = s_row[sl0], *r10 = s_row[sl0 + 1], *r01 = s_row[sl1];
 	uint32_t nv = 0, nt = 0, nc = 0;
 	const size_t rowbase = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
 	for (int x = xs + threadIdx.x; x < xe; x += LW_BS) {
 		const int i = x - xs;
-		const float c0 = s_row[0][i];
+		const float c0 = r00[i];
 		float cn[3];
-		cn[0] = x < g.sx - 1 ? s_row[0][i + 1] : c0;
-		cn[1] = y < g.sy - 1 ? s_row[1][i] : c0;
-		cn[2] = z < g.sz - 1 ? s_row[2][i] : c0;
+		cn[0] = x < g.sx - 1 ? r00[i + 1] : c0;
+		cn[1] = y < g.sy - 1 ? r10[i] : c0;
+		cn[2] = z < g.sz - 1 ? r01[i] : c0;
 		uint32_t f = 0;
 #pragma unroll
 		for (int a = 0; a < 3; a++) {
@@ -400,7 +392,7 @@ __device__ __forceinline__ void lw_classify_row(const float *__restrict__ img, c
 			int lut = 0;
 #pragma unroll
 			for (int p = 0; p < 8; p++) {
-				c[p] = s_row[c_cy[p] + 2 * c_cz[p]][i + c_cx[p]];
+				c[p] = s_row[c_cy[p] + (c_cz[p] ? sl1 : sl0)][i + c_cx[p]];
 				if (c[p] > 0.f)
 					lut += 1 << p;
 			}
@@ -423,14 +415,58 @@ __device__ __forceinline__ void lw_classify_row(const float *__restrict__ img, c
 	}
 }
 
+// plane zz of rows y and y+1 of segment seg into slots slot, slot + 1
+__device__ __forceinline__ void lw_stage_plane(float (*s_row)[LW_SEG + 1], int slot, const float *__restrict__ img,
+                                               const LwGeo &g, int seg, int y, int zz)
+{
+	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
+	const int xl = min(xe + 1, g.sx) - xs; // staged points per row: [xs, xs + xl)
+	if (zz >= g.sz)
+		return;
+#pragma unroll
+	for (int dy = 0; dy < 2; dy++) {
+		const int yy = y + dy;
+		if (yy >= g.sy)
+			continue;
+		for (int i = threadIdx.x; i < xl; i += LW_BS)
+			s_row[slot + dy][i] = lw_data(img, g, xs + i, yy, zz);
+	}
+}
+
+// one row segment, its four rows staged here (the atlas path)
+__device__ __forceinline__ void lw_classify_row(const float *__restrict__ img, const LwGeo &g, int seg, int y, int z,
+                                                uint8_t *__restrict__ pflags, uint16_t *__restrict__ crow,
+                                                uint32_t *__restrict__ cnt, size_t cstride)
+{
+	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
+	__shared__ float s_row[4][LW_SEG + 1];
+	lw_stage_plane(s_row, 0, img, g, seg, y, z);
+	lw_stage_plane(s_row, 2, img, g, seg, y, z + 1);
+	__syncthreads();
+	lw_classify_core(s_row, 0, g, seg, y, z, pflags, crow, cnt, cstride, scratch);
+}
+
+// The whole-volume pass: a workgroup walks LW_ZC planes of one (segment, y)
+// column, loading one new plane per step (each point row is read about
+// once, instead of once per neighbouring row segment).
+constexpr int LW_ZC = 8;
 __global__ void __launch_bounds__(LW_BS)
 k_lw_classify(const float *__restrict__ img, LwGeo g, uint8_t *__restrict__ pflags,
               uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt /* 3 arrays of nseg*sy*sz */)
 {
-	const int seg = blockIdx.x, y = blockIdx.y, z = blockIdx.z;
-	size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
-	size_t ns = (size_t)g.nseg * g.sy * g.sz;
-	lw_classify_row(img, g, seg, y, z, pflags, crow, segcnt + si, ns);
+	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
+	__shared__ float s_row[4][LW_SEG + 1];
+	const int seg = blockIdx.x, y = blockIdx.y, z0 = blockIdx.z * LW_ZC, z1 = min(z0 + LW_ZC, g.sz);
+	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
+	lw_stage_plane(s_row, 0, img, g, seg, y, z0);
+	for (int z = z0; z < z1; z++) {
+		const int par = (z - z0) & 1;
+		lw_stage_plane(s_row, 2 * ((1 + par) & 1), img, g, seg, y, z + 1);
+		__syncthreads();
+		const size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
+		lw_classify_core(s_row, par, g, seg, y, z, pflags, crow, segcnt + si, ns, scratch);
+		__syncthreads(); // the slots of plane z are overwritten next
+	}
 }
 
 // edge vertex position in sub-box coordinates, src/MarchingCubes.c:935-1026
@@ -713,7 +749,8 @@ void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3
 {
 	LwGeo g;
 	lw_geo(g, nx, ny, nz, lo, hi, iso, mnv, originalMC);
-	hipLaunchKernelGGL(k_lw_classify, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags, crow, segcnt);
+	hipLaunchKernelGGL(k_lw_classify, dim3(g.nseg, g.sy, (g.sz + LW_ZC - 1) / LW_ZC), dim3(LW_BS), 0, s, img, g, pflags,
+	                   crow, segcnt);
 }
 
 void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
