@@ -525,6 +525,7 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 {
 	static_assert(LW_BS == 4 * N2M_WAVE, "one wave per point row");
 	__shared__ uint32_t s_pre[4][LW_SEG + 1]; // popcount prefix for the 4 point rows
+	__shared__ uint16_t s_crow[LW_SEG];
 	__shared__ uint32_t s_base[4];
 	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
 	// a segment without edge vertices, triangles or centre vertices of its
@@ -535,7 +536,17 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 	const int len = xe - xs; // points owned; prefix covers [xs, xe]
 	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
 	const int nrows = cuberow ? 4 : 1;
-	// row r = wave r: (y + (r&1), z + (r>>1)); exclusive popcount prefix
+	const int ncube = min(xe, g.sx - 1) - xs;
+	// the segment's tiling rows, staged while the prefixes are computed
+	if (cuberow) {
+		const size_t rb0 = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
+		for (int i = threadIdx.x; i < ncube; i += LW_BS)
+			s_crow[i] = crow[rb0 + xs + i];
+	}
+	// row r = wave r: (y + (r&1), z + (r>>1)); exclusive popcount prefix.
+	// All the row's flags are loaded first; s_pre is written only for
+	// 64-point chunks with a flag (it is read only at flagged points) and at
+	// len.
 	{
 		const int r = wave_id(), lane = lane_id();
 		if (r < nrows) {
@@ -543,15 +554,30 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 			if (lane == 0)
 				s_base[r] = edge_add + bases[(size_t)seg + (size_t)g.nseg * ((size_t)yy + (size_t)g.sy * zz)];
 			const size_t rb = (size_t)g.sx * ((size_t)yy + (size_t)g.sy * zz);
+			constexpr int NIT = LW_SEG / N2M_WAVE + 1; // chunks of [0, len]
+			uint32_t f[NIT];
+#pragma unroll
+			for (int k = 0; k < NIT; k++) {
+				const int i = k * N2M_WAVE + lane;
+				f[k] = i < len ? __popc(pflags[rb + xs + i]) : 0u;
+			}
 			uint32_t carry = 0;
-			for (int x0 = 0; x0 <= len; x0 += N2M_WAVE) {
-				const int i = x0 + lane;
-				uint32_t f = i < len ? __popc(pflags[rb + xs + i]) : 0u;
-				uint32_t incl = wave_incl_scan(f);
+#pragma unroll
+			for (int k = 0; k < NIT; k++) {
+				if (k * N2M_WAVE > len)
+					break;
+				if (!__ballot(f[k] != 0u))
+					continue; // no flag in this chunk: carry unchanged
+				const int i = k * N2M_WAVE + lane;
+				const uint32_t incl = wave_incl_scan(f[k]);
 				if (i <= len)
-					s_pre[r][i] = carry + incl - f;
+					s_pre[r][i] = carry + incl - f[k];
 				carry += __shfl(incl, N2M_WAVE - 1, N2M_WAVE);
 			}
+			// the row total, at point len (the next segment's first point,
+			// whose edges close this segment's last cube)
+			if (lane == 0)
+				s_pre[r][len] = carry;
 		}
 	}
 	__syncthreads();
@@ -590,14 +616,12 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 	const size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
 	const uint32_t tbase = bases[bstride + si];
 	const uint32_t cbase = centre_add + bases[2 * bstride + si];
-	const size_t rb0 = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
 	uint32_t tcarry = 0, ccarry = 0;
-	const int ncube = min(xe, g.sx - 1) - xs;
 	for (int i0 = 0; i0 < ncube; i0 += LW_BS) {
 		int i = i0 + threadIdx.x;
 		int row = -1;
 		if (i < ncube)
-			row = (int)crow[rb0 + xs + i] - 1;
+			row = (int)s_crow[i] - 1;
 		if (!__syncthreads_or(row >= 0))
 			continue; // no cube of this chunk has triangles
 		uint32_t nt = row >= 0 ? n2m_lw_row_ntri[row] : 0u;
