@@ -235,11 +235,43 @@ k_ccl_local(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint
 	}
 }
 
-// Pass 2: links whose two runs lie in different tiles, through global
-// union-find on run names.
+// Pass 2 links runs of different tiles through the global union-find.
+// Along a tile face most links join the same two tile components, so each
+// link first takes the runs' current parents (ancestors: linking them links
+// the runs) and a workgroup keeps the pairs it has linked in a small LDS
+// hash set; a pair already there is skipped.
+constexpr int CC_SEEN = 512;
+
+__device__ __forceinline__ void ccl_link(uint32_t *L, unsigned long long *seen, uint32_t a, uint32_t b)
+{
+	uint32_t pa = ld_agent(&L[a]), pb = ld_agent(&L[b]);
+	if (pa == pb)
+		return;
+	if (pa > pb) {
+		const uint32_t t = pa;
+		pa = pb;
+		pb = t;
+	}
+	const unsigned long long key = ((unsigned long long)pa << 32) | pb; // never ~0: ids < 2^32 - 1
+	uint32_t h = (pa * 0x9E3779B1u ^ pb * 0x85EBCA77u) >> 23;         // 9 bits: CC_SEEN slots
+	for (int probe = 0; probe < 8; probe++) {
+		const unsigned long long old = atomicCAS(&seen[h], ~0ull, key);
+		if (old == key)
+			return; // another thread of the workgroup links this pair
+		if (old == ~0ull)
+			break; // inserted: this thread links it
+		h = (h + 1) & (CC_SEEN - 1);
+	}
+	uf_union(L, pa, pb);
+}
+
 __global__ void __launch_bounds__(CC_BS)
 k_ccl_merge(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint32_t *__restrict__ L)
 {
+	__shared__ unsigned long long s_seen[CC_SEEN];
+	for (int i = threadIdx.x; i < CC_SEEN; i += CC_BS)
+		s_seen[i] = ~0ull;
+	__syncthreads();
 	const int k = threadIdx.x % CC_KW, r = threadIdx.x / CC_KW;
 	const int ry = r % CC_TY, rz = r / CC_TY;
 	const int wx = blockIdx.x * CC_KW + k, y = blockIdx.y * CC_TY + ry, z = blockIdx.z * CC_TZ + rz;
@@ -253,7 +285,7 @@ k_ccl_merge(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint
 	if (k == 0 && wx > 0 && (w & 1u)) {
 		uint32_t pw = g.word(bits, wx - 1, y, z, invert);
 		if (pw >> 31)
-			uf_union(L, rb0, g.rb[wi - 1] + run_index(run_starts(pw), 31));
+			ccl_link(L, s_seen, rb0, g.rb[wi - 1] + run_index(run_starts(pw), 31));
 	}
 	int nn;
 	const NbrRow *nb = nbr_rows(conn, &nn);
@@ -276,7 +308,8 @@ k_ccl_merge(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint
 				if (!other)
 					return;
 				uint32_t uw = wo == 0 ? u : (wo < 0 ? up : un);
-				uf_union(L, rb0 + run_index(st, bw), g.rb[g.widx(wx + wo, y2, z2)] + run_index(run_starts(uw), bu));
+				ccl_link(L, s_seen, rb0 + run_index(st, bw),
+				         g.rb[g.widx(wx + wo, y2, z2)] + run_index(run_starts(uw), bu));
 			});
 	}
 }
