@@ -122,6 +122,7 @@ enum Slot {
 	S_HC_INV,
 	S_HC_OUT,
 	S_HC_SLOTS,
+	S_HC_RANK,
 	S_HC_HALO,
 	S_HC_HCNT,
 	S_AT_STATS,
@@ -779,12 +780,12 @@ Csr build_csr(n2m_ctx *C, const int *tris, uint32_t nv, uint32_t nt)
 	c.start = C->buf<uint32_t>(S_HC_START, nv + 1);
 	c.ref = C->buf<uint32_t>(S_HC_REF, (size_t)3 * nt + 1);
 	c.pairs = C->buf<int2>(S_HC_PAIRS, (size_t)3 * nt + 1);
+	uint32_t *rank = C->buf<uint32_t>(S_HC_RANK, (size_t)3 * nt + 1);
 	HIPX(hipMemsetAsync(cnt, 0, 4 * ((size_t)nv + 1), s));
 	C->tbegin("csr_build", 12.0 * nt * 4 + 8.0 * nv);
-	launch_csr_count(tris, nt, cnt, s);
+	launch_csr_count(tris, nt, cnt, rank, s);
 	scan_excl_u32(cnt, c.start, nv + 1, nullptr, scan_scratch(C, nv + 1), s);
-	HIPX(hipMemsetAsync(cnt, 0, 4 * ((size_t)nv + 1), s));
-	launch_csr_fill(tris, nt, c.start, cnt, c.ref, s);
+	launch_csr_fill(tris, nt, c.start, rank, c.ref, s);
 	launch_csr_pairs(tris, nv, c.start, c.ref, c.pairs, s);
 	C->tend();
 	return c;

@@ -416,22 +416,23 @@ void launch_compact_tris(const int *tris, const uint32_t *keep, const uint32_t *
 // (t*3+k) in ascending order -- the order src/quadric.c:321-332 adds them --
 // stored as the pair of the other two corners (lower corner first).
 // ---------------------------------------------------------------------------
-__global__ void k_csr_count(const int *__restrict__ tris, uint32_t nt, uint32_t *__restrict__ cnt)
+// references per vertex; each corner keeps its arrival rank among its
+// vertex's references (any order: k_csr_pairs sorts each vertex's list)
+__global__ void k_csr_count(const int *__restrict__ tris, uint32_t nt, uint32_t *__restrict__ cnt,
+                            uint32_t *__restrict__ rank)
 {
 	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < 3 * nt)
-		atomicAdd(&cnt[tris[i]], 1u);
+		rank[i] = atomicAdd(&cnt[tris[i]], 1u);
 }
 
 __global__ void k_csr_fill(const int *__restrict__ tris, uint32_t nt, const uint32_t *__restrict__ start,
-                           uint32_t *__restrict__ cursor, uint32_t *__restrict__ ref)
+                           const uint32_t *__restrict__ rank, uint32_t *__restrict__ ref)
 {
 	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= 3 * nt)
 		return;
-	int v = tris[i];
-	uint32_t o = start[v] + atomicAdd(&cursor[v], 1u);
-	ref[o] = i;
+	ref[start[tris[i]] + rank[i]] = i;
 }
 
 // sort each vertex's references, then expand to neighbour pairs
@@ -944,14 +945,14 @@ __global__ void k_hc_store(const d3 *__restrict__ p, const d3 *__restrict__ o, c
 		dst[perm ? perm[i] : i] = (border && border[i]) ? o[i] : p[i];
 }
 
-void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, hipStream_t s)
+void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, uint32_t *rank, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_csr_count, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, cnt);
+	hipLaunchKernelGGL(k_csr_count, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, cnt, rank);
 }
-void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, uint32_t *cursor, uint32_t *ref,
+void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, const uint32_t *rank, uint32_t *ref,
                      hipStream_t s)
 {
-	hipLaunchKernelGGL(k_csr_fill, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, start, cursor, ref);
+	hipLaunchKernelGGL(k_csr_fill, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, start, rank, ref);
 }
 void launch_csr_pairs(const int *tris, uint32_t nv, const uint32_t *start, uint32_t *ref, int2 *pairs,
                       hipStream_t s)

@@ -170,8 +170,10 @@ void launch_degen_flags(const double *pts, const int *tris, uint32_t nt, uint32_
                         hipStream_t s);
 void launch_compact_tris(const int *tris, const uint32_t *keep, const uint32_t *pos, uint32_t nt, int *out,
                          hipStream_t s);
-void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, hipStream_t s);
-void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, uint32_t *cursor, uint32_t *ref,
+// CSR of the references: count (with each corner's arrival rank), scan,
+// fill at start + rank, then sort each vertex's list and expand to pairs
+void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, uint32_t *rank, hipStream_t s);
+void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, const uint32_t *rank, uint32_t *ref,
                      hipStream_t s);
 void launch_csr_pairs(const int *tris, uint32_t nv, const uint32_t *start, uint32_t *ref, int2 *pairs,
                       hipStream_t s);
