@@ -1181,23 +1181,23 @@ __global__ void k_border_pairs(const uint32_t *__restrict__ start, const int2 *_
 		return;
 	uint32_t b = start[v], e = start[v + 1];
 	uint8_t r = 0;
-	constexpr int RB = 16;
-	if (e - b <= RB) { // the common case: the pairs held in registers
-		int2 q[RB];
+	constexpr int RB = 12;
+	if (e - b <= RB) { // the common case: the pair ids held in registers
+		int id[2 * RB];
 #pragma unroll
-		for (int k = 0; k < RB; k++)
-			q[k] = b + k < e ? pairs[b + k] : make_int2(-1, -1); // ids are >= 0
+		for (int k = 0; k < RB; k++) {
+			const int2 q = b + k < e ? pairs[b + k] : make_int2(-1, -1); // ids are >= 0
+			id[2 * k] = q.x;
+			id[2 * k + 1] = q.y;
+		}
 #pragma unroll
-		for (int i = 0; i < RB; i++)
+		for (int i = 0; i < 2 * RB; i++) {
+			uint32_t cnt = 0;
 #pragma unroll
-			for (int s = 0; s < 2; s++) {
-				const int a = s ? q[i].y : q[i].x;
-				uint32_t cnt = 0;
-#pragma unroll
-				for (int k = 0; k < RB; k++)
-					cnt += (q[k].x == a) + (q[k].y == a);
-				r |= (a >= 0 && cnt == 1);
-			}
+			for (int k = 0; k < 2 * RB; k++)
+				cnt += id[k] == id[i];
+			r |= (id[i] >= 0 && cnt == 1);
+		}
 		border[v] = r;
 		return;
 	}
