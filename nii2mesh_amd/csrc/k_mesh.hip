@@ -309,7 +309,10 @@ __global__ void k_remap_tris(const int *__restrict__ tris_in, const int *__restr
 	tris_out[i] = (v >= 0 && (uint32_t)v < npt) ? cls[v] : -1;
 }
 
-static unsigned g1(size_t n) { return (unsigned)((n + 255) / 256); }
+// workgroups of 256 for n items; at least one, since an empty mesh (no
+// triangle, no vertex) still launches and every kernel here returns for
+// items at or beyond its count
+static unsigned g1(size_t n) { return n ? (unsigned)((n + 255) / 256) : 1u; }
 
 __global__ void k_iota(int *__restrict__ out, uint32_t n)
 {

@@ -176,6 +176,58 @@ def test_hc_gather_forms(gpu, oracle, monkeypatch, form, iters, lock):
     assert ('hc_halo' in stats) == (form == 'halo')
 
 
+def test_hc_halo_overflow_falls_back(gpu, oracle):
+    """A mesh without locality (random triangles over 40000 vertices): every
+    1024-vertex block has far more than 1024 outside neighbours, so the halo
+    build overflows and the window kernels run; the result is still the
+    oracle's, bit for bit."""
+    import torch
+    rng = np.random.default_rng(17)
+    nv, nt = 40000, 30000
+    pts = rng.normal(size=(nv, 3)) * 10.0
+    tris = np.stack([rng.permutation(nv)[:nt], rng.permutation(nv)[:nt], rng.permutation(nv)[:nt]], 1)
+    tris = tris[(tris[:, 0] != tris[:, 1]) & (tris[:, 1] != tris[:, 2]) & (tris[:, 0] != tris[:, 2])]
+    tris = np.ascontiguousarray(tris, np.int32)
+    a, b = pts.copy(), pts.copy()
+    gpu.lib().laplacian_smoothHC(a, tris, 0.1, 0.5, 6, True)
+    oracle.laplacian_smoothHC(b, tris, 0.1, 0.5, 6, True)
+    assert np.array_equal(a, b)
+    c, d = pts.copy(), pts.copy()
+    gpu.load().n2m_laplacian_smooth(c, tris, len(c), len(tris), 4)
+    oracle.laplacian_smooth(d, tris, 4)
+    assert np.array_equal(c, d)
+
+
+@pytest.mark.parametrize('case', ['no_iterations', 'no_triangles', 'one_triangle', 'isolated_vertices',
+                                  'ragged_blocks'])
+def test_hc_small_and_ragged(gpu, oracle, case):
+    """laplacian_smoothHC edge cases: 0 iterations, no triangle at all, a
+    single triangle (all border), vertices no triangle uses, and a vertex
+    count that is not a multiple of the 1024-vertex block."""
+    rng = np.random.default_rng(23)
+    iters = 5
+    if case == 'no_iterations':
+        img = support.blobs((20, 22, 24), 2, noise=1.0)
+        _, tris, pts = oracle.meshify(img.copy(), float(np.percentile(img, 50)), True, False, False, cubes=1)
+        iters = 0
+    elif case == 'no_triangles':
+        pts, tris = rng.normal(size=(50, 3)), np.zeros((0, 3), np.int32)
+    elif case == 'one_triangle':
+        pts, tris = rng.normal(size=(3, 3)), np.array([[0, 1, 2]], np.int32)
+    elif case == 'isolated_vertices':
+        pts = rng.normal(size=(12, 3))
+        tris = np.array([[0, 1, 2], [2, 1, 3], [3, 1, 4], [4, 1, 0]], np.int32)
+    else:
+        img = support.blobs((70, 74, 78), 6, noise=2.0)
+        _, tris, pts = oracle.meshify(img.copy(), float(np.percentile(img, 50)), True, False, False, cubes=0)
+        assert len(pts) % 1024 != 0 and len(pts) > 32768
+    for lock in (True, False):
+        a, b = pts.copy(), pts.copy()
+        gpu.lib().laplacian_smoothHC(a, tris, 0.1, 0.5, iters, lock)
+        oracle.laplacian_smoothHC(b, tris, 0.1, 0.5, iters, lock)
+        assert np.array_equal(a, b), (case, lock)
+
+
 @pytest.mark.parametrize('seed', [11, 12, 13])
 @pytest.mark.parametrize('mode', [1, 2, 3])
 def test_set_threshold(gpu, oracle, seed, mode):
