@@ -289,3 +289,11 @@ def nan_eq(a, b):
         return False
     u = np.uint64 if a.dtype.itemsize == 8 else np.uint32
     return np.array_equal(np.where(na, 0, a).view(u), np.where(nb, 0, b).view(u))
+
+
+def tiny_volume(shape):
+    """A small test volume: blobs() when every side is at least 2, else
+    seeded normal noise (blobs needs room for its shapes)."""
+    if min(shape) > 1:
+        return blobs(shape, 3, noise=1.0)
+    return np.random.default_rng(1).normal(size=shape).astype(np.float32)

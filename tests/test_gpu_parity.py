@@ -286,6 +286,22 @@ def test_meshify_nan_volume(gpu, oracle, flags, flavour):
     assert ra[1].min() >= 0 and ra[1].max() < len(ra[2])
 
 
+@pytest.mark.parametrize('shape', [(1, 6, 7), (2, 2, 2), (3, 1, 4), (5, 5, 1), (1, 1, 1), (6, 5, 4)])
+@pytest.mark.parametrize('cubes', [0, 1])
+def test_meshify_tiny_volumes(gpu, oracle, shape, cubes):
+    """Volumes with a dimension of 1 or 2: the same return codes, meshes and
+    mutated volume as the oracle."""
+    img = support.tiny_volume(shape)
+    iso = float(np.float32(np.median(img)))
+    for flags in [(True, True, True), (False, False, False)]:
+        a, b = img.copy(), img.copy()
+        ra = gpu.lib(cubes).meshify(a, iso, *flags)
+        rb = oracle.meshify(b, iso, *flags, cubes=cubes)
+        assert ra[0] == rb[0]
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert _eq(ra[1], rb[1]) and _eq(ra[2], rb[2])
+
+
 def test_meshify_iso_fallback_and_failures(gpu, oracle):
     """Isolevel outside the range -> midpoint (src/meshify.c:316-319);
     constant volume -> failure (:312-315)."""
