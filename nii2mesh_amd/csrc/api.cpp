@@ -116,11 +116,9 @@ enum Slot {
 	S_HC_P,
 	S_HC_PN,
 	S_HC_B,
-	S_HC_Q,
 	S_HIST,
 	S_HC_TRIS,
 	S_HC_INV,
-	S_HC_OUT,
 	S_HC_SLOTS,
 	S_HC_RANK,
 	S_HC_HALO,
@@ -134,8 +132,6 @@ enum Slot {
 	S_AT_VTASK,
 	S_AT_K2,
 	S_AT_V2,
-	S_AT_K3,
-	S_AT_V3,
 	S_AT_CBASE,
 	S_AT_IDENT,
 	S_AT_IDX,

@@ -1138,22 +1138,6 @@ __global__ void k_invert_perm(const uint32_t *__restrict__ perm, uint32_t n, uin
 		inv[perm[i]] = i;
 }
 
-__global__ void k_gather_rows(const d3 *__restrict__ src, const uint32_t *__restrict__ perm, uint32_t n,
-                              d3 *__restrict__ dst)
-{
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n)
-		dst[i] = src[perm[i]];
-}
-
-__global__ void k_scatter_rows(const d3 *__restrict__ src, const uint32_t *__restrict__ perm, uint32_t n,
-                               d3 *__restrict__ dst)
-{
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n)
-		dst[perm[i]] = src[i];
-}
-
 __global__ void k_relabel_tris(const int *__restrict__ tris, const uint32_t *__restrict__ inv, uint32_t n3,
                                int *__restrict__ out)
 {
@@ -1232,14 +1216,6 @@ void launch_morton(const double *pts, uint32_t n, const uint32_t *bb, uint32_t *
 void launch_invert_perm(const uint32_t *perm, uint32_t n, uint32_t *inv, hipStream_t s)
 {
 	hipLaunchKernelGGL(k_invert_perm, dim3(g1(n)), dim3(256), 0, s, perm, n, inv);
-}
-void launch_gather_rows(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s)
-{
-	hipLaunchKernelGGL(k_gather_rows, dim3(g1(n)), dim3(256), 0, s, (const d3 *)src, perm, n, (d3 *)dst);
-}
-void launch_scatter_rows(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s)
-{
-	hipLaunchKernelGGL(k_scatter_rows, dim3(g1(n)), dim3(256), 0, s, (const d3 *)src, perm, n, (d3 *)dst);
 }
 void launch_relabel_tris(const int *tris, const uint32_t *inv, uint32_t nt, int *out, hipStream_t s)
 {

@@ -208,8 +208,6 @@ void launch_pts_bbox(const double *pts, uint32_t n, uint32_t *bb, hipStream_t s)
 void launch_morton(const double *pts, uint32_t n, const uint32_t *bb, uint32_t *keys, uint32_t *vals,
                    hipStream_t s);
 void launch_invert_perm(const uint32_t *perm, uint32_t n, uint32_t *inv, hipStream_t s);
-void launch_gather_rows(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s);
-void launch_scatter_rows(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s);
 void launch_relabel_tris(const int *tris, const uint32_t *inv, uint32_t nt, int *out, hipStream_t s);
 void launch_tris_repeat(const int *tris, uint32_t nt, uint32_t *flag, hipStream_t s);
 void launch_border_pairs(const uint32_t *start, const int2 *pairs, uint32_t nv, uint8_t *border, hipStream_t s);
