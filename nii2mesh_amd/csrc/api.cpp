@@ -216,6 +216,7 @@ struct n2m_ctx {
 	int cur_stat = -1;
 	hipEvent_t cur_a = nullptr;
 	double cur_bytes = 0;
+	n2m_ctx *aux = nullptr; // second stream and scratch: smoothing overlapped with meshing
 
 	void *get(Slot s, size_t bytes)
 	{
@@ -317,6 +318,24 @@ struct n2m_ctx {
 		}
 		pending.clear();
 	}
+	// add the (resolved) stage statistics of another context into these
+	void absorb(n2m_ctx &o)
+	{
+		o.resolve();
+		for (auto &st : o.stats) {
+			size_t k = 0;
+			while (k < stats.size() && stats[k].name != st.name)
+				k++;
+			if (k == stats.size()) {
+				stats.push_back(Stat());
+				stats.back().name = st.name;
+			}
+			stats[k].ms += st.ms;
+			stats[k].n += st.n;
+			stats[k].bytes += st.bytes;
+		}
+		o.stats.clear();
+	}
 };
 
 namespace {
@@ -345,6 +364,7 @@ void ctx_free(n2m_ctx *c)
 {
 	if (!c)
 		return;
+	ctx_free(c->aux);
 	(void)hipStreamSynchronize(c->stream);
 	for (auto &b : c->bufs)
 		if (b.p)
@@ -888,6 +908,10 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 	double *o = C->buf<double>(S_HC_O, n3), *p = C->buf<double>(S_HC_P, n3);
 	double *pn = C->buf<double>(S_HC_PN, n3), *bq = C->buf<double>(S_HC_B, n3);
 	launch_hc_load(pts, R.on ? R.perm : nullptr, nv, o, s);
+	// the border flags depend on the topology only; computed before the
+	// iterations, so that every host round trip comes first and the
+	// iterations are the last work enqueued
+	uint8_t *border = lockEdges ? border_flags(C, c, T, nv, nt) : nullptr;
 	// algorithmic bytes: CSR offsets, one pair word per reference (4 bytes
 	// as LDS slots, 8 as vertex ids), each position array once, the outputs
 	const double pair_bytes = (H.on ? 4.0 : 8.0) * 3 * nt;
@@ -906,7 +930,6 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 			launch_hc_b(c.start, c.pairs, pn, bq, p, nv, beta, s);
 		C->tend();
 	}
-	uint8_t *border = lockEdges ? border_flags(C, c, T, nv, nt) : nullptr;
 	launch_hc_store(iter > 0 ? p : o, o, border, R.on ? R.perm : nullptr, nv, pts, s);
 	C->check();
 }
@@ -2000,6 +2023,21 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 			range[0] = r[0];
 			range[1] = r[1];
 		}
+		// With N2M_PIPE_OVERLAP=1, mesh k is smoothed on a second context
+		// (own stream and scratch) while isolevel k+1 is meshed; the second
+		// stream waits for mesh k on an event.  Off by default: on the 512^3
+		// three-isolevel workload it saved 1.4 % of the step, while the HC
+		// kernels, sharing the CUs, ran 15 % slower.
+		const char *ov = getenv("N2M_PIPE_OVERLAP");
+		n2m_ctx *D = C;
+		if (hcIter > 0 && ov && *ov == '1') {
+			if (!C->aux)
+				C->aux = ctx_new();
+			D = C->aux;
+			D->timing = C->timing;
+			D->timing_only = C->timing_only;
+		}
+		std::vector<hipEvent_t> ready;
 		int ok = 0;
 		for (int k = 0; k < niso; k++) {
 			n2m_params P = *prm;
@@ -2012,12 +2050,25 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 			VolOut vo;
 			if (!volume_stages(C, vol, dim, P, smoothed, false, vo) || !surface_stages(C, vol, dim, vo, P, m))
 				continue;
-			if (hcIter > 0)
-				smooth_hc_device(C, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
+			if (hcIter > 0) {
+				if (D != C) {
+					hipEvent_t e = C->ev();
+					HIPX(hipEventRecord(e, s));
+					HIPX(hipStreamWaitEvent(D->stream, e, 0));
+					ready.push_back(e);
+				}
+				smooth_hc_device(D, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
 				                 true);
+			}
 			ok++;
 		}
+		if (D != C) {
+			D->sync();
+			C->absorb(*D);
+		}
 		C->sync();
+		for (hipEvent_t e : ready)
+			C->evpool.push_back(e);
 		return ok == niso ? EXIT_SUCCESS : EXIT_FAILURE;
 	});
 }

@@ -176,6 +176,31 @@ def test_hc_gather_forms(gpu, oracle, monkeypatch, form, iters, lock):
     assert ('hc_halo' in stats) == (form == 'halo')
 
 
+@pytest.mark.parametrize('overlap', ['1', '0'])
+@pytest.mark.parametrize('cubes', [0, 1])
+def test_pipeline_isolevels(gpu, oracle, monkeypatch, overlap, cubes):
+    """The device pipeline (one pre-smooth shared by several isolevels, each
+    meshed and HC-smoothed; with overlap on, mesh k is smoothed on a second
+    stream while isolevel k+1 is meshed) gives, per isolevel, the oracle's
+    meshify + laplacian_smoothHC(lockEdges) bit for bit."""
+    import torch
+    monkeypatch.setenv('N2M_PIPE_OVERLAP', overlap)
+    img = support.blobs((60, 64, 68), 8, noise=2.0)
+    isos = [float(np.float32(np.percentile(img, q))) for q in (40, 60, 80)]
+    ctx = gpu.DeviceContext()
+    d = torch.from_numpy(img).cuda()
+    torch.cuda.synchronize()  # the library runs on its own stream
+    prm = ctx.params(0.0, cubes=cubes, preSmooth=True, onlyLargest=True, fillBubbles=True)
+    assert ctx.pipeline(d.data_ptr(), (68, 64, 60), prm, isos, hc_iters=10) == 0
+    got = [ctx.mesh(k) for k in range(len(isos))]
+    ctx.close()
+    for k, iso in enumerate(isos):
+        rc, to, po = oracle.meshify(img.copy(), iso, True, True, True, cubes=cubes)
+        assert rc == 0
+        oracle.laplacian_smoothHC(po, to, 0.1, 0.5, 10, True)
+        assert _eq(got[k][0], to) and np.array_equal(got[k][1], po), k
+
+
 def test_hc_halo_overflow_falls_back(gpu, oracle):
     """A mesh without locality (random triangles over 40000 vertices): every
     1024-vertex block has far more than 1024 outside neighbours, so the halo
