@@ -58,15 +58,18 @@ double now_ms()
 	return t.tv_sec * 1000.0 + t.tv_nsec / 1.0e6;
 }
 
-int g_cubes = -1; // process-wide marching-cubes flavour
+std::atomic<int> g_cubes{-1}; // process-wide marching-cubes flavour (-1: not chosen yet)
 
 int cubes_flavour()
 {
-	if (g_cubes < 0) {
+	int v = g_cubes.load();
+	if (v < 0) {
 		const char *e = getenv("N2M_CUBES");
-		g_cubes = (e && (e[0] == 'l' || e[0] == 'L' || e[0] == '1')) ? N2M_CUBES_LEWINER : N2M_CUBES_CLASSIC;
+		int want = (e && (e[0] == 'l' || e[0] == 'L' || e[0] == '1')) ? N2M_CUBES_LEWINER : N2M_CUBES_CLASSIC;
+		g_cubes.compare_exchange_strong(v, want); // keeps a value set meanwhile
+		v = g_cubes.load();
 	}
-	return g_cubes;
+	return v;
 }
 
 // buffer slots of a context

@@ -973,13 +973,11 @@ void launch_border(const int *tris, uint32_t nv, const uint32_t *start, const ui
 
 static int hc_shape()
 {
-	static int v = -1;
-	if (v < 0) {
+	static const int v = [] { // read once, thread-safe initialisation
 		const char *e = getenv("N2M_HC_SHAPE");
-		v = e ? atoi(e) : 0;
-		if (v < 0 || v > 3)
-			v = 0;
-	}
+		const int k = e ? atoi(e) : 0;
+		return k < 0 || k > 3 ? 0 : k;
+	}();
 	return v;
 }
 

@@ -327,6 +327,38 @@ def test_meshify_tiny_volumes(gpu, oracle, shape, cubes):
         assert _eq(ra[1], rb[1]) and _eq(ra[2], rb[2])
 
 
+@pytest.mark.parametrize('cubes', [0, 1])
+def test_meshify_concurrent_threads(gpu, oracle, cubes):
+    """meshify() and laplacian_smoothHC() called from several host threads
+    at once, as the atlas mode's OpenMP loop does (src/nii2mesh.c:546-551):
+    each thread uses its own context and stream, and every result is the
+    oracle's."""
+    import threading
+    lib = gpu.lib(cubes)
+    vols = [support.blobs((36 + 4 * i, 44, 48), 30 + i, noise=2.0) for i in range(6)]
+    isos = [float(np.float32(np.percentile(v, 55))) for v in vols]
+    out = [None] * len(vols)
+
+    def work(i):
+        r = lib.meshify(vols[i].copy(), isos[i], True, True, True)
+        p = r[2].copy()
+        if r[0] == 0:
+            lib.laplacian_smoothHC(p, r[1], 0.1, 0.5, 10, True)
+        out[i] = (r[0], r[1], p)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(len(vols))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for i, v in enumerate(vols):
+        rc, to, po = oracle.meshify(v.copy(), isos[i], True, True, True, cubes=cubes)
+        assert rc == 0
+        oracle.laplacian_smoothHC(po, to, 0.1, 0.5, 10, True)
+        assert out[i] is not None and out[i][0] == rc
+        assert _eq(out[i][1], to) and np.array_equal(out[i][2], po), i
+
+
 def test_meshify_iso_fallback_and_failures(gpu, oracle):
     """Isolevel outside the range -> midpoint (src/meshify.c:316-319);
     constant volume -> failure (:312-315)."""
