@@ -17,6 +17,8 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <utility>
 #include <vector>
 
 #include <zlib.h>
@@ -219,7 +221,7 @@ struct n2m_ctx {
 	int cur_stat = -1;
 	hipEvent_t cur_a = nullptr;
 	double cur_bytes = 0;
-	n2m_ctx *aux = nullptr; // second stream and scratch: smoothing overlapped with meshing
+	std::vector<n2m_ctx *> workers; // per-isolevel contexts of the pipeline's threaded meshing
 
 	void *get(Slot s, size_t bytes)
 	{
@@ -367,7 +369,8 @@ void ctx_free(n2m_ctx *c)
 {
 	if (!c)
 		return;
-	ctx_free(c->aux);
+	for (n2m_ctx *w : c->workers)
+		ctx_free(w);
 	(void)hipStreamSynchronize(c->stream);
 	for (auto &b : c->bufs)
 		if (b.p)
@@ -2013,7 +2016,6 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 		hipStream_t s = C->stream;
 		C->meshes.resize(niso);
 		float *sm = C->buf<float>(S_VOL2, nvox); // smoothed volume, shared by all isolevels
-		float *vol = C->buf<float>(S_VOL, nvox);
 		bool smoothed = prm->preSmooth && NX >= 5 && NY >= 5 && NZ >= 5;
 		uint32_t range[2];
 		if (smoothed) {
@@ -2026,52 +2028,69 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 			range[0] = r[0];
 			range[1] = r[1];
 		}
-		// With N2M_PIPE_OVERLAP=1, mesh k is smoothed on a second context
-		// (own stream and scratch) while isolevel k+1 is meshed; the second
-		// stream waits for mesh k on an event.  Off by default: on the 512^3
-		// three-isolevel workload it saved 1.4 % of the step, while the HC
-		// kernels, sharing the CUs, ran 15 % slower.
-		const char *ov = getenv("N2M_PIPE_OVERLAP");
-		n2m_ctx *D = C;
-		if (hcIter > 0 && ov && *ov == '1') {
-			if (!C->aux)
-				C->aux = ctx_new();
-			D = C->aux;
-			D->timing = C->timing;
-			D->timing_only = C->timing_only;
-		}
-		std::vector<hipEvent_t> ready;
-		int ok = 0;
-		for (int k = 0; k < niso; k++) {
+		// The isolevels are meshed by one host thread and context (stream and
+		// scratch; C->workers, kept between calls) each: the meshing stages
+		// are mostly bound by latency, so they overlap well.  The HC passes,
+		// bound by bandwidth, then run one after another on this context.
+		// N2M_PIPE_THREADS=0 meshes the isolevels one after another.
+		const char *pt = getenv("N2M_PIPE_THREADS");
+		const bool threaded = niso > 1 && !(pt && *pt == '0');
+		std::vector<char> done(niso, 0);
+		auto mesh_one = [&](n2m_ctx *D, int k) {
 			n2m_params P = *prm;
 			P.isolevel = isolevels[k];
-			HIPX(hipMemcpyAsync(vol, smoothed ? sm : d_img, 4 * nvox, hipMemcpyDeviceToDevice, s));
+			float *v = D->buf<float>(S_VOL, nvox);
+			HIPX(hipMemcpyAsync(v, smoothed ? sm : d_img, 4 * nvox, hipMemcpyDeviceToDevice, D->stream));
 			if (smoothed)
-				HIPX(hipMemcpyAsync(C->d_small + SM_RANGE, range, sizeof range, hipMemcpyHostToDevice, s));
+				HIPX(hipMemcpyAsync(D->d_small + SM_RANGE, range, sizeof range, hipMemcpyHostToDevice, D->stream));
 			Mesh &m = C->meshes[k];
 			m.nt = m.np = 0;
 			VolOut vo;
-			if (!volume_stages(C, vol, dim, P, smoothed, false, vo) || !surface_stages(C, vol, dim, vo, P, m))
-				continue;
-			if (hcIter > 0) {
-				if (D != C) {
-					hipEvent_t e = C->ev();
-					HIPX(hipEventRecord(e, s));
-					HIPX(hipStreamWaitEvent(D->stream, e, 0));
-					ready.push_back(e);
-				}
-				smooth_hc_device(D, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
-				                 true);
+			if (volume_stages(D, v, dim, P, smoothed, false, vo) && surface_stages(D, v, dim, vo, P, m))
+				done[k] = 1;
+			D->sync();
+		};
+		if (threaded) {
+			C->sync(); // the shared smoothed volume is complete
+			while ((int)C->workers.size() < niso)
+				C->workers.push_back(ctx_new());
+			std::vector<std::string> err(niso);
+			std::vector<std::thread> th;
+			for (int k = 0; k < niso; k++) {
+				n2m_ctx *D = C->workers[k];
+				D->timing = C->timing;
+				D->timing_only = C->timing_only;
+				th.emplace_back([&, D, k] {
+					try {
+						HIPX(hipSetDevice(C->device));
+						mesh_one(D, k);
+					} catch (const std::exception &e) {
+						err[k] = e.what();
+					}
+				});
 			}
+			for (auto &t : th)
+				t.join();
+			for (int k = 0; k < niso; k++) {
+				C->absorb(*C->workers[k]);
+				if (!err[k].empty())
+					throw std::runtime_error(err[k]);
+			}
+		} else {
+			for (int k = 0; k < niso; k++)
+				mesh_one(C, k);
+		}
+		int ok = 0;
+		for (int k = 0; k < niso; k++) {
+			if (!done[k])
+				continue;
+			Mesh &m = C->meshes[k];
+			if (hcIter > 0)
+				smooth_hc_device(C, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
+				                 true);
 			ok++;
 		}
-		if (D != C) {
-			D->sync();
-			C->absorb(*D);
-		}
 		C->sync();
-		for (hipEvent_t e : ready)
-			C->evpool.push_back(e);
 		return ok == niso ? EXIT_SUCCESS : EXIT_FAILURE;
 	});
 }

@@ -176,15 +176,15 @@ def test_hc_gather_forms(gpu, oracle, monkeypatch, form, iters, lock):
     assert ('hc_halo' in stats) == (form == 'halo')
 
 
-@pytest.mark.parametrize('overlap', ['1', '0'])
+@pytest.mark.parametrize('threads', ['1', '0'])
 @pytest.mark.parametrize('cubes', [0, 1])
-def test_pipeline_isolevels(gpu, oracle, monkeypatch, overlap, cubes):
+def test_pipeline_isolevels(gpu, oracle, monkeypatch, threads, cubes):
     """The device pipeline (one pre-smooth shared by several isolevels, each
-    meshed and HC-smoothed; with overlap on, mesh k is smoothed on a second
-    stream while isolevel k+1 is meshed) gives, per isolevel, the oracle's
-    meshify + laplacian_smoothHC(lockEdges) bit for bit."""
+    meshed and HC-smoothed -- by one host thread and context per isolevel,
+    or one after the other) gives, per isolevel, the oracle's meshify +
+    laplacian_smoothHC(lockEdges) bit for bit."""
     import torch
-    monkeypatch.setenv('N2M_PIPE_OVERLAP', overlap)
+    monkeypatch.setenv('N2M_PIPE_THREADS', threads)
     img = support.blobs((60, 64, 68), 8, noise=2.0)
     isos = [float(np.float32(np.percentile(img, q))) for q in (40, 60, 80)]
     ctx = gpu.DeviceContext()
