@@ -48,3 +48,25 @@ def test_large_f32_512_thresholds_and_mid_level(gpu, oracle):
     gpu.lib().laplacian_smoothHC(pa, ra[1], 0.1, 0.5, 3, True)
     oracle.laplacian_smoothHC(pb, rb[1], 0.1, 0.5, 3, True)
     assert np.array_equal(pa, pb)
+
+
+def test_large_f32_512_slab_all_levels(gpu, oracle):
+    """All three isolevels of the 512^3 volume (including the bright one,
+    whose mesh is the largest) on its 512 x 512 x 64 slab z in [224, 288):
+    meshify (pre-smooth, largest cluster, bubble fill, Lewiner) and 3 HC
+    iterations equal the oracle's."""
+    from nii2mesh_amd import synth
+    img = synth.large_f32()
+    thr = [gpu.lib().setThreshold(img, m) for m in (1, 2, 3)]
+    slab = np.ascontiguousarray(img[224:288])
+    del img
+    for iso in thr:
+        a, b = slab.copy(), slab.copy()
+        ra = gpu.lib(1).meshify(a, iso, True, True, True)
+        rb = oracle.meshify(b, iso, True, True, True, cubes=1)
+        assert ra[0] == rb[0] == 0
+        assert _eq(ra[1], rb[1]) and _eq(ra[2], rb[2]), iso
+        pa, pb = ra[2].copy(), rb[2].copy()
+        gpu.lib().laplacian_smoothHC(pa, ra[1], 0.1, 0.5, 3, True)
+        oracle.laplacian_smoothHC(pb, rb[1], 0.1, 0.5, 3, True)
+        assert np.array_equal(pa, pb), iso
