@@ -128,6 +128,11 @@ enum Slot {
 	S_HC_RANK,
 	S_HC_HALO,
 	S_HC_HCNT,
+	S_HC_RING,
+	S_HC_RCNT,
+	S_HC_ROFF,
+	S_HC_RSLOTS,
+	S_HC_RO,
 	S_AT_STATS,
 	S_AT_TASKS,
 	S_AT_BMAP,
@@ -178,6 +183,7 @@ enum Small {
 	SM_PBOX = 32,  // 6 words: float box of vertex positions
 	SM_FLAG = 40,  // 1 word
 	SM_HALO = 44,  // 1 word: a smoothing block overflowed its halo
+	SM_RING = 45,  // 1 word: a smoothing block overflowed its two rings
 	SM_WORDS = 64
 };
 
@@ -899,6 +905,36 @@ Halo build_halo(n2m_ctx *C, const Csr &c, uint32_t nv, uint32_t nt)
 	return h;
 }
 
+struct Ring {
+	bool on = false;
+	uint32_t *slots = nullptr, *ring = nullptr, *rcnt = nullptr, *rslots = nullptr;
+	uint16_t *roff = nullptr;
+	double *o1 = nullptr;
+};
+
+// o: the original positions in the smoothing order
+Ring build_ring(n2m_ctx *C, const Csr &c, uint32_t nv, uint32_t nt, const double *o)
+{
+	Ring r;
+	const char *e = getenv("N2M_HC_RING");
+	if ((e && *e == '0') || nv == 0)
+		return r;
+	hipStream_t s = C->stream;
+	r.slots = C->buf<uint32_t>(S_HC_SLOTS, (size_t)3 * nt + 1);
+	r.ring = C->buf<uint32_t>(S_HC_RING, ring_words(nv));
+	r.rcnt = C->buf<uint32_t>(S_HC_RCNT, 2 * (size_t)ring_blocks(nv));
+	r.roff = C->buf<uint16_t>(S_HC_ROFF, ring_off_words(nv));
+	r.rslots = C->buf<uint32_t>(S_HC_RSLOTS, ring_pair_words(nv));
+	r.o1 = C->buf<double>(S_HC_RO, 3 * ring_o_rows(nv));
+	HIPX(hipMemsetAsync(C->d_small + SM_RING, 0, 4, s));
+	C->tbegin("hc_rings", 8.0 * nv + 12.0 * 3 * nt);
+	launch_ring_build(c.start, c.pairs, nv, o, r.ring, r.rcnt, r.slots, r.roff, r.rslots, r.o1,
+	                  C->d_small + SM_RING, s);
+	C->tend();
+	r.on = C->fetch(SM_RING, 1)[0] == 0;
+	return r;
+}
+
 // laplacian_smoothHC on a device mesh in place, src/quadric.c:343-394
 void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint32_t nt, double alpha,
                       double beta, int iter, bool lockEdges)
@@ -909,34 +945,54 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 	Relabel R = morton_relabel(C, pts, tris, nv, nt);
 	const int *T = R.on ? R.tris : tris;
 	Csr c = build_csr(C, T, nv, nt);
-	const Halo H = iter > 0 ? build_halo(C, c, nv, nt) : Halo();
+	// one launch per iteration (two-ring form) where every block's rings fit,
+	// else the halo pair of launches, else the window pair
 	const size_t n3 = 3 * (size_t)nv;
 	double *o = C->buf<double>(S_HC_O, n3), *p = C->buf<double>(S_HC_P, n3);
-	double *pn = C->buf<double>(S_HC_PN, n3), *bq = C->buf<double>(S_HC_B, n3);
+	double *pn = C->buf<double>(S_HC_PN, n3);
 	launch_hc_load(pts, R.on ? R.perm : nullptr, nv, o, s);
+	const Ring G = iter > 0 ? build_ring(C, c, nv, nt, o) : Ring();
+	const Halo H = iter > 0 && !G.on ? build_halo(C, c, nv, nt) : Halo();
 	// the border flags depend on the topology only; computed before the
 	// iterations, so that every host round trip comes first and the
 	// iterations are the last work enqueued
 	uint8_t *border = lockEdges ? border_flags(C, c, T, nv, nt) : nullptr;
-	// algorithmic bytes: CSR offsets, one pair word per reference (4 bytes
-	// as LDS slots, 8 as vertex ids), each position array once, the outputs
-	const double pair_bytes = (H.on ? 4.0 : 8.0) * 3 * nt;
-	for (int it = 0; it < iter; it++) {
-		const double *cur = it == 0 ? o : p; // p starts as the original
-		C->tbegin("hc_a", 4.0 * nv + pair_bytes + 24.0 * nv * 2 + 48.0 * nv);
-		if (H.on)
-			launch_hc_a_halo(c.start, H.slots, H.halo, H.hcount, o, cur, pn, bq, nv, alpha, s);
-		else
-			launch_hc_a(c.start, c.pairs, o, cur, pn, bq, nv, alpha, s);
-		C->tend();
-		C->tbegin("hc_b", 4.0 * nv + pair_bytes + 24.0 * nv * 2 + 24.0 * nv);
-		if (H.on)
-			launch_hc_b_halo(c.start, H.slots, H.halo, H.hcount, pn, bq, p, nv, beta, s);
-		else
-			launch_hc_b(c.start, c.pairs, pn, bq, p, nv, beta, s);
-		C->tend();
+	const double *res = o;
+	if (G.on) {
+		// algorithmic bytes: CSR offsets, one 4-byte slot pair per reference,
+		// the original and current positions in, the new positions out
+		for (int it = 0; it < iter; it++) {
+			double *nxt = it % 2 == 0 ? p : pn;
+			C->tbegin("hc_ring", 4.0 * nv + 12.0 * nt + 72.0 * nv);
+			launch_hc_ring(c.start, G.slots, G.roff, G.rslots, G.ring, G.rcnt, o, G.o1, res, nxt, nv, alpha, beta,
+			               s);
+			C->tend();
+			res = nxt;
+		}
+	} else {
+		double *bq = C->buf<double>(S_HC_B, n3);
+		// algorithmic bytes: CSR offsets, one pair word per reference (4
+		// bytes as LDS slots, 8 as vertex ids), each position array once,
+		// the outputs
+		const double pair_bytes = (H.on ? 4.0 : 8.0) * 3 * nt;
+		for (int it = 0; it < iter; it++) {
+			const double *cur = it == 0 ? o : p; // p starts as the original
+			C->tbegin("hc_a", 4.0 * nv + pair_bytes + 24.0 * nv * 2 + 48.0 * nv);
+			if (H.on)
+				launch_hc_a_halo(c.start, H.slots, H.halo, H.hcount, o, cur, pn, bq, nv, alpha, s);
+			else
+				launch_hc_a(c.start, c.pairs, o, cur, pn, bq, nv, alpha, s);
+			C->tend();
+			C->tbegin("hc_b", 4.0 * nv + pair_bytes + 24.0 * nv * 2 + 24.0 * nv);
+			if (H.on)
+				launch_hc_b_halo(c.start, H.slots, H.halo, H.hcount, pn, bq, p, nv, beta, s);
+			else
+				launch_hc_b(c.start, c.pairs, pn, bq, p, nv, beta, s);
+			C->tend();
+			res = p;
+		}
 	}
-	launch_hc_store(iter > 0 ? p : o, o, border, R.on ? R.perm : nullptr, nv, pts, s);
+	launch_hc_store(res, o, border, R.on ? R.perm : nullptr, nv, pts, s);
 	C->check();
 }
 

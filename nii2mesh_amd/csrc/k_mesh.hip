@@ -823,8 +823,25 @@ __device__ __forceinline__ void stage_halo(double *lds, const d3 *__restrict__ p
 	__syncthreads();
 }
 
+// row s of an LDS array of W-double rows (W = 4: x, y as one 16-byte read)
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) dbl2 lds_dbl2;
+template <int W> __device__ __forceinline__ d3 lds_row_w(const lds_double *L, uint32_t s)
+{
+	if constexpr (W == 4) {
+		const dbl2 xy = ((const lds_dbl2 *)L)[2 * s];
+		d3 r;
+		r.x = xy.x;
+		r.y = xy.y;
+		r.z = L[4 * s + 2];
+		return r;
+	} else {
+		return lds_row(L, s);
+	}
+}
+
 // laplace_at() over LDS slots: same terms, same order
-template <int K>
+template <int K, int W = 3>
 __device__ __forceinline__ d3 laplace_slots(const SlotRefs<K> &r, const uint32_t *__restrict__ slots,
                                             const lds_double *L, const d3 self)
 {
@@ -835,7 +852,7 @@ __device__ __forceinline__ d3 laplace_slots(const SlotRefs<K> &r, const uint32_t
 #pragma unroll
 	for (int k = 0; k < K; k++) {
 		if ((uint32_t)k < n) {
-			const d3 P = lds_row(L, r.q[k] & 0xffffu), Q = lds_row(L, r.q[k] >> 16);
+			const d3 P = lds_row_w<W>(L, r.q[k] & 0xffffu), Q = lds_row_w<W>(L, r.q[k] >> 16);
 			sx = sx + (P.x + Q.x);
 			sy = sy + (P.y + Q.y);
 			sz = sz + (P.z + Q.z);
@@ -843,7 +860,7 @@ __device__ __forceinline__ d3 laplace_slots(const SlotRefs<K> &r, const uint32_t
 	}
 	for (uint32_t i = r.b + K; i < r.e; i++) {
 		const uint32_t s = slots[i];
-		const d3 P = lds_row(L, s & 0xffffu), Q = lds_row(L, s >> 16);
+		const d3 P = lds_row_w<W>(L, s & 0xffffu), Q = lds_row_w<W>(L, s >> 16);
 		sx = sx + (P.x + Q.x);
 		sy = sy + (P.y + Q.y);
 		sz = sz + (P.z + Q.z);
@@ -928,6 +945,323 @@ k_laplace_h(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slo
 		return;
 	const lds_double *L = (const lds_double *)lds;
 	pout[v] = laplace_slots<HK>(r, slots, L, lds_row(L, threadIdx.x));
+}
+
+// ---------------------------------------------------------------------------
+// Two-ring form: one launch per HC iteration.  A workgroup of RT threads owns
+// RB consecutive vertices.  Ring 1 is the sorted distinct neighbours outside
+// the block, ring 2 the neighbours of ring 1 outside block and ring 1.  The
+// kernel stages the positions of block, ring 1 and ring 2 in LDS (row slots:
+// block 0..RB-1, ring 1 from RB, ring 2 after it), computes the Laplacian and
+// b for block and ring-1 vertices (the hc_a step), replaces the rows with b,
+// and finishes the block's vertices (the hc_b step).  Only the new positions
+// go to memory: no pn / b arrays.  Ring-1 vertices are computed by every block
+// that borders them, with the same terms in the same order, so the values are
+// those of the two-launch form.  Ring-1 pair lists are kept per block as LDS
+// slots (roff: 16-bit offsets, rslots: slot pairs).
+// ---------------------------------------------------------------------------
+constexpr int RB = 768, RT = 1024;
+constexpr int R1CAP = 1024;      // ring-1 vertices per block (RB + R1CAP <= 2 * RT)
+constexpr int RROWS = 2304;      // LDS rows: RB + ring 1 + ring 2
+constexpr int RCAP = RROWS - RB; // ring words per block
+constexpr int RPCAP = 4096;      // ring-1 pair words per block
+constexpr int ROFFW = R1CAP + 2; // 16-bit ring-1 offsets per block
+constexpr int RRAW = 4 * RT;     // raw outside references collected per pass
+
+__device__ __forceinline__ bool in_sorted(const uint32_t *h, uint32_t n, uint32_t x, uint32_t *at)
+{
+	uint32_t lo = 0, hi = n;
+	while (lo < hi) {
+		uint32_t mid = (lo + hi) >> 1;
+		if (h[mid] < x)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	*at = lo;
+	return lo < n && h[lo] == x;
+}
+
+// sort raw[0, n) ascending (bitonic, padded with ~0 to a power of two; n <=
+// RRAW) and keep the first of each run of equal ids, compacted to raw[0, H).
+// Every thread returns H.  Contains barriers.
+__device__ __forceinline__ uint32_t sort_dedup(uint32_t *raw, uint32_t n, uint32_t *scratch)
+{
+	uint32_t P = 1;
+	while (P < n)
+		P <<= 1;
+	for (uint32_t i = n + threadIdx.x; i < P; i += RT)
+		raw[i] = 0xffffffffu;
+	__syncthreads();
+	for (uint32_t size = 2; size <= P; size <<= 1)
+		for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+			for (uint32_t i = threadIdx.x; i < P; i += RT) {
+				const uint32_t j = i ^ stride;
+				if (j > i) {
+					const uint32_t a = raw[i], c = raw[j];
+					if ((a > c) == ((i & size) == 0)) {
+						raw[i] = c;
+						raw[j] = a;
+					}
+				}
+			}
+			__syncthreads();
+		}
+	uint32_t val[4], cnt = 0;
+	bool keep[4];
+#pragma unroll
+	for (int q = 0; q < 4; q++) {
+		const uint32_t i = 4 * threadIdx.x + q;
+		val[q] = i < n ? raw[i] : 0u;
+		keep[q] = i < n && (i == 0 || raw[i] != raw[i - 1]);
+		cnt += keep[q];
+	}
+	uint32_t H;
+	uint32_t off = block_excl_scan<RT>(cnt, scratch, &H);
+#pragma unroll
+	for (int q = 0; q < 4; q++)
+		if (keep[q])
+			raw[off++] = val[q];
+	__syncthreads();
+	return H;
+}
+
+// Per block: ring 1, ring 2, the block's pairs as row slots (slots, indexed
+// like pairs) and ring 1's pairs (roff / rslots).  A block whose rings exceed
+// the caps raises *overflow and the caller keeps the two-launch form.
+__global__ void __launch_bounds__(RT)
+k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, uint32_t nv,
+             const d3 *__restrict__ o, uint32_t *__restrict__ ring, uint2 *__restrict__ rcnt,
+             uint32_t *__restrict__ slots, uint16_t *__restrict__ roff, uint32_t *__restrict__ rslots,
+             d3 *__restrict__ o1, uint32_t *__restrict__ overflow)
+{
+	__shared__ uint32_t r1[RRAW], r2[RRAW];
+	__shared__ uint32_t scratch[RT / N2M_WAVE + 1];
+	__shared__ uint32_t s_n;
+	const uint32_t blk = blockIdx.x, base = blk * RB, tid = threadIdx.x;
+	if (tid == 0)
+		s_n = 0;
+	__syncthreads();
+	uint32_t b = 0, e = 0;
+	if (tid < RB && base + tid < nv) {
+		b = start[base + tid];
+		e = start[base + tid + 1];
+	}
+	// ring 1: every reference from the block to a vertex outside it
+	for (uint32_t k = b; k < e; k++) {
+		const int2 q = pairs[k];
+		const uint32_t x0 = (uint32_t)q.x, x1 = (uint32_t)q.y;
+		if (x0 - base >= (uint32_t)RB) {
+			const uint32_t i = atomicAdd(&s_n, 1u);
+			if (i < RRAW)
+				r1[i] = x0;
+		}
+		if (x1 - base >= (uint32_t)RB) {
+			const uint32_t i = atomicAdd(&s_n, 1u);
+			if (i < RRAW)
+				r1[i] = x1;
+		}
+	}
+	__syncthreads();
+	const uint32_t n1raw = s_n;
+	if (n1raw > RRAW) { // uniform
+		if (tid == 0)
+			atomicOr(overflow, 1u);
+		return;
+	}
+	const uint32_t n1 = sort_dedup(r1, n1raw, scratch);
+	if (n1 > R1CAP) { // uniform
+		if (tid == 0)
+			atomicOr(overflow, 1u);
+		return;
+	}
+	// ring 2: references from ring 1 to vertices outside block and ring 1;
+	// ring-1 vertex i = tid (n1 <= RT)
+	uint32_t hb = 0, he = 0;
+	if (tid < n1) {
+		hb = start[r1[tid]];
+		he = start[r1[tid] + 1];
+	}
+	if (tid == 0)
+		s_n = 0;
+	__syncthreads();
+	for (uint32_t k = hb; k < he; k++) {
+		const int2 q = pairs[k];
+		uint32_t at;
+#pragma unroll
+		for (int c = 0; c < 2; c++) {
+			const uint32_t x = (uint32_t)(c ? q.y : q.x);
+			if (x - base >= (uint32_t)RB && !in_sorted(r1, n1, x, &at)) {
+				const uint32_t i = atomicAdd(&s_n, 1u);
+				if (i < RRAW)
+					r2[i] = x;
+			}
+		}
+	}
+	__syncthreads();
+	const uint32_t n2raw = s_n;
+	if (n2raw > RRAW) { // uniform
+		if (tid == 0)
+			atomicOr(overflow, 1u);
+		return;
+	}
+	const uint32_t n2 = sort_dedup(r2, n2raw, scratch);
+	// ring-1 pair offsets (one ring-1 vertex per thread)
+	uint32_t ptot;
+	const uint32_t poff = block_excl_scan<RT>(he - hb, scratch, &ptot);
+	if (n1 + n2 > (uint32_t)RCAP || ptot > (uint32_t)RPCAP) { // uniform
+		if (tid == 0)
+			atomicOr(overflow, 1u);
+		return;
+	}
+	uint32_t *rg = ring + (size_t)blk * RCAP;
+	for (uint32_t i = tid; i < n1; i += RT) {
+		rg[i] = r1[i];
+		o1[(size_t)blk * R1CAP + i] = o[r1[i]];
+	}
+	for (uint32_t i = tid; i < n2; i += RT)
+		rg[n1 + i] = r2[i];
+	if (tid == 0)
+		rcnt[blk] = make_uint2(n1, n2);
+	uint16_t *ro = roff + (size_t)blk * ROFFW;
+	if (tid < n1)
+		ro[tid] = (uint16_t)poff;
+	if (tid == 0)
+		ro[n1] = (uint16_t)ptot;
+	// row slot of a vertex of block, ring 1 or ring 2
+	auto row = [&](uint32_t x) -> uint32_t {
+		if (x - base < (uint32_t)RB)
+			return x - base;
+		uint32_t at;
+		if (in_sorted(r1, n1, x, &at))
+			return RB + at;
+		in_sorted(r2, n2, x, &at);
+		return RB + n1 + at;
+	};
+	for (uint32_t k = b; k < e; k++) {
+		const int2 q = pairs[k];
+		slots[k] = row((uint32_t)q.x) | (row((uint32_t)q.y) << 16);
+	}
+	uint32_t *rs = rslots + (size_t)blk * RPCAP + poff;
+	for (uint32_t k = hb; k < he; k++) {
+		const int2 q = pairs[k];
+		rs[k - hb] = row((uint32_t)q.x) | (row((uint32_t)q.y) << 16);
+	}
+}
+
+// one HC iteration (hc_a then hc_b) in two-ring form; pout != p.  W: LDS row
+// width in doubles (4: padded rows, read as 16 + 8 bytes)
+template <int W>
+__global__ void __launch_bounds__(RT)
+k_hc_ring(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots, const uint16_t *__restrict__ roff,
+          const uint32_t *__restrict__ rslots, const uint32_t *__restrict__ ring, const uint2 *__restrict__ rcnt,
+          const d3 *__restrict__ o, const d3 *__restrict__ o1, const d3 *__restrict__ p, d3 *__restrict__ pout,
+          uint32_t nv, double alpha, double alpha1, double beta, double beta1)
+{
+	__shared__ double lds[W * RROWS];
+	const uint32_t blk = blockIdx.x, base = blk * RB, tid = threadIdx.x;
+	const uint2 nn = rcnt[blk];
+	const uint32_t n1 = nn.x, n2 = nn.y;
+	const uint32_t *rg = ring + (size_t)blk * RCAP;
+	const uint16_t *ro = roff + (size_t)blk * ROFFW;
+	const uint32_t *rs = rslots + (size_t)blk * RPCAP;
+	const d3 *ob = o1 + (size_t)blk * R1CAP; // original positions of ring 1
+	// row tid: a block vertex (tid < RB) or ring-1 vertex tid - RB
+	const bool own = tid < RB && base + tid < nv;
+	const bool r1a = tid >= RB && tid - RB < n1;
+	SlotRefs<HK> r;
+	const uint32_t *sl = own ? slots : rs;
+	d3 ov = {0.0, 0.0, 0.0}, pv = {0.0, 0.0, 0.0};
+	{
+		uint32_t gid = 0, b = 0, e = 0;
+		if (own) {
+			gid = base + tid;
+			b = start[gid];
+			e = start[gid + 1];
+		} else if (r1a) {
+			gid = rg[tid - RB];
+			b = ro[tid - RB];
+			e = ro[tid - RB + 1];
+		}
+		r.b = b;
+		r.e = e;
+#pragma unroll
+		for (int k = 0; k < HK; k++)
+			r.q[k] = b + k < e ? sl[b + k] : 0u;
+		if (own)
+			ov = o[gid];
+		else if (r1a)
+			ov = ob[tid - RB];
+		if (W == 4 && own)
+			pv = p[gid];
+	}
+	// stage block, ring-1 and ring-2 positions
+	if constexpr (W == 4) {
+		if (tid < RB) {
+			lds[4 * tid] = pv.x;
+			lds[4 * tid + 1] = pv.y;
+			lds[4 * tid + 2] = pv.z;
+		}
+	} else {
+		const double *src = reinterpret_cast<const double *>(p);
+		const size_t lo = 3 * (size_t)base, hi = 3 * (size_t)nv;
+		for (int i = tid; i < 3 * RB; i += RT) {
+			const size_t e = lo + i;
+			lds[i] = e < hi ? src[e] : 0.0;
+		}
+	}
+	for (uint32_t i = tid; i < n1 + n2; i += RT) {
+		const d3 q = p[rg[i]];
+		lds[W * (RB + i)] = q.x;
+		lds[W * (RB + i) + 1] = q.y;
+		lds[W * (RB + i) + 2] = q.z;
+	}
+	__syncthreads();
+	const lds_double *L = (const lds_double *)lds;
+	// hc_a on row tid and, for rings longer than RT - RB, on row tid + RT
+	d3 Lp = {0.0, 0.0, 0.0}, bb = {0.0, 0.0, 0.0}, bb2 = {0.0, 0.0, 0.0};
+	if (own || r1a) {
+		if (W != 4 || !own)
+			pv = lds_row_w<W>(L, tid);
+		Lp = laplace_slots<HK, W>(r, sl, L, pv);
+		bb.x = Lp.x - (ov.x * alpha + pv.x * alpha1);
+		bb.y = Lp.y - (ov.y * alpha + pv.y * alpha1);
+		bb.z = Lp.z - (ov.z * alpha + pv.z * alpha1);
+	}
+	const uint32_t u2 = tid + RT; // always a ring-1 row (RT >= RB)
+	const bool r2a = u2 - RB < n1;
+	if (r2a) {
+		SlotRefs<0> r2;
+		r2.b = ro[u2 - RB];
+		r2.e = ro[u2 - RB + 1];
+		const d3 o2 = ob[u2 - RB];
+		const d3 pv2 = lds_row_w<W>(L, u2);
+		const d3 L2 = laplace_slots<0, W>(r2, rs, L, pv2);
+		bb2.x = L2.x - (o2.x * alpha + pv2.x * alpha1);
+		bb2.y = L2.y - (o2.y * alpha + pv2.y * alpha1);
+		bb2.z = L2.z - (o2.z * alpha + pv2.z * alpha1);
+	}
+	__syncthreads(); // every position row has been read
+	if (own || r1a) {
+		lds[W * tid] = bb.x;
+		lds[W * tid + 1] = bb.y;
+		lds[W * tid + 2] = bb.z;
+	}
+	if (r2a) {
+		lds[W * u2] = bb2.x;
+		lds[W * u2 + 1] = bb2.y;
+		lds[W * u2 + 2] = bb2.z;
+	}
+	__syncthreads();
+	if (!own)
+		return;
+	// hc_b on the block's vertices: their pairs reach block and ring 1 only
+	const d3 Lb = laplace_slots<HK, W>(r, sl, L, bb);
+	d3 out;
+	out.x = Lp.x - (bb.x * beta + Lb.x * beta1);
+	out.y = Lp.y - (bb.y * beta + Lb.y * beta1);
+	out.z = Lp.z - (bb.z * beta + Lb.z * beta1);
+	pout[base + tid] = out;
 }
 
 // gather through a permutation: dst[i] = src[perm ? perm[i] : i]
@@ -1053,6 +1387,43 @@ void launch_laplace_halo(const uint32_t *start, const uint32_t *slots, const uin
 		hipLaunchKernelGGL(k_laplace_h, dim3(halo_blocks(nv)), dim3(HB), 0, s, start, slots, halo, hcount,
 		                   (const d3 *)p, (d3 *)pout, nv);
 }
+uint32_t ring_blocks(uint32_t nv) { return (nv + RB - 1) / RB; }
+size_t ring_words(uint32_t nv) { return (size_t)ring_blocks(nv) * RCAP; }
+size_t ring_pair_words(uint32_t nv) { return (size_t)ring_blocks(nv) * RPCAP; }
+size_t ring_off_words(uint32_t nv) { return (size_t)ring_blocks(nv) * ROFFW; }
+
+size_t ring_o_rows(uint32_t nv) { return (size_t)ring_blocks(nv) * R1CAP; }
+
+void launch_ring_build(const uint32_t *start, const int2 *pairs, uint32_t nv, const double *o, uint32_t *ring,
+                       uint32_t *rcnt, uint32_t *slots, uint16_t *roff, uint32_t *rslots, double *o1,
+                       uint32_t *overflow, hipStream_t s)
+{
+	if (nv)
+		hipLaunchKernelGGL(k_ring_build, dim3(ring_blocks(nv)), dim3(RT), 0, s, start, pairs, nv, (const d3 *)o,
+		                   ring, (uint2 *)rcnt, slots, roff, rslots, (d3 *)o1, overflow);
+}
+void launch_hc_ring(const uint32_t *start, const uint32_t *slots, const uint16_t *roff, const uint32_t *rslots,
+                    const uint32_t *ring, const uint32_t *rcnt, const double *o, const double *o1, const double *p,
+                    double *pout, uint32_t nv, double alpha, double beta, hipStream_t s)
+{
+	static const int w4 = [] {
+		const char *e = getenv("N2M_HC_ROW4");
+		return e ? atoi(e) : 1;
+	}();
+	if (!nv)
+		return;
+	if (w4)
+		hipLaunchKernelGGL(k_hc_ring<4>, dim3(ring_blocks(nv)), dim3(RT), 0, s, start, slots, roff, rslots, ring,
+		                   (const uint2 *)rcnt, (const d3 *)o, (const d3 *)o1, (const d3 *)p, (d3 *)pout, nv, alpha,
+		                   1.0 - alpha,
+		                   beta, 1.0 - beta);
+	else
+		hipLaunchKernelGGL(k_hc_ring<3>, dim3(ring_blocks(nv)), dim3(RT), 0, s, start, slots, roff, rslots, ring,
+		                   (const uint2 *)rcnt, (const d3 *)o, (const d3 *)o1, (const d3 *)p, (d3 *)pout, nv, alpha,
+		                   1.0 - alpha,
+		                   beta, 1.0 - beta);
+}
+
 void launch_hc_load(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s)
 {
 	hipLaunchKernelGGL(k_hc_load, dim3(g1(n)), dim3(256), 0, s, (const d3 *)src, perm, n, (d3 *)dst);

@@ -146,17 +146,23 @@ def test_hc_smooth(gpu, oracle, flavour, lock):
     assert np.array_equal(c, d)
 
 
-@pytest.mark.parametrize('form', ['window', 'halo'])
+FORM_ENV = {'window': ('0', '0'), 'halo': ('0', '1'), 'ring': ('1', '1')}  # N2M_HC_RING, N2M_HC_HALO
+
+
+@pytest.mark.parametrize('form', ['window', 'halo', 'ring'])
 @pytest.mark.parametrize('iters', [1, 2, 12])
 @pytest.mark.parametrize('lock', [True, False])
 def test_hc_gather_forms(gpu, oracle, monkeypatch, form, iters, lock):
-    """Both forms of the HC gathers -- LDS window with global escapes, and
-    LDS block + halo with 16-bit slots -- give the oracle's
+    """The three forms of the HC iteration -- two launches with an LDS window
+    and global escapes, two launches with LDS block + halo and 16-bit slots,
+    and one launch with block + two rings in LDS -- give the oracle's
     laplacian_smoothHC bit for bit on a Morton-relabelled mesh of several
     blocks.  The stage statistics show which form ran."""
     import torch
-    monkeypatch.setenv('N2M_HC_HALO', '0' if form == 'window' else '1')
-    # 34865 vertices: Morton-relabelled (>= 32768), 35 blocks of 1024
+    ring, halo = FORM_ENV[form]
+    monkeypatch.setenv('N2M_HC_RING', ring)
+    monkeypatch.setenv('N2M_HC_HALO', halo)
+    # 34865 vertices: Morton-relabelled (>= 32768), 35 blocks of 1024, 46 of 768
     img = support.blobs((80, 84, 88), 4, noise=2.0)
     iso = float(np.float32(np.percentile(img, 50)))
     ctx = gpu.DeviceContext()
@@ -174,6 +180,35 @@ def test_hc_gather_forms(gpu, oracle, monkeypatch, form, iters, lock):
     oracle.laplacian_smoothHC(po, to, 0.1, 0.5, iters, lock)
     assert _eq(t, to) and np.array_equal(p, po)
     assert ('hc_halo' in stats) == (form == 'halo')
+    assert ('hc_ring' in stats) == (form == 'ring')
+    assert ('hc_a' in stats) == (form != 'ring')
+
+
+@pytest.mark.parametrize('lock', [True, False])
+def test_hc_ring_long_rings(gpu, oracle, lock):
+    """A noisy surface (55000 vertices) on which 18 of the 72 blocks of 768
+    vertices have more than 256 ring-1 neighbours, so the one-launch HC
+    iteration computes those blocks' ring-1 rows in a second pass of its
+    workgroup; the rings still fit, the one-launch form runs, and the result
+    is the oracle's bit for bit."""
+    import torch
+    img = support.blobs((80, 84, 88), 4, noise=8.0)
+    iso = float(np.float32(np.percentile(img, 50)))
+    ctx = gpu.DeviceContext()
+    ctx.timing(True)
+    d = torch.from_numpy(img).cuda()
+    torch.cuda.synchronize()  # the library runs on its own stream
+    prm = ctx.params(iso, cubes=1, preSmooth=True, onlyLargest=False, fillBubbles=False)
+    assert ctx.meshify(d.data_ptr(), (88, 84, 80), prm) == 0
+    assert ctx.smooth_hc(0, 0.1, 0.5, 9, lock) == 0
+    t, p = ctx.mesh(0)
+    stats = ctx.kernel_stats()
+    ctx.close()
+    rc, to, po = oracle.meshify(img.copy(), iso, True, False, False, cubes=1)
+    assert rc == 0 and len(po) == 55000
+    oracle.laplacian_smoothHC(po, to, 0.1, 0.5, 9, lock)
+    assert _eq(t, to) and np.array_equal(p, po)
+    assert 'hc_ring' in stats and 'hc_a' not in stats
 
 
 @pytest.mark.parametrize('threads', ['1', '0'])
@@ -203,9 +238,9 @@ def test_pipeline_isolevels(gpu, oracle, monkeypatch, threads, cubes):
 
 def test_hc_halo_overflow_falls_back(gpu, oracle):
     """A mesh without locality (random triangles over 40000 vertices): every
-    1024-vertex block has far more than 1024 outside neighbours, so the halo
-    build overflows and the window kernels run; the result is still the
-    oracle's, bit for bit."""
+    block has far more outside neighbours than its rings or halo hold, so the
+    two-ring build and the halo build overflow and the window kernels run; the
+    result is still the oracle's, bit for bit."""
     import torch
     rng = np.random.default_rng(17)
     nv, nt = 40000, 30000
