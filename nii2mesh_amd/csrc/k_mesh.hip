@@ -438,16 +438,45 @@ __global__ void k_csr_fill(const int *__restrict__ tris, uint32_t nt, const uint
 	ref[start[tris[i]] + rank[i]] = i;
 }
 
-// sort each vertex's references, then expand to neighbour pairs
-__global__ void k_csr_pairs(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restrict__ start,
-                            uint32_t *__restrict__ ref, int2 *__restrict__ pairs)
+// sort each vertex's references, then expand to neighbour pairs.  Lists of
+// up to CP_MAX references are sorted in LDS (element k of thread t at
+// k * CP_BS + t); longer ones in place in global memory.  ref itself is left
+// unsorted: its only other reader (k_border) does not depend on the order.
+constexpr int CP_BS = 256, CP_MAX = 16;
+__global__ void __launch_bounds__(CP_BS)
+k_csr_pairs(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restrict__ start,
+            uint32_t *__restrict__ ref, int2 *__restrict__ pairs)
 {
-	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ uint32_t s_ref[CP_MAX * CP_BS];
+	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
 	if (v >= nv)
 		return;
-	uint32_t b = start[v], e = start[v + 1];
-	for (uint32_t i = b + 1; i < e; i++) { // insertion sort (valence is small)
-		uint32_t x = ref[i];
+	const uint32_t b = start[v], e = start[v + 1], n = e - b;
+	auto expand = [&](uint32_t i, uint32_t r) {
+		const uint32_t t = r / 3, k = r % 3;
+		const int c0 = tris[3 * t], c1 = tris[3 * t + 1], c2 = tris[3 * t + 2];
+		int2 p;
+		p.x = k == 0 ? c1 : c0;
+		p.y = k == 2 ? c1 : c2;
+		pairs[i] = p;
+	};
+	if (n <= (uint32_t)CP_MAX) {
+		uint32_t *L = s_ref + threadIdx.x;
+		for (uint32_t i = 0; i < n; i++) { // insertion sort (valence is small)
+			const uint32_t x = ref[b + i];
+			uint32_t j = i;
+			while (j > 0 && L[(j - 1) * CP_BS] > x) {
+				L[j * CP_BS] = L[(j - 1) * CP_BS];
+				j--;
+			}
+			L[j * CP_BS] = x;
+		}
+		for (uint32_t i = 0; i < n; i++)
+			expand(b + i, L[i * CP_BS]);
+		return;
+	}
+	for (uint32_t i = b + 1; i < e; i++) {
+		const uint32_t x = ref[i];
 		uint32_t j = i;
 		while (j > b && ref[j - 1] > x) {
 			ref[j] = ref[j - 1];
@@ -455,14 +484,8 @@ __global__ void k_csr_pairs(const int *__restrict__ tris, uint32_t nv, const uin
 		}
 		ref[j] = x;
 	}
-	for (uint32_t i = b; i < e; i++) {
-		uint32_t r = ref[i], t = r / 3, k = r % 3;
-		int c0 = tris[3 * t], c1 = tris[3 * t + 1], c2 = tris[3 * t + 2];
-		int2 p;
-		p.x = k == 0 ? c1 : c0;
-		p.y = k == 2 ? c1 : c2;
-		pairs[i] = p;
-	}
+	for (uint32_t i = b; i < e; i++)
+		expand(i, ref[i]);
 }
 
 // Border vertices as update_mesh(iteration 0) marks them: for vertex v,
@@ -960,13 +983,14 @@ k_laplace_h(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slo
 // those of the two-launch form.  Ring-1 pair lists are kept per block as LDS
 // slots (roff: 16-bit offsets, rslots: slot pairs).
 // ---------------------------------------------------------------------------
-constexpr int RB = 768, RT = 1024;
+// RB (vertices per block) is a template parameter: 512, 640 or 768.
+constexpr int RT = 1024;
 constexpr int R1CAP = 1024;      // ring-1 vertices per block (RB + R1CAP <= 2 * RT)
-constexpr int RROWS = 2304;      // LDS rows: RB + ring 1 + ring 2
-constexpr int RCAP = RROWS - RB; // ring words per block
+constexpr int RROWS = 2304;      // LDS rows: RB + ring 1 + ring 2 (ring words per block: RROWS - RB)
 constexpr int RPCAP = 4096;      // ring-1 pair words per block
 constexpr int ROFFW = R1CAP + 2; // 16-bit ring-1 offsets per block
 constexpr int RRAW = 4 * RT;     // raw outside references collected per pass
+constexpr int RK = 8;            // pairs per vertex held in registers by k_ring_build
 
 __device__ __forceinline__ bool in_sorted(const uint32_t *h, uint32_t n, uint32_t x, uint32_t *at)
 {
@@ -1029,7 +1053,8 @@ __device__ __forceinline__ uint32_t sort_dedup(uint32_t *raw, uint32_t n, uint32
 // Per block: ring 1, ring 2, the block's pairs as row slots (slots, indexed
 // like pairs) and ring 1's pairs (roff / rslots).  A block whose rings exceed
 // the caps raises *overflow and the caller keeps the two-launch form.
-__global__ void __launch_bounds__(RT)
+template <int RB>
+__global__ void __launch_bounds__(RT, 8)
 k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, uint32_t nv,
              const d3 *__restrict__ o, uint32_t *__restrict__ ring, uint2 *__restrict__ rcnt,
              uint32_t *__restrict__ slots, uint16_t *__restrict__ roff, uint32_t *__restrict__ rslots,
@@ -1038,18 +1063,25 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
 	__shared__ uint32_t r1[RRAW], r2[RRAW];
 	__shared__ uint32_t scratch[RT / N2M_WAVE + 1];
 	__shared__ uint32_t s_n;
+	constexpr int RCAP = RROWS - RB;
 	const uint32_t blk = blockIdx.x, base = blk * RB, tid = threadIdx.x;
 	if (tid == 0)
 		s_n = 0;
-	__syncthreads();
 	uint32_t b = 0, e = 0;
 	if (tid < RB && base + tid < nv) {
 		b = start[base + tid];
 		e = start[base + tid + 1];
 	}
+	// the first RK pairs of a list are loaded at once (in-block filler beyond
+	// the list); each pass loads them again rather than holding them across
+	// the sorts
+	auto load_pairs = [&](int2 *q, uint32_t lb, uint32_t le) {
+#pragma unroll
+		for (int k = 0; k < RK; k++)
+			q[k] = lb + k < le ? pairs[lb + k] : make_int2((int)base, (int)base);
+	};
 	// ring 1: every reference from the block to a vertex outside it
-	for (uint32_t k = b; k < e; k++) {
-		const int2 q = pairs[k];
+	auto collect1 = [&](const int2 q) {
 		const uint32_t x0 = (uint32_t)q.x, x1 = (uint32_t)q.y;
 		if (x0 - base >= (uint32_t)RB) {
 			const uint32_t i = atomicAdd(&s_n, 1u);
@@ -1061,7 +1093,17 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
 			if (i < RRAW)
 				r1[i] = x1;
 		}
+	};
+	{
+		int2 qa[RK];
+		load_pairs(qa, b, e);
+		__syncthreads();
+#pragma unroll
+		for (int k = 0; k < RK; k++)
+			collect1(qa[k]);
 	}
+	for (uint32_t k = b + RK; k < e; k++)
+		collect1(pairs[k]);
 	__syncthreads();
 	const uint32_t n1raw = s_n;
 	if (n1raw > RRAW) { // uniform
@@ -1082,11 +1124,12 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
 		hb = start[r1[tid]];
 		he = start[r1[tid] + 1];
 	}
+	int2 qr[RK];
+	load_pairs(qr, hb, he);
 	if (tid == 0)
 		s_n = 0;
 	__syncthreads();
-	for (uint32_t k = hb; k < he; k++) {
-		const int2 q = pairs[k];
+	auto collect2 = [&](const int2 q) {
 		uint32_t at;
 #pragma unroll
 		for (int c = 0; c < 2; c++) {
@@ -1097,7 +1140,12 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
 					r2[i] = x;
 			}
 		}
-	}
+	};
+#pragma unroll
+	for (int k = 0; k < RK; k++)
+		collect2(qr[k]);
+	for (uint32_t k = hb + RK; k < he; k++)
+		collect2(pairs[k]);
 	__syncthreads();
 	const uint32_t n2raw = s_n;
 	if (n2raw > RRAW) { // uniform
@@ -1138,27 +1186,37 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
 		in_sorted(r2, n2, x, &at);
 		return RB + n1 + at;
 	};
-	for (uint32_t k = b; k < e; k++) {
-		const int2 q = pairs[k];
-		slots[k] = row((uint32_t)q.x) | (row((uint32_t)q.y) << 16);
-	}
+	auto slot = [&](const int2 q) { return row((uint32_t)q.x) | (row((uint32_t)q.y) << 16); };
+	int2 qa[RK];
+	load_pairs(qa, b, e);
+	load_pairs(qr, hb, he);
+#pragma unroll
+	for (int k = 0; k < RK; k++)
+		if (b + k < e)
+			slots[b + k] = slot(qa[k]);
+	for (uint32_t k = b + RK; k < e; k++)
+		slots[k] = slot(pairs[k]);
 	uint32_t *rs = rslots + (size_t)blk * RPCAP + poff;
-	for (uint32_t k = hb; k < he; k++) {
-		const int2 q = pairs[k];
-		rs[k - hb] = row((uint32_t)q.x) | (row((uint32_t)q.y) << 16);
-	}
+#pragma unroll
+	for (int k = 0; k < RK; k++)
+		if (hb + k < he)
+			rs[k] = slot(qr[k]);
+	for (uint32_t k = hb + RK; k < he; k++)
+		rs[k - hb] = slot(pairs[k]);
 }
 
-// one HC iteration (hc_a then hc_b) in two-ring form; pout != p.  W: LDS row
-// width in doubles (4: padded rows, read as 16 + 8 bytes)
-template <int W>
+// one HC iteration (hc_a then hc_b) in two-ring form; pout != p.  LDS rows
+// are padded to 4 doubles (read as 16 + 8 bytes).
+template <int RB>
 __global__ void __launch_bounds__(RT)
 k_hc_ring(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots, const uint16_t *__restrict__ roff,
           const uint32_t *__restrict__ rslots, const uint32_t *__restrict__ ring, const uint2 *__restrict__ rcnt,
           const d3 *__restrict__ o, const d3 *__restrict__ o1, const d3 *__restrict__ p, d3 *__restrict__ pout,
           uint32_t nv, double alpha, double alpha1, double beta, double beta1)
 {
+	constexpr int W = 4;
 	__shared__ double lds[W * RROWS];
+	constexpr int RCAP = RROWS - RB;
 	const uint32_t blk = blockIdx.x, base = blk * RB, tid = threadIdx.x;
 	const uint2 nn = rcnt[blk];
 	const uint32_t n1 = nn.x, n2 = nn.y;
@@ -1192,23 +1250,15 @@ k_hc_ring(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots
 			ov = o[gid];
 		else if (r1a)
 			ov = ob[tid - RB];
-		if (W == 4 && own)
+		if (own)
 			pv = p[gid];
 	}
-	// stage block, ring-1 and ring-2 positions
-	if constexpr (W == 4) {
-		if (tid < RB) {
-			lds[4 * tid] = pv.x;
-			lds[4 * tid + 1] = pv.y;
-			lds[4 * tid + 2] = pv.z;
-		}
-	} else {
-		const double *src = reinterpret_cast<const double *>(p);
-		const size_t lo = 3 * (size_t)base, hi = 3 * (size_t)nv;
-		for (int i = tid; i < 3 * RB; i += RT) {
-			const size_t e = lo + i;
-			lds[i] = e < hi ? src[e] : 0.0;
-		}
+	// stage block (the own rows, already in registers), ring-1 and ring-2
+	// positions
+	if (tid < RB) {
+		lds[W * tid] = pv.x;
+		lds[W * tid + 1] = pv.y;
+		lds[W * tid + 2] = pv.z;
 	}
 	for (uint32_t i = tid; i < n1 + n2; i += RT) {
 		const d3 q = p[rg[i]];
@@ -1221,7 +1271,7 @@ k_hc_ring(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots
 	// hc_a on row tid and, for rings longer than RT - RB, on row tid + RT
 	d3 Lp = {0.0, 0.0, 0.0}, bb = {0.0, 0.0, 0.0}, bb2 = {0.0, 0.0, 0.0};
 	if (own || r1a) {
-		if (W != 4 || !own)
+		if (!own)
 			pv = lds_row_w<W>(L, tid);
 		Lp = laplace_slots<HK, W>(r, sl, L, pv);
 		bb.x = Lp.x - (ov.x * alpha + pv.x * alpha1);
@@ -1230,13 +1280,16 @@ k_hc_ring(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots
 	}
 	const uint32_t u2 = tid + RT; // always a ring-1 row (RT >= RB)
 	const bool r2a = u2 - RB < n1;
-	if (r2a) {
-		SlotRefs<0> r2;
+	if (r2a) { // its first HK slots are loaded at once
+		SlotRefs<HK> r2;
 		r2.b = ro[u2 - RB];
 		r2.e = ro[u2 - RB + 1];
+#pragma unroll
+		for (int k = 0; k < HK; k++)
+			r2.q[k] = r2.b + k < r2.e ? rs[r2.b + k] : 0u;
 		const d3 o2 = ob[u2 - RB];
 		const d3 pv2 = lds_row_w<W>(L, u2);
-		const d3 L2 = laplace_slots<0, W>(r2, rs, L, pv2);
+		const d3 L2 = laplace_slots<HK, W>(r2, rs, L, pv2);
 		bb2.x = L2.x - (o2.x * alpha + pv2.x * alpha1);
 		bb2.y = L2.y - (o2.y * alpha + pv2.y * alpha1);
 		bb2.z = L2.z - (o2.z * alpha + pv2.z * alpha1);
@@ -1294,7 +1347,7 @@ void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, const 
 void launch_csr_pairs(const int *tris, uint32_t nv, const uint32_t *start, uint32_t *ref, int2 *pairs,
                       hipStream_t s)
 {
-	hipLaunchKernelGGL(k_csr_pairs, dim3(g1(nv)), dim3(256), 0, s, tris, nv, start, ref, pairs);
+	hipLaunchKernelGGL(k_csr_pairs, dim3(g1(nv)), dim3(CP_BS), 0, s, tris, nv, start, ref, pairs);
 }
 void launch_border(const int *tris, uint32_t nv, const uint32_t *start, const uint32_t *ref, uint8_t *border,
                    hipStream_t s)
@@ -1387,41 +1440,56 @@ void launch_laplace_halo(const uint32_t *start, const uint32_t *slots, const uin
 		hipLaunchKernelGGL(k_laplace_h, dim3(halo_blocks(nv)), dim3(HB), 0, s, start, slots, halo, hcount,
 		                   (const d3 *)p, (d3 *)pout, nv);
 }
-uint32_t ring_blocks(uint32_t nv) { return (nv + RB - 1) / RB; }
-size_t ring_words(uint32_t nv) { return (size_t)ring_blocks(nv) * RCAP; }
+// vertices per two-ring block: N2M_HC_RB = 512, 640 (default) or 768
+int ring_rb()
+{
+	static const int rb = [] {
+		const char *e = getenv("N2M_HC_RB");
+		const int v = e ? atoi(e) : 640;
+		return v == 512 || v == 768 ? v : 640;
+	}();
+	return rb;
+}
+uint32_t ring_blocks(uint32_t nv) { return (nv + ring_rb() - 1) / ring_rb(); }
+size_t ring_words(uint32_t nv) { return (size_t)ring_blocks(nv) * (RROWS - ring_rb()); }
 size_t ring_pair_words(uint32_t nv) { return (size_t)ring_blocks(nv) * RPCAP; }
 size_t ring_off_words(uint32_t nv) { return (size_t)ring_blocks(nv) * ROFFW; }
-
 size_t ring_o_rows(uint32_t nv) { return (size_t)ring_blocks(nv) * R1CAP; }
+
+#define N2M_RING_RBS(X) X(512) X(640) X(768)
 
 void launch_ring_build(const uint32_t *start, const int2 *pairs, uint32_t nv, const double *o, uint32_t *ring,
                        uint32_t *rcnt, uint32_t *slots, uint16_t *roff, uint32_t *rslots, double *o1,
                        uint32_t *overflow, hipStream_t s)
 {
-	if (nv)
-		hipLaunchKernelGGL(k_ring_build, dim3(ring_blocks(nv)), dim3(RT), 0, s, start, pairs, nv, (const d3 *)o,
-		                   ring, (uint2 *)rcnt, slots, roff, rslots, (d3 *)o1, overflow);
+	if (!nv)
+		return;
+	switch (ring_rb()) {
+#define X(rb)                                                                                                      \
+	case rb:                                                                                                     \
+		hipLaunchKernelGGL(k_ring_build<rb>, dim3(ring_blocks(nv)), dim3(RT), 0, s, start, pairs, nv,               \
+		                   (const d3 *)o, ring, (uint2 *)rcnt, slots, roff, rslots, (d3 *)o1, overflow);           \
+		break;
+		N2M_RING_RBS(X)
+#undef X
+	}
 }
 void launch_hc_ring(const uint32_t *start, const uint32_t *slots, const uint16_t *roff, const uint32_t *rslots,
                     const uint32_t *ring, const uint32_t *rcnt, const double *o, const double *o1, const double *p,
                     double *pout, uint32_t nv, double alpha, double beta, hipStream_t s)
 {
-	static const int w4 = [] {
-		const char *e = getenv("N2M_HC_ROW4");
-		return e ? atoi(e) : 1;
-	}();
 	if (!nv)
 		return;
-	if (w4)
-		hipLaunchKernelGGL(k_hc_ring<4>, dim3(ring_blocks(nv)), dim3(RT), 0, s, start, slots, roff, rslots, ring,
-		                   (const uint2 *)rcnt, (const d3 *)o, (const d3 *)o1, (const d3 *)p, (d3 *)pout, nv, alpha,
-		                   1.0 - alpha,
-		                   beta, 1.0 - beta);
-	else
-		hipLaunchKernelGGL(k_hc_ring<3>, dim3(ring_blocks(nv)), dim3(RT), 0, s, start, slots, roff, rslots, ring,
-		                   (const uint2 *)rcnt, (const d3 *)o, (const d3 *)o1, (const d3 *)p, (d3 *)pout, nv, alpha,
-		                   1.0 - alpha,
-		                   beta, 1.0 - beta);
+	switch (ring_rb()) {
+#define X(rb)                                                                                                      \
+	case rb:                                                                                                     \
+		hipLaunchKernelGGL(k_hc_ring<rb>, dim3(ring_blocks(nv)), dim3(RT), 0, s, start, slots, roff, rslots,  \
+		                   ring, (const uint2 *)rcnt, (const d3 *)o, (const d3 *)o1, (const d3 *)p, (d3 *)pout, nv, \
+		                   alpha, 1.0 - alpha, beta, 1.0 - beta);                                                  \
+		break;
+		N2M_RING_RBS(X)
+#undef X
+	}
 }
 
 void launch_hc_load(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s)
