@@ -989,70 +989,68 @@ constexpr int R1CAP = 1024;      // ring-1 vertices per block (RB + R1CAP <= 2 *
 constexpr int RROWS = 2304;      // LDS rows: RB + ring 1 + ring 2 (ring words per block: RROWS - RB)
 constexpr int RPCAP = 4096;      // ring-1 pair words per block
 constexpr int ROFFW = R1CAP + 2; // 16-bit ring-1 offsets per block
-constexpr int RRAW = 4 * RT;     // raw outside references collected per pass
-constexpr int RK = 8;            // pairs per vertex held in registers by k_ring_build
+constexpr int RK = 6;            // pairs per vertex held in registers by k_ring_build
 
-__device__ __forceinline__ bool in_sorted(const uint32_t *h, uint32_t n, uint32_t x, uint32_t *at)
+// LDS hash set of vertex ids (open addressing, RHS slots) used by
+// k_ring_build: key RH_EMPTY marks a free slot, val the id's LDS row.
+constexpr int RHS = 4096;
+constexpr uint32_t RH_EMPTY = 0xffffffffu;
+__device__ __forceinline__ uint32_t rh_hash(uint32_t x) { return (x * 2654435761u) >> 20; } // 12 bits
+
+// insert x; a new id is appended (as its slot) to list[*count] when *count <
+// cap.  Returns false when the table is full or the list overflows.
+__device__ __forceinline__ bool rh_insert(uint32_t *key, uint32_t x, uint32_t *list, uint32_t *count, uint32_t cap)
 {
-	uint32_t lo = 0, hi = n;
-	while (lo < hi) {
-		uint32_t mid = (lo + hi) >> 1;
-		if (h[mid] < x)
-			lo = mid + 1;
-		else
-			hi = mid;
+	uint32_t h = rh_hash(x);
+	for (int p = 0; p < RHS; p++, h = (h + 1) & (RHS - 1)) {
+		const uint32_t prev = atomicCAS(&key[h], RH_EMPTY, x);
+		if (prev == RH_EMPTY) {
+			const uint32_t i = atomicAdd(count, 1u);
+			if (i >= cap)
+				return false;
+			list[i] = h;
+			return true;
+		}
+		if (prev == x)
+			return true;
 	}
-	*at = lo;
-	return lo < n && h[lo] == x;
+	return false;
 }
 
-// sort raw[0, n) ascending (bitonic, padded with ~0 to a power of two; n <=
-// RRAW) and keep the first of each run of equal ids, compacted to raw[0, H).
-// Every thread returns H.  Contains barriers.
-__device__ __forceinline__ uint32_t sort_dedup(uint32_t *raw, uint32_t n, uint32_t *scratch)
+// slot of x, which is present
+__device__ __forceinline__ uint32_t rh_find(const uint32_t *key, uint32_t x)
 {
-	uint32_t P = 1;
-	while (P < n)
-		P <<= 1;
-	for (uint32_t i = n + threadIdx.x; i < P; i += RT)
-		raw[i] = 0xffffffffu;
+	uint32_t h = rh_hash(x);
+	for (int p = 0; p < RHS && key[h] != x; p++)
+		h = (h + 1) & (RHS - 1);
+	return h;
+}
+
+// The n ids at key[list[0, n)] in ascending order into out[0, n), and each
+// id's row (row0 + its rank) into val.  Rank = number of smaller ids (the
+// ids are distinct); O(n^2 / RT) LDS reads, broadcast across the wave.
+__device__ __forceinline__ void rh_rank(const uint32_t *key, const uint32_t *list, uint32_t n, uint32_t *ids,
+                                        uint32_t *out, uint32_t *val, uint32_t row0)
+{
+	for (uint32_t i = threadIdx.x; i < n; i += RT)
+		ids[i] = key[list[i]];
 	__syncthreads();
-	for (uint32_t size = 2; size <= P; size <<= 1)
-		for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-			for (uint32_t i = threadIdx.x; i < P; i += RT) {
-				const uint32_t j = i ^ stride;
-				if (j > i) {
-					const uint32_t a = raw[i], c = raw[j];
-					if ((a > c) == ((i & size) == 0)) {
-						raw[i] = c;
-						raw[j] = a;
-					}
-				}
-			}
-			__syncthreads();
-		}
-	uint32_t val[4], cnt = 0;
-	bool keep[4];
-#pragma unroll
-	for (int q = 0; q < 4; q++) {
-		const uint32_t i = 4 * threadIdx.x + q;
-		val[q] = i < n ? raw[i] : 0u;
-		keep[q] = i < n && (i == 0 || raw[i] != raw[i - 1]);
-		cnt += keep[q];
+	for (uint32_t i = threadIdx.x; i < n; i += RT) {
+		const uint32_t x = ids[i];
+		uint32_t r = 0;
+		for (uint32_t j = 0; j < n; j++)
+			r += ids[j] < x;
+		out[r] = x;
+		val[list[i]] = row0 + r;
 	}
-	uint32_t H;
-	uint32_t off = block_excl_scan<RT>(cnt, scratch, &H);
-#pragma unroll
-	for (int q = 0; q < 4; q++)
-		if (keep[q])
-			raw[off++] = val[q];
 	__syncthreads();
-	return H;
 }
 
 // Per block: ring 1, ring 2, the block's pairs as row slots (slots, indexed
-// like pairs) and ring 1's pairs (roff / rslots).  A block whose rings exceed
-// the caps raises *overflow and the caller keeps the two-launch form.
+// like pairs) and ring 1's pairs (roff / rslots).  The rings are collected in
+// an LDS hash set (an id already present -- block, ring 1 or ring 2 -- is
+// skipped) and then ordered by id.  A block whose rings exceed the caps
+// raises *overflow and the caller keeps the two-launch form.
 template <int RB>
 __global__ void __launch_bounds__(RT, 8)
 k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs, uint32_t nv,
@@ -1060,38 +1058,38 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
              uint32_t *__restrict__ slots, uint16_t *__restrict__ roff, uint32_t *__restrict__ rslots,
              d3 *__restrict__ o1, uint32_t *__restrict__ overflow)
 {
-	__shared__ uint32_t r1[RRAW], r2[RRAW];
-	__shared__ uint32_t scratch[RT / N2M_WAVE + 1];
-	__shared__ uint32_t s_n;
 	constexpr int RCAP = RROWS - RB;
+	__shared__ uint32_t hkey[RHS], hval[RHS];
+	__shared__ uint32_t lst[RCAP], ids[RCAP], srt[RCAP]; // hash slots, ids, sorted ids (ring 1, then ring 2)
+	__shared__ uint32_t scratch[RT / N2M_WAVE + 1];
+	__shared__ uint32_t s_n1, s_n2, s_bad;
 	const uint32_t blk = blockIdx.x, base = blk * RB, tid = threadIdx.x;
-	if (tid == 0)
-		s_n = 0;
+	for (uint32_t i = tid; i < RHS; i += RT)
+		hkey[i] = RH_EMPTY;
+	if (tid == 0) {
+		s_n1 = 0;
+		s_n2 = 0;
+		s_bad = 0;
+	}
 	uint32_t b = 0, e = 0;
 	if (tid < RB && base + tid < nv) {
 		b = start[base + tid];
 		e = start[base + tid + 1];
 	}
 	// the first RK pairs of a list are loaded at once (in-block filler beyond
-	// the list); each pass loads them again rather than holding them across
-	// the sorts
+	// the list); each pass loads them again rather than holding them
 	auto load_pairs = [&](int2 *q, uint32_t lb, uint32_t le) {
 #pragma unroll
 		for (int k = 0; k < RK; k++)
 			q[k] = lb + k < le ? pairs[lb + k] : make_int2((int)base, (int)base);
 	};
-	// ring 1: every reference from the block to a vertex outside it
-	auto collect1 = [&](const int2 q) {
-		const uint32_t x0 = (uint32_t)q.x, x1 = (uint32_t)q.y;
-		if (x0 - base >= (uint32_t)RB) {
-			const uint32_t i = atomicAdd(&s_n, 1u);
-			if (i < RRAW)
-				r1[i] = x0;
-		}
-		if (x1 - base >= (uint32_t)RB) {
-			const uint32_t i = atomicAdd(&s_n, 1u);
-			if (i < RRAW)
-				r1[i] = x1;
+	// ring 1: the block's references to vertices outside it
+	auto add1 = [&](const int2 q) {
+#pragma unroll
+		for (int c = 0; c < 2; c++) {
+			const uint32_t x = (uint32_t)(c ? q.y : q.x);
+			if (x - base >= (uint32_t)RB && !rh_insert(hkey, x, lst, &s_n1, R1CAP))
+				s_bad = 1;
 		}
 	};
 	{
@@ -1100,75 +1098,63 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
 		__syncthreads();
 #pragma unroll
 		for (int k = 0; k < RK; k++)
-			collect1(qa[k]);
+			add1(qa[k]);
 	}
 	for (uint32_t k = b + RK; k < e; k++)
-		collect1(pairs[k]);
+		add1(pairs[k]);
 	__syncthreads();
-	const uint32_t n1raw = s_n;
-	if (n1raw > RRAW) { // uniform
+	if (s_bad) { // uniform
 		if (tid == 0)
 			atomicOr(overflow, 1u);
 		return;
 	}
-	const uint32_t n1 = sort_dedup(r1, n1raw, scratch);
-	if (n1 > R1CAP) { // uniform
-		if (tid == 0)
-			atomicOr(overflow, 1u);
-		return;
-	}
-	// ring 2: references from ring 1 to vertices outside block and ring 1;
-	// ring-1 vertex i = tid (n1 <= RT)
+	const uint32_t n1 = s_n1;
+	rh_rank(hkey, lst, n1, ids, srt, hval, RB);
+	// ring 2: ring 1's references to vertices outside block and ring 1
+	// (ring-1 vertex i = tid, n1 <= RT)
 	uint32_t hb = 0, he = 0;
 	if (tid < n1) {
-		hb = start[r1[tid]];
-		he = start[r1[tid] + 1];
+		hb = start[srt[tid]];
+		he = start[srt[tid] + 1];
 	}
-	int2 qr[RK];
-	load_pairs(qr, hb, he);
-	if (tid == 0)
-		s_n = 0;
-	__syncthreads();
-	auto collect2 = [&](const int2 q) {
-		uint32_t at;
+	auto add2 = [&](const int2 q) {
 #pragma unroll
 		for (int c = 0; c < 2; c++) {
 			const uint32_t x = (uint32_t)(c ? q.y : q.x);
-			if (x - base >= (uint32_t)RB && !in_sorted(r1, n1, x, &at)) {
-				const uint32_t i = atomicAdd(&s_n, 1u);
-				if (i < RRAW)
-					r2[i] = x;
-			}
+			if (x - base >= (uint32_t)RB && !rh_insert(hkey, x, lst + n1, &s_n2, RCAP - n1))
+				s_bad = 1;
 		}
 	};
+	{
+		int2 qr[RK];
+		load_pairs(qr, hb, he);
 #pragma unroll
-	for (int k = 0; k < RK; k++)
-		collect2(qr[k]);
+		for (int k = 0; k < RK; k++)
+			add2(qr[k]);
+	}
 	for (uint32_t k = hb + RK; k < he; k++)
-		collect2(pairs[k]);
+		add2(pairs[k]);
 	__syncthreads();
-	const uint32_t n2raw = s_n;
-	if (n2raw > RRAW) { // uniform
+	if (s_bad) { // uniform
 		if (tid == 0)
 			atomicOr(overflow, 1u);
 		return;
 	}
-	const uint32_t n2 = sort_dedup(r2, n2raw, scratch);
+	const uint32_t n2 = s_n2;
+	rh_rank(hkey, lst + n1, n2, ids + n1, srt + n1, hval, RB + n1);
 	// ring-1 pair offsets (one ring-1 vertex per thread)
 	uint32_t ptot;
 	const uint32_t poff = block_excl_scan<RT>(he - hb, scratch, &ptot);
-	if (n1 + n2 > (uint32_t)RCAP || ptot > (uint32_t)RPCAP) { // uniform
+	if (ptot > (uint32_t)RPCAP) { // uniform
 		if (tid == 0)
 			atomicOr(overflow, 1u);
 		return;
 	}
 	uint32_t *rg = ring + (size_t)blk * RCAP;
-	for (uint32_t i = tid; i < n1; i += RT) {
-		rg[i] = r1[i];
-		o1[(size_t)blk * R1CAP + i] = o[r1[i]];
-	}
-	for (uint32_t i = tid; i < n2; i += RT)
-		rg[n1 + i] = r2[i];
+	for (uint32_t i = tid; i < n1 + n2; i += RT)
+		rg[i] = srt[i];
+	for (uint32_t i = tid; i < n1; i += RT)
+		o1[(size_t)blk * R1CAP + i] = o[srt[i]];
 	if (tid == 0)
 		rcnt[blk] = make_uint2(n1, n2);
 	uint16_t *ro = roff + (size_t)blk * ROFFW;
@@ -1177,30 +1163,27 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
 	if (tid == 0)
 		ro[n1] = (uint16_t)ptot;
 	// row slot of a vertex of block, ring 1 or ring 2
-	auto row = [&](uint32_t x) -> uint32_t {
-		if (x - base < (uint32_t)RB)
-			return x - base;
-		uint32_t at;
-		if (in_sorted(r1, n1, x, &at))
-			return RB + at;
-		in_sorted(r2, n2, x, &at);
-		return RB + n1 + at;
-	};
+	auto row = [&](uint32_t x) -> uint32_t { return x - base < (uint32_t)RB ? x - base : hval[rh_find(hkey, x)]; };
 	auto slot = [&](const int2 q) { return row((uint32_t)q.x) | (row((uint32_t)q.y) << 16); };
-	int2 qa[RK];
-	load_pairs(qa, b, e);
-	load_pairs(qr, hb, he);
+	{
+		int2 qa[RK];
+		load_pairs(qa, b, e);
 #pragma unroll
-	for (int k = 0; k < RK; k++)
-		if (b + k < e)
-			slots[b + k] = slot(qa[k]);
+		for (int k = 0; k < RK; k++)
+			if (b + k < e)
+				slots[b + k] = slot(qa[k]);
+	}
 	for (uint32_t k = b + RK; k < e; k++)
 		slots[k] = slot(pairs[k]);
 	uint32_t *rs = rslots + (size_t)blk * RPCAP + poff;
+	{
+		int2 qr[RK];
+		load_pairs(qr, hb, he);
 #pragma unroll
-	for (int k = 0; k < RK; k++)
-		if (hb + k < he)
-			rs[k] = slot(qr[k]);
+		for (int k = 0; k < RK; k++)
+			if (hb + k < he)
+				rs[k] = slot(qr[k]);
+	}
 	for (uint32_t k = hb + RK; k < he; k++)
 		rs[k - hb] = slot(pairs[k]);
 }
