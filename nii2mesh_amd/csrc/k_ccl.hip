@@ -333,8 +333,19 @@ k_ccl_flatten(const uint32_t *__restrict__ bits, Geo g, int invert, uint32_t *__
 	uint32_t gi = w ? g.rb[g.widx(wx, y, z)] : 0u;
 	for (uint32_t t = run_starts(w); t; t &= t - 1, gi++) {
 		int b = __ffs(t) - 1;
-		uint32_t root = uf_find(L, gi);
-		L[gi] = root;
+		// find the root, then point the walked path at it.  All unions are
+		// done, so every thread writes only true roots and a run's final
+		// value is its root; later walks through a giant component's long
+		// chains are short.
+		uint32_t root = gi;
+		for (uint32_t p = ld_agent(&L[root]); p != root; p = ld_agent(&L[root]))
+			root = p;
+		for (uint32_t x = gi; x != root;) {
+			const uint32_t nx = ld_agent(&L[x]);
+			if (nx != root)
+				__hip_atomic_store(&L[x], root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			x = nx;
+		}
 		if (mode == 0) {
 			if (root != cur) {
 				if (cnt)
@@ -343,20 +354,34 @@ k_ccl_flatten(const uint32_t *__restrict__ bits, Geo g, int invert, uint32_t *__
 				cnt = 0;
 			}
 			cnt += (uint32_t)run_len(w, b);
-			if (root == gi)
-				rootlist[atomicAdd(nroots, 1u)] = gi;
 		} else {
 			int x0 = wx * 32 + b, x1 = x0 + run_len(w, b) - 1;
 			if ((face_row || x0 == 0 || x1 == g.nx - 1) && AUX[root] == 0)
 				AUX[root] = 1;
-			nroot_local += root == gi;
 		}
+		nroot_local += root == gi;
 	}
 	if (mode == 1) {
 		nroot_local = wave_sum(nroot_local);
 		if (lane_id() == 0 && nroot_local)
 			atomicAdd(nroots, nroot_local);
 		return;
+	}
+	// the wave's roots go to the root list with one atomic per wave (a mask
+	// with millions of single-run components would serialise on nroots);
+	// the word's runs are walked again, a root being a run with L[run] == run
+	{
+		const uint32_t incl = wave_incl_scan(nroot_local);
+		uint32_t base = 0;
+		if (lane_id() == N2M_WAVE - 1 && incl)
+			base = atomicAdd(nroots, incl);
+		uint32_t at = __shfl(base, N2M_WAVE - 1, N2M_WAVE) + incl - nroot_local;
+		if (nroot_local) {
+			uint32_t g2 = g.rb[g.widx(wx, y, z)];
+			for (uint32_t t = run_starts(w); t; t &= t - 1, g2++)
+				if (ld_agent(&L[g2]) == g2)
+					rootlist[at++] = g2;
+		}
 	}
 	// wave: lanes holding the same root add their counts
 	bool pend = cnt != 0;

@@ -141,16 +141,6 @@ __device__ __forceinline__ uint32_t uf_find_live(uint32_t *L, uint32_t x)
 	}
 	return x;
 }
-// For kernels launched after all unions completed.
-__device__ __forceinline__ uint32_t uf_find(const uint32_t *L, uint32_t x)
-{
-	uint32_t p = L[x];
-	while (p != x) {
-		x = p;
-		p = L[x];
-	}
-	return x;
-}
 __device__ __forceinline__ void uf_union(uint32_t *L, uint32_t a, uint32_t b)
 {
 	for (;;) {
