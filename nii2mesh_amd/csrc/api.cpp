@@ -910,6 +910,7 @@ struct Ring {
 	uint32_t *slots = nullptr, *ring = nullptr, *rcnt = nullptr, *rslots = nullptr;
 	uint16_t *roff = nullptr;
 	double *o1 = nullptr;
+	int rb = 0; // vertices per block
 };
 
 // o: the original positions in the smoothing order
@@ -920,16 +921,17 @@ Ring build_ring(n2m_ctx *C, const Csr &c, uint32_t nv, uint32_t nt, const double
 	if ((e && *e == '0') || nv == 0)
 		return r;
 	hipStream_t s = C->stream;
+	r.rb = ring_rb();
 	r.slots = C->buf<uint32_t>(S_HC_SLOTS, (size_t)3 * nt + 1);
-	r.ring = C->buf<uint32_t>(S_HC_RING, ring_words(nv));
-	r.rcnt = C->buf<uint32_t>(S_HC_RCNT, 2 * (size_t)ring_blocks(nv));
-	r.roff = C->buf<uint16_t>(S_HC_ROFF, ring_off_words(nv));
-	r.rslots = C->buf<uint32_t>(S_HC_RSLOTS, ring_pair_words(nv));
-	r.o1 = C->buf<double>(S_HC_RO, 3 * ring_o_rows(nv));
+	r.ring = C->buf<uint32_t>(S_HC_RING, ring_words(nv, r.rb));
+	r.rcnt = C->buf<uint32_t>(S_HC_RCNT, 2 * (size_t)ring_blocks(nv, r.rb));
+	r.roff = C->buf<uint16_t>(S_HC_ROFF, ring_off_words(nv, r.rb));
+	r.rslots = C->buf<uint32_t>(S_HC_RSLOTS, ring_pair_words(nv, r.rb));
+	r.o1 = C->buf<double>(S_HC_RO, 3 * ring_o_rows(nv, r.rb));
 	HIPX(hipMemsetAsync(C->d_small + SM_RING, 0, 4, s));
 	C->tbegin("hc_rings", 8.0 * nv + 12.0 * 3 * nt);
 	launch_ring_build(c.start, c.pairs, nv, o, r.ring, r.rcnt, r.slots, r.roff, r.rslots, r.o1,
-	                  C->d_small + SM_RING, s);
+	                  C->d_small + SM_RING, r.rb, s);
 	C->tend();
 	r.on = C->fetch(SM_RING, 1)[0] == 0;
 	return r;
@@ -965,7 +967,7 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 			double *nxt = it % 2 == 0 ? p : pn;
 			C->tbegin("hc_ring", 4.0 * nv + 12.0 * nt + 72.0 * nv);
 			launch_hc_ring(c.start, G.slots, G.roff, G.rslots, G.ring, G.rcnt, o, G.o1, res, nxt, nv, alpha, beta,
-			               s);
+			               G.rb, s);
 			C->tend();
 			res = nxt;
 		}

@@ -1423,34 +1423,32 @@ void launch_laplace_halo(const uint32_t *start, const uint32_t *slots, const uin
 		hipLaunchKernelGGL(k_laplace_h, dim3(halo_blocks(nv)), dim3(HB), 0, s, start, slots, halo, hcount,
 		                   (const d3 *)p, (d3 *)pout, nv);
 }
-// vertices per two-ring block: N2M_HC_RB = 512, 640 (default) or 768
+// vertices per two-ring block: N2M_HC_RB = 512, 640 (default) or 768, read
+// by each smoothing call
 int ring_rb()
 {
-	static const int rb = [] {
-		const char *e = getenv("N2M_HC_RB");
-		const int v = e ? atoi(e) : 640;
-		return v == 512 || v == 768 ? v : 640;
-	}();
-	return rb;
+	const char *e = getenv("N2M_HC_RB");
+	const int v = e ? atoi(e) : 640;
+	return v == 512 || v == 768 ? v : 640;
 }
-uint32_t ring_blocks(uint32_t nv) { return (nv + ring_rb() - 1) / ring_rb(); }
-size_t ring_words(uint32_t nv) { return (size_t)ring_blocks(nv) * (RROWS - ring_rb()); }
-size_t ring_pair_words(uint32_t nv) { return (size_t)ring_blocks(nv) * RPCAP; }
-size_t ring_off_words(uint32_t nv) { return (size_t)ring_blocks(nv) * ROFFW; }
-size_t ring_o_rows(uint32_t nv) { return (size_t)ring_blocks(nv) * R1CAP; }
+uint32_t ring_blocks(uint32_t nv, int rb) { return (nv + rb - 1) / rb; }
+size_t ring_words(uint32_t nv, int rb) { return (size_t)ring_blocks(nv, rb) * (RROWS - rb); }
+size_t ring_pair_words(uint32_t nv, int rb) { return (size_t)ring_blocks(nv, rb) * RPCAP; }
+size_t ring_off_words(uint32_t nv, int rb) { return (size_t)ring_blocks(nv, rb) * ROFFW; }
+size_t ring_o_rows(uint32_t nv, int rb) { return (size_t)ring_blocks(nv, rb) * R1CAP; }
 
 #define N2M_RING_RBS(X) X(512) X(640) X(768)
 
 void launch_ring_build(const uint32_t *start, const int2 *pairs, uint32_t nv, const double *o, uint32_t *ring,
                        uint32_t *rcnt, uint32_t *slots, uint16_t *roff, uint32_t *rslots, double *o1,
-                       uint32_t *overflow, hipStream_t s)
+                       uint32_t *overflow, int rbv, hipStream_t s)
 {
 	if (!nv)
 		return;
-	switch (ring_rb()) {
+	switch (rbv) {
 #define X(rb)                                                                                                      \
 	case rb:                                                                                                     \
-		hipLaunchKernelGGL(k_ring_build<rb>, dim3(ring_blocks(nv)), dim3(RT), 0, s, start, pairs, nv,               \
+		hipLaunchKernelGGL(k_ring_build<rb>, dim3(ring_blocks(nv, rb)), dim3(RT), 0, s, start, pairs, nv,           \
 		                   (const d3 *)o, ring, (uint2 *)rcnt, slots, roff, rslots, (d3 *)o1, overflow);           \
 		break;
 		N2M_RING_RBS(X)
@@ -1459,14 +1457,14 @@ void launch_ring_build(const uint32_t *start, const int2 *pairs, uint32_t nv, co
 }
 void launch_hc_ring(const uint32_t *start, const uint32_t *slots, const uint16_t *roff, const uint32_t *rslots,
                     const uint32_t *ring, const uint32_t *rcnt, const double *o, const double *o1, const double *p,
-                    double *pout, uint32_t nv, double alpha, double beta, hipStream_t s)
+                    double *pout, uint32_t nv, double alpha, double beta, int rbv, hipStream_t s)
 {
 	if (!nv)
 		return;
-	switch (ring_rb()) {
+	switch (rbv) {
 #define X(rb)                                                                                                      \
 	case rb:                                                                                                     \
-		hipLaunchKernelGGL(k_hc_ring<rb>, dim3(ring_blocks(nv)), dim3(RT), 0, s, start, slots, roff, rslots,  \
+		hipLaunchKernelGGL(k_hc_ring<rb>, dim3(ring_blocks(nv, rb)), dim3(RT), 0, s, start, slots, roff, rslots, \
 		                   ring, (const uint2 *)rcnt, (const d3 *)o, (const d3 *)o1, (const d3 *)p, (d3 *)pout, nv, \
 		                   alpha, 1.0 - alpha, beta, 1.0 - beta);                                                  \
 		break;

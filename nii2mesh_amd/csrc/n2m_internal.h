@@ -199,22 +199,23 @@ void launch_hc_b_halo(const uint32_t *start, const uint32_t *slots, const uint32
                       const double *pn, const double *bq, double *pout, uint32_t nv, double beta, hipStream_t s);
 void launch_laplace_halo(const uint32_t *start, const uint32_t *slots, const uint32_t *halo,
                          const uint32_t *hcount, const double *p, double *pout, uint32_t nv, hipStream_t s);
-// two-ring form of an HC iteration: per 768-vertex block, ring 1 and ring 2
-// (ring_words, stride 1792; counts as uint2 in rcnt, 2 words per block), the
+// two-ring form of an HC iteration: per block of rb vertices, ring 1 and ring 2
+// (ring_words, stride 2304 - rb; counts as uint2 in rcnt, 2 words per block), the
 // block's pairs as 16-bit LDS row slots (slots, 3*nt words) and ring 1's pairs
 // (roff: ring_off_words 16-bit offsets, rslots: ring_pair_words words), and
 // ring 1's original positions (o1)
-uint32_t ring_blocks(uint32_t nv);
-size_t ring_words(uint32_t nv);
-size_t ring_pair_words(uint32_t nv);
-size_t ring_off_words(uint32_t nv);
-size_t ring_o_rows(uint32_t nv); // ring 1's original positions, stride 1024 rows
+int ring_rb(); // vertices per block (N2M_HC_RB: 512, 640 or 768)
+uint32_t ring_blocks(uint32_t nv, int rb);
+size_t ring_words(uint32_t nv, int rb);
+size_t ring_pair_words(uint32_t nv, int rb);
+size_t ring_off_words(uint32_t nv, int rb);
+size_t ring_o_rows(uint32_t nv, int rb); // ring 1's original positions, stride 1024 rows
 void launch_ring_build(const uint32_t *start, const int2 *pairs, uint32_t nv, const double *o, uint32_t *ring,
                        uint32_t *rcnt, uint32_t *slots, uint16_t *roff, uint32_t *rslots, double *o1,
-                       uint32_t *overflow, hipStream_t s);
+                       uint32_t *overflow, int rb, hipStream_t s);
 void launch_hc_ring(const uint32_t *start, const uint32_t *slots, const uint16_t *roff, const uint32_t *rslots,
                     const uint32_t *ring, const uint32_t *rcnt, const double *o, const double *o1, const double *p,
-                    double *pout, uint32_t nv, double alpha, double beta, hipStream_t s);
+                    double *pout, uint32_t nv, double alpha, double beta, int rb, hipStream_t s);
 // smoothing state: 3 doubles per vertex, optionally gathered / scattered
 // through a vertex permutation
 void launch_hc_load(const double *src, const uint32_t *perm, uint32_t n, double *dst, hipStream_t s);

@@ -184,14 +184,16 @@ def test_hc_gather_forms(gpu, oracle, monkeypatch, form, iters, lock):
     assert ('hc_a' in stats) == (form != 'ring')
 
 
+@pytest.mark.parametrize('rb', ['512', '640', '768'])
 @pytest.mark.parametrize('lock', [True, False])
-def test_hc_ring_long_rings(gpu, oracle, lock):
-    """A noisy surface (55000 vertices) on which 18 of the 72 blocks of 768
-    vertices have more than 256 ring-1 neighbours, so the one-launch HC
-    iteration computes those blocks' ring-1 rows in a second pass of its
-    workgroup; the rings still fit, the one-launch form runs, and the result
-    is the oracle's bit for bit."""
+def test_hc_ring_block_sizes(gpu, oracle, monkeypatch, rb, lock):
+    """The one-launch HC iteration at each block size (N2M_HC_RB) on a noisy
+    surface of 55000 vertices.  With 768-vertex blocks, 18 of the 72 blocks
+    have more than 256 ring-1 neighbours (1024 threads - 768), so their
+    workgroups compute ring-1 rows in a second pass.  The rings fit, the
+    one-launch form runs, and the result is the oracle's bit for bit."""
     import torch
+    monkeypatch.setenv('N2M_HC_RB', rb)
     img = support.blobs((80, 84, 88), 4, noise=8.0)
     iso = float(np.float32(np.percentile(img, 50)))
     ctx = gpu.DeviceContext()
