@@ -197,6 +197,7 @@ struct Pending {
 	int stat;
 	hipEvent_t a, b;
 	double bytes;
+	int launches; // kernel launches between the two events
 };
 
 } // namespace
@@ -303,14 +304,16 @@ struct n2m_ctx {
 		cur_bytes = bytes;
 		HIPX(hipEventRecord(cur_a, stream));
 	}
-	void tend()
+	// launches: how many launches of the stage the bracket holds (a loop of
+	// back-to-back launches is timed with one event pair)
+	void tend(int launches = 1)
 	{
 		check();
 		if (!timing || cur_stat < 0)
 			return;
 		hipEvent_t b = ev();
 		HIPX(hipEventRecord(b, stream));
-		pending.push_back({cur_stat, cur_a, b, cur_bytes});
+		pending.push_back({cur_stat, cur_a, b, cur_bytes, launches});
 		cur_stat = -1;
 	}
 	void resolve()
@@ -322,7 +325,7 @@ struct n2m_ctx {
 			float ms = 0;
 			HIPX(hipEventElapsedTime(&ms, p.a, p.b));
 			stats[p.stat].ms += ms;
-			stats[p.stat].n += 1;
+			stats[p.stat].n += p.launches;
 			stats[p.stat].bytes += p.bytes;
 			evpool.push_back(p.a);
 			evpool.push_back(p.b);
@@ -962,15 +965,16 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 	const double *res = o;
 	if (G.on) {
 		// algorithmic bytes: CSR offsets, one 4-byte slot pair per reference,
-		// the original and current positions in, the new positions out
+		// the original and current positions in, the new positions out.  The
+		// iterations are back-to-back launches, timed with one event pair.
+		C->tbegin("hc_ring", (4.0 * nv + 12.0 * nt + 72.0 * nv) * iter);
 		for (int it = 0; it < iter; it++) {
 			double *nxt = it % 2 == 0 ? p : pn;
-			C->tbegin("hc_ring", 4.0 * nv + 12.0 * nt + 72.0 * nv);
 			launch_hc_ring(c.start, G.slots, G.roff, G.rslots, G.ring, G.rcnt, o, G.o1, res, nxt, nv, alpha, beta,
 			               G.rb, s);
-			C->tend();
 			res = nxt;
 		}
+		C->tend(iter);
 	} else {
 		double *bq = C->buf<double>(S_HC_B, n3);
 		// algorithmic bytes: CSR offsets, one pair word per reference (4
