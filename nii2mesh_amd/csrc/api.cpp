@@ -15,6 +15,7 @@
 
 #include <atomic>
 #include <limits>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -457,8 +458,11 @@ struct VolOut {
 };
 
 bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P, bool have_range,
-                   bool spec_bits_valid_for_iso, VolOut &out)
+                   bool spec_bits_valid_for_iso, VolOut &out, const float *src = nullptr)
 {
+	// src (when given) holds the voxels; the stages read it and the apply
+	// stage writes every voxel of vol, so vol needs no copy of src
+	const float *in = src ? src : vol;
 	const int NX = dim[0], NY = dim[1], NZ = dim[2];
 	const size_t nvox = (size_t)NX * NY * NZ;
 	hipStream_t s = C->stream;
@@ -466,7 +470,7 @@ bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P
 		uint32_t init[2] = {0xffffffffu, 0u};
 		HIPX(hipMemcpyAsync(C->d_small + SM_RANGE, init, sizeof init, hipMemcpyHostToDevice, s));
 		C->tbegin("minmax", 4.0 * nvox);
-		launch_minmax(vol, nvox, C->d_small + SM_RANGE, s);
+		launch_minmax(in, nvox, C->d_small + SM_RANGE, s);
 		C->tend();
 	}
 	uint32_t *r = C->fetch(SM_RANGE, 2);
@@ -496,7 +500,7 @@ bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P
 		uint32_t *bits = C->buf<uint32_t>(S_BITS, nwords);
 		if (!spec_bits_valid_for_iso || iso_changed) {
 			C->tbegin("mask_bits", 4.0 * nvox + nvox / 8.0);
-			launch_mask_bits(vol, NX, NY, NZ, isolevel, 0, bits, s);
+			launch_mask_bits(in, NX, NY, NZ, isolevel, 0, bits, s);
 			C->tend();
 		}
 		keep = bits;
@@ -545,7 +549,7 @@ bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P
 	HIPX(hipMemcpyAsync(C->d_small + SM_BBOX, init, 6 * sizeof(int), hipMemcpyHostToDevice, s));
 	HIPX(hipMemcpyAsync(C->d_small + SM_VMIN, &minit, 4, hipMemcpyHostToDevice, s));
 	C->tbegin("apply", 8.0 * nvox);
-	launch_apply(vol, NX, NY, NZ, isolevel, mn, edgeMax, P.fillBubbles ? keep : nullptr, dil, P.fillBubbles ? 1 : 0,
+	launch_apply(in, vol, NX, NY, NZ, isolevel, mn, edgeMax, P.fillBubbles ? keep : nullptr, dil, P.fillBubbles ? 1 : 0,
 	             (int *)(C->d_small + SM_BBOX), C->d_small + SM_VMIN, s);
 	C->tend();
 	uint32_t *bb = C->fetch(SM_BBOX, 7);
@@ -2097,19 +2101,36 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 		// N2M_PIPE_THREADS=0 meshes the isolevels one after another.
 		const char *pt = getenv("N2M_PIPE_THREADS");
 		const bool threaded = niso > 1 && !(pt && *pt == '0');
+		// N2M_PIPE_HC=1 (with threads): a worker smooths its mesh as soon as
+		// it is meshed, while other isolevels are still being meshed; a mutex
+		// keeps the HC passes of different meshes from overlapping on the GPU
+		// (each waits for the previous one to finish).  By default all meshes
+		// are smoothed here after the meshing: the step is within 1 % either
+		// way, and the HC kernels then run on a GPU of their own.
+		const char *ph = getenv("N2M_PIPE_HC");
+		const bool hc_in_worker = threaded && hcIter > 0 && ph && *ph == '1';
+		std::mutex hc_mu;
 		std::vector<char> done(niso, 0);
 		auto mesh_one = [&](n2m_ctx *D, int k) {
 			n2m_params P = *prm;
 			P.isolevel = isolevels[k];
-			float *v = D->buf<float>(S_VOL, nvox);
-			HIPX(hipMemcpyAsync(v, smoothed ? sm : d_img, 4 * nvox, hipMemcpyDeviceToDevice, D->stream));
+			float *v = D->buf<float>(S_VOL, nvox); // written whole by the apply stage
 			if (smoothed)
 				HIPX(hipMemcpyAsync(D->d_small + SM_RANGE, range, sizeof range, hipMemcpyHostToDevice, D->stream));
 			Mesh &m = C->meshes[k];
 			m.nt = m.np = 0;
 			VolOut vo;
-			if (volume_stages(D, v, dim, P, smoothed, false, vo) && surface_stages(D, v, dim, vo, P, m))
+			if (volume_stages(D, v, dim, P, smoothed, false, vo, smoothed ? sm : d_img) &&
+			    surface_stages(D, v, dim, vo, P, m)) {
 				done[k] = 1;
+				if (hc_in_worker) {
+					D->sync();
+					std::lock_guard<std::mutex> lock(hc_mu);
+					smooth_hc_device(D, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
+					                 true);
+					D->sync();
+				}
+			}
 			D->sync();
 		};
 		if (threaded) {
@@ -2147,7 +2168,7 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 			if (!done[k])
 				continue;
 			Mesh &m = C->meshes[k];
-			if (hcIter > 0)
+			if (hcIter > 0 && !hc_in_worker)
 				smooth_hc_device(C, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
 				                 true);
 			ok++;

@@ -576,7 +576,7 @@ void launch_dilate_bits(const uint32_t *in, uint32_t *out, int nx, int ny, int n
 // bbox: int[6] = {minx,miny,minz,maxx,maxy,maxz}.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_apply(float *__restrict__ img, int nx, int ny, int nz, int W, int xpad, float iso, float mn,
+k_apply(const float *src, float *img, int nx, int ny, int nz, int W, int xpad, float iso, float mn,
         float edgeMax, const uint32_t *__restrict__ keep, const uint32_t *__restrict__ dil, int fill,
         int *__restrict__ bbox, uint32_t *__restrict__ vmin)
 {
@@ -589,11 +589,12 @@ k_apply(float *__restrict__ img, int nx, int ny, int nz, int W, int xpad, float 
 	for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
 		const int y = (int)(row % ny), z = (int)(row / ny);
 		const bool face_row = y == 0 || z == 0 || y == ny - 1 || z == nz - 1;
+		const float *in = src + row * nx;
 		float *line = img + row * nx;
 		const uint32_t *krow = fill && keep ? keep + (size_t)W * row : nullptr;
 		const uint32_t *drow = dil ? dil + (size_t)W * row : nullptr;
 		for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-			float val = line[x];
+			float val = in[x];
 			if (krow && ((krow[x >> 5] >> (x & 31)) & 1u))
 				val = (float)fmax((double)val, (double)iso);
 			if (drow && !((drow[x >> 5] >> (x & 31)) & 1u))
@@ -646,12 +647,12 @@ k_apply(float *__restrict__ img, int nx, int ny, int nz, int W, int xpad, float 
 	}
 }
 
-void launch_apply(float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax,
+void launch_apply(const float *src, float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax,
                   const uint32_t *keep, const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s)
 {
 	int W = (nx + 31) / 32, xpad = (nx + 63) / 64 * 64;
-	hipLaunchKernelGGL(k_apply, dim3(row_blocks((size_t)ny * nz)), dim3(256), 0, s, img, nx, ny, nz, W, xpad, iso,
-	                   mn, edgeMax, keep, dil, fill, bbox, vmin);
+	hipLaunchKernelGGL(k_apply, dim3(row_blocks((size_t)ny * nz)), dim3(256), 0, s, src ? src : img, img, nx, ny,
+	                   nz, W, xpad, iso, mn, edgeMax, keep, dil, fill, bbox, vmin);
 }
 
 // ---------------------------------------------------------------------------

@@ -57,8 +57,9 @@ void launch_minmax(const float *img, size_t n, uint32_t *range, hipStream_t s);
 void launch_mask_bits(const float *img, int nx, int ny, int nz, float iso, int nonzero, uint32_t *bits,
                       hipStream_t s);
 void launch_dilate_bits(const uint32_t *in, uint32_t *out, int nx, int ny, int nz, hipStream_t s);
-void launch_apply(float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax, const uint32_t *keep,
-                  const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s);
+// img = apply(src) (src null: in place); every voxel of img is written
+void launch_apply(const float *src, float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax,
+                  const uint32_t *keep, const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s);
 void launch_hist_range(const float *img, size_t n, uint32_t *stats, hipStream_t s);
 // mesh output encoding (k_encode.hip)
 void launch_f64_to_f32(const double *in, size_t n, float *out, hipStream_t s);

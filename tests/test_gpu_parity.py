@@ -213,15 +213,17 @@ def test_hc_ring_block_sizes(gpu, oracle, monkeypatch, rb, lock):
     assert 'hc_ring' in stats and 'hc_a' not in stats
 
 
-@pytest.mark.parametrize('threads', ['1', '0'])
+@pytest.mark.parametrize('threads,hc_in_worker', [('1', '1'), ('1', '0'), ('0', '0')])
 @pytest.mark.parametrize('cubes', [0, 1])
-def test_pipeline_isolevels(gpu, oracle, monkeypatch, threads, cubes):
+def test_pipeline_isolevels(gpu, oracle, monkeypatch, threads, hc_in_worker, cubes):
     """The device pipeline (one pre-smooth shared by several isolevels, each
     meshed and HC-smoothed -- by one host thread and context per isolevel,
-    or one after the other) gives, per isolevel, the oracle's meshify +
+    smoothing as soon as its mesh is ready or after all meshing, or one
+    after the other) gives, per isolevel, the oracle's meshify +
     laplacian_smoothHC(lockEdges) bit for bit."""
     import torch
     monkeypatch.setenv('N2M_PIPE_THREADS', threads)
+    monkeypatch.setenv('N2M_PIPE_HC', hc_in_worker)
     img = support.blobs((60, 64, 68), 8, noise=2.0)
     isos = [float(np.float32(np.percentile(img, q))) for q in (40, 60, 80)]
     ctx = gpu.DeviceContext()
