@@ -67,12 +67,14 @@ size_t scan_scratch_words(size_t n)
 	return w + 1;
 }
 
+__global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+
 void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, uint32_t *scratch,
                    hipStream_t s)
 {
 	if (n == 0) {
-		if (total)
-			hipMemsetAsync(total, 0, sizeof(uint32_t), s);
+		if (total) // a kernel, not a memset: launch errors reach the caller's stream check
+			hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, total);
 		return;
 	}
 	size_t nb = (n + SC_TILE - 1) / SC_TILE;

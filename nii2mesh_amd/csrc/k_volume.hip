@@ -22,8 +22,7 @@ namespace n2m {
 // planes per column in registers for the z pass.  Every pass computes
 // float*double products summed left to right in double (meshify.c:189).
 // ---------------------------------------------------------------------------
-constexpr int SM_TX = 64, SM_TY = 8, SM_ZC = 32, SM_BS = 256;
-constexpr int SM_CPT = SM_TX * SM_TY / SM_BS; // columns per thread (2)
+constexpr int SM_TX = 64, SM_TY = 8, SM_ZC = 32, SM_BS = 256; // 2 columns per thread
 
 __device__ __forceinline__ float tap5(float a, float b, float c, float d, float e)
 {
