@@ -658,20 +658,43 @@ void launch_apply(const float *src, float *img, int nx, int ny, int nz, float is
 // Histograms for setThreshold() (src/isolevel.c).
 // stats: [0]=ord min, [1]=ord max over non-NaN, [2]=NaN count.
 // ---------------------------------------------------------------------------
+// f(v) for every voxel of img[0, n), grid-stride; 16-byte loads when img is
+// 16-byte aligned (the visiting order does not matter to the callers: min,
+// max and counts)
+template <typename F> __device__ __forceinline__ void each_voxel(const float *__restrict__ img, size_t n, F f)
+{
+	const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+	size_t done = 0;
+	if ((reinterpret_cast<uintptr_t>(img) & 15u) == 0) {
+		const float4 *v4 = reinterpret_cast<const float4 *>(img);
+		const size_t n4 = n / 4;
+#pragma unroll 2
+		for (size_t i = tid; i < n4; i += stride) {
+			const float4 q = v4[i];
+			f(q.x);
+			f(q.y);
+			f(q.z);
+			f(q.w);
+		}
+		done = 4 * n4;
+	}
+	for (size_t i = done + tid; i < n; i += stride)
+		f(img[i]);
+}
+
 __global__ void __launch_bounds__(256) k_hist_range(const float *__restrict__ img, size_t n, uint32_t *stats)
 {
 	__shared__ uint32_t s_red[3][4];
 	float lmin = __builtin_inff(), lmax = -__builtin_inff();
 	uint32_t nan = 0;
-	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-		float v = img[i];
+	each_voxel(img, n, [&](float v) {
 		if (v != v) {
 			nan++;
-			continue;
+			return;
 		}
 		lmin = fminf(lmin, v);
 		lmax = fmaxf(lmax, v);
-	}
+	});
 	uint32_t omin = wave_min_u32(f2ord(lmin)), omax = wave_max_u32(f2ord(lmax));
 	nan = wave_sum(nan);
 	if (lane_id() == 0) {
@@ -702,15 +725,27 @@ k_hist_bins(const float *__restrict__ img, size_t n, float mn, float scl, int nb
 	for (int i = threadIdx.x; i < nbins; i += blockDim.x)
 		s_h[i] = 0;
 	__syncthreads();
-	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-		float v = img[i];
+	// a run of voxels in the same bin (most of a background) is added with
+	// one LDS atomic
+	int cb = 0;
+	uint32_t cc = 0;
+	each_voxel(img, n, [&](float v) {
 		if (v != v)
-			continue;
+			return;
 		float f = (v - mn) * scl;
 		int b = (int)roundf(f);
 		b = b < 0 ? 0 : (b > nbins - 1 ? nbins - 1 : b);
-		atomicAdd(&s_h[b], 1u);
-	}
+		if (b == cb) {
+			cc++;
+			return;
+		}
+		if (cc)
+			atomicAdd(&s_h[cb], cc);
+		cb = b;
+		cc = 1;
+	});
+	if (cc)
+		atomicAdd(&s_h[cb], cc);
 	__syncthreads();
 	for (int i = threadIdx.x; i < nbins; i += blockDim.x)
 		if (s_h[i])
