@@ -525,6 +525,7 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 {
 	static_assert(LW_BS == 4 * N2M_WAVE, "one wave per point row");
 	__shared__ uint32_t s_pre[4][LW_SEG + 1]; // popcount prefix for the 4 point rows
+	__shared__ uint8_t s_flag[4][LW_SEG + 1]; // the 4 rows' point flags, [0, len]
 	__shared__ uint16_t s_crow[LW_SEG];
 	__shared__ uint32_t s_base[4];
 	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
@@ -559,7 +560,12 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 #pragma unroll
 			for (int k = 0; k < NIT; k++) {
 				const int i = k * N2M_WAVE + lane;
-				f[k] = i < len ? __popc(pflags[rb + xs + i]) : 0u;
+				// point len (the next segment's first point) closes the
+				// segment's last cube; it exists unless the row ends at len
+				const uint32_t fl = i <= len && xs + i < g.sx ? pflags[rb + xs + i] : 0u;
+				if (i <= len)
+					s_flag[r][i] = (uint8_t)fl;
+				f[k] = i < len ? __popc(fl) : 0u;
 			}
 			uint32_t carry = 0;
 #pragma unroll
@@ -586,7 +592,7 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 		const size_t rb = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
 		for (int i = threadIdx.x; i < len; i += LW_BS) {
 			int x = xs + i;
-			uint32_t f = pflags[rb + x];
+			uint32_t f = s_flag[0][i];
 			uint32_t id = s_base[0] + s_pre[0][i];
 			for (int a = 0; a < 3; a++) {
 				if (!((f >> a) & 1u))
@@ -640,8 +646,7 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 			for (int e = 0; e < 12; e++) {
 				int dx = c_code[e][0], dy = c_code[e][1], dz = c_code[e][2], a = c_code[e][3];
 				int r = dy + 2 * dz;
-				size_t pb = (size_t)g.sx * ((size_t)(y + dy) + (size_t)g.sy * (z + dz));
-				uint32_t f = pflags[pb + x + dx];
+				uint32_t f = s_flag[r][i + dx];
 				ids[e] = ((f >> a) & 1u) ? s_base[r] + s_pre[r][i + dx] + __popc(f & ((1u << a) - 1u))
 				                         : 0xffffffffu;
 			}
