@@ -499,12 +499,52 @@ k_mask_bits(const float *__restrict__ img, int nx, int ny, int nz, int W, int xp
 	}
 }
 
+// The same with 16-byte loads, for rows of a multiple of 32 voxels (nx = 32W)
+// whose 16-byte chunks (lpr = nx/4 per row) either fit a workgroup (several
+// rows per pass) or fill whole passes of it.  A lane tests 4 voxels; 8
+// neighbouring lanes OR their nibbles into a word.
+__global__ void __launch_bounds__(256)
+k_mask_bits4(const float *__restrict__ img, int nx, int ny, int nz, int W, float iso, int nonzero,
+             uint32_t *__restrict__ bits)
+{
+	const size_t rows = (size_t)ny * nz;
+	const int lpr = nx >> 2, span = lpr < (int)blockDim.x ? lpr : (int)blockDim.x;
+	const int rpi = (int)blockDim.x / span, sub = threadIdx.x / span, q0 = threadIdx.x % span;
+	for (size_t row0 = (size_t)blockIdx.x * rpi; row0 < rows; row0 += (size_t)gridDim.x * rpi) {
+		const size_t row = row0 + sub;
+		const bool live = sub < rpi && row < rows;
+		for (int q = q0; q < lpr; q += span) { // the same trip count on every lane
+			uint32_t nib = 0;
+			if (live) {
+				const float4 v = reinterpret_cast<const float4 *>(img + row * nx)[q];
+				if (nonzero)
+					nib = (v.x != 0.0f) | (v.y != 0.0f) << 1 | (v.z != 0.0f) << 2 | (v.w != 0.0f) << 3;
+				else
+					nib = (v.x >= iso) | (v.y >= iso) << 1 | (v.z >= iso) << 2 | (v.w >= iso) << 3;
+			}
+			uint32_t wv = nib << (4 * (q & 7));
+			wv |= __shfl_xor(wv, 1, N2M_WAVE);
+			wv |= __shfl_xor(wv, 2, N2M_WAVE);
+			wv |= __shfl_xor(wv, 4, N2M_WAVE);
+			if (live && (q & 7) == 0)
+				bits[(size_t)W * row + (q >> 3)] = wv;
+		}
+	}
+}
+
 static unsigned row_blocks(size_t rows) { return (unsigned)(rows < 8192 ? (rows ? rows : 1) : 8192); }
 
 void launch_mask_bits(const float *img, int nx, int ny, int nz, float iso, int nonzero, uint32_t *bits,
                       hipStream_t s)
 {
 	int W = (nx + 31) / 32, xpad = (nx + 63) / 64 * 64;
+	const int lpr = nx / 4;
+	if (nx % 32 == 0 && (reinterpret_cast<uintptr_t>(img) & 15u) == 0 && (lpr <= 256 || lpr % 256 == 0)) {
+		const size_t rows = (size_t)ny * nz, rpi = lpr < 256 ? 256 / lpr : 1;
+		hipLaunchKernelGGL(k_mask_bits4, dim3(row_blocks((rows + rpi - 1) / rpi)), dim3(256), 0, s, img, nx, ny,
+		                   nz, W, iso, nonzero, bits);
+		return;
+	}
 	hipLaunchKernelGGL(k_mask_bits, dim3(row_blocks((size_t)ny * nz)), dim3(256), 0, s, img, nx, ny, nz, W, xpad,
 	                   iso, nonzero, bits);
 }
@@ -575,7 +615,7 @@ void launch_dilate_bits(const uint32_t *in, uint32_t *out, int nx, int ny, int n
 // bbox: int[6] = {minx,miny,minz,maxx,maxy,maxz}.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_apply(const float *src, float *img, int nx, int ny, int nz, int W, int xpad, float iso, float mn,
+k_apply(const float *src, float *img, int nx, int ny, int nz, int W, int vec4, float iso, float mn,
         float edgeMax, const uint32_t *__restrict__ keep, const uint32_t *__restrict__ dil, int fill,
         int *__restrict__ bbox, uint32_t *__restrict__ vmin)
 {
@@ -584,32 +624,64 @@ k_apply(const float *src, float *img, int nx, int ny, int nz, int W, int xpad, f
 	float lmin = __builtin_inff();
 	int bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, bmax[3] = {-1, -1, -1};
 	const size_t rows = (size_t)ny * nz;
-	// whole rows per workgroup: y and z are uniform, no per-voxel division
-	for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
-		const int y = (int)(row % ny), z = (int)(row / ny);
-		const bool face_row = y == 0 || z == 0 || y == ny - 1 || z == nz - 1;
-		const float *in = src + row * nx;
-		float *line = img + row * nx;
-		const uint32_t *krow = fill && keep ? keep + (size_t)W * row : nullptr;
-		const uint32_t *drow = dil ? dil + (size_t)W * row : nullptr;
-		for (int x = threadIdx.x; x < nx; x += blockDim.x) {
-			float val = in[x];
-			if (krow && ((krow[x >> 5] >> (x & 31)) & 1u))
-				val = (float)fmax((double)val, (double)iso);
-			if (drow && !((drow[x >> 5] >> (x & 31)) & 1u))
-				val = mn;
-			if (val >= iso) {
-				bmin[0] = min(bmin[0], x);
-				bmax[0] = max(bmax[0], x);
-				bmin[1] = min(bmin[1], y);
-				bmax[1] = max(bmax[1], y);
-				bmin[2] = min(bmin[2], z);
-				bmax[2] = max(bmax[2], z);
+	// one voxel: kw / dw are the keep / dilated-mask words holding bit x & 31
+	auto voxel = [&](int x, int y, int z, bool face_row, float val, uint32_t kw, uint32_t dw, bool k_on,
+	                 bool d_on) -> float {
+		if (k_on && ((kw >> (x & 31)) & 1u))
+			val = (float)fmax((double)val, (double)iso);
+		if (d_on && !((dw >> (x & 31)) & 1u))
+			val = mn;
+		if (val >= iso) {
+			bmin[0] = min(bmin[0], x);
+			bmax[0] = max(bmax[0], x);
+			bmin[1] = min(bmin[1], y);
+			bmax[1] = max(bmax[1], y);
+			bmin[2] = min(bmin[2], z);
+			bmax[2] = max(bmax[2], z);
+		}
+		if (face_row || x == 0 || x == nx - 1)
+			val = fminf(edgeMax, val);
+		lmin = fminf(lmin, val);
+		return val;
+	};
+	const bool k_on = fill && keep, d_on = dil != nullptr;
+	if (vec4) {
+		// 16-byte loads and stores; rows of nx = 4 * lpr voxels, several rows
+		// per pass when a row has fewer 16-byte chunks than the workgroup
+		const int lpr = nx >> 2, span = lpr < (int)blockDim.x ? lpr : (int)blockDim.x;
+		const int rpi = (int)blockDim.x / span, sub = threadIdx.x / span, q0 = threadIdx.x % span;
+		for (size_t row0 = (size_t)blockIdx.x * rpi; row0 < rows; row0 += (size_t)gridDim.x * rpi) {
+			const size_t row = row0 + sub;
+			if (sub >= rpi || row >= rows)
+				continue;
+			const int y = (int)(row % ny), z = (int)(row / ny);
+			const bool face_row = y == 0 || z == 0 || y == ny - 1 || z == nz - 1;
+			const float4 *in = reinterpret_cast<const float4 *>(src + row * nx);
+			float4 *line = reinterpret_cast<float4 *>(img + row * nx);
+			for (int q = q0; q < lpr; q += span) {
+				const int x = 4 * q;
+				const uint32_t kw = k_on ? keep[(size_t)W * row + (x >> 5)] : 0u;
+				const uint32_t dw = d_on ? dil[(size_t)W * row + (x >> 5)] : 0u;
+				float4 v = in[q];
+				v.x = voxel(x, y, z, face_row, v.x, kw, dw, k_on, d_on);
+				v.y = voxel(x + 1, y, z, face_row, v.y, kw, dw, k_on, d_on);
+				v.z = voxel(x + 2, y, z, face_row, v.z, kw, dw, k_on, d_on);
+				v.w = voxel(x + 3, y, z, face_row, v.w, kw, dw, k_on, d_on);
+				line[q] = v;
 			}
-			if (face_row || x == 0 || x == nx - 1)
-				val = fminf(edgeMax, val);
-			line[x] = val;
-			lmin = fminf(lmin, val);
+		}
+	} else {
+		// whole rows per workgroup: y and z are uniform, no per-voxel division
+		for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
+			const int y = (int)(row % ny), z = (int)(row / ny);
+			const bool face_row = y == 0 || z == 0 || y == ny - 1 || z == nz - 1;
+			const float *in = src + row * nx;
+			float *line = img + row * nx;
+			for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+				const uint32_t kw = k_on ? keep[(size_t)W * row + (x >> 5)] : 0u;
+				const uint32_t dw = d_on ? dil[(size_t)W * row + (x >> 5)] : 0u;
+				line[x] = voxel(x, y, z, face_row, in[x], kw, dw, k_on, d_on);
+			}
 		}
 	}
 	for (int k = 0; k < 3; k++) {
@@ -649,9 +721,13 @@ k_apply(const float *src, float *img, int nx, int ny, int nz, int W, int xpad, f
 void launch_apply(const float *src, float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax,
                   const uint32_t *keep, const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s)
 {
-	int W = (nx + 31) / 32, xpad = (nx + 63) / 64 * 64;
-	hipLaunchKernelGGL(k_apply, dim3(row_blocks((size_t)ny * nz)), dim3(256), 0, s, src ? src : img, img, nx, ny,
-	                   nz, W, xpad, iso, mn, edgeMax, keep, dil, fill, bbox, vmin);
+	const int W = (nx + 31) / 32;
+	const float *in = src ? src : img;
+	const bool vec4 = nx % 4 == 0 && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(img)) & 15u) == 0;
+	const size_t rows = (size_t)ny * nz;
+	const size_t rpi = vec4 && nx / 4 < 256 ? 256 / (nx / 4) : 1;
+	hipLaunchKernelGGL(k_apply, dim3(row_blocks((rows + rpi - 1) / rpi)), dim3(256), 0, s, in, img, nx, ny, nz, W,
+	                   vec4 ? 1 : 0, iso, mn, edgeMax, keep, dil, fill, bbox, vmin);
 }
 
 // ---------------------------------------------------------------------------
