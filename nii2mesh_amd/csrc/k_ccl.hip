@@ -7,9 +7,10 @@
 // order and the largest-cluster tie rule (lowest label wins,
 // src/bwlabel.c:463) is "smallest root wins".
 //
-//  1. k_ccl_local: a workgroup labels a 32x8x8 tile in LDS (one bit word per
-//     tile row) and writes each voxel's tile-local root to L; it also clears
-//     AUX at every tile-local root (every global root is one of them).
+//  1. k_ccl_local: a workgroup labels the runs of a 512x4x4 tile in LDS
+//     (16 bit words per tile row) and writes each run's tile-local root to
+//     L; it also clears AUX at every tile-local root (every global root is
+//     one of them).
 //  2. k_ccl_merge: tile-boundary voxels union with their backward neighbours
 //     in other tiles through global atomics.
 //  3. k_ccl_flatten: L[v] = root(v); per-root voxel counts (wave-aggregated
@@ -24,8 +25,6 @@
 namespace n2m {
 
 constexpr int CC_KW = 16, CC_TY = 4, CC_TZ = 4, CC_BS = CC_KW * CC_TY * CC_TZ; // 256
-constexpr int CC_TXV = CC_KW * 32;                 // voxels of x per tile (512)
-constexpr int CC_NODES = CC_TXV * CC_TY * CC_TZ;   // LDS label slots (8192)
 constexpr uint32_t NONE = 0xffffffffu;
 
 // Run-based labelling.  A row word is split into maximal runs of set bits
@@ -177,17 +176,38 @@ k_run_count(const uint32_t *__restrict__ bits, Geo g, int invert, uint32_t *__re
 	cnt[t] = c;
 }
 
-// Pass 1: label runs inside a tile of 16 words x 4 rows x 4 slices in LDS
-// (node = position of the run's first bit in the tile; the smallest node of
-// a tile component is its smallest run id).  L[run] = the tile-local root
-// run; AUX[root] = 0 for every tile-local root.
+// Pass 1: label runs inside a tile of 16 words x 4 rows x 4 slices in LDS.
+// A run's tile-local node is its rank among the tile's runs in (slice, row,
+// word, bit) order -- the order of their global ids -- so the smallest node
+// of a tile component is its smallest run id.  A tile holds at most
+// 256 words x 16 runs.  L[run] = the tile-local root run; AUX[root] = 0 for
+// every tile-local root.
+constexpr int CC_RUNS = CC_BS * 16;
+
+// word of the tile (flat index r * CC_KW + k) holding tile-local run `node`:
+// the last word whose first run is at or before it
+__device__ __forceinline__ int tile_word_of(const uint32_t *pre, uint32_t node)
+{
+	int lo = 0, hi = CC_BS - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (pre[mid] <= node)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return lo;
+}
+
 __global__ void __launch_bounds__(CC_BS)
 k_ccl_local(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint32_t *__restrict__ L,
             uint32_t *__restrict__ AUX)
 {
-	__shared__ uint32_t lab[CC_NODES];
+	__shared__ uint32_t lab[CC_RUNS];
 	__shared__ uint32_t s_w[CC_TY * CC_TZ][CC_KW];
 	__shared__ uint32_t s_rb[CC_TY * CC_TZ][CC_KW];
+	__shared__ uint32_t s_pre[CC_TY * CC_TZ][CC_KW]; // first tile-local node of each word
+	__shared__ uint32_t scratch[CC_BS / N2M_WAVE + 1];
 	const int k = threadIdx.x % CC_KW, r = threadIdx.x / CC_KW; // word in tile, row in tile
 	const int ry = r % CC_TY, rz = r / CC_TY;
 	const int wx = blockIdx.x * CC_KW + k, y = blockIdx.y * CC_TY + ry, z = blockIdx.z * CC_TZ + rz;
@@ -195,17 +215,15 @@ k_ccl_local(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint
 	s_w[r][k] = w;
 	s_rb[r][k] = w ? g.rb[g.widx(wx, y, z)] : 0u;
 	const uint32_t st = run_starts(w);
-	const int base = k * 32 + CC_TXV * r;
-	for (uint32_t t = st; t; t &= t - 1) {
-		int b = __ffs(t) - 1;
-		lab[base + b] = base + b;
-	}
+	uint32_t tot;
+	const uint32_t pre = block_excl_scan<CC_BS>(__popc(st), scratch, &tot); // thread order = (r, k)
+	s_pre[r][k] = pre;
+	for (uint32_t j = 0, n = __popc(st); j < n; j++)
+		lab[pre + j] = pre + j;
 	__syncthreads();
 	// x-1 across the word boundary inside the tile
-	if (k > 0 && (w & 1u) && (s_w[r][k - 1] >> 31)) {
-		uint32_t sp = run_starts(s_w[r][k - 1]);
-		lds_union(lab, base, base - 32 + run_of(sp, 31));
-	}
+	if (k > 0 && (w & 1u) && (s_w[r][k - 1] >> 31))
+		lds_union(lab, pre, s_pre[r][k - 1] + run_index(run_starts(s_w[r][k - 1]), 31));
 	int nn;
 	const NbrRow *nb = nbr_rows(conn, &nn);
 	for (int i = 0; i < nn; i++) {
@@ -218,17 +236,15 @@ k_ccl_local(const uint32_t *__restrict__ bits, Geo g, int conn, int invert, uint
 		for (int dx = nb[i].dxlo; dx <= nb[i].dxhi; dx++)
 			for_links(w, u, up, un, dx, [&](int bw, int wo, int bu) {
 				uint32_t uw = wo == 0 ? u : (wo < 0 ? up : un);
-				lds_union(lab, base + run_of(st, bw), (k + wo) * 32 + CC_TXV * r2 + run_of(run_starts(uw), bu));
+				lds_union(lab, pre + run_index(st, bw), s_pre[r2][k + wo] + run_index(run_starts(uw), bu));
 			});
 	}
 	__syncthreads();
 	uint32_t gi = s_rb[r][k];
-	for (uint32_t t = st; t; t &= t - 1, gi++) {
-		int b = __ffs(t) - 1;
-		uint32_t root = lds_find(lab, base + b);
-		int rr = root / CC_TXV, rx = root % CC_TXV;
-		const uint32_t rw = s_w[rr][rx >> 5];
-		const uint32_t gr = s_rb[rr][rx >> 5] + run_index(run_starts(rw), rx & 31);
+	for (uint32_t j = 0, n = __popc(st); j < n; j++, gi++) {
+		const uint32_t root = lds_find(lab, pre + j);
+		const int rw = tile_word_of(&s_pre[0][0], root);
+		const uint32_t gr = (&s_rb[0][0])[rw] + (root - (&s_pre[0][0])[rw]);
 		L[gi] = gr;
 		if (gr == gi)
 			AUX[gi] = 0;
