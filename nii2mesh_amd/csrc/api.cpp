@@ -855,7 +855,7 @@ Relabel morton_relabel(n2m_ctx *C, const double *pts, const int *tris, uint32_t 
 	uint32_t *k1 = C->buf<uint32_t>(S_K1, nv), *v1 = C->buf<uint32_t>(S_V1, nv);
 	launch_morton(pts, nv, C->d_small + SM_PBOX, k0, v0, s);
 	uint32_t *rc = C->buf<uint32_t>(S_RCOUNT, radix_counts_words(nv));
-	bool in1 = radix_sort_u32(k0, v0, k1, v1, nv, 32, rc, scan_scratch(C, radix_counts_words(nv)), s);
+	bool in1 = radix_sort_u32(k0, v0, k1, v1, nv, 24, rc, scan_scratch(C, radix_counts_words(nv)), s);
 	r.perm = in1 ? v1 : v0;
 	uint32_t *inv = C->buf<uint32_t>(S_HC_INV, nv);
 	launch_invert_perm(r.perm, nv, inv, s);

@@ -1574,7 +1574,7 @@ __global__ void k_morton(const double *__restrict__ pts, uint32_t n, const uint3
 		uint32_t q = f > 0.f ? (f < 1023.f ? (uint32_t)f : 1023u) : 0u; // NaN -> 0
 		key |= spread10(q) << k;
 	}
-	keys[i] = key;
+	keys[i] = key >> 6; // 8 bits per axis: 24-bit keys, three radix passes
 	vals[i] = i;
 }
 
