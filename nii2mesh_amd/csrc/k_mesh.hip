@@ -1142,9 +1142,12 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
 	// ring 2: ring 1's references to vertices outside block and ring 1
 	// (ring-1 vertex i = tid, n1 <= RT)
 	uint32_t hb = 0, he = 0;
+	d3 ov1 = {0.0, 0.0, 0.0}; // its original position, loaded with its offsets
 	if (tid < n1) {
-		hb = start[srt[tid]];
-		he = start[srt[tid] + 1];
+		const uint32_t h = srt[tid];
+		hb = start[h];
+		he = start[h + 1];
+		ov1 = o[h];
 	}
 	auto add2 = [&](const int2 q) {
 #pragma unroll
@@ -1182,8 +1185,8 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
 	uint32_t *rg = ring + (size_t)blk * RCAP;
 	for (uint32_t i = tid; i < n1 + n2; i += RT)
 		rg[i] = srt[i];
-	for (uint32_t i = tid; i < n1; i += RT)
-		o1[(size_t)blk * R1CAP + i] = o[srt[i]];
+	if (tid < n1)
+		o1[(size_t)blk * R1CAP + tid] = ov1;
 	if (tid == 0)
 		rcnt[blk] = make_uint2(n1, n2);
 	uint16_t *ro = roff + (size_t)blk * ROFFW;
