@@ -9,7 +9,8 @@ x-fastest layout the reference uses (src/meshify.c:186).
   at 200 outside it; Gaussian noise, clipped to 0..255.
 * `plain_u8()`        128^3 uint8, noise 8, rounded          (plain CPU path)
 * `lewiner_i16()`     256^3 int16, 0..1000, noise 16          (drop-in Lewiner)
-* `atlas_u16()`       256^3 uint16 Voronoi labels (370)        (batched atlas)
+* `atlas_u16()`       256^3 uint16 weighted-Voronoi labels (370,
+                      log-normal sizes: median ~2k, max ~3-4e5)  (batched atlas)
 * `sphere_f32()`      192^3 float32 perturbed-sphere distance  (differentiable)
 * `large_f32()`       512^3 float32, noise 8                   (large pipeline)
 """
@@ -105,46 +106,70 @@ def lewiner_i16():
     return np.rint(v).astype(np.int16).astype(np.float32)
 
 
-def atlas_u16(n=256, nlabels=370):
-    """256^3 uint16: ellipsoid split into `nlabels` Voronoi cells whose seed
-    density varies smoothly in space, giving roughly log-normal cell sizes."""
-    from scipy.spatial import cKDTree
+def atlas_u16(n=256, nlabels=370, passes=4):
+    """256^3 uint16 labels: the head ellipsoid (semi-axes 50/40/45 * n/128)
+    split into `nlabels` cells of a multiplicatively weighted Voronoi
+    diagram, a voxel going to the seed with the smallest |x - s_i| / r_i.
+
+    Target cell sizes are drawn log-normal (median 2000 voxels, sigma 1.68
+    in log space, rescaled to fill the ellipsoid: about 3.0 M voxels, the
+    largest cells about 3-4e5 voxels); r_i starts at the radius of a ball of
+    the target size and is corrected `passes - 1` times by the cube root of
+    target / measured size.  Seeds sit on integer coordinates, so every
+    squared distance is an integer below 2^24 and exact in float32 whatever
+    the summation order; the only rounding is the one product with 1/r_i^2,
+    so the labels are the same on every machine."""
+    import torch
     rng = _rng(2)
     s = n / 128.0
     c = 64.0 * s
-    ax = np.array([50.0, 40.0, 45.0]) * s
-    # seed density exp(3 * smooth field): sample seeds by rejection
-    k = rng.normal(size=(4, 3)) / 40.0
-    ph = rng.uniform(0, 2 * np.pi, 4)
-
-    def dens(p):
-        f = sum(np.cos(p @ k[i] * (i + 1) + ph[i]) for i in range(4)) / 4.0
-        return np.exp(3.5 * f)
-
-    seeds = []
-    dmax = np.exp(3.5)
-    while len(seeds) < nlabels:
-        p = rng.uniform(-1, 1, 3)
-        if (p ** 2).sum() > 1.0:
-            continue
-        q = c + p * ax
-        if rng.uniform() * dmax <= dens(q[None, :])[0]:
-            seeds.append(q)
-    seeds = np.array(seeds)
-    tree = cKDTree(seeds)
-    vol = np.zeros((n, n, n), np.float32)
-    zz = np.arange(n, dtype=np.float32)
-    yy, xx = np.meshgrid(zz, zz, indexing='ij')
-    for z in range(n):
-        r = ((xx - c) / ax[0]) ** 2 + ((yy - c) / ax[1]) ** 2 + ((z - c) / ax[2]) ** 2
-        m = r <= 1.0
-        if not m.any():
-            continue
-        pts = np.stack([xx[m], yy[m], np.full(m.sum(), z, np.float32)], 1)
-        _, idx = tree.query(pts)
-        sl = vol[z]
-        sl[m] = idx + 1
-    return vol
+    ax = np.array([50.0, 40.0, 45.0]) * s  # x, y, z
+    g = np.arange(n, dtype=np.float64)
+    inside = (((g[None, None, :] - c) / ax[0]) ** 2 + ((g[None, :, None] - c) / ax[1]) ** 2
+              + ((g[:, None, None] - c) / ax[2]) ** 2) <= 1.0
+    idx = np.flatnonzero(inside)
+    P = np.stack([idx % n, (idx // n) % n, idx // (n * n)], 1).astype(np.float32)
+    nin = len(P)
+    tgt = np.exp(rng.normal(np.log(2000.0), 1.68, nlabels))
+    tgt *= nin / tgt.sum()
+    rad = np.cbrt(3.0 * tgt / (4.0 * np.pi))
+    # seeds, largest cell first, kept apart by 0.9 x the sum of the radii
+    seeds = np.zeros((nlabels, 3))
+    placed = []
+    for i in np.argsort(-tgt, kind='stable'):
+        q = None
+        for attempt in range(200):
+            cand = rng.uniform(-1, 1, (64, 3))
+            cand = np.rint(c + cand[(cand ** 2).sum(1) <= 1.0] * ax)
+            if not placed:
+                q = cand[0]
+                break
+            d = np.sqrt(((cand[:, None, :] - seeds[placed][None]) ** 2).sum(-1))
+            ok = np.flatnonzero(np.all(d >= 0.9 * (rad[i] + rad[placed])[None], axis=1))
+            if len(ok) or attempt == 199:
+                q = cand[ok[0] if len(ok) else 0]
+                break
+        seeds[i] = q
+        placed.append(i)
+    Pt = torch.from_numpy(P)
+    St = torch.from_numpy(seeds.astype(np.float32))
+    P2 = (Pt * Pt).sum(1)
+    S2 = (St * St).sum(1)
+    lab = torch.empty(nin, dtype=torch.int64)
+    w = rad.copy()
+    B = 1 << 17
+    for k in range(passes):
+        W2 = torch.from_numpy((1.0 / (w * w)).astype(np.float32))
+        for b in range(0, nin, B):
+            d = torch.addmm(S2[None], Pt[b:b + B], St.T, alpha=-2.0)  # exact integers
+            d.add_(P2[b:b + B, None]).mul_(W2[None])
+            lab[b:b + B] = d.argmin(1)
+        if k + 1 < passes:
+            sz = np.bincount(lab.numpy(), minlength=nlabels)
+            w = w * np.cbrt(np.clip(tgt / np.maximum(sz, 1), 0.25, 4.0))
+    vol = np.zeros(n * n * n, np.float32)
+    vol[idx] = (lab.numpy() + 1).astype(np.float32)
+    return vol.reshape(n, n, n)
 
 
 def _real_sph_harm(l, m, theta, phi):

@@ -171,6 +171,8 @@ class Ref(abi.MeshLib):
                                 np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'), C.c_int, C.c_int, C.c_bool]
         h.base64_encode.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
         h.base64_encode.restype = C.c_void_p
+        h.ref_nii_otsu.argtypes = [np.ctypeslib.ndpointer(np.int32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
 
     def quick_smooth(self, img):
         nz, ny, nx = img.shape
@@ -199,6 +201,34 @@ class Ref(abi.MeshLib):
     def apply_sform(self, tris, pts, sx, sy, sz):
         f = C.c_float * 4
         self.lib.apply_sform(tris, pts, len(tris), len(pts), f(*sx), f(*sy), f(*sz))
+
+    def otsu(self, hist, mode):
+        """nii_otsu (src/isolevel.c:141): (threshold, dark, mid, bright)."""
+        d, m, b = C.c_int(), C.c_int(), C.c_int()
+        h = np.ascontiguousarray(hist, np.int32).copy()
+        t = self.lib.ref_nii_otsu(h, len(h), mode, C.byref(d), C.byref(m), C.byref(b))
+        return t, d.value, m.value, b.value
+
+
+def otsu_histograms():
+    """256-bin histograms for nii_otsu: random, flat, empty, one spike, four
+    equal spikes (many equal objective values), mirror-symmetric, bimodal,
+    and sparse Poisson ones."""
+    rng = np.random.default_rng(77)
+    out = [rng.integers(0, 5000, 256), np.ones(256, int), np.zeros(256, int)]
+    h = np.zeros(256, int)
+    h[40] = 1000
+    out.append(h)
+    h = np.zeros(256, int)
+    h[[10, 80, 150, 220]] = 500
+    out.append(h)
+    sym = rng.integers(0, 100, 128)
+    out.append(np.concatenate([sym, sym[::-1]]))
+    x = np.arange(256)
+    out.append((1000 * (np.exp(-(x - 60) ** 2 / 200) + np.exp(-(x - 190) ** 2 / 300))).astype(int))
+    for k in range(6):
+        out.append(rng.poisson(rng.uniform(1, 300), 256) * (rng.uniform(size=256) < 0.7))
+    return [np.ascontiguousarray(h, np.int32) for h in out]
 
 
 def oracle():
@@ -297,3 +327,89 @@ def tiny_volume(shape):
     if min(shape) > 1:
         return blobs(shape, 3, noise=1.0)
     return np.random.default_rng(1).normal(size=shape).astype(np.float32)
+
+
+# ------------------------------------------------------------- real data
+DATA = os.path.join(ROOT, 'tests', 'golden', 'data')
+DT_BYTES = {2: (np.uint8, 1), 4: (np.int16, 2), 16: (np.float32, 4), 512: (np.uint16, 2)}
+
+
+def read_nifti(path):
+    """NIfTI-1 (.nii or .nii.gz, native little endian) as (header dict, raw
+    voxels (nz, ny, nx) in the file's datatype) -- the fields the reference
+    loader uses (src/nii2mesh.c:78-172): dim, datatype, vox_offset,
+    scl_slope (0 reads as 1), scl_inter, srow_x/y/z."""
+    import gzip
+    import struct
+    with (gzip.open(path, 'rb') if path.endswith('.gz') else open(path, 'rb')) as f:
+        b = f.read()
+    dim = struct.unpack('<8h', b[40:56])
+    dt = struct.unpack('<h', b[70:72])[0]
+    off = int(round(struct.unpack('<f', b[108:112])[0]))
+    slope, inter = struct.unpack('<2f', b[112:120])
+    srow = np.frombuffer(b[280:328], '<f4').astype(np.float32).reshape(3, 4)
+    nx, ny, nz = dim[1], dim[2], dim[3]
+    typ, _ = DT_BYTES[dt]
+    raw = np.frombuffer(b, typ, count=nx * ny * nz, offset=off).reshape(nz, ny, nx)
+    return {'dim': (nx, ny, nz), 'datatype': dt, 'slope': np.float32(slope if slope != 0 else 1.0),
+            'inter': np.float32(inter), 'srow': srow}, raw
+
+
+def loader_float(hdr, raw):
+    """The reference loader's conversion: raw * scl_slope + scl_inter in
+    float32, one rounding per operation (src/nii2mesh.c:155-172)."""
+    return np.ascontiguousarray(raw.astype(np.float32) * hdr['slope'] + hdr['inter'], dtype=np.float32)
+
+
+def mesh_digest(rc, tris, pts):
+    """rc, counts and SHA-256 of the vec3i / vec3d bytes of a mesh."""
+    import hashlib
+    t = np.ascontiguousarray(tris, '<i4')
+    p = np.ascontiguousarray(pts, '<f8')
+    return {'rc': int(rc), 'nt': int(len(t)), 'np': int(len(p)),
+            'tris': hashlib.sha256(t.tobytes()).hexdigest(), 'pts': hashlib.sha256(p.tobytes()).hexdigest()}
+
+
+def volume_digest(vol):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(vol, '<f4').tobytes()).hexdigest()
+
+
+# the README's flag sets for bet.nii.gz (README.md:76-124): (name, isolevel
+# (1/2/3 = dark/medium/bright, or a number), preSmooth, onlyLargest,
+# fillBubbles, postSmooth); the CLI defaults are medium, -p 1, -l 1, -b 0, -s 0
+BET_CASES = [
+    ('p0', 2, False, True, False, 0),
+    ('p1', 2, True, True, False, 0),
+    ('l0', 2, True, False, False, 0),
+    ('l1', 2, True, True, False, 0),
+    ('i122_l0_b0', 122.0, True, False, False, 0),
+    ('i122_l0_b1', 122.0, True, False, True, 0),
+    ('id', 1, True, True, False, 0),
+    ('im', 2, True, True, False, 0),
+    ('ib', 3, True, True, False, 0),
+    ('i122', 122.0, True, True, False, 0),
+    ('i120_p0_s0', 120.0, False, True, False, 0),
+    ('i120_p0_s100', 120.0, False, True, False, 100),
+]
+
+
+def write_nifti(path, vol, datatype=16, srow=None):
+    """Minimal NIfTI-1 (.nii, little endian, vox_offset 352) of a (nz, ny, nx)
+    volume, with an sform (identity unless srow, a 3 x 4 float array)."""
+    import struct
+    nz, ny, nx = vol.shape
+    typ, bpp = DT_BYTES[datatype]
+    h = bytearray(348)
+    struct.pack_into('<i', h, 0, 348)
+    struct.pack_into('<8h', h, 40, 3, nx, ny, nz, 1, 1, 1, 1)
+    struct.pack_into('<2h', h, 70, datatype, 8 * bpp)
+    struct.pack_into('<8f', h, 76, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0)
+    struct.pack_into('<f', h, 108, 352.0)
+    struct.pack_into('<2f', h, 112, 1.0, 0.0)
+    struct.pack_into('<h', h, 254, 1)  # sform_code
+    s = np.eye(3, 4, dtype=np.float32) if srow is None else np.asarray(srow, np.float32)
+    h[280:328] = s.astype('<f4').tobytes()
+    h[344:348] = b'n+1\0'
+    with open(path, 'wb') as f:
+        f.write(bytes(h) + b'\0\0\0\0' + np.ascontiguousarray(vol, typ).tobytes())
