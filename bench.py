@@ -51,40 +51,149 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(vol, isos, z0=224, nz=64):
-    """Reference CPU path (oracle/_ref, Lewiner build) on a z-slab of the same
-    volume: setThreshold for the three modes, then per isolevel meshify()
-    (pre-smooth, largest cluster, bubble fill) + 100 HC iterations."""
-    from tests import support  # checker bindings; the checker is only timed here
-    ref_so = support.REF_SO['lewiner']
-    kind = 'reference'
+REF_DIR = os.path.join(ROOT, 'oracle', '_ref')
+STAGES = ('pre-smooth', 'voxel clustering', 'marching cubes', 'vertex welding', 'remove degenerate triangles',
+          'post-smooth')
+
+
+def host_cpu():
+    """CPU model and the logical CPUs this process may use."""
+    model = 'unknown'
     try:
-        if not os.path.exists(ref_so):
-            raise FileNotFoundError(ref_so)
-        lib = support.Ref(C.CDLL(ref_so))
-    except Exception:
-        lib = support.Oracle(C.CDLL(support.ORACLE_SO))
-        kind = 'port'
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {'model': model, 'logical_cpus': len(os.sched_getaffinity(0))}
+
+
+def ref_exe(name):
+    path = os.path.join(REF_DIR, name)
+    if not os.path.exists(path):
+        raise SystemExit('bench.py: the reference build %s is missing (make -C oracle ref); '
+                         'run with --no-cpu to skip the CPU baseline' % path)
+    return path
+
+
+def run_verbose(args, env=None):
+    """Run a reference CLI with -v 1; returns {stage: total ms} from its
+    stage lines ('<stage>...: N ms', src/meshify.c:303-380, src/nii2mesh.c:330-355)."""
+    import re
+    import subprocess
+    r = subprocess.run(args, capture_output=True, text=True, env=env, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError('%s failed: %s' % (' '.join(args), r.stdout[-500:]))
+    ms = {}
+    for line in r.stdout.splitlines():
+        m = re.match(r'^([a-z][a-z -]*?)(?: \(.*\)| [0-9].*)?: (\d+) ms$', line.strip())
+        if m:
+            ms[m.group(1)] = ms.get(m.group(1), 0) + int(m.group(2))
+    return ms, r.stdout
+
+
+def cpu_baseline(vol, isos, z0=224, nz=64):
+    """The reference CLI built from its own sources (oracle/_ref, Lewiner
+    build) on a z-slab of the same volume, on this host, one thread (the
+    path is single-threaded): setThreshold for the three modes (timed
+    through the reference's own setThreshold), then per isolevel
+    `nii2mesh slab.nii -i <iso> -l 1 -b 1 -s 100 -r 0.25 -v 1`; the hot-path
+    time is the sum of its pre-smooth, clustering, marching-cubes, weld,
+    degenerate and post-smooth stage lines (load and save excluded), the
+    quadric reduction at 0.25 is reported beside it."""
+    import tempfile
+    from tests import support  # checker bindings; the checker is only timed here
+    exe = ref_exe('nii2mesh_lewiner')
+    lib = support.Ref(C.CDLL(os.path.join(REF_DIR, 'libn2mref_lewiner.so')))
     slab = np.ascontiguousarray(vol[z0:z0 + nz])
     t0 = time.perf_counter()
     for m in MODES:
         lib.setThreshold(slab, m)
-    ntri = 0
-    for iso in isos:
-        img = slab.copy()
-        if kind == 'reference':
-            rc, tris, pts = lib.meshify(img, iso, True, True, True)
-        else:
-            rc, tris, pts = lib.meshify(img, iso, True, True, True, cubes=1)
-        if rc == 0:
-            lib.laplacian_smoothHC(pts, tris, 0.1, 0.5, HC_ITERS, True)
-            ntri += len(tris)
-    dt = time.perf_counter() - t0
-    vox = slab.size
-    return {'value': vox / dt / 1e9, 'unit': 'GVoxel/s', 'cores': 1, 'kind': kind,
-            'sample': '512x512x%d z-slab [%d,%d) of the same volume: setThreshold x3 + 3 isolevels x '
-                      '(meshify Lewiner, largest+bubbles, + %d HC iters), %.1f s, %d triangles'
-                      % (nz, z0, z0 + nz, HC_ITERS, dt, ntri)}
+    thr_ms = (time.perf_counter() - t0) * 1e3
+    tmp = tempfile.mkdtemp(dir='/dev/shm' if os.path.isdir('/dev/shm') else None)
+    stages = {k: 0 for k in STAGES}
+    quadric_ms = 0
+    try:
+        nii = os.path.join(tmp, 'slab.nii')
+        support.write_nifti(nii, slab, 16)
+        for iso in isos:
+            ms, _ = run_verbose([exe, nii, '-i', '%.9g' % iso, '-l', '1', '-b', '1', '-s', str(HC_ITERS),
+                                 '-r', '0.25', '-v', '1', os.path.join(tmp, 'out.mz3')])
+            for k in STAGES:
+                stages[k] += ms.get(k, 0)
+            quadric_ms += ms.get('simplify vertices', 0)
+    finally:
+        import shutil
+        shutil.rmtree(tmp, ignore_errors=True)
+    hot_ms = thr_ms + sum(stages.values())
+    return {'value': slab.size / (hot_ms * 1e-3) / 1e9, 'unit': 'GVoxel/s', 'cores': 1, 'kind': 'reference',
+            'sample': '512x512x%d z-slab [%d,%d) of the same volume: setThreshold x3 + 3 isolevels x nii2mesh '
+                      '(Lewiner build) -l 1 -b 1 -s %d, stage lines of -v 1, load/save excluded: %.1f s'
+                      % (nz, z0, z0 + nz, HC_ITERS, hot_ms / 1e3),
+            'stages_ms': dict(stages, setThreshold=round(thr_ms, 1)),
+            'quadric_r0.25_ms': quadric_ms}
+
+
+def cpu_atlas(vol, z0=112, nz=32):
+    """The reference CLI's atlas loop (-a 1, src/nii2mesh.c:492-573; classic
+    build with OpenMP) on a z-slab of the 256^3 atlas, with one thread and
+    with every CPU this process may use; -r 1 -q 0, load and save excluded."""
+    import tempfile
+    import shutil
+    from tests import support
+    exe = ref_exe('nii2mesh_classic_omp')
+    slab = np.ascontiguousarray(vol[z0:z0 + nz])
+    tmp = tempfile.mkdtemp(dir='/dev/shm' if os.path.isdir('/dev/shm') else None)
+    out = {'sample': '256x256x%d z-slab [%d,%d) of the 256^3 atlas, labels 1..%d, nii2mesh -a 1 -r 1 -q 0'
+                     % (nz, z0, z0 + nz, int(slab.max()))}
+    try:
+        nii = os.path.join(tmp, 'atlas.nii')
+        support.write_nifti(nii, slab, 16)
+        for n in (1, len(os.sched_getaffinity(0))):
+            env = dict(os.environ, OMP_NUM_THREADS=str(n))
+            t0 = time.perf_counter()
+            ms, text = run_verbose([exe, nii, '-a', '1', '-r', '1', '-q', '0', '-v', '1',
+                                    os.path.join(tmp, 'lab.mz3')], env)
+            wall = (time.perf_counter() - t0) * 1e3
+            io = ms.get('load from disk', 0) + ms.get('save to disk', 0)
+            key = 'threads_%d_ms' % n
+            # with several threads the per-label save lines overlap: subtract
+            # them only in the one-thread run, where they are sequential
+            out[key] = round(wall - io if n == 1 else wall - ms.get('load from disk', 0), 1)
+            out['labels_meshed'] = text.count('save to disk')
+        out['threads'] = len(os.sched_getaffinity(0))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
+def pcie_inclusive(torch, vol, step, ctx):
+    """The same step fed from and returning to host memory: the volume
+    copied in from pageable memory, the three meshes copied out."""
+    host = torch.from_numpy(vol)
+    d = torch.empty(vol.shape, dtype=torch.float32, device='cuda')
+    best = None
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.copy_(host)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        step(d)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for k in range(3):
+            ctx.mesh(k)
+        t3 = time.perf_counter()
+        r = ((t1 - t0) * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3)
+        if best is None or sum(r) < sum(best):
+            best = r
+    del d
+    total = sum(best)
+    return {'value': round(vol.size / (total * 1e-3) / 1e9, 4), 'unit': 'GVoxel/s', 'ms_per_step': round(total, 3),
+            'h2d_ms': round(best[0], 3), 'step_ms': round(best[1], 3), 'd2h_ms': round(best[2], 3),
+            'note': 'volume from pageable host memory, meshes to host arrays'}
 
 
 def main():
@@ -109,7 +218,7 @@ def main():
     from nii2mesh_amd import synth
 
     if args.atlas_only:
-        print(json.dumps({'atlas': atlas_leg(torch, n2m, synth)}), flush=True)
+        print(json.dumps({'atlas': atlas_leg(torch, n2m, synth)[0]}), flush=True)
         return
     t = time.perf_counter()
     vol = synth.large_f32()
@@ -132,11 +241,12 @@ def main():
     modes = (C.c_int * 3)(*MODES)
     isos_c = (C.c_float * 3)()
 
-    def step():
-        if L.n2m_ctx_set_thresholds(ctx.h, C.c_void_p(d.data_ptr()), nvox, 3, modes, isos_c) != 0:
+    def step(dv=None):
+        dv = d if dv is None else dv
+        if L.n2m_ctx_set_thresholds(ctx.h, C.c_void_p(dv.data_ptr()), nvox, 3, modes, isos_c) != 0:
             raise RuntimeError(n2m.last_error())
         isos = [float(v) for v in isos_c]
-        if ctx.pipeline(d.data_ptr(), dims, prm, isos, 0.1, 0.5, HC_ITERS) != 0:
+        if ctx.pipeline(dv.data_ptr(), dims, prm, isos, 0.1, 0.5, HC_ITERS) != 0:
             raise RuntimeError('pipeline failed: ' + n2m.last_error())
         return isos
 
@@ -202,11 +312,21 @@ def main():
         'kernel_ms_per_step': round(tot_ms, 3) if stats else None,
         'roofline': roof,
     }
+    out['pcie_inclusive'] = pcie_inclusive(torch, vol, step, ctx)
+    log('pcie-inclusive step %.2f ms (h2d %.2f, step %.2f, d2h %.2f)' % (
+        out['pcie_inclusive']['ms_per_step'], out['pcie_inclusive']['h2d_ms'], out['pcie_inclusive']['step_ms'],
+        out['pcie_inclusive']['d2h_ms']))
+    out['host_cpu'] = host_cpu()
+    atlas_vol = None
     if not args.no_atlas:
-        out['atlas'] = atlas_leg(torch, n2m, synth)
+        out['atlas'], atlas_vol = atlas_leg(torch, n2m, synth)
     if not args.no_cpu:
         log('cpu baseline ...')
         out['cpu_baseline'] = cpu_baseline(vol, isos)
+        log('cpu baseline %.6f GVoxel/s, stages %s' % (out['cpu_baseline']['value'], out['cpu_baseline']['stages_ms']))
+        if atlas_vol is not None:
+            out['atlas']['cpu_loop'] = cpu_atlas(atlas_vol)
+            log('cpu atlas loop %s' % out['atlas']['cpu_loop'])
     print(json.dumps(out), flush=True)
 
 
@@ -243,10 +363,12 @@ def atlas_leg(torch, n2m, synth, reps=5):
     for k, (ms, n, b) in sorted(ctx.kernel_stats().items(), key=lambda kv: -kv[1][0]):
         log('  atlas %-28s %8.3f ms %5d launches' % (k, ms, n))
     ctx.close()
-    return {'workload': '256^3 Voronoi atlas, 370 labels, binarise 0.5 + pre-smooth + classic marching cubes',
+    sz = np.bincount(vol.astype(np.int64).ravel())[1:]
+    return {'workload': '256^3 weighted-Voronoi atlas, 370 labels (log-normal sizes: median %d, max %d voxels), '
+                        'binarise 0.5 + pre-smooth + classic marching cubes' % (np.median(sz), sz.max()),
             'labels_meshed': nok, 'triangles': ntri, 'batched_ms': round(batched, 3),
             'per_label_loop_ms': round(loop, 3), 'speedup': round(loop / batched, 2),
-            'batched_gvoxel_s': round(vol.size / (batched * 1e-3) / 1e9, 4)}
+            'batched_gvoxel_s': round(vol.size / (batched * 1e-3) / 1e9, 4)}, vol
 
 
 def traffic_from_profile(kernel):

@@ -134,6 +134,14 @@ void n2m_apply_sform(vec3i *t, vec3d *p, int nt, int np, float srow_x[4], float 
 /* ---------------------------------------------------------------------------
  * Device-resident context: one HIP stream, reusable device buffers.
  * All d_* pointers are device memory on the context's device.
+ *
+ * Streams: every n2m_ctx_* call runs on the context's own stream, created
+ * with hipStreamNonBlocking, and returns after that stream is idle.  It does
+ * not wait for other streams: work the caller queued on its own stream (or
+ * the legacy default stream) that writes an input must be complete before
+ * the call (hipStreamSynchronize / torch.cuda.synchronize()).  Destroy a
+ * context from a thread that may make HIP calls (not from a finaliser that
+ * can run during process teardown).
  * ------------------------------------------------------------------------- */
 typedef struct n2m_ctx n2m_ctx;
 

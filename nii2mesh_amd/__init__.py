@@ -200,25 +200,59 @@ CUBES_CLASSIC = 0
 CUBES_LEWINER = 1
 
 
+_orphans = []  # contexts dropped by the garbage collector on another thread
+
+
+def _close_orphans():
+    while _orphans:
+        load().n2m_ctx_destroy(_orphans.pop())
+
+
 class DeviceContext:
-    """Device-resident pipeline (one HIP stream, reusable HBM buffers)."""
+    """Device-resident pipeline (one HIP stream, reusable HBM buffers).
+
+    The library runs on the context's own stream (created non-blocking), so
+    work another stream queued on an input array must be complete before a
+    call (torch: torch.cuda.synchronize()).  Close a context explicitly
+    (close() or a with-block).  A context the garbage collector drops is
+    destroyed only on the thread that created it; dropped elsewhere (for
+    example on torch's autograd thread), its handle is kept and destroyed by
+    the next create() or close() on any thread."""
 
     def __init__(self):
+        import threading
         self.L = load()
+        _close_orphans()
         self.h = self.L.n2m_ctx_create()
+        self._owner = threading.get_ident()
         if not self.h:
             raise RuntimeError('n2m_ctx_create failed: ' + last_error())
 
     def close(self):
+        _close_orphans()
         if self.h:
             self.L.n2m_ctx_destroy(self.h)
             self.h = None
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        import threading
+        h = getattr(self, 'h', None)
+        if not h:
+            return
+        self.h = None
+        if threading.get_ident() == getattr(self, '_owner', None):
+            try:
+                self.L.n2m_ctx_destroy(h)
+            except Exception:
+                pass
+        else:
+            _orphans.append(h)
 
     @staticmethod
     def params(isolevel, cubes=CUBES_CLASSIC, originalMC=0, preSmooth=True, onlyLargest=True,

@@ -54,6 +54,44 @@ struct Error : std::runtime_error {
 			fail("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__);             \
 	} while (0)
 
+} // namespace
+
+// the 2 % / 98 % bins of the robust range (src/isolevel.c:102-134): lo is
+// the bin where the count from below reaches n2pct, hi likewise from above;
+// when they meet, both move outwards to the next non-empty bins
+void n2m::robust_walk(const uint32_t *h, int kBins, size_t n2pct, int *plo, int *phi)
+{
+	size_t n = 0, lo = 0;
+	while (n < n2pct)
+		n += h[lo++];
+	lo--;
+	n = 0;
+	int hi = kBins;
+	while (n < n2pct)
+		n += h[--hi];
+	if ((int)lo == hi) { // src/isolevel.c:118-134
+		int ok = -1;
+		while (ok != 0) {
+			if (lo > 0) {
+				lo--;
+				if (h[lo] > 0)
+					ok = 0;
+			}
+			if (ok != 0 && hi < kBins - 1) {
+				hi++;
+				if (h[hi] > 0)
+					ok = 0;
+			}
+			if (lo == 0 && hi == kBins - 1)
+				ok = 0;
+		}
+	}
+	*plo = (int)lo;
+	*phi = hi;
+}
+
+namespace {
+
 double now_ms()
 {
 	struct timespec t;
@@ -159,6 +197,17 @@ enum Slot {
 	S_ENC_A,
 	S_ENC_B,
 	S_ENC_C,
+	S_LG_OFF,
+	S_LG_FLAG,
+	S_LG_K0,
+	S_LG_V0,
+	S_LG_K1,
+	S_LG_V1,
+	S_LG_EK0,
+	S_LG_EV0,
+	S_LG_EK1,
+	S_LG_EV1,
+	S_LG_RC,
 	S_COUNT
 };
 
@@ -425,6 +474,22 @@ Mesh &mesh_slot(n2m_ctx *c, int i)
 	if ((int)c->meshes.size() <= i)
 		c->meshes.resize(i + 1);
 	return c->meshes[i];
+}
+
+// keep n mesh slots; the device arrays of the slots dropped are freed
+void set_mesh_count(n2m_ctx *c, int n)
+{
+	c->sync(); // no queued kernel still reads a dropped mesh
+	for (size_t i = (size_t)n; i < c->meshes.size(); i++) {
+		Mesh &m = c->meshes[i];
+		if (m.tris.p)
+			HIPX(hipFree(m.tris.p));
+		if (m.pts.p)
+			HIPX(hipFree(m.pts.p));
+		m.tris = DevBuf();
+		m.pts = DevBuf();
+	}
+	c->meshes.resize(n);
 }
 
 // scratch of the device scans: sized for 2^31 elements up front, so a later
@@ -805,7 +870,29 @@ struct Csr {
 	uint32_t *start;
 	int2 *pairs;
 	uint32_t *ref;
+	// references of the high-valence vertices (more than 16 each), sorted by
+	// vertex and, within a vertex, by corner: nlong entries
+	uint32_t nlong = 0;
+	const uint32_t *lv = nullptr, *lc = nullptr;
 };
+
+int bits_for(uint32_t n) // radix bits that cover the values 0..n-1
+{
+	int b = 8;
+	while (b < 32 && ((uint64_t)1 << b) < n)
+		b += 8;
+	return b;
+}
+
+// stable sort of n (key, value) pairs held in k0 / v0 (k1 / v1 the other
+// buffers); returns where the result is
+std::pair<uint32_t *, uint32_t *> sort_pairs(n2m_ctx *C, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1,
+                                             size_t n, int bits)
+{
+	uint32_t *rc = C->buf<uint32_t>(S_LG_RC, radix_counts_words(n));
+	const bool in1 = radix_sort_u32(k0, v0, k1, v1, n, bits, rc, scan_scratch(C, radix_counts_words(n)), C->stream);
+	return in1 ? std::make_pair(k1, v1) : std::make_pair(k0, v0);
+}
 
 Csr build_csr(n2m_ctx *C, const int *tris, uint32_t nv, uint32_t nt)
 {
@@ -823,7 +910,46 @@ Csr build_csr(n2m_ctx *C, const int *tris, uint32_t nv, uint32_t nt)
 	launch_csr_fill(tris, nt, c.start, rank, c.ref, s);
 	launch_csr_pairs(tris, nv, c.start, c.ref, c.pairs, s);
 	C->tend();
+	// lists longer than 16: ordered all at once (k_csr_pairs left them)
+	uint32_t *loff = C->buf<uint32_t>(S_LG_OFF, (size_t)nv + 1);
+	launch_long_count(c.start, nv, loff, s);
+	const uint32_t L = scan(C, loff, loff, (size_t)nv + 1, true);
+	if (L) {
+		const uint32_t n3 = 3 * nt;
+		uint32_t *pos = C->buf<uint32_t>(S_LG_FLAG, n3);
+		launch_long_flag(tris, n3, c.start, pos, s);
+		scan_excl_u32(pos, pos, n3, nullptr, scan_scratch(C, n3), s);
+		uint32_t *k0 = C->buf<uint32_t>(S_LG_K0, L), *v0 = C->buf<uint32_t>(S_LG_V0, L);
+		uint32_t *k1 = C->buf<uint32_t>(S_LG_K1, L), *v1 = C->buf<uint32_t>(S_LG_V1, L);
+		launch_long_compact(tris, n3, c.start, pos, k0, v0, s);
+		auto kv = sort_pairs(C, k0, v0, k1, v1, L, bits_for(nv));
+		launch_long_place(tris, kv.first, kv.second, L, c.start, loff, c.ref, c.pairs, s);
+		C->check();
+		c.nlong = L;
+		c.lv = kv.first;
+		c.lc = kv.second;
+	}
 	return c;
+}
+
+// border flags of the high-valence vertices (k_border / k_border_pairs
+// skip them): entries (id, vertex) sorted by (vertex, id), then the ids
+// that occur once
+void border_long(n2m_ctx *C, const Csr &c, const int *tris, uint32_t nv, bool literal, uint8_t *border)
+{
+	if (!c.nlong)
+		return;
+	hipStream_t s = C->stream;
+	const size_t n = (size_t)c.nlong * (literal ? 3 : 2);
+	uint32_t *ek0 = C->buf<uint32_t>(S_LG_EK0, n), *ev0 = C->buf<uint32_t>(S_LG_EV0, n);
+	uint32_t *ek1 = C->buf<uint32_t>(S_LG_EK1, n), *ev1 = C->buf<uint32_t>(S_LG_EV1, n);
+	launch_long_border_entries(tris, c.lv, c.lc, c.nlong, literal ? 1 : 0, ek0, ev0, s);
+	const int bits = bits_for(nv);
+	auto a = sort_pairs(C, ek0, ev0, ek1, ev1, n, bits); // by id
+	uint32_t *ok = a.first == ek0 ? ek1 : ek0, *ov = a.second == ev0 ? ev1 : ev0;
+	auto b = sort_pairs(C, a.second, a.first, ov, ok, n, bits); // then by vertex (stable)
+	launch_long_border_mark(b.first, b.second, (uint32_t)n, literal ? 1 : 0, border, s);
+	C->check();
 }
 
 void check_tris(const int *tris, int ntri, int nvert)
@@ -882,6 +1008,7 @@ uint8_t *border_flags(n2m_ctx *C, const Csr &c, const int *tris, uint32_t nv, ui
 	} else {
 		launch_border_pairs(c.start, c.pairs, nv, border, s);
 	}
+	border_long(C, c, tris, nv, repeats, border);
 	C->tend();
 	return border;
 }
@@ -1738,7 +1865,7 @@ int diff_forward(n2m_ctx *C, const float *vol, const int dim[3], float iso, int 
 	if (np < 3)
 		return EXIT_FAILURE;
 	uint32_t nt = remove_degenerate(C, pts, np, tris, nT, false);
-	C->meshes.resize(1);
+	set_mesh_count(C, 1);
 	Mesh &m = C->meshes[0];
 	int *mt = (int *)n2m_ctx::grow(m.tris, (size_t)12 * (nt ? nt : 1));
 	double *mp = (double *)n2m_ctx::grow(m.pts, (size_t)24 * np);
@@ -2064,8 +2191,8 @@ void *n2m_ctx_stream(n2m_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr;
 int n2m_ctx_meshify(n2m_ctx *C, float *d_img, const int dim[3], const n2m_params *prm)
 {
 	return guarded([&] {
+		set_mesh_count(C, 1);
 		Mesh &m = mesh_slot(C, 0);
-		C->meshes.resize(1);
 		if (!meshify_device(C, d_img, d_img, dim, *prm, m))
 			return EXIT_FAILURE;
 		C->sync();
@@ -2080,7 +2207,7 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 		const int NX = dim[0], NY = dim[1], NZ = dim[2];
 		const size_t nvox = (size_t)NX * NY * NZ;
 		hipStream_t s = C->stream;
-		C->meshes.resize(niso);
+		set_mesh_count(C, niso);
 		float *sm = C->buf<float>(S_VOL2, nvox); // smoothed volume, shared by all isolevels
 		bool smoothed = prm->preSmooth && NX >= 5 && NY >= 5 && NZ >= 5;
 		uint32_t range[2];
@@ -2660,12 +2787,14 @@ void n2m_apply_sform(vec3i *t, vec3d *p, int nt, int np, float srow_x[4], float 
 		double py = srow_y[0] + srow_y[1] + srow_y[2];
 		double pz = srow_z[0] + srow_z[1] + srow_z[2];
 		float det = px * py * pz;
-		if (det < 0.0)
-			for (int i = 0; i < nt; i++) {
-				int a = t[i].x;
-				t[i].x = t[i].y;
-				t[i].y = a;
-			}
+		if (det < 0.0 && nt > 0) {
+			int *dt = C->buf<int>(S_TRIS1, 3 * (size_t)nt);
+			HIPX(hipMemcpyAsync(dt, t, (size_t)12 * nt, hipMemcpyHostToDevice, C->stream));
+			launch_swap_winding(dt, (uint32_t)nt, C->stream);
+			C->check();
+			HIPX(hipMemcpyAsync(t, dt, (size_t)12 * nt, hipMemcpyDeviceToHost, C->stream));
+			C->sync();
+		}
 		return 0;
 	});
 }
@@ -2921,31 +3050,8 @@ bool threshold_hist(n2m_ctx *C, const float *d_img, int64_t nvox, ThrHist &out)
 				std::vector<uint32_t> h(kBins);
 				HIPX(hipMemcpyAsync(h.data(), hist, 4 * kBins, hipMemcpyDeviceToHost, s));
 				C->sync();
-				size_t n = 0, lo = 0;
-				while (n < n2pct)
-					n += h[lo++];
-				lo--;
-				n = 0;
-				int hi = kBins;
-				while (n < n2pct)
-					n += h[--hi];
-				if ((int)lo == hi) { // src/isolevel.c:118-134
-					int ok = -1;
-					while (ok != 0) {
-						if (lo > 0) {
-							lo--;
-							if (h[lo] > 0)
-								ok = 0;
-						}
-						if (ok != 0 && hi < kBins - 1) {
-							hi++;
-							if (h[hi] > 0)
-								ok = 0;
-						}
-						if (lo == 0 && hi == kBins - 1)
-							ok = 0;
-					}
-				}
+				int lo, hi;
+				robust_walk(h.data(), kBins, n2pct, &lo, &hi);
 				pct2 = (lo) / scl + mn;
 				pct98 = (hi) / scl + mn;
 			}

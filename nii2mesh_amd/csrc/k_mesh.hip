@@ -472,6 +472,19 @@ __global__ void k_csr_fill(const int *__restrict__ tris, uint32_t nt, const uint
 // k * CP_BS + t); longer ones in place in global memory.  ref itself is left
 // unsorted: its only other reader (k_border) does not depend on the order.
 constexpr int CP_BS = 256, CP_MAX = 16;
+
+// the other two corners of corner r (lower corner first)
+__device__ __forceinline__ int2 corner_pair(const int *__restrict__ tris, uint32_t r)
+{
+	const uint32_t t = r / 3, k = r % 3;
+	const int c0 = tris[3 * t], c1 = tris[3 * t + 1], c2 = tris[3 * t + 2];
+	int2 p;
+	p.x = k == 0 ? c1 : c0;
+	p.y = k == 2 ? c1 : c2;
+	return p;
+}
+
+// lists longer than CP_MAX are left to the high-valence pass below
 __global__ void __launch_bounds__(CP_BS)
 k_csr_pairs(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restrict__ start,
             uint32_t *__restrict__ ref, int2 *__restrict__ pairs)
@@ -481,14 +494,7 @@ k_csr_pairs(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restric
 	if (v >= nv)
 		return;
 	const uint32_t b = start[v], e = start[v + 1], n = e - b;
-	auto expand = [&](uint32_t i, uint32_t r) {
-		const uint32_t t = r / 3, k = r % 3;
-		const int c0 = tris[3 * t], c1 = tris[3 * t + 1], c2 = tris[3 * t + 2];
-		int2 p;
-		p.x = k == 0 ? c1 : c0;
-		p.y = k == 2 ? c1 : c2;
-		pairs[i] = p;
-	};
+	auto expand = [&](uint32_t i, uint32_t r) { pairs[i] = corner_pair(tris, r); };
 	if (n <= (uint32_t)CP_MAX) {
 		uint32_t *L = s_ref + threadIdx.x;
 		for (uint32_t i = 0; i < n; i++) { // insertion sort (valence is small)
@@ -502,19 +508,7 @@ k_csr_pairs(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restric
 		}
 		for (uint32_t i = 0; i < n; i++)
 			expand(b + i, L[i * CP_BS]);
-		return;
 	}
-	for (uint32_t i = b + 1; i < e; i++) {
-		const uint32_t x = ref[i];
-		uint32_t j = i;
-		while (j > b && ref[j - 1] > x) {
-			ref[j] = ref[j - 1];
-			j--;
-		}
-		ref[j] = x;
-	}
-	for (uint32_t i = b; i < e; i++)
-		expand(i, ref[i]);
 }
 
 // Border vertices as update_mesh(iteration 0) marks them: for vertex v,
@@ -527,6 +521,8 @@ __global__ void k_border(const int *__restrict__ tris, uint32_t nv, const uint32
 	if (v >= nv)
 		return;
 	uint32_t b = start[v], e = start[v + 1];
+	if (e - b > (uint32_t)CP_MAX)
+		return; // the high-valence pass counts this list
 	for (uint32_t i = b; i < e; i++) {
 		uint32_t t = ref[i] / 3;
 		for (int k = 0; k < 3; k++) {
@@ -1350,6 +1346,138 @@ __global__ void k_hc_store(const d3 *__restrict__ p, const d3 *__restrict__ o, c
 		dst[perm ? perm[i] : i] = (border && border[i]) ? o[i] : p[i];
 }
 
+// ---------------------------------------------------------------------------
+// High-valence vertices: more than CP_MAX references (a fan apex, say).  One
+// lane per list would take O(d^2) steps; instead all their references are
+// handled at once.  The corners of such vertices are compacted in corner
+// order, stably sorted by vertex, and placed (lists in ascending corner
+// order, the order of src/quadric.c:321-332); their border flags come from
+// a sort of (vertex, id) entries, in which an entry that occurs once is an
+// id counted once for that vertex (src/quadric.c:188-215).
+// ---------------------------------------------------------------------------
+// lcnt[v] = the list length of a long vertex, else 0 (v <= nv: a 0 sentinel)
+__global__ void k_long_count(const uint32_t *__restrict__ start, uint32_t nv, uint32_t *__restrict__ lcnt)
+{
+	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v > nv)
+		return;
+	const uint32_t n = v < nv ? start[v + 1] - start[v] : 0u;
+	lcnt[v] = n > (uint32_t)CP_MAX ? n : 0u;
+}
+
+__device__ __forceinline__ bool long_corner(const int *__restrict__ tris, const uint32_t *__restrict__ start,
+                                            uint32_t i)
+{
+	const uint32_t v = (uint32_t)tris[i];
+	return start[v + 1] - start[v] > (uint32_t)CP_MAX;
+}
+
+__global__ void k_long_flag(const int *__restrict__ tris, uint32_t n3, const uint32_t *__restrict__ start,
+                            uint32_t *__restrict__ flag)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n3)
+		flag[i] = long_corner(tris, start, i) ? 1u : 0u;
+}
+
+// pos: exclusive scan of the flags; key = vertex, val = corner
+__global__ void k_long_compact(const int *__restrict__ tris, uint32_t n3, const uint32_t *__restrict__ start,
+                               const uint32_t *__restrict__ pos, uint32_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n3 && long_corner(tris, start, i)) {
+		key[pos[i]] = (uint32_t)tris[i];
+		val[pos[i]] = i;
+	}
+}
+
+// sorted by vertex, corners ascending within a vertex; loff = exclusive scan
+// of k_long_count, so vertex v's entries start at loff[v]
+__global__ void k_long_place(const int *__restrict__ tris, const uint32_t *__restrict__ key,
+                             const uint32_t *__restrict__ val, uint32_t L, const uint32_t *__restrict__ start,
+                             const uint32_t *__restrict__ loff, uint32_t *__restrict__ ref, int2 *__restrict__ pairs)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= L)
+		return;
+	const uint32_t v = key[j], c = val[j], at = start[v] + (j - loff[v]);
+	ref[at] = c;
+	pairs[at] = corner_pair(tris, c);
+}
+
+// border entries (key id, value vertex): the two pair ids of each reference,
+// or (literal) the three ids of its triangle
+__global__ void k_long_border_entries(const int *__restrict__ tris, const uint32_t *__restrict__ key,
+                                      const uint32_t *__restrict__ val, uint32_t L, int literal,
+                                      uint32_t *__restrict__ ek, uint32_t *__restrict__ ev)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= L)
+		return;
+	const uint32_t v = key[j], c = val[j];
+	if (literal) {
+		const uint32_t t = c / 3;
+		for (int m = 0; m < 3; m++) {
+			ek[3 * (size_t)j + m] = (uint32_t)tris[3 * t + m];
+			ev[3 * (size_t)j + m] = v;
+		}
+	} else {
+		const int2 q = corner_pair(tris, c);
+		ek[2 * (size_t)j] = (uint32_t)q.x;
+		ek[2 * (size_t)j + 1] = (uint32_t)q.y;
+		ev[2 * (size_t)j] = v;
+		ev[2 * (size_t)j + 1] = v;
+	}
+}
+
+// entries sorted by (vertex, id): one that occurs once marks the vertex
+// (pair form) or the id (literal form) as border
+__global__ void k_long_border_mark(const uint32_t *__restrict__ v, const uint32_t *__restrict__ id, uint32_t n,
+                                   int literal, uint8_t *__restrict__ border)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= n)
+		return;
+	const bool first = j == 0 || v[j - 1] != v[j] || id[j - 1] != id[j];
+	const bool last = j + 1 == n || v[j + 1] != v[j] || id[j + 1] != id[j];
+	if (first && last)
+		border[literal ? id[j] : v[j]] = 1;
+}
+
+void launch_long_count(const uint32_t *start, uint32_t nv, uint32_t *lcnt, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_long_count, dim3(g1((size_t)nv + 1)), dim3(256), 0, s, start, nv, lcnt);
+}
+void launch_long_flag(const int *tris, uint32_t n3, const uint32_t *start, uint32_t *flag, hipStream_t s)
+{
+	if (n3)
+		hipLaunchKernelGGL(k_long_flag, dim3(g1(n3)), dim3(256), 0, s, tris, n3, start, flag);
+}
+void launch_long_compact(const int *tris, uint32_t n3, const uint32_t *start, const uint32_t *pos, uint32_t *key,
+                         uint32_t *val, hipStream_t s)
+{
+	if (n3)
+		hipLaunchKernelGGL(k_long_compact, dim3(g1(n3)), dim3(256), 0, s, tris, n3, start, pos, key, val);
+}
+void launch_long_place(const int *tris, const uint32_t *key, const uint32_t *val, uint32_t L, const uint32_t *start,
+                       const uint32_t *loff, uint32_t *ref, int2 *pairs, hipStream_t s)
+{
+	if (L)
+		hipLaunchKernelGGL(k_long_place, dim3(g1(L)), dim3(256), 0, s, tris, key, val, L, start, loff, ref, pairs);
+}
+void launch_long_border_entries(const int *tris, const uint32_t *key, const uint32_t *val, uint32_t L, int literal,
+                                uint32_t *ek, uint32_t *ev, hipStream_t s)
+{
+	if (L)
+		hipLaunchKernelGGL(k_long_border_entries, dim3(g1(L)), dim3(256), 0, s, tris, key, val, L, literal, ek, ev);
+}
+void launch_long_border_mark(const uint32_t *v, const uint32_t *id, uint32_t n, int literal, uint8_t *border,
+                             hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_long_border_mark, dim3(g1(n)), dim3(256), 0, s, v, id, n, literal, border);
+}
+
 void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, uint32_t *rank, hipStream_t s)
 {
 	hipLaunchKernelGGL(k_csr_count, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, cnt, rank);
@@ -1636,6 +1764,10 @@ __global__ void k_border_pairs(const uint32_t *__restrict__ start, const int2 *_
 			r |= (id[i] >= 0 && cnt == 1);
 		}
 		border[v] = r;
+		return;
+	}
+	if (e - b > (uint32_t)CP_MAX) {
+		border[v] = 0; // the high-valence pass marks it
 		return;
 	}
 	for (uint32_t i = b; i < e && !r; i++) {

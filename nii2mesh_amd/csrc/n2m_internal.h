@@ -229,6 +229,17 @@ void launch_invert_perm(const uint32_t *perm, uint32_t n, uint32_t *inv, hipStre
 void launch_relabel_tris(const int *tris, const uint32_t *inv, uint32_t nt, int *out, hipStream_t s);
 void launch_tris_repeat(const int *tris, uint32_t nt, uint32_t *flag, hipStream_t s);
 void launch_border_pairs(const uint32_t *start, const int2 *pairs, uint32_t nv, uint8_t *border, hipStream_t s);
+// high-valence vertices (more than 16 references)
+void launch_long_count(const uint32_t *start, uint32_t nv, uint32_t *lcnt, hipStream_t s);
+void launch_long_flag(const int *tris, uint32_t n3, const uint32_t *start, uint32_t *flag, hipStream_t s);
+void launch_long_compact(const int *tris, uint32_t n3, const uint32_t *start, const uint32_t *pos, uint32_t *key,
+                         uint32_t *val, hipStream_t s);
+void launch_long_place(const int *tris, const uint32_t *key, const uint32_t *val, uint32_t L, const uint32_t *start,
+                       const uint32_t *loff, uint32_t *ref, int2 *pairs, hipStream_t s);
+void launch_long_border_entries(const int *tris, const uint32_t *key, const uint32_t *val, uint32_t L, int literal,
+                                uint32_t *ek, uint32_t *ev, hipStream_t s);
+void launch_long_border_mark(const uint32_t *v, const uint32_t *id, uint32_t n, int literal, uint8_t *border,
+                             hipStream_t s);
 // batched atlas weld (k_mesh.hip)
 void launch_atlas_weld_keys(const double *pts, uint32_t n, const uint32_t *vbase, int ntasks, uint32_t *keys,
                             uint32_t *vals, uint32_t *vtask, hipStream_t s);
@@ -246,5 +257,7 @@ void launch_gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t n, uin
 void launch_add_i32(const int *src, size_t n, int add, int *dst, hipStream_t s);
 void launch_sform(double *p, uint32_t np, const float *sx, const float *sy, const float *sz, hipStream_t s);
 void launch_swap_winding(int *t, uint32_t nt, hipStream_t s);
+// the robust-range percentile walk of api.cpp
+void robust_walk(const uint32_t *h, int kBins, size_t n2pct, int *lo, int *hi);
 
 } // namespace n2m

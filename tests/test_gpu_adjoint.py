@@ -12,11 +12,13 @@ from tests import support
 
 pytestmark = [
     pytest.mark.gpu,
-    # These tests passed in full on the GPU (forward bit-exact, gradient within
-    # 1e-12 of the FP64 oracle, 64/64 central differences within 1e-3 at
-    # 192^3); the test process then stopped making progress between the last
-    # test of this module and the next module, with no cause shown yet.
-    # Skipped until that cause is found.
+    # These tests passed in full on the GPU once (forward bit-exact, gradient
+    # within 1e-12 of the FP64 oracle, 64/64 central differences within 1e-3
+    # at 192^3); the test process then stopped making progress between the
+    # last test of this module and the next module.  Reading the code has not
+    # shown a cause: the contexts are now closed explicitly and every library
+    # call follows a device synchronize, but neither gap can make a process
+    # stall, so these tests stay off the GPU until a cause is shown.
     pytest.mark.skip(reason='process stalled after this module on the GPU; cause not yet shown'),
 ]
 
@@ -29,7 +31,9 @@ def make_fn(gpu, torch):
         def forward(fctx, vol, lap_iter, exact):
             dev = gpu.DeviceContext()
             nz, ny, nx = vol.shape
+            torch.cuda.synchronize()  # the library reads vol on its own stream
             if dev.diff_forward(vol.data_ptr(), (nx, ny, nz), 0.0, 0, lap_iter, exact) != 0:
+                dev.close()
                 raise RuntimeError('diff_forward failed: ' + gpu.last_error())
             t, p = dev.mesh(0)
             fctx.dev = dev
@@ -41,8 +45,11 @@ def make_fn(gpu, torch):
             g = grad_p.contiguous().to(torch.float64)
             out = torch.empty(fctx.shape, dtype=torch.float64, device=g.device)
             torch.cuda.synchronize()
-            if fctx.dev.diff_backward(g.data_ptr(), out.data_ptr()) != 0:
-                raise RuntimeError('diff_backward failed: ' + gpu.last_error())
+            try:
+                if fctx.dev.diff_backward(g.data_ptr(), out.data_ptr()) != 0:
+                    raise RuntimeError('diff_backward failed: ' + gpu.last_error())
+            finally:
+                fctx.dev.close()
             return out.to(torch.float32), None, None
 
     return MeshFromVolume
@@ -58,10 +65,11 @@ def sphere_loss_t(torch, p, c, r0):
 def test_diff_forward_matches_oracle(gpu, oracle, exact, originalMC):
     import torch
     img = support.sphere_field(48, 15.0, 2, noise=0.02)
-    dev = gpu.DeviceContext()
     d = torch.from_numpy(img).cuda()
-    assert dev.diff_forward(d.data_ptr(), (48, 48, 48), 0.0, originalMC, 20, exact) == 0
-    t, p = dev.mesh(0)
+    torch.cuda.synchronize()
+    with gpu.DeviceContext() as dev:
+        assert dev.diff_forward(d.data_ptr(), (48, 48, 48), 0.0, originalMC, 20, exact) == 0
+        t, p = dev.mesh(0)
     rc, to, po = oracle.diff_forward(img, 0.0, lap_iter=20, originalMC=originalMC, exact=exact)
     assert rc == 0
     assert np.array_equal(t, to) and np.array_equal(p, po)
@@ -95,12 +103,13 @@ def test_gradient_matches_oracle_fp64(gpu, oracle):
     g_orc = oracle.diff_gradient(img, 0.0, gp, lap_iter=lap)
     # float32 output of the autograd wrapper; the ABI itself returns float64
     assert np.abs(g_gpu - g_orc).max() <= 1e-6 * np.abs(g_orc).max()
-    dev = gpu.DeviceContext()
     d = torch.from_numpy(img).cuda()
-    assert dev.diff_forward(d.data_ptr(), (n, n, n), 0.0, 0, lap, True) == 0
     gd = torch.from_numpy(gp).cuda()
     out = torch.empty((n, n, n), dtype=torch.float64, device='cuda')
-    assert dev.diff_backward(gd.data_ptr(), out.data_ptr()) == 0
+    torch.cuda.synchronize()
+    with gpu.DeviceContext() as dev:
+        assert dev.diff_forward(d.data_ptr(), (n, n, n), 0.0, 0, lap, True) == 0
+        assert dev.diff_backward(gd.data_ptr(), out.data_ptr()) == 0
     g64 = out.cpu().numpy()
     err = np.abs(g64 - g_orc).max() / np.abs(g_orc).max()
     print('max |gpu - oracle| / max |oracle| = %.3g' % err)
@@ -118,10 +127,19 @@ def test_gradient_central_differences_192(gpu, oracle):
     n = img.shape[0]
     c, r0, lap, eps = ((n - 1) / 2.0,) * 3, 55.0, 20, 1e-3
     dev = gpu.DeviceContext()
+    try:
+        _central_differences(gpu, oracle, torch, dev, img, c, r0, lap, eps)
+    finally:
+        dev.close()
+
+
+def _central_differences(gpu, oracle, torch, dev, img, c, r0, lap, eps):
+    n = img.shape[0]
     d = torch.from_numpy(img).cuda()
     dims = (n, n, n)
 
     def forward(vol_t):
+        torch.cuda.synchronize()  # vol_t was written on torch's stream
         assert dev.diff_forward(vol_t.data_ptr(), dims, 0.0, 0, lap, True) == 0
         return dev.mesh(0)
 
@@ -129,6 +147,7 @@ def test_gradient_central_differences_192(gpu, oracle):
     L0, gp = support.sphere_loss(p0, c, r0)
     gd = torch.from_numpy(gp).cuda()
     out = torch.empty(dims[::-1], dtype=torch.float64, device='cuda')
+    torch.cuda.synchronize()
     assert dev.diff_backward(gd.data_ptr(), out.data_ptr()) == 0
     grad = out.cpu().numpy()
     # FP64 CPU gradient of the same forward

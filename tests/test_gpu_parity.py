@@ -512,3 +512,73 @@ def test_golden_fixtures_on_gpu(gpu):
         elif kind == 'threshold':
             for mode, val in zip(g['modes'], g['values']):
                 assert np.float32(gpu.lib().setThreshold(g['img'], int(mode))) == np.float32(val), f
+
+
+@pytest.mark.parametrize('flip', [False, True])
+def test_apply_sform_matches_reference(gpu, oracle, ref_classic, flip):
+    """n2m_apply_sform (FP64 on the GPU, winding swapped on the GPU when the
+    product of the row sums is negative) against the reference's own
+    apply_sform (src/meshify.c:1025-1045), bit for bit."""
+    import ctypes as C
+    img = support.blobs((26, 24, 22), 12)
+    rc, t, p = oracle.meshify(img, float(np.percentile(img, 60)), True, True, False, cubes=0)
+    assert rc == 0 and len(t) > 100
+    srow = np.array([[0.9973992, -0.0386101, 0.0608605, -69.86039],
+                     [0.0402, 1.0213, -0.0129, -102.5],
+                     [-0.0598, 0.0155, 0.9987, -73.25]], np.float32)
+    if flip:
+        srow[0] *= -1.5  # negative product of the row sums
+    ta, pa = t.copy(), p.copy()
+    tb, pb = t.copy(), p.copy()
+    f = C.c_float * 4
+    gpu.load().n2m_apply_sform(ta, pa, len(ta), len(pa), f(*srow[0]), f(*srow[1]), f(*srow[2]))
+    ref_classic.apply_sform(tb, pb, srow[0], srow[1], srow[2])
+    assert np.array_equal(ta, tb) and np.array_equal(pa.view(np.uint64), pb.view(np.uint64))
+    assert flip == (not np.array_equal(ta, t))
+
+
+def fan_mesh(n_apex, seed, repeats=False):
+    """A disk fanned around vertex 0 (n_apex triangles), a second apex of
+    valence 20 (a short fan), a vertex of valence 14, and, with repeats,
+    triangles listing a high-valence vertex twice."""
+    rng = np.random.default_rng(seed)
+    pts = [np.array([0.0, 0.0, 0.0])]
+    tris = []
+    a = np.linspace(0, 2 * np.pi, n_apex, endpoint=False)
+    rim = np.stack([np.cos(a), np.sin(a), 0.05 * rng.normal(size=n_apex)], 1) * 10
+    pts.extend(rim)
+    for i in range(n_apex):
+        tris.append((0, 1 + i, 1 + (i + 1) % n_apex))
+    for m, val in ((20, 20), (14, 14)):
+        c = len(pts)
+        pts.append(np.array([30.0 + m, 0.0, 0.0]))
+        b = np.linspace(0, 2 * np.pi, val, endpoint=False)
+        base = len(pts)
+        pts.extend(np.stack([30.0 + m + np.cos(b), np.sin(b), 0.1 * rng.normal(size=val)], 1))
+        for i in range(val):
+            tris.append((c, base + i, base + (i + 1) % val))
+    if repeats:
+        tris.append((0, 0, 5))
+        tris.append((7, 0, 7))
+    t = np.array(tris, np.int32)
+    order = rng.permutation(len(t))  # high-valence references spread over the triangle list
+    p = np.array(pts, np.float64) + rng.normal(scale=0.01, size=(len(pts), 3))
+    return t[order], p
+
+
+@pytest.mark.parametrize('repeats', [False, True])
+@pytest.mark.parametrize('lock', [True, False])
+def test_hc_high_valence_fan(gpu, oracle, repeats, lock):
+    """A vertex in 12000 triangles (and others of valence 20 and 14): its
+    sorted reference list and its border flags come from the sort-based
+    pass, not one lane per list; HC and the plain Laplacian equal the
+    oracle's bit for bit."""
+    t, p = fan_mesh(12000, 3, repeats)
+    a, b = p.copy(), p.copy()
+    gpu.lib().laplacian_smoothHC(a, t, 0.1, 0.5, 5, lock)
+    oracle.laplacian_smoothHC(b, t, 0.1, 0.5, 5, lock)
+    assert np.array_equal(a, b)
+    a, b = p.copy(), p.copy()
+    gpu.load().n2m_laplacian_smooth(a, t, len(a), len(t), 3)
+    oracle.laplacian_smooth(b, t, 3)
+    assert np.array_equal(a, b)
