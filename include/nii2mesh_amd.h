@@ -181,6 +181,22 @@ float n2m_ctx_set_threshold(n2m_ctx *ctx, const float *d_img, int64_t nvox, int 
 int n2m_ctx_set_thresholds(n2m_ctx *ctx, const float *d_img, int64_t nvox, int n, const int *modes,
                            float *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): n2m_ctx_set_thresholds without a context and without any
+ * allocation.  Runs on `stream` (a hipStream_t; NULL for the default
+ * stream) and returns when it is done.  Its histograms, range words and Otsu
+ * table live in one fixed 1 MiB device array per device, held until the
+ * process ends; every call on a device shares that array, so calls that
+ * overlap on one device race.  Voxel counts are 32-bit in the histograms:
+ * nvox must be 1..N2M_NOALLOC_MAX_VOXELS, n 1..N2M_NOALLOC_MAX_MODES and each
+ * mode 1, 2 or 3; anything else returns EXIT_FAILURE with the reason in
+ * n2m_last_error().  out[i] equals n2m_ctx_set_thresholds' out[i]. */
+#define N2M_NOALLOC_MAX_VOXELS 4294967295ull
+#define N2M_NOALLOC_MAX_MODES 16
+int n2m_set_thresholds_noalloc(const float *d_img, int64_t nvox, int n, const int *modes, float *out,
+                               void *stream);
+
 /* HC smoothing of mesh `mesh` held by the context. */
 int n2m_ctx_smooth_hc(n2m_ctx *ctx, int mesh, double alpha, double beta, int iter, int lockEdges);
 

@@ -90,6 +90,14 @@ void n2m::robust_walk(const uint32_t *h, int kBins, size_t n2pct, int *plo, int 
 	*phi = hi;
 }
 
+void n2m::set_error(const char *msg)
+{
+	tl_err = msg;
+	fprintf(stderr, "nii2mesh_amd: %s\n", msg);
+}
+
+void n2m::clear_error() { tl_err.clear(); }
+
 namespace {
 
 double now_ms()

@@ -942,6 +942,84 @@ void launch_otsu3(const double *P, int nBin, double *bv, int *bm, int *bh, hipSt
 		hipLaunchKernelGGL(k_otsu3, dim3(nBin - 3), dim3(256), 0, s, P, nBin, bv, bm, bh);
 }
 
+// The Liao table of nii_otsu (src/isolevel.c:147-175) for a 256-bin
+// histogram, built on the device: thread 0 forms the running sums in the
+// reference's order (P[0] is the raw count H[0], a quirk the thresholds
+// depend on), then thread u fills row u: P[u][v] = (S*S)/P of the bin range
+// [u, v] (left as is when P is 0); entries with v < u are never read and
+// are set to 0.
+__global__ void __launch_bounds__(256) k_otsu_table(const uint32_t *__restrict__ H, double *__restrict__ P)
+{
+	constexpr int n = 256;
+	__shared__ double s_p[n], s_s[n];
+	if (threadIdx.x == 0) {
+		double Sum = 0.0;
+		for (int v = 0; v < n; v++)
+			Sum = Sum + (double)(int)H[v];
+		s_p[0] = (double)(int)H[0];
+		s_s[0] = (double)(int)H[0];
+		for (int v = 1; v < n; v++) {
+			const double prob = (double)(int)H[v] / Sum;
+			s_p[v] = s_p[v - 1] + prob;
+			s_s[v] = s_s[v - 1] + (v + 1) * prob;
+		}
+	}
+	__syncthreads();
+	const int u = threadIdx.x;
+	double *row = P + (size_t)u * n;
+	for (int v = 0; v < n; v++) {
+		if (v < u) {
+			row[v] = 0.0;
+			continue;
+		}
+		const double p = u == 0 ? s_p[v] : s_p[v] - s_p[u - 1];
+		const double q = u == 0 ? s_s[v] : s_s[v] - s_s[u - 1];
+		row[v] = p != 0 ? (q * q) / p : p;
+	}
+}
+
+// the table entries the host-side searches read: row 0 (P[i]), column 255
+// shifted by one row (P[i+1][255]) and the four terms of the mode-5 start
+// value (lo, mi, hi = 64, 128, 192)
+__global__ void __launch_bounds__(256) k_otsu_probe(const double *__restrict__ P, double *__restrict__ probe)
+{
+	constexpr int n = 256;
+	const int i = threadIdx.x;
+	probe[i] = P[i];
+	probe[n + i] = i < n - 1 ? P[(size_t)(i + 1) * n + (n - 1)] : 0.0;
+	if (i == 0) {
+		probe[2 * n + 0] = P[64];
+		probe[2 * n + 1] = P[(size_t)65 * n + 128];
+		probe[2 * n + 2] = P[(size_t)129 * n + 192];
+		probe[2 * n + 3] = P[(size_t)193 * n + 255];
+	}
+}
+
+// range statistics {ordered min, ordered max, NaN count} and both
+// histograms cleared in one launch (no memset nodes)
+__global__ void __launch_bounds__(256)
+k_threshold_init(uint32_t *__restrict__ stats, uint32_t *__restrict__ h1001, uint32_t *__restrict__ h256)
+{
+	for (int i = threadIdx.x; i < 1024; i += blockDim.x)
+		h1001[i] = 0u;
+	h256[threadIdx.x] = 0u;
+	if (threadIdx.x < 4)
+		stats[threadIdx.x] = threadIdx.x == 0 ? 0xffffffffu : 0u;
+}
+
+void launch_otsu_table(const uint32_t *hist, double *P, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_otsu_table, dim3(1), dim3(256), 0, s, hist, P);
+}
+void launch_otsu_probe(const double *P, double *probe, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_otsu_probe, dim3(1), dim3(256), 0, s, P, probe);
+}
+void launch_threshold_init(uint32_t *stats, uint32_t *hist1001, uint32_t *hist256, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_threshold_init, dim3(1), dim3(256), 0, s, stats, hist1001, hist256);
+}
+
 // ---------------------------------------------------------------------------
 // Raw NIfTI voxels to the FP32 working volume, src/nii2mesh.c:155-172:
 // out = (raw * slope) + inter in float (two roundings), for uint8, int16,

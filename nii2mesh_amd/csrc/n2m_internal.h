@@ -257,7 +257,14 @@ void launch_gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t n, uin
 void launch_add_i32(const int *src, size_t n, int add, int *dst, hipStream_t s);
 void launch_sform(double *p, uint32_t np, const float *sx, const float *sy, const float *sz, hipStream_t s);
 void launch_swap_winding(int *t, uint32_t nt, hipStream_t s);
-// the robust-range percentile walk of api.cpp
+// setThreshold's histograms and Otsu (k_volume.hip)
+void launch_otsu_table(const uint32_t *hist, double *P, hipStream_t s);
+void launch_otsu_probe(const double *P, double *probe, hipStream_t s);
+void launch_threshold_init(uint32_t *stats, uint32_t *hist1001, uint32_t *hist256, hipStream_t s);
+
+// host helpers of api.cpp shared with noalloc.hip
 void robust_walk(const uint32_t *h, int kBins, size_t n2pct, int *lo, int *hi);
+void set_error(const char *msg); // n2m_last_error() text of this thread
+void clear_error();
 
 } // namespace n2m
