@@ -15,6 +15,7 @@ the Python side of that boundary:
 """
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -220,7 +221,6 @@ class DeviceContext:
     the next create() or close() on any thread."""
 
     def __init__(self):
-        import threading
         self.L = load()
         _close_orphans()
         self.h = self.L.n2m_ctx_create()
@@ -241,12 +241,11 @@ class DeviceContext:
         self.close()
 
     def __del__(self):
-        import threading
         h = getattr(self, 'h', None)
         if not h:
             return
         self.h = None
-        if threading.get_ident() == getattr(self, '_owner', None):
+        if threading is not None and threading.get_ident() == getattr(self, '_owner', None):
             try:
                 self.L.n2m_ctx_destroy(h)
             except Exception:
