@@ -169,7 +169,7 @@ struct LwGeo {
 	int out0, out1, out2; // lo + origin: added (as float) to sub-box coordinates
 };
 
-constexpr int LW_SEG = 1024, LW_BS = 256;
+constexpr int LW_SEG = 512, LW_BS = 256;
 
 __device__ __forceinline__ float lw_clamp(float v) { return fabsf(v) < FLT_EPSILON ? FLT_EPSILON : v; }
 
@@ -733,6 +733,141 @@ k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pf
 	            ns);
 }
 
+// The whole-volume pass B with one wave per row segment: the output of
+// lw_emit_row (the same vertex numbers, the same triangles in the same
+// order), but the popcount prefixes and the per-chunk scans are wave scans,
+// so no workgroup barrier separates the steps.  A noisy 512-point row holds
+// a few dozen vertices: the four-wave form spent its time in barriers and
+// dependent loads, and a wave per row keeps four times as many rows in
+// flight.  The prefixes are 16-bit (a segment has at most 3 * LW_SEG edge
+// vertices).
+static_assert(3 * LW_SEG < 65536, "16-bit prefixes");
+__global__ void __launch_bounds__(N2M_WAVE)
+k_lw_emit_w(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
+            const uint16_t *__restrict__ crow, const uint32_t *__restrict__ segcnt,
+            const uint32_t *__restrict__ bases, uint32_t nedge, double *__restrict__ pts, int *__restrict__ tris)
+{
+	__shared__ uint16_t s_pre[4][LW_SEG + 1];
+	__shared__ uint8_t s_flag[4][LW_SEG + 1];
+	const int seg = blockIdx.x, y = blockIdx.y, z = blockIdx.z, lane = (int)threadIdx.x;
+	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
+	const size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
+	if (segcnt[si] == 0 && segcnt[ns + si] == 0 && segcnt[2 * ns + si] == 0)
+		return;
+	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
+	const int len = xe - xs;
+	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
+	const int nrows = cuberow ? 4 : 1;
+	const int ncube = min(xe, g.sx - 1) - xs;
+	uint32_t base[4] = {0u, 0u, 0u, 0u};
+	for (int r = 0; r < nrows; r++) {
+		const int yy = y + (r & 1), zz = z + (r >> 1);
+		base[r] = bases[(size_t)seg + (size_t)g.nseg * ((size_t)yy + (size_t)g.sy * zz)];
+		const size_t rb = (size_t)g.sx * ((size_t)yy + (size_t)g.sy * zz);
+		uint32_t carry = 0;
+		// points [0, len]: point len (the next segment's first) closes the
+		// segment's last cube; its prefix is the row total
+		for (int c = 0; c * N2M_WAVE <= len; c++) {
+			const int i = c * N2M_WAVE + lane;
+			const uint32_t fl = i <= len && xs + i < g.sx ? pflags[rb + xs + i] : 0u;
+			const uint32_t f = i < len ? (uint32_t)__popc(fl) : 0u;
+			const uint32_t incl = wave_incl_scan(f);
+			if (i <= len) {
+				s_flag[r][i] = (uint8_t)fl;
+				s_pre[r][i] = (uint16_t)(carry + incl - f);
+			}
+			carry += __shfl(incl, N2M_WAVE - 1, N2M_WAVE);
+		}
+	}
+	__syncthreads(); // one wave: orders the LDS writes before other lanes' reads
+	// edge vertices of the owned points (row 0)
+	for (int i = lane; i < len; i += N2M_WAVE) {
+		const uint32_t f = s_flag[0][i];
+		if (!f)
+			continue;
+		const int x = xs + i;
+		uint32_t id = base[0] + s_pre[0][i];
+		for (int a = 0; a < 3; a++) {
+			if (!((f >> a) & 1u))
+				continue;
+			float p[3];
+			lw_edge_pos(img, g, x, y, z, a, p);
+			double *o = pts + (size_t)id * 3;
+			o[0] = (double)(p[0] + (float)g.out0);
+			o[1] = (double)(p[1] + (float)g.out1);
+			o[2] = (double)(p[2] + (float)g.out2);
+			id++;
+		}
+	}
+	if (!cuberow)
+		return;
+	const uint32_t tbase = bases[ns + si];
+	const uint32_t cbase = nedge + bases[2 * ns + si];
+	const size_t rb0 = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
+	uint32_t tcarry = 0, ccarry = 0;
+	for (int i0 = 0; i0 < ncube; i0 += N2M_WAVE) {
+		const int i = i0 + lane;
+		const int row = i < ncube ? (int)crow[rb0 + xs + i] - 1 : -1;
+		if (!__ballot(row >= 0))
+			continue; // no cube of this chunk has triangles
+		const uint32_t nt = row >= 0 ? n2m_lw_row_ntri[row] : 0u;
+		const uint32_t hc = (row >= 0 && !g.originalMC && n2m_lw_row_centre[row]) ? 1u : 0u;
+		// a chunk holds at most 64 * 12 triangles and 64 centres
+		const uint32_t packed = nt | (hc << 16);
+		const uint32_t incl = wave_incl_scan(packed);
+		const uint32_t excl = incl - packed, tot = __shfl(incl, N2M_WAVE - 1, N2M_WAVE);
+		if (row >= 0) {
+			const int x = xs + i;
+			uint32_t ids[12];
+#pragma unroll
+			for (int e = 0; e < 12; e++) {
+				const int dx = c_code[e][0], dy = c_code[e][1], dz = c_code[e][2], a = c_code[e][3];
+				const int r = dy + 2 * dz;
+				const uint32_t f = s_flag[r][i + dx];
+				ids[e] = ((f >> a) & 1u) ? base[r] + s_pre[r][i + dx] + __popc(f & ((1u << a) - 1u)) : 0xffffffffu;
+			}
+			uint32_t cid = 0xffffffffu;
+			if (hc) { // add_c_vertex(): mean of the cube's edge vertices, in float
+				float sxv = 0.f, syv = 0.f, szv = 0.f, cn = 0.f;
+				for (int e = 0; e < 12; e++) {
+					if (ids[e] == 0xffffffffu)
+						continue;
+					float p[3];
+					lw_edge_pos(img, g, x + c_code[e][0], y + c_code[e][1], z + c_code[e][2], c_code[e][3], p);
+					++cn;
+					sxv += p[0];
+					syv += p[1];
+					szv += p[2];
+				}
+				if (cn > 0.f) {
+					sxv /= cn;
+					syv /= cn;
+					szv /= cn;
+				}
+				cid = cbase + ccarry + (excl >> 16);
+				double *o = pts + (size_t)cid * 3;
+				o[0] = (double)(sxv + (float)g.out0);
+				o[1] = (double)(syv + (float)g.out1);
+				o[2] = (double)(szv + (float)g.out2);
+			}
+			const uint8_t *codes = &n2m_lw_edges[n2m_lw_row_off[row]];
+			int *t = tris + (size_t)(tbase + tcarry + (excl & 0xffffu)) * 3;
+			for (uint32_t k = 0; k < nt; k++) {
+				uint32_t v[3];
+				for (int j = 0; j < 3; j++) {
+					const int e = codes[3 * k + j];
+					v[j] = e == 12 ? cid : ids[e];
+				}
+				t[3 * k + 0] = (int)v[2];
+				t[3 * k + 1] = (int)v[1];
+				t[3 * k + 2] = (int)v[0];
+			}
+		}
+		tcarry += tot & 0xffffu;
+		ccarry += tot >> 16;
+	}
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(LW_BS)
 k_lw_emit_rec(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
@@ -794,8 +929,16 @@ void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], c
 	g.out0 = lo[0] + origin[0];
 	g.out1 = lo[1] + origin[1];
 	g.out2 = lo[2] + origin[2];
-	hipLaunchKernelGGL(k_lw_emit, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags, crow, segcnt, bases,
-	                   nedge, pts, tris);
+	static const bool four_waves = [] { // N2M_LW_EMIT=4: the four-wave form
+		const char *e = getenv("N2M_LW_EMIT");
+		return e && *e == '4';
+	}();
+	if (four_waves)
+		hipLaunchKernelGGL(k_lw_emit, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags, crow, segcnt,
+		                   bases, nedge, pts, tris);
+	else
+		hipLaunchKernelGGL(k_lw_emit_w, dim3(g.nseg, g.sy, g.sz), dim3(N2M_WAVE), 0, s, img, g, pflags, crow, segcnt,
+		                   bases, nedge, pts, tris);
 }
 
 void launch_lw_emit_rec(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
