@@ -508,6 +508,19 @@ uint32_t *scan_scratch(n2m_ctx *C, size_t n)
 	return C->buf<uint32_t>(S_SCAN, w > ((size_t)1 << 21) ? w : ((size_t)1 << 21));
 }
 
+// three exclusive scans of n elements (in + k*n -> out + k*n) with one host
+// round trip for the three totals
+void scan3(n2m_ctx *C, const uint32_t *in, uint32_t *out, size_t n, uint32_t tot[3])
+{
+	uint32_t *scr = scan_scratch(C, n);
+	for (int k = 0; k < 3; k++)
+		scan_excl_u32(in + k * n, out + k * n, n, C->d_small + SM_TOTAL + k, scr, C->stream);
+	C->check();
+	const uint32_t *t = C->fetch(SM_TOTAL, 3);
+	for (int k = 0; k < 3; k++)
+		tot[k] = t[k];
+}
+
 uint32_t scan(n2m_ctx *C, const uint32_t *in, uint32_t *out, size_t n, bool want_total)
 {
 	uint32_t *scr = scan_scratch(C, n);
@@ -697,9 +710,9 @@ bool marching_cubes(n2m_ctx *C, const float *vol, const int dim[3], const int lo
 	C->tbegin("mc_lewiner_classify", 4.0 * npts + 3.0 * npts);
 	launch_lw_classify(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, pf, cr, cnt, s);
 	C->tend();
-	uint32_t nE = scan(C, cnt, base, nseg, true);
-	uint32_t nT = scan(C, cnt + nseg, base + nseg, nseg, true);
-	uint32_t nC = scan(C, cnt + 2 * nseg, base + 2 * nseg, nseg, true);
+	uint32_t tot[3];
+	scan3(C, cnt, base, nseg, tot);
+	const uint32_t nE = tot[0], nT = tot[1], nC = tot[2];
 	uint32_t V = nE + nC;
 	if (V < 3 || nT < 1)
 		return false;
@@ -1846,9 +1859,9 @@ int diff_forward(n2m_ctx *C, const float *vol, const int dim[3], float iso, int 
 	C->tbegin("mc_lewiner_classify", 7.0 * npts);
 	launch_lw_classify(vol, NX, NY, NZ, lo, hi, iso, 0.f, originalMC, pf, cr, cnt, s);
 	C->tend();
-	uint32_t nE = scan(C, cnt, base, nseg, true);
-	uint32_t nT = scan(C, cnt + nseg, base + nseg, nseg, true);
-	uint32_t nC = scan(C, cnt + 2 * nseg, base + 2 * nseg, nseg, true);
+	uint32_t tot[3];
+	scan3(C, cnt, base, nseg, tot);
+	const uint32_t nE = tot[0], nT = tot[1], nC = tot[2];
 	uint32_t V = nE + nC;
 	if (V < 3 || nT < 1)
 		return EXIT_FAILURE;

@@ -405,13 +405,28 @@ __device__ __forceinline__ void lw_classify_core(const float (*s_row)[LW_SEG + 1
 		}
 		crow[rowbase + x] = cr;
 	}
-	nv = block_sum_u32<LW_BS>(nv, scratch);
-	nt = block_sum_u32<LW_BS>(nt, scratch);
-	nc = block_sum_u32<LW_BS>(nc, scratch);
+	// the three segment counts: wave sums, then one barrier (scratch holds
+	// 3 words per wave)
+	constexpr int NW = LW_BS / N2M_WAVE;
+	nv = wave_sum(nv);
+	nt = wave_sum(nt);
+	nc = wave_sum(nc);
+	if (lane_id() == 0) {
+		scratch[wave_id()] = nv;
+		scratch[NW + wave_id()] = nt;
+		scratch[2 * NW + wave_id()] = nc;
+	}
+	__syncthreads();
 	if (threadIdx.x == 0) {
-		cnt[0] = nv;
-		cnt[cstride] = nt;
-		cnt[2 * cstride] = nc;
+		uint32_t a = 0, b = 0, c = 0;
+		for (int w = 0; w < NW; w++) {
+			a += scratch[w];
+			b += scratch[NW + w];
+			c += scratch[2 * NW + w];
+		}
+		cnt[0] = a;
+		cnt[cstride] = b;
+		cnt[2 * cstride] = c;
 	}
 }
 
@@ -438,7 +453,7 @@ __device__ __forceinline__ void lw_classify_row(const float *__restrict__ img, c
                                                 uint8_t *__restrict__ pflags, uint16_t *__restrict__ crow,
                                                 uint32_t *__restrict__ cnt, size_t cstride)
 {
-	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
+	__shared__ uint32_t scratch[3 * (LW_BS / N2M_WAVE)];
 	__shared__ float s_row[4][LW_SEG + 1];
 	lw_stage_plane(s_row, 0, img, g, seg, y, z);
 	lw_stage_plane(s_row, 2, img, g, seg, y, z + 1);
@@ -456,7 +471,7 @@ __global__ void __launch_bounds__(LW_BS, 7)
 k_lw_classify(const float *__restrict__ img, LwGeo g, uint8_t *__restrict__ pflags,
               uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt /* 3 arrays of nseg*sy*sz */)
 {
-	__shared__ uint32_t scratch[LW_BS / N2M_WAVE + 1];
+	__shared__ uint32_t scratch[3 * (LW_BS / N2M_WAVE)];
 	__shared__ float s_row[4][LW_SEG + 1];
 	const int seg = blockIdx.x, y = blockIdx.y, z0 = blockIdx.z * LW_ZC, z1 = min(z0 + LW_ZC, g.sz);
 	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
@@ -771,11 +786,17 @@ k_lw_emit_w(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ 
 			const int i = c * N2M_WAVE + lane;
 			const uint32_t fl = i <= len && xs + i < g.sx ? pflags[rb + xs + i] : 0u;
 			const uint32_t f = i < len ? (uint32_t)__popc(fl) : 0u;
-			const uint32_t incl = wave_incl_scan(f);
-			if (i <= len) {
+			if (i <= len)
 				s_flag[r][i] = (uint8_t)fl;
-				s_pre[r][i] = (uint16_t)(carry + incl - f);
+			// a prefix is read only at a flagged point, and at len
+			if (!__ballot(f != 0u)) {
+				if (i == len)
+					s_pre[r][i] = (uint16_t)carry;
+				continue;
 			}
+			const uint32_t incl = wave_incl_scan(f);
+			if (i <= len)
+				s_pre[r][i] = (uint16_t)(carry + incl - f);
 			carry += __shfl(incl, N2M_WAVE - 1, N2M_WAVE);
 		}
 	}
