@@ -57,6 +57,17 @@ int meshify(float *img, short dim[3], int originalMC, float isolevel, vec3i **t,
 void laplacian_smoothHC(vec3d *verts, vec3i *tris, int nvert, int ntri, double alpha, double beta,
                         int iter, bool lockEdges);
 
+/* Replaces src/quadric.h:6 quadric_simplify_mesh() (src/quadric.c:396-518):
+ * the reference's edge-collapse simplification, result bit-identical.  The
+ * collapse loop is sequential (each collapse reads what the earlier ones
+ * changed) and runs on the host; the first update pass (reference lists,
+ * border flags, plane quadrics, edge errors: src/quadric.c:144-215) runs on
+ * the GPU.  *vs and *ts are freed and replaced by malloc'd arrays, as in the
+ * reference.  When the GPU pass fails the mesh is left unchanged and the
+ * reason goes to stderr. */
+void quadric_simplify_mesh(vec3d **vs, vec3i **ts, int *nvert, int *ntri, int target_count,
+                           double agressiveness, bool verbose, bool finishLossless);
+
 /* Replaces src/bwlabel.h:10 bwlabel() (src/bwlabel.c:478-543): connected
  * components (conn 6/18/26) numbered by first voxel in x-fastest order,
  * optional largest-cluster selection and bubble fill.  Returns the number of

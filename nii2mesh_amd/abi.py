@@ -173,6 +173,31 @@ def bind_degenerate(lib, name):
     return call
 
 
+def bind_quadric(lib, name='quadric_simplify_mesh'):
+    """Bind void quadric_simplify_mesh(vec3d **vs, vec3i **ts, int *nvert,
+    int *ntri, int target_count, double agressiveness, bool verbose,
+    bool finishLossless) (src/quadric.h:6): the call takes malloc'd arrays
+    and replaces them; returns copies of the simplified (tris, pts)."""
+    f = getattr(lib, name)
+    f.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
+                  C.c_double, C.c_bool, C.c_bool]
+    f.restype = None
+
+    def call(pts, tris, target_count, agressiveness=7.0, verbose=False, finishLossless=False):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        tris = np.ascontiguousarray(tris, dtype=np.int32)
+        pv = _libc_malloc(max(len(pts), 1) * 24)
+        tv = _libc_malloc(max(len(tris), 1) * 12)
+        C.memmove(pv, pts.ctypes.data, len(pts) * 24)
+        C.memmove(tv, tris.ctypes.data, len(tris) * 12)
+        p, t = C.c_void_p(pv), C.c_void_p(tv)
+        nv, nt = C.c_int(len(pts)), C.c_int(len(tris))
+        f(C.byref(p), C.byref(t), C.byref(nv), C.byref(nt), int(target_count), float(agressiveness), bool(verbose),
+          bool(finishLossless))
+        return _take(t.value, nt.value, np.int32), _take(p.value, nv.value, np.float64)
+    return call
+
+
 _libc.malloc.argtypes = [C.c_size_t]
 _libc.malloc.restype = C.c_void_p
 

@@ -216,6 +216,9 @@ enum Slot {
 	S_LG_EK1,
 	S_LG_EV1,
 	S_LG_RC,
+	S_Q_ND,
+	S_Q_Q,
+	S_Q_ERR,
 	S_COUNT
 };
 
@@ -915,7 +918,9 @@ std::pair<uint32_t *, uint32_t *> sort_pairs(n2m_ctx *C, uint32_t *k0, uint32_t 
 	return in1 ? std::make_pair(k1, v1) : std::make_pair(k0, v0);
 }
 
-Csr build_csr(n2m_ctx *C, const int *tris, uint32_t nv, uint32_t nt)
+// sorted_ref: ref holds each vertex's corners in ascending order (by
+// default only the pairs are ordered)
+Csr build_csr(n2m_ctx *C, const int *tris, uint32_t nv, uint32_t nt, bool sorted_ref = false)
 {
 	hipStream_t s = C->stream;
 	Csr c;
@@ -929,7 +934,7 @@ Csr build_csr(n2m_ctx *C, const int *tris, uint32_t nv, uint32_t nt)
 	launch_csr_count(tris, nt, cnt, rank, s);
 	scan_excl_u32(cnt, c.start, nv + 1, nullptr, scan_scratch(C, nv + 1), s);
 	launch_csr_fill(tris, nt, c.start, rank, c.ref, s);
-	launch_csr_pairs(tris, nv, c.start, c.ref, c.pairs, s);
+	launch_csr_pairs(tris, nv, c.start, c.ref, c.pairs, s, sorted_ref);
 	C->tend();
 	// lists longer than 16: ordered all at once (k_csr_pairs left them)
 	uint32_t *loff = C->buf<uint32_t>(S_LG_OFF, (size_t)nv + 1);
@@ -2168,6 +2173,41 @@ int save_mesh_device(n2m_ctx *C, const Mesh &m, const char *fnm, bool gz)
 // ===========================================================================
 // C ABI
 // ===========================================================================
+int n2m::quadric_prep(const vec3d *p, const vec3i *t, int nv, int nt, QuadricPrep &g)
+{
+	return guarded([&] {
+		if (nv < 0 || nt < 0)
+			fail("quadric_simplify_mesh: %d vertices, %d triangles", nv, nt);
+		check_tris((const int *)t, nt, nv);
+		n2m_ctx *C = tl_ctx.get();
+		hipStream_t s = C->stream;
+		const uint32_t NV = (uint32_t)nv, NT = (uint32_t)nt;
+		double *dp = C->buf<double>(S_PTS1, (size_t)3 * NV);
+		int *dt = C->buf<int>(S_TRIS1, (size_t)3 * NT);
+		if (NV)
+			HIPX(hipMemcpyAsync(dp, p, (size_t)24 * NV, hipMemcpyHostToDevice, s));
+		if (NT)
+			HIPX(hipMemcpyAsync(dt, t, (size_t)12 * NT, hipMemcpyHostToDevice, s));
+		const Csr c = build_csr(C, dt, NV, NT, true);
+		uint8_t *border = border_flags(C, c, dt, NV, NT);
+		double *nd = C->buf<double>(S_Q_ND, (size_t)4 * NT);
+		double *q = C->buf<double>(S_Q_Q, (size_t)10 * NV);
+		double *err = C->buf<double>(S_Q_ERR, (size_t)4 * NT);
+		C->tbegin("quadric_first_update", 36.0 * NT + 8.0 * NV);
+		launch_q_planes(dp, dt, NT, nd, s);
+		launch_q_vertex(c.start, c.ref, NV, nd, q, s);
+		launch_q_errors(dp, dt, NT, q, border, err, s);
+		C->tend();
+		g.start = download(C, c.start, (size_t)NV + 1);
+		g.ref = download(C, c.ref, (size_t)3 * NT);
+		g.border = download(C, border, NV);
+		g.q = download(C, q, (size_t)10 * NV);
+		g.nd = download(C, nd, (size_t)4 * NT);
+		g.err = download(C, err, (size_t)4 * NT);
+		return EXIT_SUCCESS;
+	});
+}
+
 extern "C" {
 
 int n2m_set_cubes(int flavour)

@@ -485,9 +485,10 @@ __device__ __forceinline__ int2 corner_pair(const int *__restrict__ tris, uint32
 }
 
 // lists longer than CP_MAX are left to the high-valence pass below
+// (sorted_ref: the sorted lists are also written back to ref)
 __global__ void __launch_bounds__(CP_BS)
 k_csr_pairs(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restrict__ start,
-            uint32_t *__restrict__ ref, int2 *__restrict__ pairs)
+            uint32_t *__restrict__ ref, int2 *__restrict__ pairs, int sorted_ref)
 {
 	__shared__ uint32_t s_ref[CP_MAX * CP_BS];
 	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -508,6 +509,9 @@ k_csr_pairs(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restric
 		}
 		for (uint32_t i = 0; i < n; i++)
 			expand(b + i, L[i * CP_BS]);
+		if (sorted_ref)
+			for (uint32_t i = 0; i < n; i++)
+				ref[b + i] = L[i * CP_BS];
 	}
 }
 
@@ -1488,9 +1492,10 @@ void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, const 
 	hipLaunchKernelGGL(k_csr_fill, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, start, rank, ref);
 }
 void launch_csr_pairs(const int *tris, uint32_t nv, const uint32_t *start, uint32_t *ref, int2 *pairs,
-                      hipStream_t s)
+                      hipStream_t s, bool sorted_ref)
 {
-	hipLaunchKernelGGL(k_csr_pairs, dim3(g1(nv)), dim3(CP_BS), 0, s, tris, nv, start, ref, pairs);
+	hipLaunchKernelGGL(k_csr_pairs, dim3(g1(nv)), dim3(CP_BS), 0, s, tris, nv, start, ref, pairs,
+	                   sorted_ref ? 1 : 0);
 }
 void launch_border(const int *tris, uint32_t nv, const uint32_t *start, const uint32_t *ref, uint8_t *border,
                    hipStream_t s)

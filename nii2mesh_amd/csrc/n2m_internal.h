@@ -6,6 +6,10 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
+#include "../../include/nii2mesh_amd.h"
+
 namespace n2m {
 
 // inverse of the order-preserving float -> uint32 map used by the device
@@ -177,7 +181,8 @@ void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, uint32_t *ran
 void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, const uint32_t *rank, uint32_t *ref,
                      hipStream_t s);
 void launch_csr_pairs(const int *tris, uint32_t nv, const uint32_t *start, uint32_t *ref, int2 *pairs,
-                      hipStream_t s);
+                      hipStream_t s,
+                      bool sorted_ref = false);
 void launch_border(const int *tris, uint32_t nv, const uint32_t *start, const uint32_t *ref, uint8_t *border,
                    hipStream_t s);
 void launch_hc_a(const uint32_t *start, const int2 *pairs, const double *o, const double *p, double *pn,
@@ -261,6 +266,25 @@ void launch_swap_winding(int *t, uint32_t nt, hipStream_t s);
 void launch_otsu_table(const uint32_t *hist, double *P, hipStream_t s);
 void launch_otsu_probe(const double *P, double *probe, hipStream_t s);
 void launch_threshold_init(uint32_t *stats, uint32_t *hist1001, uint32_t *hist256, hipStream_t s);
+
+// quadric simplification (k_quadric.hip, quadric.cpp): the first update
+// pass of quadric_simplify_mesh on the GPU
+void launch_q_planes(const double *p, const int *tris, uint32_t nt, double *nd, hipStream_t s);
+void launch_q_vertex(const uint32_t *start, const uint32_t *ref, uint32_t nv, const double *nd, double *q,
+                     hipStream_t s);
+void launch_q_errors(const double *p, const int *tris, uint32_t nt, const double *q, const uint8_t *border,
+                     double *err, hipStream_t s);
+// update_mesh(iteration 0) of src/quadric.c:144-215 for a host mesh: the
+// vertex lists (start, nv + 1; ref, corners t*3+k in ascending order), the
+// border flags, 10 quadric terms per vertex, normal + plane offset and the
+// four edge errors per triangle.  Returns 0, or EXIT_FAILURE with the reason
+// in n2m_last_error().
+struct QuadricPrep {
+	std::vector<uint32_t> start, ref;
+	std::vector<uint8_t> border;
+	std::vector<double> q, nd, err;
+};
+int quadric_prep(const vec3d *p, const vec3i *t, int nv, int nt, QuadricPrep &g);
 
 // host helpers of api.cpp shared with noalloc.hip
 void robust_walk(const uint32_t *h, int kBins, size_t n2pct, int *lo, int *hi);
