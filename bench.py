@@ -27,7 +27,16 @@ Also printed in the same JSON line:
                 launch time (HIP events on the library's stream, during the
                 timed steps), against the 8 TB/s HBM peak;
   cpu_baseline  the reference compiled from its own sources, timed on this
-                host (1 core) on a bounded sample of the same workload.
+                host (1 core) on a bounded sample of the same workload, with
+                its stage timers (-v 1) and its quadric reduction at 0.25
+                beside the library's quadric_simplify_mesh on the same meshes;
+  pcie_inclusive  the same step fed from, and returning to, host memory;
+  host_cpu      the host CPU model and the CPUs this process may use;
+  atlas.cpu_loop  the reference CLI's -a loop (OpenMP build) at 1 thread and
+                at every CPU, on a slab of the atlas.
+The roofline's `traffic` is read from profiles/traffic_<kernel>.txt
+(rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE of the same kernel, calibrated
+against a copy kernel): a committed measurement, not one of this run.
 """
 import argparse
 import ctypes as C
@@ -127,12 +136,30 @@ def cpu_baseline(vol, isos, z0=224, nz=64):
         import shutil
         shutil.rmtree(tmp, ignore_errors=True)
     hot_ms = thr_ms + sum(stages.values())
+    # the library's quadric_simplify_mesh (first pass on the GPU) on the same
+    # slab meshes, meshed and smoothed by the library (bit-identical to the
+    # reference's)
+    import nii2mesh_amd as n2m
+    from nii2mesh_amd import abi
+    q = abi.bind_quadric(n2m.load())
+    glib = n2m.lib(n2m.CUBES_LEWINER)
+    lib_ms = 0.0
+    for iso in isos:
+        rc, t, p = glib.meshify(slab.copy(), iso, True, True, True)
+        if rc != 0:
+            continue
+        glib.laplacian_smoothHC(p, t, 0.1, 0.5, HC_ITERS, True)
+        target = int(np.floor(np.float32(len(t)) * np.float32(0.25) + 0.5))  # round((float)ntri * 0.25)
+        t0 = time.perf_counter()
+        q(p, t, target, 7.0)
+        lib_ms += (time.perf_counter() - t0) * 1e3
     return {'value': slab.size / (hot_ms * 1e-3) / 1e9, 'unit': 'GVoxel/s', 'cores': 1, 'kind': 'reference',
             'sample': '512x512x%d z-slab [%d,%d) of the same volume: setThreshold x3 + 3 isolevels x nii2mesh '
                       '(Lewiner build) -l 1 -b 1 -s %d, stage lines of -v 1, load/save excluded: %.1f s'
                       % (nz, z0, z0 + nz, HC_ITERS, hot_ms / 1e3),
             'stages_ms': dict(stages, setThreshold=round(thr_ms, 1)),
-            'quadric_r0.25_ms': quadric_ms}
+            'quadric_r0.25_ms': quadric_ms,
+            'library_quadric_r0.25_ms': round(lib_ms, 1)}
 
 
 def cpu_atlas(vol, z0=112, nz=32):
@@ -290,7 +317,8 @@ def main():
     roof = {'kernel': dom, 'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
             'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
             'bytes_per_launch': dbytes / dn if dn else 0, 'avg_launch_us': dms / dn * 1e3 if dn else 0,
-            'traffic': traffic_from_profile(dom)}
+            'traffic': traffic_from_profile(dom),
+            'traffic_source': 'profiles/traffic_%s.txt (rocprofv3 --pmc, committed)' % dom}
     out = {
         'metric': 'GVoxel/s mesh extraction at 512^3 (%HBM peak)',
         'value': round(nvox / (ms_step * 1e-3) / 1e9, 4),
