@@ -56,38 +56,48 @@ struct Error : std::runtime_error {
 
 } // namespace
 
-// the 2 % / 98 % bins of the robust range (src/isolevel.c:102-134): lo is
-// the bin where the count from below reaches n2pct, hi likewise from above;
-// when they meet, both move outwards to the next non-empty bins
-void n2m::robust_walk(const uint32_t *h, int kBins, size_t n2pct, int *plo, int *phi)
+void n2m::thresholds_from_record(const ThrRecord &r, int n, const int *modes, float *out)
 {
-	size_t n = 0, lo = 0;
-	while (n < n2pct)
-		n += h[lo++];
-	lo--;
-	n = 0;
-	int hi = kBins;
-	while (n < n2pct)
-		n += h[--hi];
-	if ((int)lo == hi) { // src/isolevel.c:118-134
-		int ok = -1;
-		while (ok != 0) {
-			if (lo > 0) {
-				lo--;
-				if (h[lo] > 0)
-					ok = 0;
+	const int nBin = 256;
+	const float mn = r.par[2], scl = r.par[3];
+	double Sum = 0.0;
+	for (int v = 0; v < nBin; v++)
+		Sum = Sum + (double)(int)r.h256[v];
+	int dark5 = 0, bright5 = 0, mid3 = 0;
+	bool need5 = false, need3 = false;
+	for (int i = 0; i < n; i++)
+		(modes[i] == 1 || modes[i] == 3 ? need5 : need3) = true;
+	if (Sum > 0) { // an empty histogram gives 0 for every threshold (src/isolevel.c:152)
+		const double *probe = r.probe;
+		if (need3) { // nii_otsu mode 3: one threshold (src/isolevel.c:225-239)
+			int t = (int)(0.25 * nBin);
+			double best = probe[t] + probe[nBin + t];
+			for (int i = 0; i < nBin - 1; i++) {
+				const double v = probe[i] + probe[nBin + i];
+				if (v > best) {
+					t = i;
+					best = v;
+				}
 			}
-			if (ok != 0 && hi < kBins - 1) {
-				hi++;
-				if (h[hi] > 0)
-					ok = 0;
-			}
-			if (lo == 0 && hi == kBins - 1)
-				ok = 0;
+			mid3 = t;
+		}
+		if (need5) { // nii_otsu mode 5: three thresholds (src/isolevel.c:177-200)
+			int lo = (int)(0.25 * nBin), hi = (int)(0.75 * nBin);
+			double best = probe[2 * nBin] + probe[2 * nBin + 1] + probe[2 * nBin + 2] + probe[2 * nBin + 3];
+			for (int l = 0; l < nBin - 3; l++) // first strict maximum in l order
+				if (r.bm[l] >= 0 && r.bv[l] > best) {
+					lo = l;
+					hi = r.bh[l];
+					best = r.bv[l];
+				}
+			dark5 = lo;
+			bright5 = hi;
 		}
 	}
-	*plo = (int)lo;
-	*phi = hi;
+	for (int i = 0; i < n; i++) { // src/isolevel.c:264-276
+		const int t = modes[i] == 1 ? dark5 : (modes[i] == 3 ? bright5 : mid3);
+		out[i] = (t / scl) + mn;
+	}
 }
 
 void n2m::set_error(const char *msg)
@@ -219,6 +229,8 @@ enum Slot {
 	S_Q_ND,
 	S_Q_Q,
 	S_Q_ERR,
+	S_PIPE_BITS,
+	S_WPOS,
 	S_COUNT
 };
 
@@ -281,6 +293,7 @@ struct n2m_ctx {
 	} diff;
 	uint32_t *d_small = nullptr;
 	uint32_t *h_small = nullptr;
+	ThrRecord *h_thr = nullptr; // pinned copy of the threshold chain's record
 	bool timing = false;
 	std::string timing_only; // when set, only this stage is timed
 	std::vector<Stat> stats;
@@ -462,6 +475,8 @@ void ctx_free(n2m_ctx *c)
 		(void)hipFree(c->d_small);
 	if (c->h_small)
 		(void)hipHostFree(c->h_small);
+	if (c->h_thr)
+		(void)hipHostFree(c->h_thr);
 	(void)hipStreamDestroy(c->stream);
 	delete c;
 }
@@ -546,8 +561,10 @@ struct VolOut {
 	float vmin; // min of the final volume (Lewiner padding)
 };
 
+// spec_bits (when given) holds the threshold mask of P.isolevel, written by
+// the smoothing; it is recomputed there if the range check moves the isolevel
 bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P, bool have_range,
-                   bool spec_bits_valid_for_iso, VolOut &out, const float *src = nullptr)
+                   uint32_t *spec_bits, VolOut &out, const float *src = nullptr)
 {
 	// src (when given) holds the voxels; the stages read it and the apply
 	// stage writes every voxel of vol, so vol needs no copy of src
@@ -586,8 +603,8 @@ bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P
 	// mask untouched (src/bwlabel.c:434-437)
 	const bool label_ok = NX >= 2 && NY >= 2 && NZ >= 1;
 	if (P.onlyLargest || P.fillBubbles) {
-		uint32_t *bits = C->buf<uint32_t>(S_BITS, nwords);
-		if (!spec_bits_valid_for_iso || iso_changed) {
+		uint32_t *bits = spec_bits ? spec_bits : C->buf<uint32_t>(S_BITS, nwords);
+		if (!spec_bits || iso_changed) {
 			C->tbegin("mask_bits", 4.0 * nvox + nvox / 8.0);
 			launch_mask_bits(in, NX, NY, NZ, isolevel, 0, bits, s);
 			C->tend();
@@ -602,12 +619,9 @@ bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P
 			if (P.fillBubbles) { // src/bwlabel.c:478-541
 				HIPX(hipMemsetAsync(C->d_small + SM_NROOTS, 0, 4, s));
 				launch_ccl(bits, NX, NY, NZ, 6, 1, L, AUX, 1, C->d_small + SM_NROOTS, nullptr, rbase, scr, s, &tm);
-				uint32_t ninv = C->fetch(SM_NROOTS, 1)[0];
-				if (ninv > 1) {
-					C->tbegin("fill_bubbles", 4.0 * nvox);
-					launch_fill_bubbles(bits, NX, NY, NZ, L, AUX, rbase, s);
-					C->tend();
-				}
+				C->tbegin("fill_bubbles", 4.0 * nvox);
+				launch_fill_bubbles(bits, NX, NY, NZ, L, AUX, rbase, C->d_small + SM_NROOTS, s);
+				C->tend();
 			}
 			if (P.onlyLargest) { // src/bwlabel.c:453-471
 				uint32_t *roots = C->buf<uint32_t>(S_ROOTS, nvox / 2 + 1024);
@@ -734,9 +748,14 @@ bool marching_cubes(n2m_ctx *C, const float *vol, const int dim[3], const int lo
 // weld (src/meshify.c:45-106) from (pts, tris) -> (ptsOut, trisOut).
 // Returns the new vertex count; trisOut always receives explicit triangles.
 // ------------------------------------------------------------------------
+// With wpos given, a weld that merges vertices leaves tris_out unfilled and
+// returns in *wpos the welded position of every input vertex (in input
+// order), for remove_degenerate_welded; *wpos is null when nothing merged.
 uint32_t weld(n2m_ctx *C, const double *pts, uint32_t npt, const int *tris_in /* null: implicit */, uint32_t nt,
-              double *&pts_out, int *&tris_out, bool verbose)
+              double *&pts_out, int *&tris_out, bool verbose, double **wpos = nullptr)
 {
+	if (wpos)
+		*wpos = nullptr;
 	hipStream_t s = C->stream;
 	double t0 = now_ms();
 	uint32_t *k0 = C->buf<uint32_t>(S_K0, npt), *v0 = C->buf<uint32_t>(S_V0, npt);
@@ -781,10 +800,18 @@ uint32_t weld(n2m_ctx *C, const double *pts, uint32_t npt, const int *tris_in /*
 	launch_weld_assign(vals, lcls, flags, segid, seedbase, npt, cls, rep, s);
 	C->tend();
 	pts_out = C->buf<double>(S_PTS1, (size_t)3 * nnew);
-	C->tbegin("weld_gather", 28.0 * nnew + 24.0 * nnew);
-	launch_gather_pts(pts, rep, nnew, pts_out, s);
-	launch_remap_tris(tris_in, cls, nt, npt, tris_out, s);
-	C->tend();
+	if (wpos) {
+		double *wp = C->buf<double>(S_WPOS, (size_t)3 * npt);
+		C->tbegin("weld_place", 8.0 * npt + 24.0 * nnew + 24.0 * npt);
+		launch_weld_place(pts, cls, rep, npt, pts_out, wp, s);
+		C->tend();
+		*wpos = wp;
+	} else {
+		C->tbegin("weld_gather", 28.0 * nnew + 24.0 * nnew);
+		launch_gather_pts(pts, rep, nnew, pts_out, s);
+		launch_remap_tris(tris_in, cls, nt, npt, tris_out, s);
+		C->tend();
+	}
 	if (verbose) {
 		C->sync();
 		printf("vertex welding %d -> %d: %ld ms\n", (int)npt, (int)nnew, (long)round(now_ms() - t0));
@@ -821,6 +848,48 @@ uint32_t remove_degenerate(n2m_ctx *C, const double *pts, uint32_t np, int *&tri
 	return k;
 }
 
+// remove_degenerate after a merging weld(..., &wpos): the remap of the
+// input triangles (tris_in, null for implicit) through the class map and the
+// degenerate flags in one pass over positions in input order, then the
+// compaction.  tris: the weld's tris_out buffer in, the result out.
+uint32_t remove_degenerate_welded(n2m_ctx *C, const int *tris_in, uint32_t npt, const double *wpos, int *&tris,
+                                  uint32_t nt, bool verbose)
+{
+	hipStream_t s = C->stream;
+	double t0 = now_ms();
+	uint32_t *keep = C->buf<uint32_t>(S_KEEP, nt), *pos = C->buf<uint32_t>(S_KEEPPOS, nt);
+	C->tbegin("degenerate", 12.0 * nt + 12.0 * nt + 72.0 * nt + 4.0 * nt);
+	launch_remap_degen(tris_in, C->buf<int>(S_CLS, npt), npt, wpos, nt, tris, keep, s);
+	C->tend();
+	uint32_t k = scan(C, keep, pos, nt, true);
+	if (k == nt)
+		return nt;
+	int *out = C->buf<int>(S_TRIS0, (size_t)3 * (k ? k : 1));
+	C->tbegin("compact_tris", 12.0 * nt + 12.0 * k);
+	launch_compact_tris(tris, keep, pos, nt, out, s);
+	C->tend();
+	tris = out;
+	if (verbose) {
+		C->sync();
+		printf("remove degenerate triangles %d -> %d: %ld ms\n", (int)nt, (int)k, (long)round(now_ms() - t0));
+	}
+	return k;
+}
+
+// The result array p (bytes long) into the mesh buffer dst: when p is a
+// scratch slot of C, the two buffers trade places (no copy; the slot reuses
+// the mesh's old buffer next time), otherwise p is copied.
+void adopt(n2m_ctx *C, DevBuf &dst, const void *p, size_t bytes)
+{
+	for (DevBuf &b : C->bufs)
+		if (b.p == p) {
+			std::swap(b, dst);
+			return;
+		}
+	void *q = n2m_ctx::grow(dst, bytes);
+	HIPX(hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, C->stream));
+}
+
 // meshify() after the volume stages: MC, weld, de-degenerate; result into mesh m
 bool surface_stages(n2m_ctx *C, const float *vol, const int dim[3], const VolOut &vo, const n2m_params &P, Mesh &m)
 {
@@ -836,15 +905,14 @@ bool surface_stages(n2m_ctx *C, const float *vol, const int dim[3], const VolOut
 	const int *tris0 = mc.implicit ? nullptr : C->buf<int>(S_TRIS0, (size_t)3 * mc.ntri);
 	double *pts = nullptr;
 	int *tris = nullptr;
-	uint32_t np = weld(C, pts0, mc.npt, tris0, mc.ntri, pts, tris, P.verbose != 0);
+	double *wpos = nullptr;
+	uint32_t np = weld(C, pts0, mc.npt, tris0, mc.ntri, pts, tris, P.verbose != 0, &wpos);
 	if (np < 3)
 		return false;
-	uint32_t nt = remove_degenerate(C, pts, np, tris, mc.ntri, P.verbose != 0);
-	// copy out of the scratch slots into the mesh's own buffers
-	int *mt = (int *)n2m_ctx::grow(m.tris, (size_t)12 * (nt ? nt : 1));
-	double *mp = (double *)n2m_ctx::grow(m.pts, (size_t)24 * np);
-	HIPX(hipMemcpyAsync(mt, tris, (size_t)12 * nt, hipMemcpyDeviceToDevice, C->stream));
-	HIPX(hipMemcpyAsync(mp, pts, (size_t)24 * np, hipMemcpyDeviceToDevice, C->stream));
+	uint32_t nt = wpos ? remove_degenerate_welded(C, tris0, mc.npt, wpos, tris, mc.ntri, P.verbose != 0)
+	                   : remove_degenerate(C, pts, np, tris, mc.ntri, P.verbose != 0);
+	adopt(C, m.tris, tris, (size_t)12 * nt);
+	adopt(C, m.pts, pts, (size_t)24 * np);
 	m.nt = (int)nt;
 	m.np = (int)np;
 	return true;
@@ -857,7 +925,8 @@ bool meshify_device(n2m_ctx *C, const float *src, float *vol, const int dim[3], 
 	const int NX = dim[0], NY = dim[1], NZ = dim[2];
 	const size_t nvox = (size_t)NX * NY * NZ;
 	hipStream_t s = C->stream;
-	bool have_range = false, spec = false;
+	bool have_range = false;
+	uint32_t *spec = nullptr;
 	if (P.preSmooth && NX >= 5 && NY >= 5 && NZ >= 5) { // src/meshify.c:299-304
 		double t0 = now_ms();
 		uint32_t init[2] = {0xffffffffu, 0u};
@@ -865,13 +934,10 @@ bool meshify_device(n2m_ctx *C, const float *src, float *vol, const int dim[3], 
 		float *out = vol;
 		if (src == vol) // the stencil cannot run in place
 			out = C->buf<float>(S_VOL2, nvox);
-		uint32_t *bits = nullptr;
-		if (P.onlyLargest || P.fillBubbles) {
-			bits = C->buf<uint32_t>(S_BITS, (size_t)((NX + 31) / 32) * NY * NZ);
-			spec = true;
-		}
+		if (P.onlyLargest || P.fillBubbles)
+			spec = C->buf<uint32_t>(S_BITS, (size_t)((NX + 31) / 32) * NY * NZ);
 		C->tbegin("smooth", 8.0 * nvox);
-		launch_smooth(src, out, NX, NY, NZ, C->d_small + SM_RANGE, P.isolevel, bits, s);
+		launch_smooth(src, out, NX, NY, NZ, C->d_small + SM_RANGE, &P.isolevel, 1, spec, s);
 		C->tend();
 		if (out != vol)
 			HIPX(hipMemcpyAsync(vol, out, 4 * nvox, hipMemcpyDeviceToDevice, s));
@@ -1208,14 +1274,8 @@ template <typename T> T *host_copy_out(n2m_ctx *C, const void *d, size_t count)
 	return h;
 }
 
-int set_threshold_device(n2m_ctx *C, const float *d_img, int64_t nvox, int mode, float *result);
+int set_thresholds_device(n2m_ctx *C, const float *d_img, int64_t nvox, int n, const int *modes, float *out);
 int otsu(n2m_ctx *C, const int *H, int nBin, int mode, int *dark, int *mid, int *bright);
-struct ThrHist {
-	float mn, scl;
-	std::vector<int> h;
-};
-bool threshold_hist(n2m_ctx *C, const float *d_img, int64_t nvox, ThrHist &out);
-void thresholds_from(n2m_ctx *C, const ThrHist &H, int n, const int *modes, float *out);
 
 } // namespace
 
@@ -2272,12 +2332,24 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 		float *sm = C->buf<float>(S_VOL2, nvox); // smoothed volume, shared by all isolevels
 		bool smoothed = prm->preSmooth && NX >= 5 && NY >= 5 && NZ >= 5;
 		uint32_t range[2];
+		// the threshold masks of the first isolevels in one pass over the
+		// smoothed volume (valid while the range check keeps the isolevel)
+		const size_t nwords = (size_t)((NX + 31) / 32) * NY * NZ;
+		const int nmask = smoothed && (prm->onlyLargest || prm->fillBubbles)
+		                      ? (niso < N2M_SMOOTH_MASKS ? niso : N2M_SMOOTH_MASKS)
+		                      : 0;
+		uint32_t *masks = nmask ? C->buf<uint32_t>(S_PIPE_BITS, nmask * nwords) : nullptr;
 		if (smoothed) {
 			uint32_t init[2] = {0xffffffffu, 0u};
 			HIPX(hipMemcpyAsync(C->d_small + SM_RANGE, init, sizeof init, hipMemcpyHostToDevice, s));
 			C->tbegin("smooth", 8.0 * nvox);
-			launch_smooth(d_img, sm, NX, NY, NZ, C->d_small + SM_RANGE, 0.f, nullptr, s);
+			launch_smooth(d_img, sm, NX, NY, NZ, C->d_small + SM_RANGE, nullptr, 0, nullptr, s);
 			C->tend();
+			if (nmask) {
+				C->tbegin("mask_bits", 4.0 * nvox + nmask * nvox / 8.0);
+				launch_mask_bits_multi(sm, NX, NY, NZ, isolevels, nmask, masks, s);
+				C->tend();
+			}
 			uint32_t *r = C->fetch(SM_RANGE, 2);
 			range[0] = r[0];
 			range[1] = r[1];
@@ -2308,7 +2380,8 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 			Mesh &m = C->meshes[k];
 			m.nt = m.np = 0;
 			VolOut vo;
-			if (volume_stages(D, v, dim, P, smoothed, false, vo, smoothed ? sm : d_img) &&
+			if (volume_stages(D, v, dim, P, smoothed, k < nmask ? masks + k * nwords : nullptr, vo,
+			                  smoothed ? sm : d_img) &&
 			    surface_stages(D, v, dim, vo, P, m)) {
 				done[k] = 1;
 				if (hc_in_worker) {
@@ -2369,22 +2442,13 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 float n2m_ctx_set_threshold(n2m_ctx *C, const float *d_img, int64_t nvox, int mode)
 {
 	float r = 1.0f;
-	guarded([&] { return set_threshold_device(C, d_img, nvox, mode, &r); });
+	guarded([&] { return set_thresholds_device(C, d_img, nvox, 1, &mode, &r); });
 	return r;
 }
 
 int n2m_ctx_set_thresholds(n2m_ctx *C, const float *d_img, int64_t nvox, int n, const int *modes, float *out)
 {
-	return guarded([&] {
-		ThrHist H;
-		if (!threshold_hist(C, d_img, nvox, H)) {
-			for (int i = 0; i < n; i++)
-				out[i] = 1.0f;
-			return EXIT_SUCCESS;
-		}
-		thresholds_from(C, H, n, modes, out);
-		return EXIT_SUCCESS;
-	});
+	return guarded([&] { return set_thresholds_device(C, d_img, nvox, n, modes, out); });
 }
 
 int n2m_ctx_smooth_hc(n2m_ctx *C, int mesh, double alpha, double beta, int iter, int lockEdges)
@@ -2766,7 +2830,7 @@ int n2m_quick_smooth(float *img, int nx, int ny, int nz)
 		float *a = C->buf<float>(S_VOL, n), *b = C->buf<float>(S_VOL2, n);
 		HIPX(hipMemcpyAsync(a, img, 4 * n, hipMemcpyHostToDevice, C->stream));
 		HIPX(hipMemsetAsync(C->d_small + SM_RANGE, 0, 8, C->stream));
-		launch_smooth(a, b, nx, ny, nz, C->d_small + SM_RANGE, 0.f, nullptr, C->stream);
+		launch_smooth(a, b, nx, ny, nz, C->d_small + SM_RANGE, nullptr, 0, nullptr, C->stream);
 		C->check();
 		HIPX(hipMemcpyAsync(img, b, 4 * n, hipMemcpyDeviceToHost, C->stream));
 		C->sync();
@@ -2925,8 +2989,7 @@ int bwlabel(float *img, int conn, size_t dim[3], bool onlyLargest, bool fillBubb
 			HIPX(hipMemsetAsync(C->d_small + SM_NROOTS, 0, 4, s));
 			launch_ccl(bits, NX, NY, NZ, 6, 1, L, AUX, 1, C->d_small + SM_NROOTS, nullptr, rbase,
 			           scan_scratch(C, nwords + 1), s, nullptr);
-			if (C->fetch(SM_NROOTS, 1)[0] > 1)
-				launch_fill_bubbles(bits, NX, NY, NZ, L, AUX, rbase, s);
+			launch_fill_bubbles(bits, NX, NY, NZ, L, AUX, rbase, C->d_small + SM_NROOTS, s);
 		}
 		HIPX(hipMemsetAsync(C->d_small + SM_NROOTS, 0, 4, s));
 		launch_ccl(bits, NX, NY, NZ, conn, 0, L, AUX, 0, C->d_small + SM_NROOTS, roots, rbase,
@@ -2964,7 +3027,7 @@ float setThreshold(float *img, int nvox, int darkMediumBright123)
 		float *vol = C->buf<float>(S_VOL, nvox > 0 ? nvox : 1);
 		if (nvox > 0)
 			HIPX(hipMemcpyAsync(vol, img, 4 * (size_t)nvox, hipMemcpyHostToDevice, C->stream));
-		return set_threshold_device(C, vol, nvox, darkMediumBright123, &r);
+		return set_thresholds_device(C, vol, nvox, 1, &darkMediumBright123, &r);
 	});
 	return r;
 }
@@ -3066,106 +3129,31 @@ int otsu(n2m_ctx *C, const int *H, int nBin, int mode, int *dark, int *mid, int 
 	return thresh;
 }
 
-// setThreshold(): GPU range + histograms, host percentiles and Otsu.
-// src/isolevel.c:35-139 (nifti_robust_range, ignoreZeroVoxels = 0), 245-277
-
-// range + the two histograms of setThreshold(); returns false where the
-// reference returns 1 early (nvox < 1)
-bool threshold_hist(n2m_ctx *C, const float *d_img, int64_t nvox, ThrHist &out)
+// setThreshold() for a device volume (src/isolevel.c:35-277): the device
+// chain, then one copy of its record to pinned host memory.  nvox < 1
+// returns 1 for every mode, as the reference does.
+int set_thresholds_device(n2m_ctx *C, const float *d_img, int64_t nvox, int n, const int *modes, float *out)
 {
-	hipStream_t s = C->stream;
-	float mn, mx;
-	{
-		float pct2 = 0.0f, pct98 = 1.0f;
-		if (nvox < 1)
-			return false;
-		uint32_t init[3] = {0xffffffffu, 0u, 0u};
-		HIPX(hipMemcpyAsync(C->d_small + SM_HSTAT, init, sizeof init, hipMemcpyHostToDevice, s));
-		C->tbegin("hist_range", 4.0 * nvox);
-		launch_hist_range(d_img, nvox, C->d_small + SM_HSTAT, s);
-		C->tend();
-		uint32_t *st = C->fetch(SM_HSTAT, 3);
-		size_t nNan = st[2];
-		mn = ord2f_host(st[0]);
-		mx = ord2f_host(st[1]);
-		// zero voxels are already part of the minimum (src/isolevel.c:65-66)
-		if ((size_t)nvox == nNan) {
-			// all NaN: the range keeps its 0..1 defaults (src/isolevel.c:67-68)
-		} else if (mn == mx) {
-			pct2 = mn;
-			pct98 = mx;
-		} else {
-			size_t valid = (size_t)nvox - nNan;
-			size_t n2pct = (size_t)round(valid * 0.02);
-			if (n2pct < 1 || valid < 100) {
-				pct2 = mn;
-				pct98 = mx;
-			} else {
-				const int kBins = 1001;
-				float scl = (kBins - 1) / (mx - mn);
-				uint32_t *hist = C->buf<uint32_t>(S_HIST, 2048);
-				HIPX(hipMemsetAsync(hist, 0, 4 * kBins, s));
-				C->tbegin("hist_1001", 4.0 * nvox);
-				launch_hist_bins(d_img, nvox, mn, scl, kBins, hist, s);
-				C->tend();
-				std::vector<uint32_t> h(kBins);
-				HIPX(hipMemcpyAsync(h.data(), hist, 4 * kBins, hipMemcpyDeviceToHost, s));
-				C->sync();
-				int lo, hi;
-				robust_walk(h.data(), kBins, n2pct, &lo, &hi);
-				pct2 = (lo) / scl + mn;
-				pct98 = (hi) / scl + mn;
-			}
-		}
-		mn = pct2;
-		mx = pct98;
-	}
-	const int kOtsuBins = 256;
-	float scl = (kOtsuBins - 1) / (mx - mn);
-	uint32_t *hist = C->buf<uint32_t>(S_HIST, 2048);
-	HIPX(hipMemsetAsync(hist, 0, 4 * kOtsuBins, s));
-	C->tbegin("hist_256", 4.0 * nvox);
-	launch_hist_bins(d_img, nvox, mn, scl, kOtsuBins, hist, s);
-	C->tend();
-	std::vector<uint32_t> hu(kOtsuBins);
-	HIPX(hipMemcpyAsync(hu.data(), hist, 4 * kOtsuBins, hipMemcpyDeviceToHost, s));
-	C->sync();
-	out.mn = mn;
-	out.scl = scl;
-	out.h.assign(hu.begin(), hu.end());
-	return true;
-}
-
-// isolevels for several darkMediumBright123 modes from one set of histograms
-// (src/isolevel.c:264-276: modes 1 and 3 share the 3-threshold search)
-void thresholds_from(n2m_ctx *C, const ThrHist &H, int n, const int *modes, float *out)
-{
-	int d5 = -1, b5 = -1, m3 = -1, dark, mid, bright;
-	for (int i = 0; i < n; i++) {
-		if (modes[i] == 1 || modes[i] == 3) {
-			if (d5 < 0) {
-				otsu(C, H.h.data(), 256, 5, &dark, &mid, &bright);
-				d5 = dark;
-				b5 = bright;
-			}
-			out[i] = modes[i] == 1 ? (d5 / H.scl) + H.mn : (b5 / H.scl) + H.mn;
-		} else {
-			if (m3 < 0) {
-				otsu(C, H.h.data(), 256, 3, &dark, &mid, &bright);
-				m3 = mid;
-			}
-			out[i] = (m3 / H.scl) + H.mn;
-		}
-	}
-}
-
-int set_threshold_device(n2m_ctx *C, const float *d_img, int64_t nvox, int mode, float *result)
-{
-	ThrHist H;
-	*result = 1.0f;
-	if (!threshold_hist(C, d_img, nvox, H))
+	if (nvox < 1) {
+		for (int i = 0; i < n; i++)
+			out[i] = 1.0f;
 		return 0;
-	thresholds_from(C, H, 1, &mode, result);
+	}
+	bool need5 = false;
+	for (int i = 0; i < n; i++)
+		need5 = need5 || modes[i] == 1 || modes[i] == 3;
+	constexpr size_t REC = (sizeof(ThrRecord) + 255) / 256 * 256, H1001 = 4096;
+	char *b = C->buf<char>(S_OTSU, REC + H1001 + (size_t)256 * 256 * 8);
+	ThrRecord *rec = (ThrRecord *)b;
+	C->tbegin("set_threshold", 12.0 * nvox);
+	launch_threshold_chain(d_img, (size_t)nvox, rec, (uint32_t *)(b + REC), (double *)(b + REC + H1001), need5,
+	                       C->stream);
+	C->tend();
+	if (!C->h_thr)
+		HIPX(hipHostMalloc((void **)&C->h_thr, sizeof(ThrRecord), hipHostMallocDefault));
+	HIPX(hipMemcpyAsync(C->h_thr, rec, sizeof(ThrRecord), hipMemcpyDeviceToHost, C->stream));
+	C->sync();
+	thresholds_from_record(*C->h_thr, n, modes, out);
 	return 0;
 }
 

@@ -466,7 +466,8 @@ __device__ __forceinline__ void lw_classify_row(const float *__restrict__ img, c
 // once, instead of once per neighbouring row segment).
 constexpr int LW_ZC = 8;
 // 7 waves per SIMD: the register cap (72) leaves no spills; the default
-// allocation (95 registers, 5 waves) was 30 % slower
+// allocation (95 registers, 5 waves) was 30 % slower.  Loading the plane two
+// steps ahead into registers (80 registers, 6 waves) was 13 % slower.
 __global__ void __launch_bounds__(LW_BS, 7)
 k_lw_classify(const float *__restrict__ img, LwGeo g, uint8_t *__restrict__ pflags,
               uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt /* 3 arrays of nseg*sy*sz */)
@@ -755,15 +756,22 @@ k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pf
 // a few dozen vertices: the four-wave form spent its time in barriers and
 // dependent loads, and a wave per row keeps four times as many rows in
 // flight.  The prefixes are 16-bit (a segment has at most 3 * LW_SEG edge
-// vertices).
+// vertices).  Memory latency, not bandwidth, bounds this pass, so every
+// load of a step is issued before the first is used: the flags of the four
+// point rows together, the tiling rows of all cube chunks together, and the
+// owned row's edge vertices one per lane (from a list built with the
+// prefixes) instead of one point per lane.
 static_assert(3 * LW_SEG < 65536, "16-bit prefixes");
 __global__ void __launch_bounds__(N2M_WAVE)
 k_lw_emit_w(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
             const uint16_t *__restrict__ crow, const uint32_t *__restrict__ segcnt,
             const uint32_t *__restrict__ bases, uint32_t nedge, double *__restrict__ pts, int *__restrict__ tris)
 {
+	constexpr int NIT = LW_SEG / N2M_WAVE + 1; // chunks of the points [0, len]
+	constexpr int NCC = LW_SEG / N2M_WAVE;     // chunks of the cubes
 	__shared__ uint16_t s_pre[4][LW_SEG + 1];
 	__shared__ uint8_t s_flag[4][LW_SEG + 1];
+	__shared__ uint16_t s_vl[3 * LW_SEG]; // the owned row's edge vertices in id order: point << 2 | axis
 	const int seg = blockIdx.x, y = blockIdx.y, z = blockIdx.z, lane = (int)threadIdx.x;
 	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
 	const size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
@@ -775,19 +783,34 @@ k_lw_emit_w(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ 
 	const int nrows = cuberow ? 4 : 1;
 	const int ncube = min(xe, g.sx - 1) - xs;
 	uint32_t base[4] = {0u, 0u, 0u, 0u};
-	for (int r = 0; r < nrows; r++) {
+	uint32_t fl[4][NIT];
+#pragma unroll
+	for (int r = 0; r < 4; r++) {
 		const int yy = y + (r & 1), zz = z + (r >> 1);
-		base[r] = bases[(size_t)seg + (size_t)g.nseg * ((size_t)yy + (size_t)g.sy * zz)];
+		const bool live = r < nrows;
+		if (live)
+			base[r] = bases[(size_t)seg + (size_t)g.nseg * ((size_t)yy + (size_t)g.sy * zz)];
 		const size_t rb = (size_t)g.sx * ((size_t)yy + (size_t)g.sy * zz);
-		uint32_t carry = 0;
-		// points [0, len]: point len (the next segment's first) closes the
-		// segment's last cube; its prefix is the row total
-		for (int c = 0; c * N2M_WAVE <= len; c++) {
+#pragma unroll
+		for (int c = 0; c < NIT; c++) {
 			const int i = c * N2M_WAVE + lane;
-			const uint32_t fl = i <= len && xs + i < g.sx ? pflags[rb + xs + i] : 0u;
-			const uint32_t f = i < len ? (uint32_t)__popc(fl) : 0u;
+			fl[r][c] = live && i <= len && xs + i < g.sx ? pflags[rb + xs + i] : 0u;
+		}
+	}
+	uint32_t nv0 = 0; // edge vertices of the owned row
+#pragma unroll
+	for (int r = 0; r < 4; r++) {
+		if (r >= nrows)
+			break;
+		uint32_t carry = 0;
+#pragma unroll
+		for (int c = 0; c < NIT; c++) {
+			if (c * N2M_WAVE > len)
+				break;
+			const int i = c * N2M_WAVE + lane;
+			const uint32_t f = i < len ? (uint32_t)__popc(fl[r][c]) : 0u;
 			if (i <= len)
-				s_flag[r][i] = (uint8_t)fl;
+				s_flag[r][i] = (uint8_t)fl[r][c];
 			// a prefix is read only at a flagged point, and at len
 			if (!__ballot(f != 0u)) {
 				if (i == len)
@@ -795,40 +818,48 @@ k_lw_emit_w(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ 
 				continue;
 			}
 			const uint32_t incl = wave_incl_scan(f);
+			const uint32_t pre = carry + incl - f;
 			if (i <= len)
-				s_pre[r][i] = (uint16_t)(carry + incl - f);
+				s_pre[r][i] = (uint16_t)pre;
+			if (r == 0 && f) {
+				uint32_t at = pre;
+				for (int a = 0; a < 3; a++)
+					if ((fl[0][c] >> a) & 1u)
+						s_vl[at++] = (uint16_t)((i << 2) | a);
+			}
 			carry += __shfl(incl, N2M_WAVE - 1, N2M_WAVE);
 		}
+		if (r == 0)
+			nv0 = carry;
 	}
 	__syncthreads(); // one wave: orders the LDS writes before other lanes' reads
-	// edge vertices of the owned points (row 0)
-	for (int i = lane; i < len; i += N2M_WAVE) {
-		const uint32_t f = s_flag[0][i];
-		if (!f)
-			continue;
-		const int x = xs + i;
-		uint32_t id = base[0] + s_pre[0][i];
-		for (int a = 0; a < 3; a++) {
-			if (!((f >> a) & 1u))
-				continue;
-			float p[3];
-			lw_edge_pos(img, g, x, y, z, a, p);
-			double *o = pts + (size_t)id * 3;
-			o[0] = (double)(p[0] + (float)g.out0);
-			o[1] = (double)(p[1] + (float)g.out1);
-			o[2] = (double)(p[2] + (float)g.out2);
-			id++;
-		}
+	// the tiling rows of all cube chunks, loaded before the first is used
+	const size_t rb0 = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
+	int rows[NCC];
+#pragma unroll
+	for (int c = 0; c < NCC; c++) {
+		const int i = c * N2M_WAVE + lane;
+		rows[c] = cuberow && i < ncube ? (int)crow[rb0 + xs + i] - 1 : -1;
+	}
+	// edge vertices of the owned points (row 0), one vertex per lane
+	for (uint32_t v = lane; v < nv0; v += N2M_WAVE) {
+		const uint32_t e = s_vl[v];
+		float p[3];
+		lw_edge_pos(img, g, xs + (int)(e >> 2), y, z, (int)(e & 3u), p);
+		double *o = pts + (size_t)(base[0] + v) * 3;
+		o[0] = (double)(p[0] + (float)g.out0);
+		o[1] = (double)(p[1] + (float)g.out1);
+		o[2] = (double)(p[2] + (float)g.out2);
 	}
 	if (!cuberow)
 		return;
 	const uint32_t tbase = bases[ns + si];
 	const uint32_t cbase = nedge + bases[2 * ns + si];
-	const size_t rb0 = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
 	uint32_t tcarry = 0, ccarry = 0;
-	for (int i0 = 0; i0 < ncube; i0 += N2M_WAVE) {
-		const int i = i0 + lane;
-		const int row = i < ncube ? (int)crow[rb0 + xs + i] - 1 : -1;
+#pragma unroll
+	for (int c = 0; c < NCC; c++) {
+		const int i = c * N2M_WAVE + lane;
+		const int row = rows[c];
 		if (!__ballot(row >= 0))
 			continue; // no cube of this chunk has triangles
 		const uint32_t nt = row >= 0 ? n2m_lw_row_ntri[row] : 0u;

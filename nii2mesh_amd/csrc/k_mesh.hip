@@ -92,9 +92,12 @@ void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total,
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort of (uint32 key, uint32 value), 8-bit digits.
 // Each workgroup ranks a tile of RS_TILE keys in RS_IPT rounds of RS_BS keys
-// (tile order is preserved: round-major, thread order inside a round).
+// (tile order is preserved: round-major, thread order inside a round),
+// places them in LDS in digit order, then writes each digit's keys as one
+// contiguous run at its global offset (coalesced, instead of one scattered
+// word per key).
 // ---------------------------------------------------------------------------
-constexpr int RS_BS = 256, RS_IPT = 8, RS_TILE = RS_BS * RS_IPT, RS_NW = RS_BS / N2M_WAVE;
+constexpr int RS_BS = 256, RS_IPT = 16, RS_TILE = RS_BS * RS_IPT, RS_NW = RS_BS / N2M_WAVE;
 
 __global__ void __launch_bounds__(RS_BS)
 k_radix_hist(const uint32_t *__restrict__ keys, size_t n, int shift, uint32_t *__restrict__ counts, size_t nblocks)
@@ -116,30 +119,44 @@ __global__ void __launch_bounds__(RS_BS)
 k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint32_t *__restrict__ kout,
                 uint32_t *__restrict__ vout, size_t n, int shift, const uint32_t *__restrict__ offs, size_t nblocks)
 {
-	__shared__ uint32_t run[256];
-	__shared__ uint32_t wcnt[RS_NW][256];
+	__shared__ uint32_t s_k[RS_TILE], s_v[RS_TILE];
+	__shared__ uint32_t run[256];  // keys of each digit placed so far
+	__shared__ uint32_t lofs[256]; // tile-local start of each digit
 	__shared__ uint32_t gbase[256];
+	__shared__ uint32_t wcnt[RS_NW][256];
+	__shared__ uint32_t scratch[RS_BS / N2M_WAVE + 1];
+	const size_t base = blockIdx.x * (size_t)RS_TILE;
+	const int tn = (int)(n - base < (size_t)RS_TILE ? n - base : (size_t)RS_TILE);
 	run[threadIdx.x] = 0;
 	gbase[threadIdx.x] = offs[(size_t)threadIdx.x * nblocks + blockIdx.x];
 	for (int w = 0; w < RS_NW; w++)
 		wcnt[w][threadIdx.x] = 0;
 	__syncthreads();
-	const size_t base = blockIdx.x * (size_t)RS_TILE;
+	for (int k = 0; k < RS_IPT; k++) {
+		const int i = k * RS_BS + threadIdx.x;
+		if (i < tn)
+			atomicAdd(&run[(kin[base + i] >> shift) & 0xFF], 1u);
+	}
+	__syncthreads();
+	uint32_t tot;
+	lofs[threadIdx.x] = block_excl_scan<RS_BS>(run[threadIdx.x], scratch, &tot); // thread t <-> digit t
+	run[threadIdx.x] = 0;
+	__syncthreads();
 	const int lane = lane_id(), wv = wave_id();
 	const unsigned long long lt = (1ull << lane) - 1ull;
 	for (int k = 0; k < RS_IPT; k++) {
-		size_t i = base + k * RS_BS + threadIdx.x;
-		bool act = i < n;
-		uint32_t key = act ? kin[i] : 0u, val = act ? vin[i] : 0u;
-		uint32_t d = (key >> shift) & 0xFF;
+		const int i = k * RS_BS + threadIdx.x;
+		const bool act = i < tn;
+		const uint32_t key = act ? kin[base + i] : 0u, val = act ? vin[base + i] : 0u;
+		const uint32_t d = (key >> shift) & 0xFF;
 		unsigned long long m = __ballot(act);
 #pragma unroll
 		for (int b = 0; b < 8; b++) {
-			bool bit = (d >> b) & 1u;
-			unsigned long long bb = __ballot(bit);
+			const bool bit = (d >> b) & 1u;
+			const unsigned long long bb = __ballot(bit);
 			m &= bit ? bb : ~bb;
 		}
-		uint32_t rk = __popcll(m & lt);
+		const uint32_t rk = __popcll(m & lt);
 		if (act && rk == 0)
 			wcnt[wv][d] = __popcll(m);
 		__syncthreads();
@@ -147,9 +164,8 @@ k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ v
 			uint32_t pos = run[d] + rk;
 			for (int w = 0; w < wv; w++)
 				pos += wcnt[w][d];
-			uint32_t o = gbase[d] + pos;
-			kout[o] = key;
-			vout[o] = val;
+			s_k[lofs[d] + pos] = key;
+			s_v[lofs[d] + pos] = val;
 		}
 		__syncthreads();
 		uint32_t add = 0;
@@ -159,6 +175,12 @@ k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ v
 		}
 		run[threadIdx.x] += add;
 		__syncthreads();
+	}
+	for (int i = threadIdx.x; i < tn; i += RS_BS) {
+		const uint32_t key = s_k[i], d = (key >> shift) & 0xFF;
+		const uint32_t o = gbase[d] + (uint32_t)(i - (int)lofs[d]);
+		kout[o] = key;
+		vout[o] = s_v[i];
 	}
 }
 
@@ -391,6 +413,82 @@ __global__ void k_degen_flags(const double *__restrict__ pts, const int *__restr
 			k = 0;
 	}
 	keep[i] = k;
+}
+
+// The weld's output positions by scatter: each class's representative
+// (rep[c], its last member in index order) writes out[c]; every vertex also
+// gets its welded position in its own (pre-weld) order, wpos[v] =
+// pts[rep[cls[v]]], so that triangles can read positions in emit order.
+__global__ void k_weld_place(const double *__restrict__ pts, const int *__restrict__ cls,
+                             const int *__restrict__ rep, uint32_t npt, double *__restrict__ out,
+                             double *__restrict__ wpos)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= npt)
+		return;
+	const int c = cls[v];
+	const uint32_t r = (uint32_t)rep[c];
+	const double x = pts[(size_t)r * 3], y = pts[(size_t)r * 3 + 1], z = pts[(size_t)r * 3 + 2];
+	if (r == v) {
+		out[(size_t)c * 3] = x;
+		out[(size_t)c * 3 + 1] = y;
+		out[(size_t)c * 3 + 2] = z;
+	}
+	wpos[(size_t)v * 3] = x;
+	wpos[(size_t)v * 3 + 1] = y;
+	wpos[(size_t)v * 3 + 2] = z;
+}
+
+// k_remap_tris and k_degen_flags in one pass: the welded ids of a
+// triangle's corners (an id outside [0, npt) stays -1 and marks the
+// triangle degenerate) and the degenerate test on the welded positions,
+// read as wpos[pre-weld id] -- the same doubles as out[welded id].
+__global__ void k_remap_degen(const int *__restrict__ tris_in, const int *__restrict__ cls, uint32_t npt,
+                              const double *__restrict__ wpos, uint32_t nt, int *__restrict__ tris_out,
+                              uint32_t *__restrict__ keep)
+{
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nt)
+		return;
+	int v[3];
+	bool ok = true;
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+		v[k] = tris_in ? tris_in[3 * (size_t)i + k] : (int)(3 * i + k);
+		const bool valid = v[k] >= 0 && (uint32_t)v[k] < npt;
+		tris_out[3 * (size_t)i + k] = valid ? cls[v[k]] : -1;
+		ok = ok && valid;
+	}
+	if (!ok) {
+		keep[i] = 0;
+		return;
+	}
+	const double *A = wpos + (size_t)v[0] * 3, *B = wpos + (size_t)v[1] * 3, *C = wpos + (size_t)v[2] * 3;
+	double l = dist3(A, B), m = dist3(A, C), n = dist3(B, C);
+	double c = fmin(fmin(l, m), n);
+	double a = fmax(fmax(l, m), n);
+	double b = l + m + n - a - c;
+	uint32_t k = 1;
+	if ((c - (a - b)) <= 0.0) {
+		k = 0;
+	} else {
+		double area4 = 0.25 * sqrt((a + (b + c)) * (c - (a - b)) * (c + (a - b)) * (a + (b - c)));
+		if (area4 < FLT_EPSILON)
+			k = 0;
+	}
+	keep[i] = k;
+}
+
+void launch_weld_place(const double *pts, const int *cls, const int *rep, uint32_t npt, double *out, double *wpos,
+                       hipStream_t s)
+{
+	hipLaunchKernelGGL(k_weld_place, dim3(g1(npt)), dim3(256), 0, s, pts, cls, rep, npt, out, wpos);
+}
+
+void launch_remap_degen(const int *tris_in, const int *cls, uint32_t npt, const double *wpos, uint32_t nt,
+                        int *tris_out, uint32_t *keep, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_remap_degen, dim3(g1(nt)), dim3(256), 0, s, tris_in, cls, npt, wpos, nt, tris_out, keep);
 }
 
 __global__ void k_compact_tris(const int *__restrict__ tris, const uint32_t *__restrict__ keep,

@@ -35,15 +35,23 @@ __device__ __forceinline__ float tap5(float a, float b, float c, float d, float 
 	return (float)acc;
 }
 
+// Isolevels whose threshold masks the smoothing writes as it goes: mask k
+// (out >= v[k]) goes to bits + k * nwords.
+struct IsoMasks {
+	float v[N2M_SMOOTH_MASKS];
+	int n;
+	size_t nwords;
+};
+
 // Smooth the TX x TY tile (x0, y0, [z0, z1)) of an nx*ny*nz volume read
 // through `src(x, y, z)` (only called inside the volume) into out; returns
 // the tile's min/max through lmin/lmax (per thread) and, for TX = 64 (one
-// wave per tile row), writes speculative mask bits (out >= iso) when
-// bits != null.
+// wave per tile row), writes speculative mask bits for the isolevels of
+// `iso` when bits != null.
 template <int TX, int TY, class Src>
 __device__ __forceinline__ void smooth_tile(const Src &src, float *__restrict__ out, int nx, int ny, int nz,
-                                            int x0, int y0, int z0, int z1, float iso, uint32_t *__restrict__ bits,
-                                            int W, float &lmin, float &lmax)
+                                            int x0, int y0, int z0, int z1, const IsoMasks &iso,
+                                            uint32_t *__restrict__ bits, int W, float &lmin, float &lmax)
 {
 	constexpr int CPT = TX * TY / SM_BS, RPP = SM_BS / TX; // columns per thread, rows per pass
 	static_assert(CPT * SM_BS == TX * TY && RPP * TX == SM_BS, "tile must tile the workgroup");
@@ -117,15 +125,18 @@ __device__ __forceinline__ void smooth_tile(const Src &src, float *__restrict__ 
 					lmin = fminf(lmin, o);
 					lmax = fmaxf(lmax, o);
 				}
-				if (TX == 64 && bits) { // speculative threshold mask (the common iso)
-					unsigned long long m = __ballot(valid && o >= iso);
-					if (y < ny) {
-						const int wx = x0 / 32;
-						size_t wi = (size_t)wx + (size_t)W * ((size_t)y + (size_t)ny * zo);
-						if (tx == 0)
-							bits[wi] = (uint32_t)m;
-						else if (tx == 32 && wx + 1 < W)
-							bits[wi + 1] = (uint32_t)(m >> 32);
+				if (TX == 64 && bits) { // speculative threshold masks
+					const int wx = x0 / 32;
+					const size_t wi = (size_t)wx + (size_t)W * ((size_t)y + (size_t)ny * zo);
+					for (int k = 0; k < iso.n; k++) {
+						const unsigned long long m = __ballot(valid && o >= iso.v[k]);
+						uint32_t *bk = bits + k * iso.nwords;
+						if (y < ny) {
+							if (tx == 0)
+								bk[wi] = (uint32_t)m;
+							else if (tx == 32 && wx + 1 < W)
+								bk[wi + 1] = (uint32_t)(m >> 32);
+						}
 					}
 				}
 			}
@@ -166,7 +177,7 @@ struct ImgSrc {
 
 __global__ void __launch_bounds__(SM_BS)
 k_smooth(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, int nz,
-         uint32_t *__restrict__ range /* [0]=ord min, [1]=ord max */, float iso,
+         uint32_t *__restrict__ range /* [0]=ord min, [1]=ord max */, IsoMasks iso,
          uint32_t *__restrict__ bits, int W)
 {
 	const int x0 = blockIdx.x * SM_TX, y0 = blockIdx.y * SM_TY;
@@ -283,7 +294,8 @@ k_atlas_smooth(const float *__restrict__ labels, int gnx, int gny, const AtlasTa
 		const int tx = (t.nx + TX - 1) / TX, ty = (t.ny + TY - 1) / TY;
 		const int bx = local % tx, by = (local / tx) % ty, bz = local / (tx * ty);
 		const int z0 = bz * ZC, z1 = min(z0 + ZC, t.nz);
-		smooth_tile<TX, TY>(src, out, t.nx, t.ny, t.nz, bx * TX, by * TY, z0, z1, 0.f, nullptr, 0, lmin, lmax);
+		smooth_tile<TX, TY>(src, out, t.nx, t.ny, t.nz, bx * TX, by * TY, z0, z1, IsoMasks{}, nullptr, 0, lmin,
+		                    lmax);
 	} else {
 		const size_t n = (size_t)t.nx * t.ny * t.nz;
 		const size_t v = (size_t)local * SM_BS + threadIdx.x;
@@ -415,12 +427,17 @@ void launch_atlas_apply(float *crops, int gnx, int gny, int gnz, const AtlasTask
 		                   bbox, vmin);
 }
 
-void launch_smooth(const float *in, float *out, int nx, int ny, int nz, uint32_t *range, float iso,
-                   uint32_t *bits, hipStream_t s)
+void launch_smooth(const float *in, float *out, int nx, int ny, int nz, uint32_t *range, const float *isos,
+                   int nmask, uint32_t *bits, hipStream_t s)
 {
 	dim3 grid((nx + SM_TX - 1) / SM_TX, (ny + SM_TY - 1) / SM_TY, (nz + SM_ZC - 1) / SM_ZC);
 	int W = (nx + 31) / 32;
-	hipLaunchKernelGGL(k_smooth, grid, dim3(SM_BS), 0, s, in, out, nx, ny, nz, range, iso, bits, W);
+	IsoMasks m{};
+	m.n = bits ? (nmask < N2M_SMOOTH_MASKS ? nmask : N2M_SMOOTH_MASKS) : 0;
+	for (int k = 0; k < m.n; k++)
+		m.v[k] = isos[k];
+	m.nwords = (size_t)W * ny * nz;
+	hipLaunchKernelGGL(k_smooth, grid, dim3(SM_BS), 0, s, in, out, nx, ny, nz, range, m, bits, W);
 }
 
 // ---------------------------------------------------------------------------
@@ -532,7 +549,60 @@ k_mask_bits4(const float *__restrict__ img, int nx, int ny, int nz, int W, float
 	}
 }
 
+// The masks of several isolevels in one read of the volume (same layout
+// and conditions as k_mask_bits4): mask k to bits + k * iso.nwords.
+__global__ void __launch_bounds__(256)
+k_mask_bits4_multi(const float *__restrict__ img, int nx, int ny, int nz, int W, IsoMasks iso,
+                   uint32_t *__restrict__ bits)
+{
+	const size_t rows = (size_t)ny * nz;
+	const int lpr = nx >> 2, span = lpr < (int)blockDim.x ? lpr : (int)blockDim.x;
+	const int rpi = (int)blockDim.x / span, sub = threadIdx.x / span, q0 = threadIdx.x % span;
+	for (size_t row0 = (size_t)blockIdx.x * rpi; row0 < rows; row0 += (size_t)gridDim.x * rpi) {
+		const size_t row = row0 + sub;
+		const bool live = sub < rpi && row < rows;
+		for (int q = q0; q < lpr; q += span) { // the same trip count on every lane
+			float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+			if (live)
+				v = reinterpret_cast<const float4 *>(img + row * nx)[q];
+			for (int k = 0; k < iso.n; k++) {
+				const float t = iso.v[k];
+				uint32_t nib = 0;
+				if (live)
+					nib = (v.x >= t) | (v.y >= t) << 1 | (v.z >= t) << 2 | (v.w >= t) << 3;
+				uint32_t wv = nib << (4 * (q & 7));
+				wv |= __shfl_xor(wv, 1, N2M_WAVE);
+				wv |= __shfl_xor(wv, 2, N2M_WAVE);
+				wv |= __shfl_xor(wv, 4, N2M_WAVE);
+				if (live && (q & 7) == 0)
+					bits[k * iso.nwords + (size_t)W * row + (q >> 3)] = wv;
+			}
+		}
+	}
+}
+
 static unsigned row_blocks(size_t rows) { return (unsigned)(rows < 8192 ? (rows ? rows : 1) : 8192); }
+
+void launch_mask_bits_multi(const float *img, int nx, int ny, int nz, const float *isos, int n, uint32_t *bits,
+                            hipStream_t s)
+{
+	const int W = (nx + 31) / 32, lpr = nx / 4;
+	const size_t nwords = (size_t)W * ny * nz;
+	if (n <= N2M_SMOOTH_MASKS && nx % 32 == 0 && (reinterpret_cast<uintptr_t>(img) & 15u) == 0 &&
+	    (lpr <= 256 || lpr % 256 == 0)) {
+		IsoMasks m{};
+		m.n = n;
+		for (int k = 0; k < n; k++)
+			m.v[k] = isos[k];
+		m.nwords = nwords;
+		const size_t rows = (size_t)ny * nz, rpi = lpr < 256 ? 256 / lpr : 1;
+		hipLaunchKernelGGL(k_mask_bits4_multi, dim3(row_blocks((rows + rpi - 1) / rpi)), dim3(256), 0, s, img, nx,
+		                   ny, nz, W, m, bits);
+		return;
+	}
+	for (int k = 0; k < n; k++)
+		launch_mask_bits(img, nx, ny, nz, isos[k], 0, bits + k * nwords, s);
+}
 
 void launch_mask_bits(const float *img, int nx, int ny, int nz, float iso, int nonzero, uint32_t *bits,
                       hipStream_t s)
@@ -794,8 +864,8 @@ __global__ void __launch_bounds__(256) k_hist_range(const float *__restrict__ im
 
 // bin = round((v - mn) * scl) in float, clamped to [0, nbins-1] (the
 // 1001-bin pass never leaves that range for finite data).
-__global__ void __launch_bounds__(256)
-k_hist_bins(const float *__restrict__ img, size_t n, float mn, float scl, int nbins, uint32_t *hist)
+__device__ __forceinline__ void hist_bins_body(const float *__restrict__ img, size_t n, float mn, float scl, int nbins,
+                                               uint32_t *hist)
 {
 	extern __shared__ uint32_t s_h[];
 	for (int i = threadIdx.x; i < nbins; i += blockDim.x)
@@ -826,6 +896,23 @@ k_hist_bins(const float *__restrict__ img, size_t n, float mn, float scl, int nb
 	for (int i = threadIdx.x; i < nbins; i += blockDim.x)
 		if (s_h[i])
 			atomicAdd(&hist[i], s_h[i]);
+}
+
+__global__ void __launch_bounds__(256)
+k_hist_bins(const float *__restrict__ img, size_t n, float mn, float scl, int nbins, uint32_t *hist)
+{
+	hist_bins_body(img, n, mn, scl, nbins, hist);
+}
+
+// the same with mn = par[0], scl = par[1] from the device; nothing when
+// *enable is 0
+__global__ void __launch_bounds__(256)
+k_hist_bins_dev(const float *__restrict__ img, size_t n, const float *__restrict__ par,
+                const int32_t *__restrict__ enable, int nbins, uint32_t *hist)
+{
+	if (enable && *enable == 0)
+		return;
+	hist_bins_body(img, n, par[0], par[1], nbins, hist);
 }
 
 static unsigned hist_blocks(size_t n)
@@ -943,39 +1030,44 @@ void launch_otsu3(const double *P, int nBin, double *bv, int *bm, int *bh, hipSt
 }
 
 // The Liao table of nii_otsu (src/isolevel.c:147-175) for a 256-bin
-// histogram, built on the device: thread 0 forms the running sums in the
-// reference's order (P[0] is the raw count H[0], a quirk the thresholds
-// depend on), then thread u fills row u: P[u][v] = (S*S)/P of the bin range
-// [u, v] (left as is when P is 0); entries with v < u are never read and
-// are set to 0.
+// histogram, built on the device, one workgroup per row u.  Each workgroup
+// forms the running sums in the reference's order (the bin probabilities in
+// parallel, the sums by one thread; P[0] is the raw count H[0], a quirk the
+// thresholds depend on), then thread v writes P[u][v] = (S*S)/P of the bin
+// range [u, v] (left as is when P is 0); entries with v < u are never read
+// and are set to 0.
 __global__ void __launch_bounds__(256) k_otsu_table(const uint32_t *__restrict__ H, double *__restrict__ P)
 {
 	constexpr int n = 256;
-	__shared__ double s_p[n], s_s[n];
+	__shared__ double s_p[n], s_s[n], s_prob[n];
+	__shared__ double s_sum;
 	if (threadIdx.x == 0) {
 		double Sum = 0.0;
 		for (int v = 0; v < n; v++)
 			Sum = Sum + (double)(int)H[v];
+		s_sum = Sum;
+	}
+	__syncthreads();
+	s_prob[threadIdx.x] = (double)(int)H[threadIdx.x] / s_sum;
+	__syncthreads();
+	if (threadIdx.x == 0) {
 		s_p[0] = (double)(int)H[0];
 		s_s[0] = (double)(int)H[0];
 		for (int v = 1; v < n; v++) {
-			const double prob = (double)(int)H[v] / Sum;
+			const double prob = s_prob[v];
 			s_p[v] = s_p[v - 1] + prob;
 			s_s[v] = s_s[v - 1] + (v + 1) * prob;
 		}
 	}
 	__syncthreads();
-	const int u = threadIdx.x;
-	double *row = P + (size_t)u * n;
-	for (int v = 0; v < n; v++) {
-		if (v < u) {
-			row[v] = 0.0;
-			continue;
-		}
+	const int u = blockIdx.x, v = threadIdx.x;
+	double val = 0.0;
+	if (v >= u) {
 		const double p = u == 0 ? s_p[v] : s_p[v] - s_p[u - 1];
 		const double q = u == 0 ? s_s[v] : s_s[v] - s_s[u - 1];
-		row[v] = p != 0 ? (q * q) / p : p;
+		val = p != 0 ? (q * q) / p : p;
 	}
+	P[(size_t)u * n + v] = val;
 }
 
 // the table entries the host-side searches read: row 0 (P[i]), column 255
@@ -995,30 +1087,86 @@ __global__ void __launch_bounds__(256) k_otsu_probe(const double *__restrict__ P
 	}
 }
 
-// range statistics {ordered min, ordered max, NaN count} and both
-// histograms cleared in one launch (no memset nodes)
-__global__ void __launch_bounds__(256)
-k_threshold_init(uint32_t *__restrict__ stats, uint32_t *__restrict__ h1001, uint32_t *__restrict__ h256)
+// ---------------------------------------------------------------------------
+// setThreshold() as one chain of launches (src/isolevel.c:35-277), results
+// in a ThrRecord.
+// ---------------------------------------------------------------------------
+// range statistics {ordered min, ordered max, NaN count}, the parameters and
+// both histograms cleared in one launch (no memset nodes)
+__global__ void __launch_bounds__(256) k_thr_init(ThrRecord *__restrict__ r, uint32_t *__restrict__ h1001)
 {
 	for (int i = threadIdx.x; i < 1024; i += blockDim.x)
 		h1001[i] = 0u;
-	h256[threadIdx.x] = 0u;
-	if (threadIdx.x < 4)
-		stats[threadIdx.x] = threadIdx.x == 0 ? 0xffffffffu : 0u;
+	r->h256[threadIdx.x] = 0u;
+	if (threadIdx.x < 4) {
+		r->stats[threadIdx.x] = threadIdx.x == 0 ? 0xffffffffu : 0u;
+		r->par[threadIdx.x] = 0.0f;
+	}
+	if (threadIdx.x == 0)
+		r->do1001 = 0;
 }
 
-void launch_otsu_table(const uint32_t *hist, double *P, hipStream_t s)
+// One thread: the cases of the robust range (src/isolevel.c:59-139).  First
+// call (final = 0): whether the 1001-bin pass runs, with its mn and scl.
+// Second call: the percentiles (the walk over that histogram when it ran)
+// and the 256-bin pass's mn and scl (src/isolevel.c:245-250).  All float
+// and double arithmetic in the host code's order.
+__global__ void k_thr_range(ThrRecord *__restrict__ r, const uint32_t *__restrict__ h1001, size_t nvox, int final)
 {
-	hipLaunchKernelGGL(k_otsu_table, dim3(1), dim3(256), 0, s, hist, P);
+	const size_t nNan = r->stats[2];
+	const float mn = ord2f(r->stats[0]), mx = ord2f(r->stats[1]);
+	float pct2 = 0.0f, pct98 = 1.0f;
+	if (nvox == nNan) {
+		// all NaN: the range keeps its 0..1 defaults
+	} else if (mn == mx) {
+		pct2 = mn;
+		pct98 = mx;
+	} else {
+		const size_t valid = nvox - nNan;
+		const size_t n2pct = (size_t)round(valid * 0.02);
+		if (n2pct < 1 || valid < 100) {
+			pct2 = mn;
+			pct98 = mx;
+		} else {
+			const int kBins = 1001;
+			const float scl = (kBins - 1) / (mx - mn);
+			if (!final) {
+				r->do1001 = 1;
+				r->par[0] = mn;
+				r->par[1] = scl;
+				return;
+			}
+			int lo, hi;
+			robust_walk(h1001, kBins, n2pct, &lo, &hi);
+			pct2 = (lo) / scl + mn;
+			pct98 = (hi) / scl + mn;
+		}
+	}
+	if (final) {
+		const int nBin = 256;
+		r->par[2] = pct2;
+		r->par[3] = (nBin - 1) / (pct98 - pct2);
+	}
 }
-void launch_otsu_probe(const double *P, double *probe, hipStream_t s)
+
+void launch_threshold_chain(const float *img, size_t nvox, ThrRecord *rec, uint32_t *h1001, double *P, bool need5,
+                            hipStream_t s)
 {
-	hipLaunchKernelGGL(k_otsu_probe, dim3(1), dim3(256), 0, s, P, probe);
+	const unsigned hb = hist_blocks(nvox);
+	hipLaunchKernelGGL(k_thr_init, dim3(1), dim3(256), 0, s, rec, h1001);
+	hipLaunchKernelGGL(k_hist_range, dim3(hb), dim3(256), 0, s, img, nvox, rec->stats);
+	hipLaunchKernelGGL(k_thr_range, dim3(1), dim3(1), 0, s, rec, (const uint32_t *)h1001, nvox, 0);
+	hipLaunchKernelGGL(k_hist_bins_dev, dim3(hb), dim3(256), 1001 * sizeof(uint32_t), s, img, nvox,
+	                   (const float *)rec->par, (const int32_t *)&rec->do1001, 1001, h1001);
+	hipLaunchKernelGGL(k_thr_range, dim3(1), dim3(1), 0, s, rec, (const uint32_t *)h1001, nvox, 1);
+	hipLaunchKernelGGL(k_hist_bins_dev, dim3(hb), dim3(256), 256 * sizeof(uint32_t), s, img, nvox,
+	                   (const float *)(rec->par + 2), (const int32_t *)nullptr, 256, rec->h256);
+	hipLaunchKernelGGL(k_otsu_table, dim3(256), dim3(256), 0, s, (const uint32_t *)rec->h256, P);
+	hipLaunchKernelGGL(k_otsu_probe, dim3(1), dim3(256), 0, s, (const double *)P, rec->probe);
+	if (need5)
+		launch_otsu3(P, 256, rec->bv, rec->bm, rec->bh, s);
 }
-void launch_threshold_init(uint32_t *stats, uint32_t *hist1001, uint32_t *hist256, hipStream_t s)
-{
-	hipLaunchKernelGGL(k_threshold_init, dim3(1), dim3(256), 0, s, stats, hist1001, hist256);
-}
+
 
 // ---------------------------------------------------------------------------
 // Raw NIfTI voxels to the FP32 working volume, src/nii2mesh.c:155-172:

@@ -55,8 +55,16 @@ struct CclTimer {
 };
 
 // k_volume.hip
-void launch_smooth(const float *in, float *out, int nx, int ny, int nz, uint32_t *range, float iso,
-                   uint32_t *bits, hipStream_t s);
+// pre-smooth with the range scan; when bits != null it also writes the
+// threshold masks (out >= isos[k]) of the first min(nmask,
+// N2M_SMOOTH_MASKS) isolevels to bits + k * ceil(nx/32) * ny * nz
+constexpr int N2M_SMOOTH_MASKS = 4;
+// threshold masks (img >= isos[k]) of n isolevels to bits + k * ceil(nx/32) *
+// ny * nz, one read of the volume when n <= N2M_SMOOTH_MASKS
+void launch_mask_bits_multi(const float *img, int nx, int ny, int nz, const float *isos, int n, uint32_t *bits,
+                            hipStream_t s);
+void launch_smooth(const float *in, float *out, int nx, int ny, int nz, uint32_t *range, const float *isos,
+                   int nmask, uint32_t *bits, hipStream_t s);
 void launch_minmax(const float *img, size_t n, uint32_t *range, hipStream_t s);
 void launch_mask_bits(const float *img, int nx, int ny, int nz, float iso, int nonzero, uint32_t *bits,
                       hipStream_t s);
@@ -100,8 +108,9 @@ void launch_ccl_argmax(const uint32_t *rootlist, const uint32_t *nroots, const u
                        unsigned long long *best, hipStream_t s);
 void launch_select_root(const uint32_t *bits, int nx, int ny, int nz, const uint32_t *L,
                         const unsigned long long *best, uint32_t *out, const uint32_t *rbase, hipStream_t s);
+// ninv: device word holding the background component count (no fill at <= 1)
 void launch_fill_bubbles(uint32_t *bits, int nx, int ny, int nz, const uint32_t *L, const uint32_t *AUX,
-                         const uint32_t *rbase, hipStream_t s);
+                         const uint32_t *rbase, const uint32_t *ninv, hipStream_t s);
 void launch_write_labels(const uint32_t *bits, int nx, int ny, int nz, const uint32_t *L, const uint32_t *rank,
                          float *img, const uint32_t *rbase, hipStream_t s);
 void launch_scatter_rank(const uint32_t *sorted, uint32_t n, uint32_t *rank, hipStream_t s);
@@ -168,6 +177,13 @@ void launch_weld_greedy(const uint32_t *starts, uint32_t nseg, uint32_t n, const
 void launch_weld_assign(const uint32_t *vals, const int *lcls, const uint32_t *flags, const uint32_t *segid,
                         const uint32_t *seedbase, uint32_t n, int *cls, int *rep, hipStream_t s);
 void launch_gather_pts(const double *pts, const int *rep, uint32_t n, double *out, hipStream_t s);
+// out[cls[v]] = pts[v] for each class representative v; wpos[v] = the
+// welded position of every pre-weld vertex v
+void launch_weld_place(const double *pts, const int *cls, const int *rep, uint32_t npt, double *out, double *wpos,
+                       hipStream_t s);
+// remap (tris_in null: implicit 3t+k) and degenerate flags in one pass
+void launch_remap_degen(const int *tris_in, const int *cls, uint32_t npt, const double *wpos, uint32_t nt,
+                        int *tris_out, uint32_t *keep, hipStream_t s);
 void launch_iota(int *out, uint32_t n, hipStream_t s);
 void launch_remap_tris(const int *tris_in, const int *cls, uint32_t nt, uint32_t npt, int *tris_out,
                        hipStream_t s);
@@ -262,10 +278,26 @@ void launch_gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t n, uin
 void launch_add_i32(const int *src, size_t n, int add, int *dst, hipStream_t s);
 void launch_sform(double *p, uint32_t np, const float *sx, const float *sy, const float *sz, hipStream_t s);
 void launch_swap_winding(int *t, uint32_t nt, hipStream_t s);
-// setThreshold's histograms and Otsu (k_volume.hip)
-void launch_otsu_table(const uint32_t *hist, double *P, hipStream_t s);
-void launch_otsu_probe(const double *P, double *probe, hipStream_t s);
-void launch_threshold_init(uint32_t *stats, uint32_t *hist1001, uint32_t *hist256, hipStream_t s);
+// setThreshold() on the device (k_volume.hip): range, percentile walk, the
+// two histograms, the Liao table and the Otsu searches run as one chain of
+// launches, whose results land in one record for a single copy to the host.
+struct ThrRecord {
+	uint32_t stats[4]; // ordered min, ordered max, NaN count, 0
+	float par[4];      // mn, scl of the 1001-bin pass; mn, scl of the 256-bin pass
+	int32_t do1001;    // the 1001-bin pass ran
+	int32_t pad[3];
+	uint32_t h256[256];
+	double probe[2 * 256 + 4]; // P[i], P[i+1][255], the four mode-5 start terms
+	double bv[256];            // mode-5 search: best value, m and h per first threshold l
+	int32_t bm[256], bh[256];
+};
+// rec: device record; h1001: 1024 words; P: 256 x 256 doubles.  need5: run
+// the three-threshold search (modes 1 and 3).  Uses no host round trip.
+void launch_threshold_chain(const float *img, size_t nvox, ThrRecord *rec, uint32_t *h1001, double *P, bool need5,
+                            hipStream_t s);
+// the isolevels of setThreshold(img, nvox, modes[i]) from the chain's record
+// (src/isolevel.c:141-277): mode 1 dark, 3 bright, anything else medium
+void thresholds_from_record(const ThrRecord &r, int n, const int *modes, float *out);
 
 // quadric simplification (k_quadric.hip, quadric.cpp): the first update
 // pass of quadric_simplify_mesh on the GPU
@@ -286,8 +318,42 @@ struct QuadricPrep {
 };
 int quadric_prep(const vec3d *p, const vec3i *t, int nv, int nt, QuadricPrep &g);
 
+// the 2 % / 98 % bins of the robust range (src/isolevel.c:102-134): lo is
+// the bin where the count from below reaches n2pct, hi likewise from above;
+// when they meet, both move outwards to the next non-empty bins.  Shared by
+// the host and the device chain.
+__host__ __device__ inline void robust_walk(const uint32_t *h, int kBins, size_t n2pct, int *plo, int *phi)
+{
+	size_t n = 0, lo = 0;
+	while (n < n2pct)
+		n += h[lo++];
+	lo--;
+	n = 0;
+	int hi = kBins;
+	while (n < n2pct)
+		n += h[--hi];
+	if ((int)lo == hi) { // src/isolevel.c:118-134
+		int ok = -1;
+		while (ok != 0) {
+			if (lo > 0) {
+				lo--;
+				if (h[lo] > 0)
+					ok = 0;
+			}
+			if (ok != 0 && hi < kBins - 1) {
+				hi++;
+				if (h[hi] > 0)
+					ok = 0;
+			}
+			if (lo == 0 && hi == kBins - 1)
+				ok = 0;
+		}
+	}
+	*plo = (int)lo;
+	*phi = hi;
+}
+
 // host helpers of api.cpp shared with noalloc.hip
-void robust_walk(const uint32_t *h, int kBins, size_t n2pct, int *lo, int *hi);
 void set_error(const char *msg); // n2m_last_error() text of this thread
 void clear_error();
 
