@@ -719,13 +719,13 @@ bool marching_cubes(n2m_ctx *C, const float *vol, const int dim[3], const int lo
 	size_t sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
 	size_t npts = sx * sy * sz;
 	size_t nseg = lw_nseg_total(NX, NY, NZ, lo, hi);
-	uint8_t *pf = C->buf<uint8_t>(S_PFLAGS, npts);
+	unsigned long long *fm = C->buf<unsigned long long>(S_PFLAGS, lw_fmask_words(NX, NY, NZ, lo, hi));
 	uint16_t *cr = C->buf<uint16_t>(S_CROW, npts);
 	uint32_t *cnt = C->buf<uint32_t>(S_SEGCNT, 3 * nseg);
 	uint32_t *base = C->buf<uint32_t>(S_SEGBASE, 3 * nseg);
 	float mnv = vmin - iso;
-	C->tbegin("mc_lewiner_classify", 4.0 * npts + 3.0 * npts);
-	launch_lw_classify(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, pf, cr, cnt, s);
+	C->tbegin("mc_lewiner_classify", 4.0 * npts + 2.0 * npts + 3.0 * npts / 8);
+	launch_lw_classify(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, nullptr, cr, cnt, s, fm);
 	C->tend();
 	uint32_t tot[3];
 	scan3(C, cnt, base, nseg, tot);
@@ -735,8 +735,8 @@ bool marching_cubes(n2m_ctx *C, const float *vol, const int dim[3], const int lo
 		return false;
 	double *pts = C->buf<double>(S_PTS0, (size_t)3 * V);
 	int *tris = C->buf<int>(S_TRIS0, (size_t)3 * nT);
-	C->tbegin("mc_lewiner_emit", 3.0 * npts + 24.0 * V + 12.0 * nT);
-	launch_lw_emit(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, pf, cr, cnt, base, nE, pts, tris, origin, s);
+	C->tbegin("mc_lewiner_emit", 2.0 * npts + 4 * 3.0 * npts / 8 + 24.0 * V + 12.0 * nT);
+	launch_lw_emit(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, fm, cr, cnt, base, nE, pts, tris, origin, s);
 	C->tend();
 	o.npt = V;
 	o.ntri = nT;
@@ -802,8 +802,9 @@ uint32_t weld(n2m_ctx *C, const double *pts, uint32_t npt, const int *tris_in /*
 	pts_out = C->buf<double>(S_PTS1, (size_t)3 * nnew);
 	if (wpos) {
 		double *wp = C->buf<double>(S_WPOS, (size_t)3 * npt);
-		C->tbegin("weld_place", 8.0 * npt + 24.0 * nnew + 24.0 * npt);
-		launch_weld_place(pts, cls, rep, npt, pts_out, wp, s);
+		C->tbegin("weld_gather", 28.0 * nnew + 24.0 * nnew + 8.0 * npt + 24.0 * npt + 24.0 * npt);
+		launch_gather_pts(pts, rep, nnew, pts_out, s);
+		launch_weld_wpos(pts, cls, rep, npt, wp, s);
 		C->tend();
 		*wpos = wp;
 	} else {

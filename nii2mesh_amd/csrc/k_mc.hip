@@ -259,12 +259,14 @@ __device__ bool lw_interior(const float *c, int cs, int cf, int sub, int s)
 	}
 }
 
-// process_cube() decision tree, src/MarchingCubes.c:458-795 -> tiling row or -1
-__device__ int lw_select(const float c[8], int lut, int originalMC)
+// process_cube() decision tree, src/MarchingCubes.c:458-795 -> tiling row or -1.
+// cs_tab / cf_tab: the case and configuration tables (or a copy in LDS).
+__device__ int lw_select(const float c[8], int lut, int originalMC, const uint8_t *cs_tab = n2m_lw_case,
+                         const uint8_t *cf_tab = n2m_lw_config)
 {
 	if (originalMC)
 		return N2M_LW_CLASSIC + lut;
-	const int cs = n2m_lw_case[lut], cf = n2m_lw_config[lut];
+	const int cs = cs_tab[lut], cf = cf_tab[lut];
 	int sub = 0;
 	switch (cs) {
 	case 0:
@@ -360,10 +362,22 @@ __device__ int lw_select(const float c[8], int lut, int originalMC)
 // are staged in LDS (sub-box values, clamped as lw_data returns them); row r
 // sits in slot (r & 1) + 2 * (((r >> 1) + par) & 1), so that a z walk can
 // keep the shared plane where it is.
+// The small tables a cube's classification reads: the case and
+// configuration of each corner pattern, the triangles and centre-vertex flag
+// of each tiling row (global, or copies in LDS).
+struct LwTabs {
+	const uint8_t *cs, *cf, *ntri, *centre;
+};
+constexpr LwTabs LW_GLOBAL_TABS = {n2m_lw_case, n2m_lw_config, n2m_lw_row_ntri, n2m_lw_row_centre};
+
+// The point flags go either to pflags (one byte per point) or, when fmask is
+// given, to per-axis bit masks: fmask[(row * nchunk + chunk) * 3 + a] holds
+// bit b for the point x = 64 * chunk + b of sub-box row (y + sy * z).
 __device__ __forceinline__ void lw_classify_core(const float (*s_row)[LW_SEG + 1], int par, const LwGeo &g, int seg,
                                                  int y, int z, uint8_t *__restrict__ pflags,
                                                  uint16_t *__restrict__ crow, uint32_t *__restrict__ cnt,
-                                                 size_t cstride, uint32_t *scratch)
+                                                 size_t cstride, uint32_t *scratch, const LwTabs &tab = LW_GLOBAL_TABS,
+                                                 unsigned long long *__restrict__ fmask = nullptr)
 {
 	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
 	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
@@ -384,7 +398,21 @@ __device__ __forceinline__ void lw_classify_core(const float (*s_row)[LW_SEG + 1
 			bool cross = c0 < 0.f ? cn[a] > 0.f : cn[a] < 0.f;
 			f |= (uint32_t)cross << a;
 		}
-		pflags[rowbase + x] = (uint8_t)f;
+		if (fmask) {
+			// one wave holds one 64-point chunk (xs is a multiple of 64 and
+			// a lane past the row end is inactive); its first lane writes
+			const unsigned long long mx = __ballot(f & 1u), my = __ballot(f & 2u), mz = __ballot(f & 4u);
+			if (lane_id() == 0) {
+				const int nchunk = (g.sx + N2M_WAVE - 1) / N2M_WAVE;
+				unsigned long long *o =
+				    fmask + ((size_t)(y + (size_t)g.sy * z) * nchunk + (size_t)(x / N2M_WAVE)) * 3;
+				o[0] = mx;
+				o[1] = my;
+				o[2] = mz;
+			}
+		} else {
+			pflags[rowbase + x] = (uint8_t)f;
+		}
 		nv += __popc(f);
 		uint16_t cr = 0;
 		if (cuberow && x < g.sx - 1) {
@@ -396,11 +424,13 @@ __device__ __forceinline__ void lw_classify_core(const float (*s_row)[LW_SEG + 1
 				if (c[p] > 0.f)
 					lut += 1 << p;
 			}
-			int row = lw_select(c, lut, g.originalMC);
+			// all corners on one side: no triangle (case 0, or a classic row
+			// without triangles for -o 1), the table walk is skipped
+			const int row = (lut == 0 || lut == 255) ? -1 : lw_select(c, lut, g.originalMC, tab.cs, tab.cf);
 			if (row >= 0) {
 				cr = (uint16_t)(row + 1);
-				nt += n2m_lw_row_ntri[row];
-				nc += (!g.originalMC && n2m_lw_row_centre[row]) ? 1u : 0u;
+				nt += tab.ntri[row];
+				nc += (!g.originalMC && tab.centre[row]) ? 1u : 0u;
 			}
 		}
 		crow[rowbase + x] = cr;
@@ -470,10 +500,21 @@ constexpr int LW_ZC = 8;
 // steps ahead into registers (80 registers, 6 waves) was 13 % slower.
 __global__ void __launch_bounds__(LW_BS, 7)
 k_lw_classify(const float *__restrict__ img, LwGeo g, uint8_t *__restrict__ pflags,
-              uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt /* 3 arrays of nseg*sy*sz */)
+              uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt /* 3 arrays of nseg*sy*sz */,
+              unsigned long long *__restrict__ fmask)
 {
 	__shared__ uint32_t scratch[3 * (LW_BS / N2M_WAVE)];
 	__shared__ float s_row[4][LW_SEG + 1];
+	// the tables a surface cube reads, in LDS (global reads of them, in the
+	// loop over points, were a dependent memory round trip per cube)
+	__shared__ uint8_t s_cs[256], s_cf[256], s_ntri[N2M_LW_NROWS], s_centre[N2M_LW_NROWS];
+	s_cs[threadIdx.x] = n2m_lw_case[threadIdx.x];
+	s_cf[threadIdx.x] = n2m_lw_config[threadIdx.x];
+	for (int i = threadIdx.x; i < N2M_LW_NROWS; i += LW_BS) {
+		s_ntri[i] = n2m_lw_row_ntri[i];
+		s_centre[i] = n2m_lw_row_centre[i];
+	}
+	const LwTabs tab = {s_cs, s_cf, s_ntri, s_centre};
 	const int seg = blockIdx.x, y = blockIdx.y, z0 = blockIdx.z * LW_ZC, z1 = min(z0 + LW_ZC, g.sz);
 	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
 	lw_stage_plane(s_row, 0, img, g, seg, y, z0);
@@ -482,7 +523,7 @@ k_lw_classify(const float *__restrict__ img, LwGeo g, uint8_t *__restrict__ pfla
 		lw_stage_plane(s_row, 2 * ((1 + par) & 1), img, g, seg, y, z + 1);
 		__syncthreads();
 		const size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
-		lw_classify_core(s_row, par, g, seg, y, z, pflags, crow, segcnt + si, ns, scratch);
+		lw_classify_core(s_row, par, g, seg, y, z, pflags, crow, segcnt + si, ns, scratch, tab, fmask);
 		__syncthreads(); // the slots of plane z are overwritten next
 	}
 }
@@ -738,40 +779,47 @@ __device__ __forceinline__ void lw_emit_row(const float *__restrict__ img, const
 	}
 }
 
-__global__ void __launch_bounds__(LW_BS)
-k_lw_emit(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
-          const uint16_t *__restrict__ crow, const uint32_t *__restrict__ segcnt, const uint32_t *__restrict__ bases,
-          uint32_t nedge, double *__restrict__ pts, int *__restrict__ tris)
+// The whole-volume pass B with one wave per row segment, from the per-axis
+// flag masks of pass A: the output of lw_emit_row (the same vertex numbers,
+// the same triangles in the same order).  A point's vertex number is its
+// row's base plus the edge vertices before it: those of the chunks before
+// (a running sum of mask popcounts) and those of its chunk below it
+// (popcounts of the masks under its bit), so no per-point prefix pass is
+// needed; the four point rows' masks (27 words each) are read into LDS with
+// two loads per lane.
+static_assert(LW_SEG % N2M_WAVE == 0, "segments are whole chunks");
+constexpr int LW_SCH = LW_SEG / N2M_WAVE; // chunks per segment
+
+struct LwMaskRows {
+	unsigned long long m[4][LW_SCH + 1][3]; // rows (dy + 2 dz) x chunks (the last: the next segment's first) x axes
+	uint16_t cb[4][LW_SCH + 1];             // edge vertices of the row segment before each chunk
+};
+
+// number (from the row's base) of the a-edge vertex of point i of row r,
+// or 0xffffffff when that edge has none
+__device__ __forceinline__ uint32_t lw_mask_vid(const LwMaskRows &M, int r, int i, int a)
 {
-	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
-	const size_t si = (size_t)blockIdx.x + (size_t)g.nseg * ((size_t)blockIdx.y + (size_t)g.sy * blockIdx.z);
-	lw_emit_row(img, g, blockIdx.x, blockIdx.y, blockIdx.z, pflags, crow, bases, ns, 0u, nedge, pts, tris, segcnt + si,
-	            ns);
+	const int c = i >> 6, b = i & 63;
+	const unsigned long long m0 = M.m[r][c][0], m1 = M.m[r][c][1], m2 = M.m[r][c][2];
+	const unsigned long long ma = a == 0 ? m0 : (a == 1 ? m1 : m2);
+	if (!((ma >> b) & 1ull))
+		return 0xffffffffu;
+	const unsigned long long lo = (1ull << b) - 1ull;
+	uint32_t id = M.cb[r][c] + (uint32_t)(__popcll(m0 & lo) + __popcll(m1 & lo) + __popcll(m2 & lo));
+	if (a > 0)
+		id += (uint32_t)((m0 >> b) & 1ull);
+	if (a > 1)
+		id += (uint32_t)((m1 >> b) & 1ull);
+	return id;
 }
 
-// The whole-volume pass B with one wave per row segment: the output of
-// lw_emit_row (the same vertex numbers, the same triangles in the same
-// order), but the popcount prefixes and the per-chunk scans are wave scans,
-// so no workgroup barrier separates the steps.  A noisy 512-point row holds
-// a few dozen vertices: the four-wave form spent its time in barriers and
-// dependent loads, and a wave per row keeps four times as many rows in
-// flight.  The prefixes are 16-bit (a segment has at most 3 * LW_SEG edge
-// vertices).  Memory latency, not bandwidth, bounds this pass, so every
-// load of a step is issued before the first is used: the flags of the four
-// point rows together, the tiling rows of all cube chunks together, and the
-// owned row's edge vertices one per lane (from a list built with the
-// prefixes) instead of one point per lane.
-static_assert(3 * LW_SEG < 65536, "16-bit prefixes");
 __global__ void __launch_bounds__(N2M_WAVE)
-k_lw_emit_w(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ pflags,
+k_lw_emit_m(const float *__restrict__ img, LwGeo g, const unsigned long long *__restrict__ fmask,
             const uint16_t *__restrict__ crow, const uint32_t *__restrict__ segcnt,
             const uint32_t *__restrict__ bases, uint32_t nedge, double *__restrict__ pts, int *__restrict__ tris)
 {
-	constexpr int NIT = LW_SEG / N2M_WAVE + 1; // chunks of the points [0, len]
-	constexpr int NCC = LW_SEG / N2M_WAVE;     // chunks of the cubes
-	__shared__ uint16_t s_pre[4][LW_SEG + 1];
-	__shared__ uint8_t s_flag[4][LW_SEG + 1];
-	__shared__ uint16_t s_vl[3 * LW_SEG]; // the owned row's edge vertices in id order: point << 2 | axis
+	__shared__ LwMaskRows M;
+	__shared__ uint16_t s_vl[3 * LW_SEG]; // the owned row's edge vertices in number order: point << 2 | axis
 	const int seg = blockIdx.x, y = blockIdx.y, z = blockIdx.z, lane = (int)threadIdx.x;
 	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
 	const size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
@@ -782,65 +830,64 @@ k_lw_emit_w(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ 
 	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
 	const int nrows = cuberow ? 4 : 1;
 	const int ncube = min(xe, g.sx - 1) - xs;
+	const int nchunk = (g.sx + N2M_WAVE - 1) / N2M_WAVE;
+	// the masks of the four point rows; chunks past the row end are empty
+	constexpr int NW = 4 * (LW_SCH + 1) * 3;
+	unsigned long long *mflat = &M.m[0][0][0];
+	for (int e = lane; e < NW; e += N2M_WAVE) {
+		const int r = e / ((LW_SCH + 1) * 3), c = (e / 3) % (LW_SCH + 1), a = e % 3;
+		const int ch = seg * LW_SCH + c;
+		unsigned long long v = 0ull;
+		if (r < nrows && ch < nchunk) {
+			const int yy = y + (r & 1), zz = z + (r >> 1);
+			v = fmask[((size_t)(yy + (size_t)g.sy * zz) * nchunk + ch) * 3 + a];
+		}
+		mflat[e] = v;
+	}
 	uint32_t base[4] = {0u, 0u, 0u, 0u};
-	uint32_t fl[4][NIT];
 #pragma unroll
-	for (int r = 0; r < 4; r++) {
-		const int yy = y + (r & 1), zz = z + (r >> 1);
-		const bool live = r < nrows;
-		if (live)
-			base[r] = bases[(size_t)seg + (size_t)g.nseg * ((size_t)yy + (size_t)g.sy * zz)];
-		const size_t rb = (size_t)g.sx * ((size_t)yy + (size_t)g.sy * zz);
-#pragma unroll
-		for (int c = 0; c < NIT; c++) {
-			const int i = c * N2M_WAVE + lane;
-			fl[r][c] = live && i <= len && xs + i < g.sx ? pflags[rb + xs + i] : 0u;
-		}
-	}
-	uint32_t nv0 = 0; // edge vertices of the owned row
-#pragma unroll
-	for (int r = 0; r < 4; r++) {
-		if (r >= nrows)
-			break;
-		uint32_t carry = 0;
-#pragma unroll
-		for (int c = 0; c < NIT; c++) {
-			if (c * N2M_WAVE > len)
-				break;
-			const int i = c * N2M_WAVE + lane;
-			const uint32_t f = i < len ? (uint32_t)__popc(fl[r][c]) : 0u;
-			if (i <= len)
-				s_flag[r][i] = (uint8_t)fl[r][c];
-			// a prefix is read only at a flagged point, and at len
-			if (!__ballot(f != 0u)) {
-				if (i == len)
-					s_pre[r][i] = (uint16_t)carry;
-				continue;
-			}
-			const uint32_t incl = wave_incl_scan(f);
-			const uint32_t pre = carry + incl - f;
-			if (i <= len)
-				s_pre[r][i] = (uint16_t)pre;
-			if (r == 0 && f) {
-				uint32_t at = pre;
-				for (int a = 0; a < 3; a++)
-					if ((fl[0][c] >> a) & 1u)
-						s_vl[at++] = (uint16_t)((i << 2) | a);
-			}
-			carry += __shfl(incl, N2M_WAVE - 1, N2M_WAVE);
-		}
-		if (r == 0)
-			nv0 = carry;
-	}
+	for (int r = 0; r < 4; r++)
+		if (r < nrows)
+			base[r] = bases[(size_t)seg + (size_t)g.nseg * ((size_t)(y + (r & 1)) + (size_t)g.sy * (z + (r >> 1)))];
 	__syncthreads(); // one wave: orders the LDS writes before other lanes' reads
+	// per row, the edge vertices before each chunk (only points below len
+	// are owned: the bits past len in the last chunk are zero, and the
+	// chunk after the segment is not counted)
+	if (lane < 4) {
+		uint32_t run = 0;
+		for (int c = 0; c <= LW_SCH; c++) {
+			M.cb[lane][c] = (uint16_t)run;
+			if (c < LW_SCH)
+				run += (uint32_t)(__popcll(M.m[lane][c][0]) + __popcll(M.m[lane][c][1]) + __popcll(M.m[lane][c][2]));
+		}
+	}
+	__syncthreads();
+	// the owned row's edge vertices, listed in number order
+	const uint32_t nv0 = M.cb[0][LW_SCH];
+	for (int c = 0; c < LW_SCH; c++) {
+		const unsigned long long m0 = M.m[0][c][0], m1 = M.m[0][c][1], m2 = M.m[0][c][2];
+		if (!(m0 | m1 | m2))
+			continue;
+		const uint32_t f = (uint32_t)((m0 >> lane) & 1ull) | (uint32_t)((m1 >> lane) & 1ull) << 1 |
+		                   (uint32_t)((m2 >> lane) & 1ull) << 2;
+		if (f) {
+			const unsigned long long lo = (1ull << lane) - 1ull;
+			uint32_t at = M.cb[0][c] + (uint32_t)(__popcll(m0 & lo) + __popcll(m1 & lo) + __popcll(m2 & lo));
+			const int i = c * N2M_WAVE + lane;
+			for (int a = 0; a < 3; a++)
+				if ((f >> a) & 1u)
+					s_vl[at++] = (uint16_t)((i << 2) | a);
+		}
+	}
 	// the tiling rows of all cube chunks, loaded before the first is used
 	const size_t rb0 = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
-	int rows[NCC];
+	int rows[LW_SCH];
 #pragma unroll
-	for (int c = 0; c < NCC; c++) {
+	for (int c = 0; c < LW_SCH; c++) {
 		const int i = c * N2M_WAVE + lane;
 		rows[c] = cuberow && i < ncube ? (int)crow[rb0 + xs + i] - 1 : -1;
 	}
+	__syncthreads();
 	// edge vertices of the owned points (row 0), one vertex per lane
 	for (uint32_t v = lane; v < nv0; v += N2M_WAVE) {
 		const uint32_t e = s_vl[v];
@@ -857,7 +904,7 @@ k_lw_emit_w(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ 
 	const uint32_t cbase = nedge + bases[2 * ns + si];
 	uint32_t tcarry = 0, ccarry = 0;
 #pragma unroll
-	for (int c = 0; c < NCC; c++) {
+	for (int c = 0; c < LW_SCH; c++) {
 		const int i = c * N2M_WAVE + lane;
 		const int row = rows[c];
 		if (!__ballot(row >= 0))
@@ -875,8 +922,8 @@ k_lw_emit_w(const float *__restrict__ img, LwGeo g, const uint8_t *__restrict__ 
 			for (int e = 0; e < 12; e++) {
 				const int dx = c_code[e][0], dy = c_code[e][1], dz = c_code[e][2], a = c_code[e][3];
 				const int r = dy + 2 * dz;
-				const uint32_t f = s_flag[r][i + dx];
-				ids[e] = ((f >> a) & 1u) ? base[r] + s_pre[r][i + dx] + __popc(f & ((1u << a) - 1u)) : 0xffffffffu;
+				const uint32_t k = lw_mask_vid(M, r, i + dx, a);
+				ids[e] = k == 0xffffffffu ? k : base[r] + k;
 			}
 			uint32_t cid = 0xffffffffu;
 			if (hc) { // add_c_vertex(): mean of the cube's edge vertices, in float
@@ -963,34 +1010,33 @@ size_t lw_nseg_total(int nx, int ny, int nz, const int lo[3], const int hi[3])
 
 void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                         float mnv, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
-                        hipStream_t s)
+                        hipStream_t s, unsigned long long *fmask)
 {
 	LwGeo g;
 	lw_geo(g, nx, ny, nz, lo, hi, iso, mnv, originalMC);
 	hipLaunchKernelGGL(k_lw_classify, dim3(g.nseg, g.sy, (g.sz + LW_ZC - 1) / LW_ZC), dim3(LW_BS), 0, s, img, g, pflags,
-	                   crow, segcnt);
+	                   crow, segcnt, fmask);
+}
+
+size_t lw_fmask_words(int nx, int ny, int nz, const int lo[3], const int hi[3])
+{
+	LwGeo g;
+	lw_geo(g, nx, ny, nz, lo, hi, 0.f, 0.f, 0);
+	return (size_t)g.sy * g.sz * ((g.sx + N2M_WAVE - 1) / N2M_WAVE) * 3;
 }
 
 void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
-                    float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow, const uint32_t *segcnt,
-                    const uint32_t *bases, uint32_t nedge, double *pts, int *tris, const int origin[3],
-                    hipStream_t s)
+                    float mnv, int originalMC, const unsigned long long *fmask, const uint16_t *crow,
+                    const uint32_t *segcnt, const uint32_t *bases, uint32_t nedge, double *pts, int *tris,
+                    const int origin[3], hipStream_t s)
 {
 	LwGeo g;
 	lw_geo(g, nx, ny, nz, lo, hi, iso, mnv, originalMC);
 	g.out0 = lo[0] + origin[0];
 	g.out1 = lo[1] + origin[1];
 	g.out2 = lo[2] + origin[2];
-	static const bool four_waves = [] { // N2M_LW_EMIT=4: the four-wave form
-		const char *e = getenv("N2M_LW_EMIT");
-		return e && *e == '4';
-	}();
-	if (four_waves)
-		hipLaunchKernelGGL(k_lw_emit, dim3(g.nseg, g.sy, g.sz), dim3(LW_BS), 0, s, img, g, pflags, crow, segcnt,
-		                   bases, nedge, pts, tris);
-	else
-		hipLaunchKernelGGL(k_lw_emit_w, dim3(g.nseg, g.sy, g.sz), dim3(N2M_WAVE), 0, s, img, g, pflags, crow, segcnt,
-		                   bases, nedge, pts, tris);
+	hipLaunchKernelGGL(k_lw_emit_m, dim3(g.nseg, g.sy, g.sz), dim3(N2M_WAVE), 0, s, img, g, fmask, crow, segcnt,
+	                   bases, nedge, pts, tris);
 }
 
 void launch_lw_emit_rec(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,

@@ -14,7 +14,7 @@ namespace n2m {
 // ---------------------------------------------------------------------------
 // Exclusive scan of uint32 (multi-level: reduce, scan the partials, apply).
 // ---------------------------------------------------------------------------
-constexpr int SC_BS = 256, SC_IPT = 8, SC_TILE = SC_BS * SC_IPT;
+constexpr int SC_BS = 256, SC_IPT = 16, SC_TILE = SC_BS * SC_IPT;
 
 __global__ void __launch_bounds__(SC_BS) k_scan_reduce(const uint32_t *__restrict__ in, size_t n, uint32_t *sums)
 {
@@ -415,28 +415,19 @@ __global__ void k_degen_flags(const double *__restrict__ pts, const int *__restr
 	keep[i] = k;
 }
 
-// The weld's output positions by scatter: each class's representative
-// (rep[c], its last member in index order) writes out[c]; every vertex also
-// gets its welded position in its own (pre-weld) order, wpos[v] =
-// pts[rep[cls[v]]], so that triangles can read positions in emit order.
-__global__ void k_weld_place(const double *__restrict__ pts, const int *__restrict__ cls,
-                             const int *__restrict__ rep, uint32_t npt, double *__restrict__ out,
-                             double *__restrict__ wpos)
+// The welded position of every vertex in its own (pre-weld) order, wpos[v] =
+// pts[rep[cls[v]]] (the weld's output position of v's class), so that
+// triangles can read positions in emit order.  rep[c] is mostly v itself.
+__global__ void k_weld_wpos(const double *__restrict__ pts, const int *__restrict__ cls,
+                            const int *__restrict__ rep, uint32_t npt, double *__restrict__ wpos)
 {
 	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
 	if (v >= npt)
 		return;
-	const int c = cls[v];
-	const uint32_t r = (uint32_t)rep[c];
-	const double x = pts[(size_t)r * 3], y = pts[(size_t)r * 3 + 1], z = pts[(size_t)r * 3 + 2];
-	if (r == v) {
-		out[(size_t)c * 3] = x;
-		out[(size_t)c * 3 + 1] = y;
-		out[(size_t)c * 3 + 2] = z;
-	}
-	wpos[(size_t)v * 3] = x;
-	wpos[(size_t)v * 3 + 1] = y;
-	wpos[(size_t)v * 3 + 2] = z;
+	const size_t r = (size_t)(uint32_t)rep[cls[v]];
+	wpos[(size_t)v * 3] = pts[r * 3];
+	wpos[(size_t)v * 3 + 1] = pts[r * 3 + 1];
+	wpos[(size_t)v * 3 + 2] = pts[r * 3 + 2];
 }
 
 // k_remap_tris and k_degen_flags in one pass: the welded ids of a
@@ -479,10 +470,9 @@ __global__ void k_remap_degen(const int *__restrict__ tris_in, const int *__rest
 	keep[i] = k;
 }
 
-void launch_weld_place(const double *pts, const int *cls, const int *rep, uint32_t npt, double *out, double *wpos,
-                       hipStream_t s)
+void launch_weld_wpos(const double *pts, const int *cls, const int *rep, uint32_t npt, double *wpos, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_weld_place, dim3(g1(npt)), dim3(256), 0, s, pts, cls, rep, npt, out, wpos);
+	hipLaunchKernelGGL(k_weld_wpos, dim3(g1(npt)), dim3(256), 0, s, pts, cls, rep, npt, wpos);
 }
 
 void launch_remap_degen(const int *tris_in, const int *cls, uint32_t npt, const double *wpos, uint32_t nt,

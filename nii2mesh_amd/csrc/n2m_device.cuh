@@ -12,6 +12,11 @@
 namespace n2m {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (N2M_WAVE - 1); }
+// set bits of the wave mask m at lanes below this one
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m)
+{
+	return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 __device__ __forceinline__ int wave_id() { return threadIdx.x / N2M_WAVE; }
 
 // Order-preserving map float <-> uint32 for atomic min/max on floats.

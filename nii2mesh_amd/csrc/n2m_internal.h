@@ -124,13 +124,18 @@ void launch_classic_emit(const float *img, int nx, int ny, int nz, const int lo[
                          const uint8_t *cfgs, const uint32_t *blockOffs, double *pts, const int origin[3],
                          hipStream_t s);
 size_t lw_nseg_total(int nx, int ny, int nz, const int lo[3], const int hi[3]);
+// pass A of the whole-volume Lewiner cubes: point flags to pflags (one byte
+// each), or, when fmask is given, to per-axis bit masks (lw_fmask_words
+// words); tiling rows to crow, segment counts to segcnt
 void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                         float mnv, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
-                        hipStream_t s);
+                        hipStream_t s, unsigned long long *fmask = nullptr);
+size_t lw_fmask_words(int nx, int ny, int nz, const int lo[3], const int hi[3]);
+// pass B from the flag masks
 void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
-                    float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow, const uint32_t *segcnt,
-                    const uint32_t *bases, uint32_t nedge, double *pts, int *tris, const int origin[3],
-                    hipStream_t s);
+                    float mnv, int originalMC, const unsigned long long *fmask, const uint16_t *crow,
+                    const uint32_t *segcnt, const uint32_t *bases, uint32_t nedge, double *pts, int *tris,
+                    const int origin[3], hipStream_t s);
 
 void launch_lw_emit_rec(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                         float mnv, int originalMC, const uint8_t *pflags, const uint16_t *crow,
@@ -177,10 +182,8 @@ void launch_weld_greedy(const uint32_t *starts, uint32_t nseg, uint32_t n, const
 void launch_weld_assign(const uint32_t *vals, const int *lcls, const uint32_t *flags, const uint32_t *segid,
                         const uint32_t *seedbase, uint32_t n, int *cls, int *rep, hipStream_t s);
 void launch_gather_pts(const double *pts, const int *rep, uint32_t n, double *out, hipStream_t s);
-// out[cls[v]] = pts[v] for each class representative v; wpos[v] = the
-// welded position of every pre-weld vertex v
-void launch_weld_place(const double *pts, const int *cls, const int *rep, uint32_t npt, double *out, double *wpos,
-                       hipStream_t s);
+// wpos[v] = pts[rep[cls[v]]]: the welded position of every pre-weld vertex v
+void launch_weld_wpos(const double *pts, const int *cls, const int *rep, uint32_t npt, double *wpos, hipStream_t s);
 // remap (tris_in null: implicit 3t+k) and degenerate flags in one pass
 void launch_remap_degen(const int *tris_in, const int *cls, uint32_t npt, const double *wpos, uint32_t nt,
                         int *tris_out, uint32_t *keep, hipStream_t s);
