@@ -365,14 +365,17 @@ __device__ int lw_select(const float c[8], int lut, int originalMC, const uint8_
 // The small tables a cube's classification reads: the case and
 // configuration of each corner pattern, the triangles and centre-vertex flag
 // of each tiling row (global, or copies in LDS).
+constexpr int LW_FMW = 4; // mask words per 64-point chunk: x, y, z edges, cubes with triangles
+
 struct LwTabs {
 	const uint8_t *cs, *cf, *ntri, *centre;
 };
 constexpr LwTabs LW_GLOBAL_TABS = {n2m_lw_case, n2m_lw_config, n2m_lw_row_ntri, n2m_lw_row_centre};
 
 // The point flags go either to pflags (one byte per point) or, when fmask is
-// given, to per-axis bit masks: fmask[(row * nchunk + chunk) * 3 + a] holds
-// bit b for the point x = 64 * chunk + b of sub-box row (y + sy * z).
+// given, to per-axis bit masks: fmask[(row * nchunk + chunk) * 4 + a] holds
+// bit b for the point x = 64 * chunk + b of sub-box row (y + sy * z), and
+// word 3 the cubes with a tiling row; crow is then written only for those.
 __device__ __forceinline__ void lw_classify_core(const float (*s_row)[LW_SEG + 1], int par, const LwGeo &g, int seg,
                                                  int y, int z, uint8_t *__restrict__ pflags,
                                                  uint16_t *__restrict__ crow, uint32_t *__restrict__ cnt,
@@ -398,21 +401,8 @@ __device__ __forceinline__ void lw_classify_core(const float (*s_row)[LW_SEG + 1
 			bool cross = c0 < 0.f ? cn[a] > 0.f : cn[a] < 0.f;
 			f |= (uint32_t)cross << a;
 		}
-		if (fmask) {
-			// one wave holds one 64-point chunk (xs is a multiple of 64 and
-			// a lane past the row end is inactive); its first lane writes
-			const unsigned long long mx = __ballot(f & 1u), my = __ballot(f & 2u), mz = __ballot(f & 4u);
-			if (lane_id() == 0) {
-				const int nchunk = (g.sx + N2M_WAVE - 1) / N2M_WAVE;
-				unsigned long long *o =
-				    fmask + ((size_t)(y + (size_t)g.sy * z) * nchunk + (size_t)(x / N2M_WAVE)) * 3;
-				o[0] = mx;
-				o[1] = my;
-				o[2] = mz;
-			}
-		} else {
+		if (!fmask)
 			pflags[rowbase + x] = (uint8_t)f;
-		}
 		nv += __popc(f);
 		uint16_t cr = 0;
 		if (cuberow && x < g.sx - 1) {
@@ -433,7 +423,25 @@ __device__ __forceinline__ void lw_classify_core(const float (*s_row)[LW_SEG + 1
 				nc += (!g.originalMC && tab.centre[row]) ? 1u : 0u;
 			}
 		}
-		crow[rowbase + x] = cr;
+		if (fmask) {
+			// one wave holds one 64-point chunk (xs is a multiple of 64 and
+			// a lane past the row end is inactive); its first lane writes
+			const unsigned long long mx = __ballot(f & 1u), my = __ballot(f & 2u), mz = __ballot(f & 4u),
+			                         mc = __ballot(cr != 0);
+			if (lane_id() == 0) {
+				const int nchunk = (g.sx + N2M_WAVE - 1) / N2M_WAVE;
+				unsigned long long *o =
+				    fmask + ((size_t)(y + (size_t)g.sy * z) * nchunk + (size_t)(x / N2M_WAVE)) * LW_FMW;
+				o[0] = mx;
+				o[1] = my;
+				o[2] = mz;
+				o[3] = mc;
+			}
+			if (cr)
+				crow[rowbase + x] = cr;
+		} else {
+			crow[rowbase + x] = cr;
+		}
 	}
 	// the three segment counts: wave sums, then one barrier (scratch holds
 	// 3 words per wave)
@@ -792,6 +800,7 @@ constexpr int LW_SCH = LW_SEG / N2M_WAVE; // chunks per segment
 
 struct LwMaskRows {
 	unsigned long long m[4][LW_SCH + 1][3]; // rows (dy + 2 dz) x chunks (the last: the next segment's first) x axes
+	unsigned long long cubes[LW_SCH];       // the owned row's cubes with a tiling row
 	uint16_t cb[4][LW_SCH + 1];             // edge vertices of the row segment before each chunk
 };
 
@@ -825,11 +834,9 @@ k_lw_emit_m(const float *__restrict__ img, LwGeo g, const unsigned long long *__
 	const size_t si = (size_t)seg + (size_t)g.nseg * ((size_t)y + (size_t)g.sy * z);
 	if (segcnt[si] == 0 && segcnt[ns + si] == 0 && segcnt[2 * ns + si] == 0)
 		return;
-	const int xs = seg * LW_SEG, xe = min(xs + LW_SEG, g.sx);
-	const int len = xe - xs;
+	const int xs = seg * LW_SEG;
 	const bool cuberow = y < g.sy - 1 && z < g.sz - 1;
 	const int nrows = cuberow ? 4 : 1;
-	const int ncube = min(xe, g.sx - 1) - xs;
 	const int nchunk = (g.sx + N2M_WAVE - 1) / N2M_WAVE;
 	// the masks of the four point rows; chunks past the row end are empty
 	constexpr int NW = 4 * (LW_SCH + 1) * 3;
@@ -840,9 +847,14 @@ k_lw_emit_m(const float *__restrict__ img, LwGeo g, const unsigned long long *__
 		unsigned long long v = 0ull;
 		if (r < nrows && ch < nchunk) {
 			const int yy = y + (r & 1), zz = z + (r >> 1);
-			v = fmask[((size_t)(yy + (size_t)g.sy * zz) * nchunk + ch) * 3 + a];
+			v = fmask[((size_t)(yy + (size_t)g.sy * zz) * nchunk + ch) * LW_FMW + a];
 		}
 		mflat[e] = v;
+	}
+	if (lane < LW_SCH) {
+		const int ch = seg * LW_SCH + lane;
+		M.cubes[lane] =
+		    cuberow && ch < nchunk ? fmask[((size_t)(y + (size_t)g.sy * z) * nchunk + ch) * LW_FMW + 3] : 0ull;
 	}
 	uint32_t base[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
@@ -850,9 +862,9 @@ k_lw_emit_m(const float *__restrict__ img, LwGeo g, const unsigned long long *__
 		if (r < nrows)
 			base[r] = bases[(size_t)seg + (size_t)g.nseg * ((size_t)(y + (r & 1)) + (size_t)g.sy * (z + (r >> 1)))];
 	__syncthreads(); // one wave: orders the LDS writes before other lanes' reads
-	// per row, the edge vertices before each chunk (only points below len
-	// are owned: the bits past len in the last chunk are zero, and the
-	// chunk after the segment is not counted)
+	// per row, the edge vertices before each chunk (the segment owns its
+	// points: bits past the row end are zero, and the chunk after the
+	// segment, the next one's, is not counted)
 	if (lane < 4) {
 		uint32_t run = 0;
 		for (int c = 0; c <= LW_SCH; c++) {
@@ -879,13 +891,14 @@ k_lw_emit_m(const float *__restrict__ img, LwGeo g, const unsigned long long *__
 					s_vl[at++] = (uint16_t)((i << 2) | a);
 		}
 	}
-	// the tiling rows of all cube chunks, loaded before the first is used
+	// the tiling rows of the cubes that have one (crow holds only those),
+	// all loaded before the first is used
 	const size_t rb0 = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
 	int rows[LW_SCH];
 #pragma unroll
 	for (int c = 0; c < LW_SCH; c++) {
 		const int i = c * N2M_WAVE + lane;
-		rows[c] = cuberow && i < ncube ? (int)crow[rb0 + xs + i] - 1 : -1;
+		rows[c] = ((M.cubes[c] >> lane) & 1ull) ? (int)crow[rb0 + xs + i] - 1 : -1;
 	}
 	__syncthreads();
 	// edge vertices of the owned points (row 0), one vertex per lane
@@ -1022,7 +1035,7 @@ size_t lw_fmask_words(int nx, int ny, int nz, const int lo[3], const int hi[3])
 {
 	LwGeo g;
 	lw_geo(g, nx, ny, nz, lo, hi, 0.f, 0.f, 0);
-	return (size_t)g.sy * g.sz * ((g.sx + N2M_WAVE - 1) / N2M_WAVE) * 3;
+	return (size_t)g.sy * g.sz * ((g.sx + N2M_WAVE - 1) / N2M_WAVE) * LW_FMW;
 }
 
 void launch_lw_emit(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,

@@ -14,7 +14,7 @@ namespace n2m {
 // ---------------------------------------------------------------------------
 // Exclusive scan of uint32 (multi-level: reduce, scan the partials, apply).
 // ---------------------------------------------------------------------------
-constexpr int SC_BS = 256, SC_IPT = 16, SC_TILE = SC_BS * SC_IPT;
+constexpr int SC_BS = 256, SC_IPT = 8, SC_TILE = SC_BS * SC_IPT;
 
 __global__ void __launch_bounds__(SC_BS) k_scan_reduce(const uint32_t *__restrict__ in, size_t n, uint32_t *sums)
 {
@@ -69,6 +69,31 @@ size_t scan_scratch_words(size_t n)
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
+// In-place exclusive scan of up to SC_SMALL words by one workgroup: the
+// partial sums level of a scan of up to SC_TILE * SC_SMALL words, in one
+// launch instead of two or three.
+constexpr int SC_SBS = 1024, SC_SIPT = 16, SC_SMALL = SC_SBS * SC_SIPT;
+
+__global__ void __launch_bounds__(SC_SBS) k_scan_sums(uint32_t *__restrict__ a, uint32_t n)
+{
+	__shared__ uint32_t scratch[SC_SBS / N2M_WAVE + 1];
+	const uint32_t base = threadIdx.x * SC_SIPT;
+	uint32_t v[SC_SIPT], s = 0;
+#pragma unroll
+	for (int k = 0; k < SC_SIPT; k++) {
+		v[k] = base + k < n ? a[base + k] : 0u;
+		s += v[k];
+	}
+	uint32_t tot;
+	uint32_t run = block_excl_scan<SC_SBS>(s, scratch, &tot);
+#pragma unroll
+	for (int k = 0; k < SC_SIPT; k++) {
+		if (base + k < n)
+			a[base + k] = run;
+		run += v[k];
+	}
+}
+
 void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, uint32_t *scratch,
                    hipStream_t s)
 {
@@ -84,7 +109,10 @@ void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total,
 	}
 	uint32_t *sums = scratch;
 	hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_BS), 0, s, in, n, sums);
-	scan_excl_u32(sums, sums, nb, nullptr, scratch + nb, s);
+	if (nb <= (size_t)SC_SMALL)
+		hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_SBS), 0, s, sums, (uint32_t)nb);
+	else
+		scan_excl_u32(sums, sums, nb, nullptr, scratch + nb, s);
 	hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SC_BS), 0, s, in, out, n, (const uint32_t *)sums,
 	                   total);
 }
