@@ -120,10 +120,13 @@ int n2m_remove_degenerate_triangles(vec3d *pts, vec3i **tris, int ntri);
 /* src/quadric.c:316-341 laplacian_smooth(), applied iter times. */
 void n2m_laplacian_smooth(vec3d *verts, vec3i *tris, int nvert, int ntri, int iter);
 
-/* save_mesh() (src/meshify.c:984-1019) for the binary formats .mz3, .gii and
- * .ply, byte-identical files; the float32 vertex copy, the PLY face records
- * and the GIfTI base64 run on the GPU, zlib / gzip (isGz) on the host.  Other
- * extensions fail: the reference's text writers stay on the CPU. */
+/* save_mesh() (src/meshify.c:984-1019), every format the reference writes,
+ * byte-identical files (FreeSurfer: but its time stamp).  The binary formats
+ * (.mz3, .gii, .ply, .stl, .vtk, .pial / .inflated) are encoded on the GPU:
+ * float32 vertices, PLY face records, STL facets, big-endian words, GIfTI
+ * base64; zlib / gzip (isGz) runs on the host.  The text formats (.obj,
+ * .off, .json) are formatted by the host C library with the reference's
+ * format strings, in parallel chunks. */
 int n2m_save_mesh(const char *fnm, vec3i *tris, vec3d *pts, int ntri, int npt, bool isGz);
 
 /* src/nii2mesh.c:119,155-172 (the image loader): raw voxels to the FP32

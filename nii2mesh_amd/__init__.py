@@ -153,8 +153,10 @@ def decode(raw, datatype, slope=1.0, inter=0.0):
 
 
 def save_mesh(fnm, tris, pts, gz=False):
-    """save_mesh() for .mz3 / .gii / .ply with the byte streams encoded on
-    the GPU (src/meshify.c:984-1019); returns 0 on success."""
+    """save_mesh() (src/meshify.c:984-1019) for every format of the
+    reference: the binary byte streams (.mz3, .gii, .ply, .stl, .vtk,
+    FreeSurfer) encoded on the GPU, the text formats (.obj, .off, .json)
+    formatted by the host C library; returns 0 on success."""
     L = load()
     return L.n2m_save_mesh(str(fnm).encode(), np.ascontiguousarray(tris, np.int32),
                            np.ascontiguousarray(pts, np.float64), len(tris), len(pts), bool(gz))

@@ -2201,6 +2201,151 @@ bool write_mz3(n2m_ctx *C, const Mesh &m, const char *fnm, bool gz) // src/meshi
 }
 
 // save_mesh() dispatch (src/meshify.c:984-1019) for the binary formats
+// FreeSurfer binary surface (src/meshify.c:418-456): magic, a time stamp
+// line, then counts, float32 vertices and triangles, all big-endian (the
+// reference swaps on every platform: its endianness test is a function
+// address, never null)
+bool write_freesurfer(n2m_ctx *C, const Mesh &m, const char *fnm)
+{
+	FILE *fp = fopen(fnm, "wb");
+	if (!fp)
+		return false;
+	const uint8_t magic[3] = {0xFF, 0xFF, 0xFE};
+	fwrite(magic, 3, 1, fp);
+	time_t t = time(nullptr);
+	char stamp[128] = "";
+	struct tm tmv;
+	localtime_r(&t, &tmv);
+	strftime(stamp, sizeof stamp, "created by niimath on %c\n\n", &tmv);
+	fwrite(stamp, strlen(stamp), 1, fp);
+	const uint32_t counts[2] = {__builtin_bswap32((uint32_t)m.np), __builtin_bswap32((uint32_t)m.nt)};
+	fwrite(counts, 4, 2, fp);
+	const float *p32 = pts_f32(C, m);
+	uint32_t *a = C->buf<uint32_t>(S_ENC_B, (size_t)3 * (m.np ? m.np : 1));
+	uint32_t *b = C->buf<uint32_t>(S_ENC_C, (size_t)3 * (m.nt ? m.nt : 1));
+	C->tbegin("encode_swap", 24.0 * m.np + 24.0 * m.nt);
+	launch_bswap32((const uint32_t *)p32, (size_t)3 * m.np, a, C->stream);
+	launch_bswap32((const uint32_t *)m.tris.p, (size_t)3 * m.nt, b, C->stream);
+	C->tend();
+	std::vector<uint8_t> ha = dl_bytes(C, a, (size_t)12 * m.np), hb = dl_bytes(C, b, (size_t)12 * m.nt);
+	fwrite(ha.data(), 1, ha.size(), fp);
+	fwrite(hb.data(), 1, hb.size(), fp);
+	fclose(fp);
+	return true;
+}
+
+// binary STL (src/meshify.c:725-760): 80 zero bytes, the facet count, then
+// 50-byte facets
+bool write_stl(n2m_ctx *C, const Mesh &m, const char *fnm)
+{
+	FILE *fp = fopen(fnm, "wb");
+	if (!fp)
+		return false;
+	const uint8_t hdr[80] = {0};
+	fwrite(hdr, 80, 1, fp);
+	const int32_t nf = m.nt;
+	fwrite(&nf, sizeof nf, 1, fp);
+	uint16_t *f = C->buf<uint16_t>(S_ENC_B, (size_t)25 * (m.nt ? m.nt : 1));
+	C->tbegin("encode_stl", 12.0 * m.nt + 72.0 * m.nt + 50.0 * m.nt);
+	launch_stl_facets((const double *)m.pts.p, (const int *)m.tris.p, (size_t)m.nt, f, C->stream);
+	C->tend();
+	std::vector<uint8_t> h = dl_bytes(C, f, (size_t)50 * m.nt);
+	fwrite(h.data(), 1, h.size(), fp);
+	fclose(fp);
+	return true;
+}
+
+// legacy VTK polydata, binary (src/meshify.c:932-968): big-endian float32
+// points and (3, v0, v1, v2) polygons
+bool write_vtk(n2m_ctx *C, const Mesh &m, const char *fnm)
+{
+	FILE *fp = fopen(fnm, "wb");
+	if (!fp)
+		return false;
+	fputs("# vtk DataFile Version 3.0\n", fp);
+	fputs("this file was written using niimath\n", fp);
+	fputs("BINARY\n", fp);
+	fputs("DATASET POLYDATA\n", fp);
+	fprintf(fp, "POINTS %d float\n", m.np);
+	const float *p32 = pts_f32(C, m);
+	uint32_t *a = C->buf<uint32_t>(S_ENC_B, (size_t)3 * (m.np ? m.np : 1));
+	uint32_t *b = C->buf<uint32_t>(S_ENC_C, (size_t)4 * (m.nt ? m.nt : 1));
+	C->tbegin("encode_swap", 24.0 * m.np + 28.0 * m.nt);
+	launch_bswap32((const uint32_t *)p32, (size_t)3 * m.np, a, C->stream);
+	launch_vtk_polys((const int *)m.tris.p, (size_t)m.nt, b, C->stream);
+	C->tend();
+	std::vector<uint8_t> ha = dl_bytes(C, a, (size_t)12 * m.np), hb = dl_bytes(C, b, (size_t)16 * m.nt);
+	fwrite(ha.data(), 1, ha.size(), fp);
+	fprintf(fp, "POLYGONS %d %d\n", m.nt, m.nt * 4);
+	fwrite(hb.data(), 1, hb.size(), fp);
+	fclose(fp);
+	return true;
+}
+
+// rows [0, n) formatted by fmt(i, buf) (returning the length), in parallel
+// chunks on the host threads, written in order
+template <typename F> void write_rows(FILE *fp, size_t n, F fmt)
+{
+	unsigned nth = std::thread::hardware_concurrency();
+	nth = n < 65536 ? 1u : (nth < 1 ? 1u : (nth > 16 ? 16u : nth));
+	std::vector<std::string> out(nth);
+	auto run = [&](unsigned k) {
+		const size_t b = n * k / nth, e = n * (k + 1) / nth;
+		std::string &o = out[k];
+		o.reserve((e - b) * 36);
+		char buf[160];
+		for (size_t i = b; i < e; i++)
+			o.append(buf, (size_t)fmt(i, buf));
+	};
+	std::vector<std::thread> th;
+	for (unsigned k = 1; k < nth; k++)
+		th.emplace_back(run, k);
+	run(0);
+	for (auto &t : th)
+		t.join();
+	for (auto &o : out)
+		fwrite(o.data(), 1, o.size(), fp);
+}
+
+// the text formats (src/meshify.c:683-723): numbers are formatted by the
+// host C library with the reference's format strings (%g of each float64
+// coordinate, %d of each 1-based index); the GPU only supplies the arrays
+enum TextFmt { TXT_JSON, TXT_OFF, TXT_OBJ };
+bool write_text(n2m_ctx *C, const Mesh &m, const char *fnm, TextFmt kind)
+{
+	std::vector<uint8_t> hp = dl_bytes(C, m.pts.p, (size_t)24 * m.np), ht = dl_bytes(C, m.tris.p, (size_t)12 * m.nt);
+	const double *p = (const double *)hp.data();
+	const int *t = (const int *)ht.data();
+	FILE *fp = fopen(fnm, "w");
+	if (!fp)
+		return false;
+	if (kind == TXT_JSON) {
+		fprintf(fp, "{\n");
+		fprintf(fp, "\t\"_DataInfo_\":{\n\t\t\"JMeshVersion\":\"0.5\",\n\t\t\"Comment\":\"Created by nii2mesh\"\n\t},\n");
+		fprintf(fp, "\t\"MeshVertex3\":[\n");
+		write_rows(fp, (size_t)m.np, [&](size_t i, char *b) {
+			return snprintf(b, 160, "[%g,\t%g,\t%g],\n", p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+		});
+		fprintf(fp, "\t],\n\t\"MeshTri3\":[\n");
+		write_rows(fp, (size_t)m.nt, [&](size_t i, char *b) {
+			return snprintf(b, 160, "[%d,\t%d,\t%d],\n", t[3 * i] + 1, t[3 * i + 1] + 1, t[3 * i + 2] + 1);
+		});
+		fprintf(fp, "\t]\n}\n");
+	} else {
+		if (kind == TXT_OFF)
+			fprintf(fp, "OFF\n%d\t%d\t0\n", m.np, m.nt);
+		const char *vf = kind == TXT_OFF ? "%g %g %g\n" : "v %g %g %g\n";
+		const char *ff = kind == TXT_OFF ? "%d %d %d\n" : "f %d %d %d\n";
+		write_rows(fp, (size_t)m.np,
+		           [&](size_t i, char *b) { return snprintf(b, 160, vf, p[3 * i], p[3 * i + 1], p[3 * i + 2]); });
+		write_rows(fp, (size_t)m.nt, [&](size_t i, char *b) {
+			return snprintf(b, 160, ff, t[3 * i] + 1, t[3 * i + 1] + 1, t[3 * i + 2] + 1);
+		});
+	}
+	fclose(fp);
+	return true;
+}
+
 int save_mesh_device(n2m_ctx *C, const Mesh &m, const char *fnm, bool gz)
 {
 	// extension = what strip_ext() (src/meshify.c:973-982) removes: the text
@@ -2212,18 +2357,27 @@ int save_mesh_device(n2m_ctx *C, const Mesh &m, const char *fnm, bool gz)
 		--e;
 	if (e > 0 && e < f.size() && f[e] == '.' && f[e - 1] != '\\' && f[e - 1] != '/')
 		ext = f.substr(e);
+	// the reference's order of tests (src/meshify.c:989-1018)
+	auto has = [&](const char *x) { return ext.find(x) != std::string::npos; };
 	bool ok;
-	if (ext.find(".mz3") != std::string::npos)
+	if (has(".mz3"))
 		ok = write_mz3(C, m, fnm, gz);
-	else if (ext.find(".gii") != std::string::npos)
+	else if (has(".gii"))
 		ok = write_gii(C, m, fnm, gz);
-	else if (ext.find(".ply") != std::string::npos)
+	else if (has(".inflated") || has(".pial"))
+		ok = write_freesurfer(C, m, fnm);
+	else if (has(".json"))
+		ok = write_text(C, m, fnm, TXT_JSON);
+	else if (has(".off"))
+		ok = write_text(C, m, fnm, TXT_OFF);
+	else if (has(".obj"))
+		ok = write_text(C, m, fnm, TXT_OBJ);
+	else if (has(".ply"))
 		ok = write_ply(C, m, fnm);
-	else if (ext.find(".inflated") != std::string::npos || ext.find(".pial") != std::string::npos ||
-	         ext.find(".json") != std::string::npos || ext.find(".off") != std::string::npos ||
-	         ext.find(".obj") != std::string::npos || ext.find(".stl") != std::string::npos ||
-	         ext.find(".vtk") != std::string::npos)
-		fail("%s: the device writer handles .mz3, .gii and .ply; use the reference's save_mesh", fnm);
+	else if (has(".stl"))
+		ok = write_stl(C, m, fnm);
+	else if (has(".vtk"))
+		ok = write_vtk(C, m, fnm);
 	else
 		ok = write_mz3(C, m, (f + ".mz3").c_str(), gz);
 	return ok ? EXIT_SUCCESS : EXIT_FAILURE;

@@ -4,7 +4,12 @@
 //   * PLY face records: uint8 3 then three little-endian int32
 //     (src/meshify.c:806-813);
 //   * base64 without line breaks, '=' padded (src/base64.c, RFC 1341), for
-//     the GIfTI data arrays.
+//     the GIfTI data arrays;
+//   * binary STL facets: a zero normal, the three corners as float32 and a
+//     zero 16-bit spacer, 50 bytes (src/meshify.c:725-760);
+//   * big-endian 32-bit words (FreeSurfer and VTK, whose writers swap every
+//     word: src/meshify.c:418-456, 932-968) and VTK's polygon records
+//     (3, v0, v1, v2).
 #include "n2m_device.cuh"
 #include "n2m_internal.h"
 
@@ -52,6 +57,63 @@ __global__ void k_base64(const uint8_t *__restrict__ src, size_t len, uint8_t *_
 		o[2] = left > 1 ? b64(((b & 15u) << 2) | (c >> 6)) : (uint8_t)'=';
 		o[3] = left > 2 ? b64(c & 63u) : (uint8_t)'=';
 	}
+}
+
+// one thread per facet; records are 2-byte aligned, so they are written as
+// 25 16-bit halves
+__global__ void k_stl_facets(const double *__restrict__ pts, const int *__restrict__ tris, size_t nt,
+                             uint16_t *__restrict__ out)
+{
+	for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nt; t += (size_t)gridDim.x * blockDim.x) {
+		uint16_t *o = out + 25 * t;
+		for (int k = 0; k < 6; k++)
+			o[k] = 0; // the normal, not written by the reference
+		for (int c = 0; c < 3; c++) {
+			const double *p = pts + 3 * (size_t)tris[3 * t + c];
+			for (int k = 0; k < 3; k++) {
+				const uint32_t u = __float_as_uint((float)p[k]);
+				o[6 + 6 * c + 2 * k] = (uint16_t)u;
+				o[7 + 6 * c + 2 * k] = (uint16_t)(u >> 16);
+			}
+		}
+		o[24] = 0; // spacer
+	}
+}
+
+__device__ __forceinline__ uint32_t bswap32(uint32_t v)
+{
+	return (v >> 24) | ((v >> 8) & 0xff00u) | ((v << 8) & 0xff0000u) | (v << 24);
+}
+
+__global__ void k_bswap32(const uint32_t *__restrict__ in, size_t n, uint32_t *__restrict__ out)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		out[i] = bswap32(in[i]);
+}
+
+__global__ void k_vtk_polys(const int *__restrict__ tris, size_t nt, uint32_t *__restrict__ out)
+{
+	for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nt; t += (size_t)gridDim.x * blockDim.x) {
+		out[4 * t] = bswap32(3u);
+		for (int k = 0; k < 3; k++)
+			out[4 * t + 1 + k] = bswap32((uint32_t)tris[3 * t + k]);
+	}
+}
+
+void launch_stl_facets(const double *pts, const int *tris, size_t nt, uint16_t *out, hipStream_t s)
+{
+	if (nt)
+		hipLaunchKernelGGL(k_stl_facets, dim3(blocks_for(nt)), dim3(256), 0, s, pts, tris, nt, out);
+}
+void launch_bswap32(const uint32_t *in, size_t n, uint32_t *out, hipStream_t s)
+{
+	if (n)
+		hipLaunchKernelGGL(k_bswap32, dim3(blocks_for(n)), dim3(256), 0, s, in, n, out);
+}
+void launch_vtk_polys(const int *tris, size_t nt, uint32_t *out, hipStream_t s)
+{
+	if (nt)
+		hipLaunchKernelGGL(k_vtk_polys, dim3(blocks_for(nt)), dim3(256), 0, s, tris, nt, out);
 }
 
 void launch_f64_to_f32(const double *in, size_t n, float *out, hipStream_t s)

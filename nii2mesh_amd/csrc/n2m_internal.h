@@ -77,6 +77,11 @@ void launch_hist_range(const float *img, size_t n, uint32_t *stats, hipStream_t 
 void launch_f64_to_f32(const double *in, size_t n, float *out, hipStream_t s);
 void launch_ply_faces(const int *tris, size_t nt, uint8_t *out, hipStream_t s);
 void launch_base64(const uint8_t *src, size_t len, uint8_t *out, hipStream_t s);
+// binary STL facets (50 bytes each, written as 25 16-bit halves)
+void launch_stl_facets(const double *pts, const int *tris, size_t nt, uint16_t *out, hipStream_t s);
+// byte-swapped 32-bit words; VTK polygon records (3, v0, v1, v2), big-endian
+void launch_bswap32(const uint32_t *in, size_t n, uint32_t *out, hipStream_t s);
+void launch_vtk_polys(const int *tris, size_t nt, uint32_t *out, hipStream_t s);
 // raw NIfTI voxels -> float (datatype 2 uint8, 4 int16, 512 uint16, 16 float32);
 // returns -1 for another datatype
 int launch_decode(const void *raw, int datatype, size_t n, float slope, float inter, float *out, hipStream_t s);
