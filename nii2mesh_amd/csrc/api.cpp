@@ -257,6 +257,9 @@ enum Small {
 	SM_FLAG = 40,  // 1 word
 	SM_HALO = 44,  // 1 word: a smoothing block overflowed its halo
 	SM_RING = 45,  // 1 word: a smoothing block overflowed its two rings
+	SM_WNSEG = 48, // 1 word: weld segment count (left on the device)
+	SM_WNNEW = 49, // 1 word: welded vertex count (left on the device)
+	SM_DEGK = 50,  // 1 word: triangles kept by the degenerate test (follows SM_WNNEW)
 	SM_WORDS = 64
 };
 
@@ -877,6 +880,67 @@ uint32_t remove_degenerate_welded(n2m_ctx *C, const int *tris_in, uint32_t npt, 
 	return k;
 }
 
+// weld + remove_degenerate with every count left on the device (no host
+// round trip between the stages): the buffers are sized for the upper bounds
+// (npt vertices, nt triangles), the launches cover them and read the counts
+// from the device, and a weld that merges nothing numbers each vertex as
+// itself (the reference's unchanged output).  pts_out / tris_out receive the
+// result arrays; the vertex and triangle counts are left in SM_WNNEW and
+// SM_DEGK, for one copy to the host.  The same kernels and order of
+// operations as weld() and remove_degenerate_welded().
+void weld_clean_deferred(n2m_ctx *C, const double *pts, uint32_t npt, const int *tris_in, uint32_t nt,
+                         double *&pts_out, int *&tris_out)
+{
+	hipStream_t s = C->stream;
+	uint32_t *k0 = C->buf<uint32_t>(S_K0, npt), *v0 = C->buf<uint32_t>(S_V0, npt);
+	uint32_t *k1 = C->buf<uint32_t>(S_K1, npt), *v1 = C->buf<uint32_t>(S_V1, npt);
+	C->tbegin("weld_keys", 24.0 * npt + 8.0 * npt);
+	launch_weld_keys(pts, npt, k0, v0, s);
+	C->tend();
+	uint32_t *rc = C->buf<uint32_t>(S_RCOUNT, radix_counts_words(npt));
+	uint32_t *scr = scan_scratch(C, radix_counts_words(npt) > npt ? radix_counts_words(npt) : npt);
+	C->tbegin("radix_sort", 4 * 16.0 * npt);
+	bool in1 = radix_sort_u32(k0, v0, k1, v1, npt, 32, rc, scr, s);
+	C->tend();
+	uint32_t *keys = in1 ? k1 : k0, *vals = in1 ? v1 : v0;
+	uint32_t *flags = C->buf<uint32_t>(S_FLAGS, npt), *segid = C->buf<uint32_t>(S_SEGID, npt);
+	uint32_t *d_nseg = C->d_small + SM_WNSEG, *d_nnew = C->d_small + SM_WNNEW, *d_k = C->d_small + SM_DEGK;
+	C->tbegin("weld_segments", 8.0 * npt);
+	launch_weld_segflags(keys, npt, flags, s);
+	scan_excl_u32(flags, segid, npt, d_nseg, scr, s);
+	C->tend();
+	uint32_t *starts = C->buf<uint32_t>(S_STARTS, npt);
+	int *lcls = C->buf<int>(S_LCLS, npt);
+	uint32_t *seeds = C->buf<uint32_t>(S_SEEDS, npt), *seedbase = C->buf<uint32_t>(S_SEEDBASE, npt);
+	C->tbegin("weld_greedy", 4.0 * npt + 4.0 * npt + 24.0 * npt + 4.0 * npt);
+	launch_weld_segstarts(flags, segid, npt, starts, s);
+	launch_weld_greedy(starts, npt, npt, keys, vals, pts, lcls, seeds, s, d_nseg);
+	scan_excl_u32(seeds, seedbase, npt, d_nnew, scr, s);
+	C->tend();
+	int *cls = C->buf<int>(S_CLS, npt), *rep = C->buf<int>(S_REP, npt);
+	HIPX(hipMemsetAsync(rep, 0xff, (size_t)4 * npt, s));
+	C->tbegin("weld_assign", 4.0 * npt * 3 + 8.0 * npt);
+	launch_weld_assign(vals, lcls, flags, segid, seedbase, npt, cls, rep, s, d_nnew);
+	C->tend();
+	pts_out = C->buf<double>(S_PTS1, (size_t)3 * npt);
+	double *wpos = C->buf<double>(S_WPOS, (size_t)3 * npt);
+	C->tbegin("weld_gather", 28.0 * npt + 24.0 * npt + 8.0 * npt + 24.0 * npt + 24.0 * npt);
+	launch_gather_pts(pts, rep, npt, pts_out, s, d_nnew);
+	launch_weld_wpos(pts, cls, rep, npt, wpos, s);
+	C->tend();
+	int *t1 = C->buf<int>(S_TRIS1, (size_t)3 * (nt ? nt : 1));
+	uint32_t *keep = C->buf<uint32_t>(S_KEEP, nt), *pos = C->buf<uint32_t>(S_KEEPPOS, nt);
+	C->tbegin("degenerate", 12.0 * nt + 12.0 * nt + 72.0 * nt + 4.0 * nt);
+	launch_remap_degen(tris_in, cls, npt, wpos, nt, t1, keep, s);
+	scan_excl_u32(keep, pos, nt, d_k, scan_scratch(C, nt), s);
+	C->tend();
+	int *out = C->buf<int>(S_TRIS0, (size_t)3 * (nt ? nt : 1));
+	C->tbegin("compact_tris", 12.0 * nt + 12.0 * nt);
+	launch_compact_tris(t1, keep, pos, nt, out, s);
+	C->tend();
+	tris_out = out;
+}
+
 // The result array p (bytes long) into the mesh buffer dst: when p is a
 // scratch slot of C, the two buffers trade places (no copy; the slot reuses
 // the mesh's old buffer next time), otherwise p is copied.
@@ -906,6 +970,18 @@ bool surface_stages(n2m_ctx *C, const float *vol, const int dim[3], const VolOut
 	const int *tris0 = mc.implicit ? nullptr : C->buf<int>(S_TRIS0, (size_t)3 * mc.ntri);
 	double *pts = nullptr;
 	int *tris = nullptr;
+	if (!P.verbose) { // the counts come back once, at the end
+		weld_clean_deferred(C, pts0, mc.npt, tris0, mc.ntri, pts, tris);
+		const uint32_t *cnt = C->fetch(SM_WNNEW, 2);
+		const uint32_t np = cnt[0], nt = cnt[1];
+		if (np < 3) // src/meshify.c:380
+			return false;
+		adopt(C, m.tris, tris, (size_t)12 * nt);
+		adopt(C, m.pts, pts, (size_t)24 * np);
+		m.nt = (int)nt;
+		m.np = (int)np;
+		return true;
+	}
 	double *wpos = nullptr;
 	uint32_t np = weld(C, pts0, mc.npt, tris0, mc.ntri, pts, tris, P.verbose != 0, &wpos);
 	if (np < 3)

@@ -292,13 +292,25 @@ __global__ void k_weld_segstarts(const uint32_t *__restrict__ flags, const uint3
 		starts[segid[i]] = i;
 }
 
+// d_nseg (when given) holds the segment count on the device: the launch
+// covers nseg = n threads, and those past the count write a zero seed count
+// (so that the scan of seeds can run over n words without a host round trip)
 __global__ void k_weld_greedy(const uint32_t *__restrict__ starts, uint32_t nseg, uint32_t n,
                               const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-                              const double *__restrict__ pts, int *__restrict__ lcls, uint32_t *__restrict__ seeds)
+                              const double *__restrict__ pts, int *__restrict__ lcls, uint32_t *__restrict__ seeds,
+                              const uint32_t *__restrict__ d_nseg)
 {
 	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= nseg)
 		return;
+	if (d_nseg) {
+		const uint32_t ns_dev = *d_nseg;
+		if (s >= ns_dev) {
+			seeds[s] = 0u;
+			return;
+		}
+		nseg = ns_dev;
+	}
 	const float tol = 0.00001f;
 	uint32_t b = starts[s], e = s + 1 < nseg ? starts[s + 1] : n;
 	for (uint32_t i = b; i < e; i++)
@@ -322,26 +334,29 @@ __global__ void k_weld_greedy(const uint32_t *__restrict__ starts, uint32_t nseg
 	seeds[s] = (uint32_t)ns;
 }
 
+// d_nnew (when given): the class count on the device; when it equals n no
+// vertex was merged and, as the reference then keeps its input unchanged
+// (src/meshify.c:82-87), every vertex keeps its own number
 __global__ void k_weld_assign(const uint32_t *__restrict__ vals, const int *__restrict__ lcls,
                               const uint32_t *__restrict__ flags, const uint32_t *__restrict__ segid,
                               const uint32_t *__restrict__ seedbase, uint32_t n, int *__restrict__ cls,
-                              int *__restrict__ rep)
+                              int *__restrict__ rep, const uint32_t *__restrict__ d_nnew)
 {
 	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
 	uint32_t sg = segid[i] + flags[i] - 1; // inclusive segment index
-	int c = (int)seedbase[sg] + lcls[i];
 	uint32_t v = vals[i];
+	const int c = (d_nnew && *d_nnew == n) ? (int)v : (int)seedbase[sg] + lcls[i];
 	cls[v] = c;
 	atomicMax(&rep[c], (int)v); // last member in index order provides the position
 }
 
 __global__ void k_gather_pts(const double *__restrict__ pts, const int *__restrict__ rep, uint32_t n,
-                             double *__restrict__ out)
+                             double *__restrict__ out, const uint32_t *__restrict__ d_n)
 {
 	uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= n)
+	if (c >= n || (d_n && c >= *d_n)) // d_n: the class count on the device
 		return;
 	const double *p = pts + (size_t)rep[c] * 3;
 	out[(size_t)c * 3 + 0] = p[0];
@@ -392,20 +407,23 @@ void launch_weld_segstarts(const uint32_t *flags, const uint32_t *segid, uint32_
 	hipLaunchKernelGGL(k_weld_segstarts, dim3(g1(n)), dim3(256), 0, s, flags, segid, n, starts);
 }
 void launch_weld_greedy(const uint32_t *starts, uint32_t nseg, uint32_t n, const uint32_t *keys,
-                        const uint32_t *vals, const double *pts, int *lcls, uint32_t *seeds, hipStream_t s)
+                        const uint32_t *vals, const double *pts, int *lcls, uint32_t *seeds, hipStream_t s,
+                        const uint32_t *d_nseg)
 {
 	hipLaunchKernelGGL(k_weld_greedy, dim3(g1(nseg)), dim3(256), 0, s, starts, nseg, n, keys, vals, pts, lcls,
-	                   seeds);
+	                   seeds, d_nseg);
 }
 void launch_weld_assign(const uint32_t *vals, const int *lcls, const uint32_t *flags, const uint32_t *segid,
-                        const uint32_t *seedbase, uint32_t n, int *cls, int *rep, hipStream_t s)
+                        const uint32_t *seedbase, uint32_t n, int *cls, int *rep, hipStream_t s,
+                        const uint32_t *d_nnew)
 {
 	hipLaunchKernelGGL(k_weld_assign, dim3(g1(n)), dim3(256), 0, s, vals, lcls, flags, segid, seedbase, n, cls,
-	                   rep);
+	                   rep, d_nnew);
 }
-void launch_gather_pts(const double *pts, const int *rep, uint32_t n, double *out, hipStream_t s)
+void launch_gather_pts(const double *pts, const int *rep, uint32_t n, double *out, hipStream_t s,
+                       const uint32_t *d_n)
 {
-	hipLaunchKernelGGL(k_gather_pts, dim3(g1(n)), dim3(256), 0, s, pts, rep, n, out);
+	hipLaunchKernelGGL(k_gather_pts, dim3(g1(n)), dim3(256), 0, s, pts, rep, n, out, d_n);
 }
 void launch_remap_tris(const int *tris_in, const int *cls, uint32_t nt, uint32_t npt, int *tris_out,
                        hipStream_t s)

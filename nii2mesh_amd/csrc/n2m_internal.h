@@ -182,11 +182,16 @@ void launch_weld_keys(const double *pts, uint32_t n, uint32_t *keys, uint32_t *v
 void launch_weld_segflags(const uint32_t *keys, uint32_t n, uint32_t *flags, hipStream_t s);
 void launch_weld_segstarts(const uint32_t *flags, const uint32_t *segid, uint32_t n, uint32_t *starts,
                            hipStream_t s);
+// d_nseg / d_nnew / d_n: counts left on the device (no host round trip);
+// the launches then cover the upper bound n
 void launch_weld_greedy(const uint32_t *starts, uint32_t nseg, uint32_t n, const uint32_t *keys,
-                        const uint32_t *vals, const double *pts, int *lcls, uint32_t *seeds, hipStream_t s);
+                        const uint32_t *vals, const double *pts, int *lcls, uint32_t *seeds, hipStream_t s,
+                        const uint32_t *d_nseg = nullptr);
 void launch_weld_assign(const uint32_t *vals, const int *lcls, const uint32_t *flags, const uint32_t *segid,
-                        const uint32_t *seedbase, uint32_t n, int *cls, int *rep, hipStream_t s);
-void launch_gather_pts(const double *pts, const int *rep, uint32_t n, double *out, hipStream_t s);
+                        const uint32_t *seedbase, uint32_t n, int *cls, int *rep, hipStream_t s,
+                        const uint32_t *d_nnew = nullptr);
+void launch_gather_pts(const double *pts, const int *rep, uint32_t n, double *out, hipStream_t s,
+                       const uint32_t *d_n = nullptr);
 // wpos[v] = pts[rep[cls[v]]]: the welded position of every pre-weld vertex v
 void launch_weld_wpos(const double *pts, const int *cls, const int *rep, uint32_t npt, double *wpos, hipStream_t s);
 // remap (tris_in null: implicit 3t+k) and degenerate flags in one pass
