@@ -2598,8 +2598,11 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 		// (each waits for the previous one to finish).  By default all meshes
 		// are smoothed here after the meshing: the step is within 1 % either
 		// way, and the HC kernels then run on a GPU of their own.
+		// N2M_PIPE_HC=2: as 1, but the HC passes of different meshes run
+		// concurrently (one stream each).
 		const char *ph = getenv("N2M_PIPE_HC");
-		const bool hc_in_worker = threaded && hcIter > 0 && ph && *ph == '1';
+		const bool hc_in_worker = threaded && hcIter > 0 && ph && (*ph == '1' || *ph == '2');
+		const bool hc_serial = !(ph && *ph == '2');
 		std::mutex hc_mu;
 		std::vector<char> done(niso, 0);
 		auto mesh_one = [&](n2m_ctx *D, int k) {
@@ -2617,7 +2620,9 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 				done[k] = 1;
 				if (hc_in_worker) {
 					D->sync();
-					std::lock_guard<std::mutex> lock(hc_mu);
+					std::unique_lock<std::mutex> lock(hc_mu, std::defer_lock);
+					if (hc_serial)
+						lock.lock();
 					smooth_hc_device(D, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
 					                 true);
 					D->sync();
