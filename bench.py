@@ -25,7 +25,11 @@ reference's per-label loop run on the same GPU; both give the same meshes.
 Also printed in the same JSON line:
   roofline      the dominant kernel's algorithmic bytes per launch / its mean
                 launch time (HIP events on the library's stream, during the
-                timed steps), against the 8 TB/s HBM peak;
+                timed steps), against the 8 TB/s HBM peak.  The dominant
+                kernel is k_hc_ring on the largest mesh (stage "hc_ring"),
+                whose 100 iterations run alone on the GPU; the smaller
+                meshes' iterations overlap its preparation and are timed as
+                "hc_ring_side";
   cpu_baseline  the reference compiled from its own sources, timed on this
                 host (1 core) on a bounded sample of the same workload, with
                 its stage timers (-v 1) and its quadric reduction at 0.25
@@ -317,6 +321,7 @@ def main():
     roof = {'kernel': dom, 'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
             'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
             'bytes_per_launch': dbytes / dn if dn else 0, 'avg_launch_us': dms / dn * 1e3 if dn else 0,
+            'launches_per_step': dn // max(args.steps, 1),
             'traffic': traffic_from_profile(dom),
             'traffic_source': 'profiles/traffic_%s.txt (rocprofv3 --pmc, committed)' % dom}
     out = {

@@ -14,6 +14,7 @@
 #include <time.h>
 
 #include <atomic>
+#include <functional>
 #include <limits>
 #include <mutex>
 #include <stdexcept>
@@ -1241,8 +1242,12 @@ Ring build_ring(n2m_ctx *C, const Csr &c, uint32_t nv, uint32_t nt, const double
 }
 
 // laplacian_smoothHC on a device mesh in place, src/quadric.c:343-394
+// before_iter (when given) runs on the host after the preparation (every host
+// round trip done) and before the iterations are enqueued; stat names the
+// timing bracket of the one-launch iterations
 void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint32_t nt, double alpha,
-                      double beta, int iter, bool lockEdges)
+                      double beta, int iter, bool lockEdges, const std::function<void()> *before_iter = nullptr,
+                      const char *stat = "hc_ring")
 {
 	if (nv == 0)
 		return;
@@ -1262,12 +1267,14 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 	// iterations, so that every host round trip comes first and the
 	// iterations are the last work enqueued
 	uint8_t *border = lockEdges ? border_flags(C, c, T, nv, nt) : nullptr;
+	if (before_iter)
+		(*before_iter)();
 	const double *res = o;
 	if (G.on) {
 		// algorithmic bytes: CSR offsets, one 4-byte slot pair per reference,
 		// the original and current positions in, the new positions out.  The
 		// iterations are back-to-back launches, timed with one event pair.
-		C->tbegin("hc_ring", (4.0 * nv + 12.0 * nt + 72.0 * nv) * iter);
+		C->tbegin(stat, (4.0 * nv + 12.0 * nt + 72.0 * nv) * iter);
 		for (int it = 0; it < iter; it++) {
 			double *nxt = it % 2 == 0 ? p : pn;
 			launch_hc_ring(c.start, G.slots, G.roff, G.rslots, G.ring, G.rcnt, o, G.o1, res, nxt, nv, alpha, beta,
@@ -2660,15 +2667,65 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 			for (int k = 0; k < niso; k++)
 				mesh_one(C, k);
 		}
-		int ok = 0;
-		for (int k = 0; k < niso; k++) {
-			if (!done[k])
-				continue;
-			Mesh &m = C->meshes[k];
-			if (hcIter > 0 && !hc_in_worker)
-				smooth_hc_device(C, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
-				                 true);
-			ok++;
+		int ok = 0, big = -1;
+		for (int k = 0; k < niso; k++)
+			if (done[k]) {
+				ok++;
+				if (big < 0 || C->meshes[k].nt > C->meshes[big].nt)
+					big = k;
+			}
+		if (hcIter > 0 && !hc_in_worker && big >= 0) {
+			// The largest mesh is smoothed on this context.  With worker
+			// contexts, the other meshes are smoothed on theirs while its
+			// preparation (relabelling, CSR, rings, border flags) runs, and
+			// they have finished before its iterations are enqueued, so the
+			// iterations of the largest mesh have the GPU to themselves.
+			// Their timing brackets are "hc_ring_side" (they share the GPU),
+			// the largest mesh's "hc_ring".
+			std::vector<std::thread> th;
+			std::vector<std::string> err(niso);
+			for (int k = 0; k < niso; k++) {
+				if (!done[k] || k == big)
+					continue;
+				Mesh &m = C->meshes[k];
+				if (!threaded) {
+					smooth_hc_device(C, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
+					                 true, nullptr, "hc_ring_side");
+					continue;
+				}
+				n2m_ctx *D = C->workers[k];
+				th.emplace_back([&, D, k] {
+					try {
+						HIPX(hipSetDevice(C->device));
+						Mesh &mk = C->meshes[k];
+						smooth_hc_device(D, (double *)mk.pts.p, (const int *)mk.tris.p, mk.np, mk.nt, alpha, beta,
+						                 hcIter, true, nullptr, "hc_ring_side");
+						D->sync();
+					} catch (const std::exception &e) {
+						err[k] = e.what();
+					}
+				});
+			}
+			const std::function<void()> join = [&] {
+				for (auto &t : th)
+					t.join();
+				th.clear();
+			};
+			Mesh &mb = C->meshes[big];
+			try {
+				smooth_hc_device(C, (double *)mb.pts.p, (const int *)mb.tris.p, mb.np, mb.nt, alpha, beta, hcIter,
+				                 true, &join);
+			} catch (...) {
+				join();
+				throw;
+			}
+			join();
+			if (threaded)
+				for (int k = 0; k < niso; k++) {
+					C->absorb(*C->workers[k]);
+					if (!err[k].empty())
+						throw std::runtime_error(err[k]);
+				}
 		}
 		C->sync();
 		return ok == niso ? EXIT_SUCCESS : EXIT_FAILURE;
