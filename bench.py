@@ -396,12 +396,14 @@ def atlas_leg(torch, n2m, synth, reps=5):
     for k, (ms, n, b) in sorted(ctx.kernel_stats().items(), key=lambda kv: -kv[1][0]):
         log('  atlas %-28s %8.3f ms %5d launches' % (k, ms, n))
     ctx.close()
+    tuned = {k: {'tile': v[0], 'blocks_per_cu': v[1]} for k, v in n2m.atlas_tuned(0).items() if v[1] >= 0}
+    log('atlas launch shapes (tuned): %s' % tuned)
     sz = np.bincount(vol.astype(np.int64).ravel())[1:]
     return {'workload': '256^3 weighted-Voronoi atlas, 370 labels (log-normal sizes: median %d, max %d voxels), '
                         'binarise 0.5 + pre-smooth + classic marching cubes' % (np.median(sz), sz.max()),
             'labels_meshed': nok, 'triangles': ntri, 'batched_ms': round(batched, 3),
             'per_label_loop_ms': round(loop, 3), 'speedup': round(loop / batched, 2),
-            'batched_gvoxel_s': round(vol.size / (batched * 1e-3) / 1e9, 4)}, vol
+            'batched_gvoxel_s': round(vol.size / (batched * 1e-3) / 1e9, 4), 'tuned': tuned}, vol
 
 
 def traffic_from_profile(kernel):

@@ -309,6 +309,14 @@ void n2m_ctx_reset_timing(n2m_ctx *ctx);
 int n2m_ctx_kernel_stat(n2m_ctx *ctx, int idx, const char **name, double *ms_total, int *launches,
                         double *bytes_total);
 
+/* Launch shape the batched atlas's autotuner chose for stage `stage` on
+   `device` (no reference counterpart: it reports the tuner of
+   n2m_atlas / n2m_ctx_atlas).  Returns the number of stages; for a valid
+   stage fills its name, its crop tile (-1 for stages without one) and its
+   blocks per CU (0: one workgroup per work item), both -1 until that stage
+   has been tuned on the device. */
+int n2m_atlas_tuned(int device, int stage, const char **name, int *tile, int *blocks_per_cu);
+
 #ifdef __cplusplus
 }
 #endif

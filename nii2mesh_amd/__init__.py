@@ -87,6 +87,7 @@ def _bind_ext(L):
     L.n2m_ctx_timing_only.argtypes = [C.c_void_p, C.c_char_p]
     L.n2m_ctx_kernel_stat.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                                       C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.n2m_atlas_tuned.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.n2m_quick_smooth.argtypes = [np.ctypeslib.ndpointer(np.float32, flags='C_CONTIGUOUS'), C.c_int, C.c_int,
                                    C.c_int]
     L.n2m_laplacian_smooth.argtypes = [np.ctypeslib.ndpointer(np.float64, flags='C_CONTIGUOUS'),
@@ -197,6 +198,19 @@ def otsu(hist, mode):
 
 def last_error():
     return load().n2m_last_error().decode()
+
+
+def atlas_tuned(device=0):
+    """The batched atlas's tuned launch shapes on `device`: {stage: (tile,
+    blocks_per_cu)}, -1 for a stage not tuned yet (tile -1: the stage has no
+    tile choice; blocks per CU 0: one workgroup per work item)."""
+    L = load()
+    out = {}
+    name, tile, bpc = C.c_char_p(), C.c_int(), C.c_int()
+    for k in range(L.n2m_atlas_tuned(device, -1, None, None, None)):
+        L.n2m_atlas_tuned(device, k, C.byref(name), C.byref(tile), C.byref(bpc))
+        out[name.value.decode()] = (tile.value, bpc.value)
+    return out
 
 
 CUBES_CLASSIC = 0

@@ -14,6 +14,7 @@
 #include <time.h>
 
 #include <atomic>
+#include <algorithm>
 #include <functional>
 #include <limits>
 #include <mutex>
@@ -198,8 +199,9 @@ enum Slot {
 	S_AT_RED,
 	S_AT_VBASE,
 	S_AT_VTASK,
-	S_AT_K2,
-	S_AT_V2,
+	S_AT_RSTILES,
+	S_LREP,
+	S_AT_NEXT,
 	S_AT_CBASE,
 	S_AT_IDENT,
 	S_AT_IDX,
@@ -339,8 +341,40 @@ struct n2m_ctx {
 		return b.p;
 	}
 
-	void sync() { HIPX(hipStreamSynchronize(stream)); }
+	// Pinned staging of host-to-device uploads: the copy runs asynchronously
+	// from here, so an upload costs no round trip.  Space is handed out in
+	// stream order and reclaimed at the next stream synchronisation, when
+	// every copy from it has completed.
+	uint8_t *h_stage = nullptr;
+	size_t stage_cap = 0, stage_used = 0;
+
+	void sync()
+	{
+		HIPX(hipStreamSynchronize(stream));
+		stage_used = 0;
+	}
 	void check() { HIPX(hipGetLastError()); }
+
+	void *stage(const void *src, size_t bytes)
+	{
+		const size_t need = (bytes + 255) & ~(size_t)255;
+		if (stage_used + need > stage_cap) {
+			sync();
+			if (need > stage_cap) {
+				if (h_stage)
+					HIPX(hipHostFree(h_stage));
+				h_stage = nullptr;
+				stage_cap = 0;
+				size_t want = need > ((size_t)1 << 20) ? need : ((size_t)1 << 20);
+				HIPX(hipHostMalloc((void **)&h_stage, want, hipHostMallocDefault));
+				stage_cap = want;
+			}
+		}
+		void *p = h_stage + stage_used;
+		stage_used += need;
+		memcpy(p, src, bytes);
+		return p;
+	}
 
 	// read `n` small words at offset `off` back to the host
 	uint32_t *fetch(int off, int n)
@@ -481,6 +515,8 @@ void ctx_free(n2m_ctx *c)
 		(void)hipHostFree(c->h_small);
 	if (c->h_thr)
 		(void)hipHostFree(c->h_thr);
+	if (c->h_stage)
+		(void)hipHostFree(c->h_stage);
 	(void)hipStreamDestroy(c->stream);
 	delete c;
 }
@@ -781,9 +817,10 @@ uint32_t weld(n2m_ctx *C, const double *pts, uint32_t npt, const int *tris_in /*
 	uint32_t *starts = C->buf<uint32_t>(S_STARTS, nseg);
 	int *lcls = C->buf<int>(S_LCLS, npt);
 	uint32_t *seeds = C->buf<uint32_t>(S_SEEDS, nseg), *seedbase = C->buf<uint32_t>(S_SEEDBASE, nseg);
-	C->tbegin("weld_greedy", 4.0 * npt + 4.0 * npt + 24.0 * npt + 4.0 * npt);
+	uint32_t *lrep = C->buf<uint32_t>(S_LREP, npt);
+	C->tbegin("weld_greedy", 4.0 * npt + 4.0 * npt + 24.0 * npt + 8.0 * npt);
 	launch_weld_segstarts(flags, segid, npt, starts, s);
-	launch_weld_greedy(starts, nseg, npt, keys, vals, pts, lcls, seeds, s);
+	launch_weld_greedy(starts, nseg, npt, keys, vals, pts, lcls, seeds, lrep, s);
 	C->tend();
 	uint32_t nnew = scan(C, seeds, seedbase, nseg, true);
 	tris_out = C->buf<int>(S_TRIS1, (size_t)3 * nt);
@@ -799,9 +836,8 @@ uint32_t weld(n2m_ctx *C, const double *pts, uint32_t npt, const int *tris_in /*
 		return npt;
 	}
 	int *cls = C->buf<int>(S_CLS, npt), *rep = C->buf<int>(S_REP, nnew);
-	HIPX(hipMemsetAsync(rep, 0xff, (size_t)4 * nnew, s));
-	C->tbegin("weld_assign", 4.0 * npt * 3 + 8.0 * npt);
-	launch_weld_assign(vals, lcls, flags, segid, seedbase, npt, cls, rep, s);
+	C->tbegin("weld_assign", 4.0 * npt * 5 + 4.0 * npt + 4.0 * nnew);
+	launch_weld_assign(vals, lcls, flags, segid, seedbase, lrep, npt, cls, rep, s);
 	C->tend();
 	pts_out = C->buf<double>(S_PTS1, (size_t)3 * nnew);
 	if (wpos) {
@@ -913,15 +949,15 @@ void weld_clean_deferred(n2m_ctx *C, const double *pts, uint32_t npt, const int 
 	uint32_t *starts = C->buf<uint32_t>(S_STARTS, npt);
 	int *lcls = C->buf<int>(S_LCLS, npt);
 	uint32_t *seeds = C->buf<uint32_t>(S_SEEDS, npt), *seedbase = C->buf<uint32_t>(S_SEEDBASE, npt);
-	C->tbegin("weld_greedy", 4.0 * npt + 4.0 * npt + 24.0 * npt + 4.0 * npt);
+	uint32_t *lrep = C->buf<uint32_t>(S_LREP, npt);
+	C->tbegin("weld_greedy", 4.0 * npt + 4.0 * npt + 24.0 * npt + 8.0 * npt);
 	launch_weld_segstarts(flags, segid, npt, starts, s);
-	launch_weld_greedy(starts, npt, npt, keys, vals, pts, lcls, seeds, s, d_nseg);
+	launch_weld_greedy(starts, npt, npt, keys, vals, pts, lcls, seeds, lrep, s, d_nseg);
 	scan_excl_u32(seeds, seedbase, npt, d_nnew, scr, s);
 	C->tend();
 	int *cls = C->buf<int>(S_CLS, npt), *rep = C->buf<int>(S_REP, npt);
-	HIPX(hipMemsetAsync(rep, 0xff, (size_t)4 * npt, s));
-	C->tbegin("weld_assign", 4.0 * npt * 3 + 8.0 * npt);
-	launch_weld_assign(vals, lcls, flags, segid, seedbase, npt, cls, rep, s, d_nnew);
+	C->tbegin("weld_assign", 4.0 * npt * 5 + 4.0 * npt + 4.0 * npt);
+	launch_weld_assign(vals, lcls, flags, segid, seedbase, lrep, npt, cls, rep, s, d_nnew);
 	C->tend();
 	pts_out = C->buf<double>(S_PTS1, (size_t)3 * npt);
 	double *wpos = C->buf<double>(S_WPOS, (size_t)3 * npt);
@@ -1372,8 +1408,8 @@ template <typename T> void upload(n2m_ctx *C, T *d, const std::vector<T> &h)
 {
 	if (h.empty())
 		return;
-	HIPX(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, C->stream));
-	C->sync(); // h may change right after
+	const size_t bytes = h.size() * sizeof(T);
+	HIPX(hipMemcpyAsync(d, C->stage(h.data(), bytes), bytes, hipMemcpyHostToDevice, C->stream));
 }
 
 template <typename T> std::vector<T> download(n2m_ctx *C, const T *d, size_t n)
@@ -1464,43 +1500,92 @@ void sform_and_smooth(n2m_ctx *C, Mesh &m, int postSmooth, const float *srow)
 		smooth_hc_device(C, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, 0.1, 0.5, postSmooth, true);
 }
 
-// Crop tile of the atlas smoothing: N2M_ATLAS_TILE forces one; otherwise the
-// first smoothed atlas run on a device times every shape on its own crops
-// and keeps the fastest for the process.
-std::atomic<int> g_atlas_tile[64]; // per device: tuned tile + 1, 0 until tuned
+// Launch shapes of the batched atlas.  Every stage is a persistent launch of
+// `blocks per CU` x CUs workgroups over its block map (0: one workgroup per
+// item); the smoothing also picks its crop tile.  The first atlas run on a
+// device times every candidate on its own crops and keeps the fastest for
+// the process.  N2M_ATLAS_TILE / N2M_ATLAS_BPC force a tile / one blocks-per-CU
+// value for every stage.
+constexpr int kAtlasBpc[] = {0, 1, 2, 4, 8};
+constexpr int kNBpc = (int)(sizeof kAtlasBpc / sizeof kAtlasBpc[0]);
+enum AtlasStage { AST_SMOOTH, AST_APPLY, AST_CCOUNT, AST_CEMIT, AST_LWCLS, AST_LWEMIT, AST_N };
+const char *const kAtlasStageName[AST_N] = {"smooth", "apply", "classic_count", "classic_emit", "lewiner_classify",
+                                            "lewiner_emit"};
+std::atomic<int> g_atlas_cfg[64][AST_N]; // per device: tuned configuration + 1, 0 until tuned
 
-template <typename Run> int atlas_tile_for(n2m_ctx *C, Run run)
+int env_choice(const char *name, int n)
 {
-	const char *e = getenv("N2M_ATLAS_TILE");
-	if (e && *e) {
-		int t = atoi(e);
-		if (t < 0 || t >= atlas_tile_count())
-			fail("N2M_ATLAS_TILE=%s: expected 0..%d", e, atlas_tile_count() - 1);
-		return t;
+	const char *e = getenv(name);
+	if (!e || !*e)
+		return -1;
+	int v = atoi(e);
+	if (v < 0 || v >= n)
+		fail("%s=%s: expected 0..%d", name, e, n - 1);
+	return v;
+}
+
+int device_cus(int dev)
+{
+	static std::atomic<int> cus[64];
+	const int d = dev >= 0 && dev < 64 ? dev : 0;
+	int n = cus[d].load(std::memory_order_relaxed);
+	if (!n) {
+		HIPX(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+		cus[d].store(n, std::memory_order_relaxed);
 	}
+	return n;
+}
+
+// workgroups of a persistent launch over nwork items at blocks-per-CU index k
+unsigned atlas_grid(n2m_ctx *C, unsigned nwork, int k)
+{
+	if (!kAtlasBpc[k])
+		return nwork;
+	const size_t g = (size_t)kAtlasBpc[k] * device_cus(C->device);
+	return g < nwork ? (unsigned)g : nwork;
+}
+
+// configuration of a stage: cfg = tile * kNBpc + blocks-per-CU index (tile 0
+// for the stages without one).  run(cfg) must be safe to repeat.
+template <typename Run> int atlas_tune(n2m_ctx *C, int stage, int ntiles, Run run)
+{
+	const int ft = ntiles > 1 ? env_choice("N2M_ATLAS_TILE", ntiles) : -1;
+	const int fb = env_choice("N2M_ATLAS_BPC", kNBpc);
+	std::vector<int> cand;
+	for (int t = 0; t < ntiles; t++)
+		for (int k = 0; k < kNBpc; k++)
+			if ((ft < 0 || t == ft) && (fb < 0 || k == fb))
+				cand.push_back(t * kNBpc + k);
+	if (cand.size() == 1)
+		return cand[0];
 	const int dev = C->device >= 0 && C->device < 64 ? C->device : 0;
-	const int known = g_atlas_tile[dev].load(std::memory_order_relaxed);
-	if (known > 0)
+	const int known = g_atlas_cfg[dev][stage].load(std::memory_order_relaxed);
+	if (known > 0 && std::find(cand.begin(), cand.end(), known - 1) != cand.end())
 		return known - 1;
 	hipEvent_t a = C->ev(), b = C->ev();
 	float best = 0.f;
-	int pick = 0;
-	for (int t = 0; t < atlas_tile_count(); t++) {
-		run(t); // warm the shape's code object
-		HIPX(hipEventRecord(a, C->stream));
-		run(t);
-		HIPX(hipEventRecord(b, C->stream));
-		HIPX(hipEventSynchronize(b));
+	int pick = cand[0];
+	for (size_t i = 0; i < cand.size(); i++) {
+		run(cand[i]); // warms the code object and the caches
 		float ms = 0.f;
-		HIPX(hipEventElapsedTime(&ms, a, b));
-		if (t == 0 || ms < best) {
+		for (int r = 0; r < 2; r++) {
+			HIPX(hipEventRecord(a, C->stream));
+			run(cand[i]);
+			HIPX(hipEventRecord(b, C->stream));
+			HIPX(hipEventSynchronize(b));
+			float m = 0.f;
+			HIPX(hipEventElapsedTime(&m, a, b));
+			ms = r == 0 || m < ms ? m : ms;
+		}
+		if (i == 0 || ms < best) {
 			best = ms;
-			pick = t;
+			pick = cand[i];
 		}
 	}
 	C->evpool.push_back(a);
 	C->evpool.push_back(b);
-	g_atlas_tile[dev].store(pick + 1, std::memory_order_relaxed);
+	if (ft < 0 && fb < 0)
+		g_atlas_cfg[dev][stage].store(pick + 1, std::memory_order_relaxed);
 	return pick;
 }
 
@@ -1542,41 +1627,104 @@ void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_pa
 		rinit[2 * k] = 0xffffffffu;
 		rinit[2 * k + 1] = 0u;
 	}
-	auto smooth_with = [&](int tile) {
-		unsigned nb = plan([&](const AtlasTask &t) { return atlas_smooth_blocks(t.nx, t.ny, t.nz, smooth, tile); });
+	int planned_tile = -1;
+	unsigned nb = 0;
+	auto smooth_with = [&](int cfg) {
+		const int tile = cfg / kNBpc;
+		if (tile != planned_tile) {
+			nb = plan([&](const AtlasTask &t) { return atlas_smooth_blocks(t.nx, t.ny, t.nz, smooth, tile); });
+			planned_tile = tile;
+		}
 		upload(C, red, rinit);
-		launch_atlas_smooth(labels, NX, NY, NZ, dT(), dmap(), nb, smooth ? 1 : 0, tile, crops, red, s);
+		launch_atlas_smooth(labels, NX, NY, NZ, dT(), dmap(), nb, atlas_grid(C, nb, cfg % kNBpc), smooth ? 1 : 0,
+		                    tile, crops, red, s);
 		C->check();
 	};
-	const int tile = smooth ? atlas_tile_for(C, smooth_with) : 0;
+	const int scfg = atlas_tune(C, AST_SMOOTH, smooth ? atlas_tile_count() : 1, smooth_with);
 	C->tbegin("atlas_smooth", 8.0 * nfl);
-	smooth_with(tile);
+	smooth_with(scfg);
 	C->tend();
+
+	// 2. edge darkening, box of >= iso, minimum (src/meshify.c:347-371).  The
+	// isolevels come from the ranges on the device (atlas_iso), so the
+	// ranges, boxes and minima come back in one copy; a task without
+	// variability runs with a NaN isolevel (it darkens nothing and finds no
+	// voxel >= iso) and is dropped afterwards.
 	{
-		std::vector<uint32_t> rr = download(C, red, 2 * T.size());
+		const size_t n0 = T.size();
+		nb = plan([&](const AtlasTask &t) { return atlas_apply_blocks(t.nx, t.ny, t.nz); });
+		launch_atlas_iso(dT(), (int)n0, red, P.isolevel, s);
+		std::vector<uint32_t> init(7 * n0);
+		for (size_t k = 0; k < n0; k++) {
+			for (int j = 0; j < 3; j++) {
+				init[6 * k + j] = 0x7fffffffu;
+				init[6 * k + 3 + j] = 0xffffffffu; // -1
+			}
+			init[6 * n0 + k] = 0xffffffffu;
+		}
+		upload(C, red + 2 * n0, init);
+		int *bbox = (int *)red + 2 * n0;
+		uint32_t *vmin = red + 8 * n0;
+		// repeatable for the tuner: the darkening is a min, and a repeat sees
+		// a subset of the voxels >= iso, so the reduced box and minimum stay
+		auto apply_with = [&](int cfg) {
+			launch_atlas_apply(crops, NX, NY, NZ, dT(), dmap(), nb, atlas_grid(C, nb, cfg), bbox, vmin, s);
+			C->check();
+		};
+		const int acfg = atlas_tune(C, AST_APPLY, 1, apply_with);
+		C->tbegin("atlas_apply", 8.0 * nfl);
+		apply_with(acfg);
+		C->tend();
+		const std::vector<uint32_t> rb = download(C, red, 9 * n0);
+		const uint32_t *rr = rb.data(), *bb = rb.data() + 2 * n0;
 		std::vector<AtlasTask> L;
 		std::vector<int> LE;
-		for (size_t k = 0; k < T.size(); k++) {
+		std::vector<size_t> LK; // each kept task's index in T
+		for (size_t k = 0; k < n0; k++) {
 			AtlasTask t = T[k];
-			float mn = ord2f_host(rr[2 * k]), mx = ord2f_host(rr[2 * k + 1]);
-			if (mn == mx) { // src/meshify.c:312-315
+			const float mn = ord2f_host(rr[2 * k]), mx = ord2f_host(rr[2 * k + 1]);
+			bool fallback = false;
+			if (!atlas_iso(P.isolevel, mn, mx, &t.iso, &t.edgeMax, &fallback)) { // src/meshify.c:312-315
 				printf("Error: No variability in image intensity.\n");
 				failed((int)k);
 				continue;
 			}
-			float isolevel = P.isolevel;
-			if ((isolevel <= mn) || (isolevel > mx)) { // src/meshify.c:316-319
-				isolevel = 0.5 * (mn + mx);
+			if (fallback) // src/meshify.c:316-319
 				printf("Suggested isolevel out of range. Intensity range %g..%g, setting isolevel to %g\n", mn, mx,
-				       isolevel);
-			}
+				       t.iso);
 			if (P.verbose)
-				printf("intensity range %g..%g, isolevel %g\n", mn, mx, isolevel);
-			t.iso = isolevel;
+				printf("intensity range %g..%g, isolevel %g\n", mn, mx, t.iso);
 			t.mn = mn;
-			t.edgeMax = 0.75 * (mn + isolevel);
 			L.push_back(t);
 			LE.push_back(entry[k]);
+			LK.push_back(k);
+		}
+		for (size_t j = 0; j < L.size(); j++) {
+			const size_t k = LK[j];
+			AtlasTask &t = L[j];
+			const int org[3] = {t.ox, t.oy, t.oz}, csz[3] = {t.nx, t.ny, t.nz};
+			t.mnv = ord2f_host(bb[6 * n0 + k]) - t.iso;
+			if ((int)bb[6 * k + 3] < 0) {
+				// no voxel >= iso: cannot happen (the maximum is >= iso); the
+				// reference's marching cubes would find nothing
+				printf("marching cubes failed to identify triangles with an isolevel of %g\n", t.iso);
+				C->atlas[LE[j]].status = N2M_ATLAS_FAILED;
+				for (int i = 0; i < 3; i++)
+					t.lo3[i] = t.hi3[i] = 0;
+			} else {
+				for (int i = 0; i < 3; i++) {
+					int glo = (int)bb[6 * k + i] + org[i], ghi = (int)bb[6 * k + 3 + i] + org[i];
+					glo = glo - 1 > 0 ? glo - 1 : 0;
+					ghi = ghi + 2 < dim[i] ? ghi + 2 : dim[i];
+					t.lo3[i] = glo - org[i];
+					t.hi3[i] = ghi - org[i];
+					// the crop holds the label's box + 4 (clipped), the box of
+					// >= iso lies within the label's box + 2, so lo3 >= 0 and
+					// hi3 <= csz
+					if (t.lo3[i] < 0 || t.hi3[i] > csz[i])
+						fail("atlas label %d: box outside its crop", t.label);
+				}
+			}
 		}
 		T.swap(L);
 		entry.swap(LE);
@@ -1584,51 +1732,6 @@ void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_pa
 	if (T.empty())
 		return;
 	const size_t nt_ = T.size();
-
-	// 2. edge darkening, box of >= iso, minimum (src/meshify.c:347-371)
-	unsigned nb = plan([&](const AtlasTask &t) { return atlas_apply_blocks(t.nx, t.ny, t.nz); });
-	{
-		std::vector<uint32_t> init(7 * nt_);
-		for (size_t k = 0; k < nt_; k++) {
-			for (int j = 0; j < 3; j++) {
-				init[6 * k + j] = 0x7fffffffu;
-				init[6 * k + 3 + j] = 0xffffffffu; // -1
-			}
-			init[6 * nt_ + k] = 0xffffffffu;
-		}
-		upload(C, red, init);
-	}
-	C->tbegin("atlas_apply", 8.0 * nfl);
-	launch_atlas_apply(crops, NX, NY, NZ, dT(), dmap(), nb, (int *)red, red + 6 * nt_, s);
-	C->tend();
-	{
-		std::vector<uint32_t> bb = download(C, red, 7 * nt_);
-		for (size_t k = 0; k < nt_; k++) {
-			AtlasTask &t = T[k];
-			const int org[3] = {t.ox, t.oy, t.oz}, csz[3] = {t.nx, t.ny, t.nz};
-			t.mnv = ord2f_host(bb[6 * nt_ + k]) - t.iso;
-			if ((int)bb[6 * k + 3] < 0) {
-				// no voxel >= iso: cannot happen (the maximum is >= iso); the
-				// reference's marching cubes would find nothing
-				printf("marching cubes failed to identify triangles with an isolevel of %g\n", t.iso);
-				failed((int)k);
-				for (int i = 0; i < 3; i++)
-					t.lo3[i] = t.hi3[i] = 0;
-				continue;
-			}
-			for (int i = 0; i < 3; i++) {
-				int glo = (int)bb[6 * k + i] + org[i], ghi = (int)bb[6 * k + 3 + i] + org[i];
-				glo = glo - 1 > 0 ? glo - 1 : 0;
-				ghi = ghi + 2 < dim[i] ? ghi + 2 : dim[i];
-				t.lo3[i] = glo - org[i];
-				t.hi3[i] = ghi - org[i];
-				// the crop holds the label's box + 4 (clipped), the box of >= iso
-				// lies within the label's box + 2, so lo3 >= 0 and hi3 <= csz
-				if (t.lo3[i] < 0 || t.hi3[i] > csz[i])
-					fail("atlas label %d: box outside its crop", t.label);
-			}
-		}
-	}
 
 	// 3. marching cubes
 	uint32_t Vtot = 0, Ttot = 0;
@@ -1646,8 +1749,13 @@ void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_pa
 		uint8_t *cfg = C->buf<uint8_t>(S_CFG, cubes);
 		uint32_t *bs = C->buf<uint32_t>(S_BLOCK, (size_t)nb + 1), *bo = C->buf<uint32_t>(S_BLOCKOFF, (size_t)nb + 1);
 		HIPX(hipMemsetAsync(bs + nb, 0, 4, s));
+		auto count_with = [&](int k) {
+			launch_atlas_classic_count(crops, dT(), dmap(), nb, atlas_grid(C, nb, k), cfg, bs, s);
+			C->check();
+		};
+		const int ccfg = atlas_tune(C, AST_CCOUNT, 1, count_with);
 		C->tbegin("atlas_mc_classic_count", 4.0 * cubes + cubes);
-		launch_atlas_classic_count(crops, dT(), dmap(), nb, cfg, bs, s);
+		count_with(ccfg);
 		C->tend();
 		scan_excl_u32(bs, bo, (size_t)nb + 1, nullptr, scan_scratch(C, (size_t)nb + 1), s);
 		C->check();
@@ -1667,8 +1775,13 @@ void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_pa
 			}
 		if (Ttot) {
 			double *pts = C->buf<double>(S_PTS0, (size_t)9 * Ttot);
+			auto emit_with = [&](int k) {
+				launch_atlas_classic_emit(crops, dT(), dmap(), nb, atlas_grid(C, nb, k), cfg, bo, pts, s);
+				C->check();
+			};
+			const int ecfg = atlas_tune(C, AST_CEMIT, 1, emit_with);
 			C->tbegin("atlas_mc_classic_emit", 1.0 * cubes + 72.0 * Ttot);
-			launch_atlas_classic_emit(crops, dT(), dmap(), nb, cfg, bo, pts, s);
+			emit_with(ecfg);
 			C->tend();
 		}
 	} else { // src/MarchingCubes.c:1086-1143
@@ -1687,8 +1800,14 @@ void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_pa
 		uint32_t *cnt = C->buf<uint32_t>(S_SEGCNT, 3 * stride), *base = C->buf<uint32_t>(S_SEGBASE, 3 * stride);
 		for (int j = 0; j < 3; j++)
 			HIPX(hipMemsetAsync(cnt + j * stride + nb, 0, 4, s));
+		auto classify_with = [&](int k) {
+			launch_atlas_lw_classify(crops, dT(), dmap(), nb, atlas_grid(C, nb, k), P.originalMC, pf, cr, cnt, stride,
+			                         s);
+			C->check();
+		};
+		const int lcfg = atlas_tune(C, AST_LWCLS, 1, classify_with);
 		C->tbegin("atlas_mc_lewiner_classify", 7.0 * npts);
-		launch_atlas_lw_classify(crops, dT(), dmap(), nb, P.originalMC, pf, cr, cnt, stride, s);
+		classify_with(lcfg);
 		C->tend();
 		for (int j = 0; j < 3; j++)
 			scan_excl_u32(cnt + j * stride, base + j * stride, stride, nullptr, scan_scratch(C, stride), s);
@@ -1713,9 +1832,14 @@ void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_pa
 		if (Vtot) {
 			double *pts = C->buf<double>(S_PTS0, (size_t)3 * Vtot);
 			int *tris = C->buf<int>(S_TRIS0, (size_t)3 * (Ttot ? Ttot : 1));
+			auto emit_with = [&](int k) {
+				launch_atlas_lw_emit(crops, dT(), (int)nt_, dmap(), nb, atlas_grid(C, nb, k), P.originalMC, pf, cr, cnt,
+				                     base, stride, E[nt_], pts, tris, s);
+				C->check();
+			};
+			const int ecfg = atlas_tune(C, AST_LWEMIT, 1, emit_with);
 			C->tbegin("atlas_mc_lewiner_emit", 3.0 * npts + 24.0 * Vtot + 12.0 * Ttot);
-			launch_atlas_lw_emit(crops, dT(), (int)nt_, dmap(), nb, P.originalMC, pf, cr, cnt, base, stride, E[nt_],
-			                     pts, tris, s);
+			emit_with(ecfg);
 			C->tend();
 		}
 	}
@@ -1729,88 +1853,91 @@ void atlas_chunk(n2m_ctx *C, const float *labels, const int dim[3], const n2m_pa
 	uint32_t *k0 = C->buf<uint32_t>(S_K0, Vtot), *v0 = C->buf<uint32_t>(S_V0, Vtot);
 	uint32_t *k1 = C->buf<uint32_t>(S_K1, Vtot), *v1 = C->buf<uint32_t>(S_V1, Vtot);
 	uint32_t *vtask = C->buf<uint32_t>(S_AT_VTASK, Vtot);
-	uint32_t *rc = C->buf<uint32_t>(S_RCOUNT, radix_counts_words(Vtot));
-	uint32_t *scr = scan_scratch(C, radix_counts_words(Vtot));
+	// every task's vertices sorted on their own: one segmented sort, the
+	// tasks keep their places, so the task of sorted position i is vtask[i]
+	const std::vector<RsTile> tiles = radix_seg_tiles(vbase);
+	RsTile *dtiles = C->buf<RsTile>(S_AT_RSTILES, tiles.size());
+	upload(C, dtiles, tiles);
+	uint32_t *rc = C->buf<uint32_t>(S_RCOUNT, 256 * tiles.size() + 1);
+	uint32_t *scr = scan_scratch(C, 256 * tiles.size() + 1);
 	C->tbegin("atlas_weld_sort", 24.0 * Vtot + 12.0 * Vtot + 4 * 16.0 * Vtot);
 	launch_atlas_weld_keys(pts0, Vtot, dvb, (int)nt_, k0, v0, vtask, s);
-	bool in1 = radix_sort_u32(k0, v0, k1, v1, Vtot, 32, rc, scr, s);
-	uint32_t *K = in1 ? k1 : k0, *V = in1 ? v1 : v0;
-	uint32_t *tk = in1 ? k0 : k1, *pos = in1 ? v0 : v1;
-	uint32_t *tk2 = C->buf<uint32_t>(S_AT_K2, Vtot), *pos2 = C->buf<uint32_t>(S_AT_V2, Vtot);
-	launch_atlas_task_keys(V, vtask, Vtot, tk, pos, s);
-	const int tbits = nt_ <= 256 ? 8 : (nt_ <= 65536 ? 16 : 24);
-	bool in2 = radix_sort_u32(tk, pos, tk2, pos2, Vtot, tbits, rc, scr, s);
-	uint32_t *TK = in2 ? tk2 : tk, *PS = in2 ? pos2 : pos;
-	uint32_t *keys = in2 ? tk : tk2, *vals = in2 ? pos : pos2; // the free pair
-	launch_atlas_weld_gather(K, V, PS, Vtot, keys, vals, s);
+	bool in1 = radix_sort_seg_u32(k0, v0, k1, v1, Vtot, 32, dtiles, tiles.size(), rc, scr, s);
+	uint32_t *keys = in1 ? k1 : k0, *vals = in1 ? v1 : v0;
+	const uint32_t *TK = vtask;
 	C->tend();
 	uint32_t *flags = C->buf<uint32_t>(S_FLAGS, Vtot), *segid = C->buf<uint32_t>(S_SEGID, Vtot);
 	C->tbegin("atlas_weld_segments", 12.0 * Vtot);
 	launch_atlas_weld_segflags(keys, TK, Vtot, flags, s);
 	C->tend();
-	uint32_t nseg = scan(C, flags, segid, Vtot, true);
-	uint32_t *starts = C->buf<uint32_t>(S_STARTS, nseg);
+	// The counts stay on the device until one copy at the end: the launches
+	// cover their upper bound Vtot.
+	uint32_t *d_nseg = C->d_small + SM_WNSEG, *d_nnew = C->d_small + SM_WNNEW;
+	scan_excl_u32(flags, segid, Vtot, d_nseg, scr, s);
+	uint32_t *starts = C->buf<uint32_t>(S_STARTS, Vtot);
 	int *lcls = C->buf<int>(S_LCLS, Vtot);
-	uint32_t *seeds = C->buf<uint32_t>(S_SEEDS, nseg), *seedbase = C->buf<uint32_t>(S_SEEDBASE, nseg);
-	C->tbegin("atlas_weld_greedy", 4.0 * Vtot + 4.0 * Vtot + 24.0 * Vtot + 4.0 * Vtot);
+	uint32_t *seeds = C->buf<uint32_t>(S_SEEDS, Vtot), *seedbase = C->buf<uint32_t>(S_SEEDBASE, Vtot);
+	uint32_t *lrep = C->buf<uint32_t>(S_LREP, Vtot);
+	C->tbegin("atlas_weld_greedy", 4.0 * Vtot + 4.0 * Vtot + 24.0 * Vtot + 8.0 * Vtot);
 	launch_weld_segstarts(flags, segid, Vtot, starts, s);
-	launch_weld_greedy(starts, nseg, Vtot, keys, vals, pts0, lcls, seeds, s);
+	launch_weld_greedy(starts, Vtot, Vtot, keys, vals, pts0, lcls, seeds, lrep, s, d_nseg);
 	C->tend();
-	uint32_t nnew = scan(C, seeds, seedbase, nseg, true);
-	uint32_t *dcb = C->buf<uint32_t>(S_AT_CBASE, nt_);
-	int *cls = C->buf<int>(S_CLS, Vtot), *rep = C->buf<int>(S_REP, nnew);
-	HIPX(hipMemsetAsync(rep, 0xff, (size_t)4 * nnew, s));
-	C->tbegin("atlas_weld_assign", 4.0 * Vtot * 4 + 8.0 * Vtot);
+	scan_excl_u32(seeds, seedbase, Vtot, d_nnew, scr, s);
+	// per task: next task with vertices (ntasks for none)
+	std::vector<uint32_t> nextne(nt_);
+	for (size_t k = nt_, nx = nt_; k-- > 0;) {
+		nextne[k] = (uint32_t)nx;
+		if (vbase[k + 1] != vbase[k])
+			nx = k;
+	}
+	uint32_t *dnext = C->buf<uint32_t>(S_AT_NEXT, nt_);
+	upload(C, dnext, nextne);
+	// result words: [0, nt_] kept-triangle bases, [nt_ + 1, 2 nt_ + 1)
+	// class bases, [2 nt_ + 1, 3 nt_ + 1) class counts, then the total
+	uint32_t *res = C->buf<uint32_t>(S_AT_OUT, 3 * nt_ + 2);
+	uint32_t *dcb = res + nt_ + 1, *dncls = res + 2 * nt_ + 1;
+	uint8_t *di = C->buf<uint8_t>(S_AT_IDENT, nt_);
+	int *cls = C->buf<int>(S_CLS, Vtot), *rep = C->buf<int>(S_REP, Vtot);
+	C->tbegin("atlas_weld_assign", 4.0 * Vtot * 6 + 4.0 * Vtot + 4.0 * Vtot);
 	launch_atlas_class_base(TK, segid, seedbase, Vtot, dcb, s);
-	launch_weld_assign(vals, lcls, flags, segid, seedbase, Vtot, cls, rep, s);
+	launch_weld_assign(vals, lcls, flags, segid, seedbase, lrep, Vtot, cls, rep, s);
+	launch_atlas_weld_tasks(dcb, dvb, dnext, (int)nt_, d_nnew, dncls, di, s);
+	launch_atlas_weld_identity(Vtot, vtask, dvb, dcb, di, cls, rep, s);
 	C->tend();
-	std::vector<uint32_t> cb = download(C, dcb, nt_);
-	std::vector<uint32_t> ncls(nt_, 0);
-	std::vector<uint8_t> ident(nt_, 0);
-	bool any_ident = false;
-	{
-		uint32_t next = nnew;
-		for (size_t k = nt_; k-- > 0;) {
-			if (vbase[k + 1] == vbase[k]) {
-				cb[k] = next;
-				continue;
-			}
-			ncls[k] = next - cb[k];
-			next = cb[k];
-			if (ncls[k] == vbase[k + 1] - vbase[k]) {
-				ident[k] = 1;
-				any_ident = true;
-				if (P.verbose)
-					printf("Unify vertices found no shared vertices\n");
-			}
-			if (ncls[k] < 3) // src/meshify.c:384
-				failed((int)k);
-		}
-	}
-	if (any_ident) {
-		uint8_t *di = C->buf<uint8_t>(S_AT_IDENT, nt_);
-		upload(C, di, ident);
-		launch_atlas_weld_identity(Vtot, vtask, dvb, dcb, di, cls, rep, s);
-		C->check();
-	}
-	double *pts1 = C->buf<double>(S_PTS1, (size_t)3 * nnew);
+	double *pts1 = C->buf<double>(S_PTS1, (size_t)3 * Vtot);
 	int *tris1 = C->buf<int>(S_TRIS1, (size_t)3 * (Ttot ? Ttot : 1));
-	C->tbegin("atlas_weld_gather", 28.0 * nnew + 24.0 * nnew + 24.0 * Ttot);
-	launch_gather_pts(pts0, rep, nnew, pts1, s);
+	C->tbegin("atlas_weld_gather", 28.0 * Vtot + 24.0 * Vtot + 24.0 * Ttot);
+	launch_gather_pts(pts0, rep, Vtot, pts1, s, d_nnew);
 	launch_remap_tris(implicit ? nullptr : C->buf<int>(S_TRIS0, (size_t)3 * (Ttot ? Ttot : 1)), cls, Ttot, Vtot,
 	                  tris1, s);
 	C->tend();
 
-	// 5. degenerate triangles (src/meshify.c:113-168)
+	// 5. degenerate triangles (src/meshify.c:113-168); every class number is
+	// below the class count, so Vtot bounds the vertex ids as well
 	uint32_t *keep = C->buf<uint32_t>(S_KEEP, (size_t)Ttot + 1), *kpos = C->buf<uint32_t>(S_KEEPPOS, (size_t)Ttot + 1);
 	HIPX(hipMemsetAsync(keep + Ttot, 0, 4, s));
 	C->tbegin("atlas_degenerate", 12.0 * Ttot + 72.0 * Ttot + 4.0 * Ttot);
 	if (Ttot)
-		launch_degen_flags(pts1, tris1, Ttot, nnew, keep, s);
+		launch_degen_flags(pts1, tris1, Ttot, Vtot, keep, s);
 	C->tend();
 	scan_excl_u32(keep, kpos, (size_t)Ttot + 1, nullptr, scan_scratch(C, (size_t)Ttot + 1), s);
+	uint32_t *dtb = C->buf<uint32_t>(S_AT_IDX, nt_ + 1);
+	upload(C, dtb, tbase);
+	launch_gather_u32(kpos, dtb, (uint32_t)(nt_ + 1), res, s);
+	HIPX(hipMemcpyAsync(res + 3 * nt_ + 1, d_nnew, 4, hipMemcpyDeviceToDevice, s));
 	C->check();
-	std::vector<uint32_t> kb = values_at(C, kpos, tbase);
+	const std::vector<uint32_t> r = download(C, res, 3 * nt_ + 2);
+	const std::vector<uint32_t> kb(r.begin(), r.begin() + nt_ + 1);
+	const uint32_t *cb = r.data() + nt_ + 1, *ncls = r.data() + 2 * nt_ + 1;
+	const uint32_t nnew = r[3 * nt_ + 1];
+	for (size_t k = nt_; k-- > 0;) {
+		if (vbase[k + 1] == vbase[k])
+			continue;
+		if (ncls[k] == vbase[k + 1] - vbase[k] && P.verbose)
+			printf("Unify vertices found no shared vertices\n");
+		if (ncls[k] < 3) // src/meshify.c:384
+			failed((int)k);
+	}
 	const uint32_t kept = kb[nt_];
 	int *tris2 = C->buf<int>(S_TRIS0, (size_t)3 * (kept ? kept : 1));
 	if (kept && Ttot)
@@ -3032,6 +3159,20 @@ int n2m_ctx_kernel_stat(n2m_ctx *C, int idx, const char **name, double *ms_total
 		*bytes_total = C->stats[idx].bytes;
 	}
 	return (int)C->stats.size();
+}
+
+int n2m_atlas_tuned(int device, int stage, const char **name, int *tile, int *blocks_per_cu)
+{
+	if (stage >= 0 && stage < AST_N) {
+		const int k = device >= 0 && device < 64 ? g_atlas_cfg[device][stage].load(std::memory_order_relaxed) : 0;
+		if (name)
+			*name = kAtlasStageName[stage];
+		if (tile)
+			*tile = k ? (stage == AST_SMOOTH ? (k - 1) / kNBpc : -1) : -1;
+		if (blocks_per_cu)
+			*blocks_per_cu = k ? kAtlasBpc[(k - 1) % kNBpc] : -1;
+	}
+	return AST_N;
 }
 
 // ---------------------------------------------------------------------------

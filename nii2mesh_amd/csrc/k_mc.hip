@@ -1080,26 +1080,47 @@ __device__ __forceinline__ ClsGeo cls_geo_task(const AtlasTask &t)
 	              t.hi3[2] - 1 - t.lo3[2], t.iso, t.ox, t.oy, t.oz};
 }
 
+// The atlas kernels: workgroup b takes item b of the block map, or with
+// LOOP (persistent) the grid walks the `nwork` items in strides of its own
+// size, with a barrier between items (the per-item helpers reuse their LDS).
+#define N2M_ATLAS_ITEMS(ITEM)                                                                                      \
+	if (!LOOP) {                                                                                                  \
+		ITEM(blockIdx.x);                                                                                          \
+		return;                                                                                                    \
+	}                                                                                                             \
+	for (unsigned b = blockIdx.x; b < nwork; b += gridDim.x) {                                                    \
+		ITEM(b);                                                                                                   \
+		__syncthreads();                                                                                           \
+	}
+
+template <bool LOOP>
 __global__ void __launch_bounds__(MC_BS)
 k_atlas_classic_count(const float *__restrict__ crops, const AtlasTask *__restrict__ tasks,
-                      const uint32_t *__restrict__ blockTask, uint8_t *__restrict__ cfgs,
+                      const uint32_t *__restrict__ blockTask, unsigned nwork, uint8_t *__restrict__ cfgs,
                       uint32_t *__restrict__ blockSums)
 {
-	const AtlasTask t = tasks[blockTask[blockIdx.x]];
-	const ClsGeo g = cls_geo_task(t);
-	uint32_t s = classic_count_block(crops + t.off, g, cfgs + t.cube_off, blockIdx.x - t.first_block);
-	if (threadIdx.x == 0)
-		blockSums[blockIdx.x] = s;
+	auto item = [&](unsigned b) {
+		const AtlasTask t = tasks[blockTask[b]];
+		const ClsGeo g = cls_geo_task(t);
+		uint32_t s = classic_count_block(crops + t.off, g, cfgs + t.cube_off, b - t.first_block);
+		if (threadIdx.x == 0)
+			blockSums[b] = s;
+	};
+	N2M_ATLAS_ITEMS(item)
 }
 
+template <bool LOOP>
 __global__ void __launch_bounds__(MC_BS)
 k_atlas_classic_emit(const float *__restrict__ crops, const AtlasTask *__restrict__ tasks,
-                     const uint32_t *__restrict__ blockTask, const uint8_t *__restrict__ cfgs,
+                     const uint32_t *__restrict__ blockTask, unsigned nwork, const uint8_t *__restrict__ cfgs,
                      const uint32_t *__restrict__ blockOffs, double *__restrict__ pts)
 {
-	const AtlasTask t = tasks[blockTask[blockIdx.x]];
-	const ClsGeo g = cls_geo_task(t);
-	classic_emit_block(crops + t.off, g, cfgs + t.cube_off, blockIdx.x - t.first_block, blockOffs[blockIdx.x], pts);
+	auto item = [&](unsigned b) {
+		const AtlasTask t = tasks[blockTask[b]];
+		const ClsGeo g = cls_geo_task(t);
+		classic_emit_block(crops + t.off, g, cfgs + t.cube_off, b - t.first_block, blockOffs[b], pts);
+	};
+	N2M_ATLAS_ITEMS(item)
 }
 
 __device__ __forceinline__ LwGeo lw_geo_task(const AtlasTask &t, int originalMC)
@@ -1124,18 +1145,22 @@ __device__ __forceinline__ LwGeo lw_geo_task(const AtlasTask &t, int originalMC)
 	return g;
 }
 
-// segcnt: 3 arrays `stride` entries apart, one entry per workgroup (= segment)
+// segcnt: 3 arrays `stride` entries apart, one entry per item (= segment)
+template <bool LOOP>
 __global__ void __launch_bounds__(LW_BS)
 k_atlas_lw_classify(const float *__restrict__ crops, const AtlasTask *__restrict__ tasks,
-                    const uint32_t *__restrict__ blockTask, int originalMC, uint8_t *__restrict__ pflags,
-                    uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt, size_t stride)
+                    const uint32_t *__restrict__ blockTask, unsigned nwork, int originalMC,
+                    uint8_t *__restrict__ pflags, uint16_t *__restrict__ crow, uint32_t *__restrict__ segcnt,
+                    size_t stride)
 {
-	const AtlasTask t = tasks[blockTask[blockIdx.x]];
-	const LwGeo g = lw_geo_task(t, originalMC);
-	const int local = blockIdx.x - t.first_block;
-	const int seg = local % g.nseg, y = (local / g.nseg) % g.sy, z = local / (g.nseg * g.sy);
-	lw_classify_row(crops + t.off, g, seg, y, z, pflags + t.cube_off, crow + t.cube_off, segcnt + blockIdx.x,
-	                stride);
+	auto item = [&](unsigned b) {
+		const AtlasTask t = tasks[blockTask[b]];
+		const LwGeo g = lw_geo_task(t, originalMC);
+		const int local = (int)b - t.first_block;
+		const int seg = local % g.nseg, y = (local / g.nseg) % g.sy, z = local / (g.nseg * g.sy);
+		lw_classify_row(crops + t.off, g, seg, y, z, pflags + t.cube_off, crow + t.cube_off, segcnt + b, stride);
+	};
+	N2M_ATLAS_ITEMS(item)
 }
 
 // Vertex numbers follow the reference within each task: the task's edge
@@ -1143,57 +1168,65 @@ k_atlas_lw_classify(const float *__restrict__ crops, const AtlasTask *__restrict
 // the per-segment edge / centre counts over all tasks, task t (first segment
 // f, next task's first segment f') starts at E(f) + C(f); its edge vertex
 // of segment s is E(s) + C(f) and its centre vertex is E(f') + C(s).
+template <bool LOOP>
 __global__ void __launch_bounds__(LW_BS)
 k_atlas_lw_emit(const float *__restrict__ crops, const AtlasTask *__restrict__ tasks, int ntasks,
-                const uint32_t *__restrict__ blockTask, int originalMC, const uint8_t *__restrict__ pflags,
-                const uint16_t *__restrict__ crow, const uint32_t *__restrict__ segcnt,
-                const uint32_t *__restrict__ bases, size_t stride, uint32_t totalE, double *__restrict__ pts,
-                int *__restrict__ tris)
+                const uint32_t *__restrict__ blockTask, unsigned nwork, int originalMC,
+                const uint8_t *__restrict__ pflags, const uint16_t *__restrict__ crow,
+                const uint32_t *__restrict__ segcnt, const uint32_t *__restrict__ bases, size_t stride,
+                uint32_t totalE, double *__restrict__ pts, int *__restrict__ tris)
 {
-	const int ti = blockTask[blockIdx.x];
-	const AtlasTask t = tasks[ti];
-	const LwGeo g = lw_geo_task(t, originalMC);
-	const int local = blockIdx.x - t.first_block;
-	const int seg = local % g.nseg, y = (local / g.nseg) % g.sy, z = local / (g.nseg * g.sy);
-	const uint32_t cbase = bases[2 * stride + t.first_block];
-	const uint32_t enext = ti + 1 < ntasks ? bases[tasks[ti + 1].first_block] : totalE;
-	lw_emit_row(crops + t.off, g, seg, y, z, pflags + t.cube_off, crow + t.cube_off, bases + t.first_block, stride,
-	            cbase, enext, pts, tris, segcnt + blockIdx.x, stride);
+	auto item = [&](unsigned b) {
+		const int ti = (int)blockTask[b];
+		const AtlasTask t = tasks[ti];
+		const LwGeo g = lw_geo_task(t, originalMC);
+		const int local = (int)b - t.first_block;
+		const int seg = local % g.nseg, y = (local / g.nseg) % g.sy, z = local / (g.nseg * g.sy);
+		const uint32_t cbase = bases[2 * stride + t.first_block];
+		const uint32_t enext = ti + 1 < ntasks ? bases[tasks[ti + 1].first_block] : totalE;
+		lw_emit_row(crops + t.off, g, seg, y, z, pflags + t.cube_off, crow + t.cube_off, bases + t.first_block,
+		            stride, cbase, enext, pts, tris, segcnt + b, stride);
+	};
+	N2M_ATLAS_ITEMS(item)
 }
+#undef N2M_ATLAS_ITEMS
+
+// the persistent form only when the grid is smaller than the work
+#define N2M_ATLAS_LAUNCH(K, BS, ...)                                                                               \
+	if (!nblocks)                                                                                                 \
+		return;                                                                                                    \
+	if (grid < nblocks)                                                                                           \
+		hipLaunchKernelGGL(K<true>, dim3(grid), dim3(BS), 0, s, __VA_ARGS__);                                      \
+	else                                                                                                          \
+		hipLaunchKernelGGL(K<false>, dim3(nblocks), dim3(BS), 0, s, __VA_ARGS__);
 
 void launch_atlas_classic_count(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
-                                unsigned nblocks, uint8_t *cfgs, uint32_t *blockSums, hipStream_t s)
+                                unsigned nblocks, unsigned grid, uint8_t *cfgs, uint32_t *blockSums, hipStream_t s)
 {
-	if (nblocks)
-		hipLaunchKernelGGL(k_atlas_classic_count, dim3(nblocks), dim3(MC_BS), 0, s, crops, tasks, blockTask, cfgs,
-		                   blockSums);
+	N2M_ATLAS_LAUNCH(k_atlas_classic_count, MC_BS, crops, tasks, blockTask, nblocks, cfgs, blockSums)
 }
 void launch_atlas_classic_emit(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
-                               unsigned nblocks, const uint8_t *cfgs, const uint32_t *blockOffs, double *pts,
-                               hipStream_t s)
+                               unsigned nblocks, unsigned grid, const uint8_t *cfgs, const uint32_t *blockOffs,
+                               double *pts, hipStream_t s)
 {
-	if (nblocks)
-		hipLaunchKernelGGL(k_atlas_classic_emit, dim3(nblocks), dim3(MC_BS), 0, s, crops, tasks, blockTask, cfgs,
-		                   blockOffs, pts);
+	N2M_ATLAS_LAUNCH(k_atlas_classic_emit, MC_BS, crops, tasks, blockTask, nblocks, cfgs, blockOffs, pts)
 }
 void launch_atlas_lw_classify(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
-                              unsigned nblocks, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
-                              size_t stride, hipStream_t s)
+                              unsigned nblocks, unsigned grid, int originalMC, uint8_t *pflags, uint16_t *crow,
+                              uint32_t *segcnt, size_t stride, hipStream_t s)
 {
-	if (nblocks)
-		hipLaunchKernelGGL(k_atlas_lw_classify, dim3(nblocks), dim3(LW_BS), 0, s, crops, tasks, blockTask,
-		                   originalMC, pflags, crow, segcnt, stride);
+	N2M_ATLAS_LAUNCH(k_atlas_lw_classify, LW_BS, crops, tasks, blockTask, nblocks, originalMC, pflags, crow, segcnt,
+	                 stride)
 }
 void launch_atlas_lw_emit(const float *crops, const AtlasTask *tasks, int ntasks, const uint32_t *blockTask,
-                          unsigned nblocks, int originalMC, const uint8_t *pflags, const uint16_t *crow,
+                          unsigned nblocks, unsigned grid, int originalMC, const uint8_t *pflags, const uint16_t *crow,
                           const uint32_t *segcnt, const uint32_t *bases, size_t stride, uint32_t totalE,
-                          double *pts, int *tris,
-                          hipStream_t s)
+                          double *pts, int *tris, hipStream_t s)
 {
-	if (nblocks)
-		hipLaunchKernelGGL(k_atlas_lw_emit, dim3(nblocks), dim3(LW_BS), 0, s, crops, tasks, ntasks, blockTask,
-		                   originalMC, pflags, crow, segcnt, bases, stride, totalE, pts, tris);
+	N2M_ATLAS_LAUNCH(k_atlas_lw_emit, LW_BS, crops, tasks, ntasks, blockTask, nblocks, originalMC, pflags, crow,
+	                 segcnt, bases, stride, totalE, pts, tris)
 }
+#undef N2M_ATLAS_LAUNCH
 int lw_seg_len() { return LW_SEG; }
 
 } // namespace n2m

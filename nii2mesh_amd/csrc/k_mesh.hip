@@ -127,25 +127,43 @@ void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total,
 // ---------------------------------------------------------------------------
 constexpr int RS_BS = 256, RS_IPT = 16, RS_TILE = RS_BS * RS_IPT, RS_NW = RS_BS / N2M_WAVE;
 
+// The tile a workgroup sorts: keys [start, start + len), digit d's count at
+// counts[cbase + d * cstride].  Without descriptors tile b is keys
+// [b * RS_TILE, ...) and the counts are digit-major over all tiles.
+__device__ __forceinline__ RsTile rs_tile(const RsTile *__restrict__ desc, size_t n, size_t nblocks)
+{
+	if (desc)
+		return desc[blockIdx.x];
+	const size_t base = blockIdx.x * (size_t)RS_TILE;
+	RsTile t;
+	t.start = (uint32_t)base;
+	t.len = (uint32_t)(n - base < (size_t)RS_TILE ? n - base : (size_t)RS_TILE);
+	t.cbase = blockIdx.x;
+	t.cstride = (uint32_t)nblocks;
+	return t;
+}
+
 __global__ void __launch_bounds__(RS_BS)
-k_radix_hist(const uint32_t *__restrict__ keys, size_t n, int shift, uint32_t *__restrict__ counts, size_t nblocks)
+k_radix_hist(const uint32_t *__restrict__ keys, size_t n, int shift, uint32_t *__restrict__ counts, size_t nblocks,
+             const RsTile *__restrict__ desc)
 {
 	__shared__ uint32_t h[256];
 	h[threadIdx.x] = 0;
 	__syncthreads();
-	size_t base = blockIdx.x * (size_t)RS_TILE;
+	const RsTile t = rs_tile(desc, n, nblocks);
 	for (int k = 0; k < RS_IPT; k++) {
-		size_t i = base + k * RS_BS + threadIdx.x;
-		if (i < n)
-			atomicAdd(&h[(keys[i] >> shift) & 0xFF], 1u);
+		const uint32_t i = k * RS_BS + threadIdx.x;
+		if (i < t.len)
+			atomicAdd(&h[(keys[t.start + i] >> shift) & 0xFF], 1u);
 	}
 	__syncthreads();
-	counts[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+	counts[t.cbase + (size_t)threadIdx.x * t.cstride] = h[threadIdx.x];
 }
 
 __global__ void __launch_bounds__(RS_BS)
 k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint32_t *__restrict__ kout,
-                uint32_t *__restrict__ vout, size_t n, int shift, const uint32_t *__restrict__ offs, size_t nblocks)
+                uint32_t *__restrict__ vout, size_t n, int shift, const uint32_t *__restrict__ offs, size_t nblocks,
+                const RsTile *__restrict__ desc)
 {
 	__shared__ uint32_t s_k[RS_TILE], s_v[RS_TILE];
 	__shared__ uint32_t run[256];  // keys of each digit placed so far
@@ -153,10 +171,11 @@ k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ v
 	__shared__ uint32_t gbase[256];
 	__shared__ uint32_t wcnt[RS_NW][256];
 	__shared__ uint32_t scratch[RS_BS / N2M_WAVE + 1];
-	const size_t base = blockIdx.x * (size_t)RS_TILE;
-	const int tn = (int)(n - base < (size_t)RS_TILE ? n - base : (size_t)RS_TILE);
+	const RsTile t = rs_tile(desc, n, nblocks);
+	const size_t base = t.start;
+	const int tn = (int)t.len;
 	run[threadIdx.x] = 0;
-	gbase[threadIdx.x] = offs[(size_t)threadIdx.x * nblocks + blockIdx.x];
+	gbase[threadIdx.x] = offs[t.cbase + (size_t)threadIdx.x * t.cstride];
 	for (int w = 0; w < RS_NW; w++)
 		wcnt[w][threadIdx.x] = 0;
 	__syncthreads();
@@ -213,24 +232,59 @@ k_radix_scatter(const uint32_t *__restrict__ kin, const uint32_t *__restrict__ v
 }
 
 // Sorts (k0,v0) using (k1,v1) as the ping-pong buffers; returns true when
-// the result is in (k1,v1).  counts/offs hold 256*nblocks words each.
+// the result is in (k1,v1).  counts/offs hold 256*nblocks words each.  With
+// tile descriptors (radix_seg_tiles) every segment is sorted on its own and
+// stays where it is: a segment's tiles count their digits in a region of
+// their own, laid out digit-major after the regions of the earlier segments,
+// so the one scan of all counts places each key inside its segment.
+static bool radix_passes(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int bits,
+                         uint32_t *counts, uint32_t *scan_scratch, size_t nb, const RsTile *desc, hipStream_t s)
+{
+	bool flip = false;
+	for (int shift = 0; shift < bits; shift += 8) {
+		const uint32_t *ki = flip ? k1 : k0, *vi = flip ? v1 : v0;
+		uint32_t *ko = flip ? k0 : k1, *vo = flip ? v0 : v1;
+		hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(RS_BS), 0, s, ki, n, shift, counts, nb, desc);
+		scan_excl_u32(counts, counts, 256 * nb, nullptr, scan_scratch, s);
+		hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(RS_BS), 0, s, ki, vi, ko, vo, n, shift,
+		                   counts, nb, desc);
+		flip = !flip;
+	}
+	return flip;
+}
+
 bool radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int bits,
                     uint32_t *counts, uint32_t *scan_scratch, hipStream_t s)
 {
 	if (n <= 1)
 		return false;
-	size_t nb = (n + RS_TILE - 1) / RS_TILE;
-	bool flip = false;
-	for (int shift = 0; shift < bits; shift += 8) {
-		const uint32_t *ki = flip ? k1 : k0, *vi = flip ? v1 : v0;
-		uint32_t *ko = flip ? k0 : k1, *vo = flip ? v0 : v1;
-		hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(RS_BS), 0, s, ki, n, shift, counts, nb);
-		scan_excl_u32(counts, counts, 256 * nb, nullptr, scan_scratch, s);
-		hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(RS_BS), 0, s, ki, vi, ko, vo, n, shift,
-		                   counts, nb);
-		flip = !flip;
+	return radix_passes(k0, v0, k1, v1, n, bits, counts, scan_scratch, (n + RS_TILE - 1) / RS_TILE, nullptr, s);
+}
+
+std::vector<RsTile> radix_seg_tiles(const std::vector<uint32_t> &sb)
+{
+	std::vector<RsTile> d;
+	for (size_t k = 0; k + 1 < sb.size(); k++) {
+		const uint32_t len = sb[k + 1] - sb[k], nt = (len + RS_TILE - 1) / RS_TILE;
+		const uint32_t first = (uint32_t)d.size();
+		for (uint32_t j = 0; j < nt; j++) {
+			RsTile t;
+			t.start = sb[k] + j * RS_TILE;
+			t.len = len - j * RS_TILE < (uint32_t)RS_TILE ? len - j * RS_TILE : (uint32_t)RS_TILE;
+			t.cbase = 256 * first + j;
+			t.cstride = nt;
+			d.push_back(t);
+		}
 	}
-	return flip;
+	return d;
+}
+
+bool radix_sort_seg_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int bits,
+                        const RsTile *desc, size_t ntiles, uint32_t *counts, uint32_t *scan_scratch, hipStream_t s)
+{
+	if (n <= 1 || !ntiles)
+		return false;
+	return radix_passes(k0, v0, k1, v1, n, bits, counts, scan_scratch, ntiles, desc, s);
 }
 
 size_t radix_counts_words(size_t n) { return 256 * ((n + RS_TILE - 1) / RS_TILE) + 1; }
@@ -295,10 +349,29 @@ __global__ void k_weld_segstarts(const uint32_t *__restrict__ flags, const uint3
 // d_nseg (when given) holds the segment count on the device: the launch
 // covers nseg = n threads, and those past the count write a zero seed count
 // (so that the scan of seeds can run over n words without a host round trip)
+__device__ __forceinline__ double dist3v(double ax, double ay, double az, double bx, double by, double bz)
+{
+	double dx = ax - bx, dy = ay - by, dz = az - bz;
+	return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// Segments of up to WG_REG vertices (most of them: a welded vertex's copies
+// share one distance key) run the loop below from registers: their keys,
+// ids and positions are loaded up front, all loads in flight together,
+// instead of one dependent load chain per comparison.
+constexpr int WG_REG = 8;
+
+// lrep (when given): at each seed's sorted position the largest vertex id
+// whose final class is the seed's (the member whose position the reference
+// keeps, src/meshify.c:100-101), WG_NOSEED elsewhere, so that the class
+// representatives need no atomics.  A seed can claim a vertex an earlier
+// seed claimed; a segment where that happened takes the maxima again over
+// the final classes.
+constexpr uint32_t WG_NOSEED = 0xffffffffu;
 __global__ void k_weld_greedy(const uint32_t *__restrict__ starts, uint32_t nseg, uint32_t n,
                               const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
                               const double *__restrict__ pts, int *__restrict__ lcls, uint32_t *__restrict__ seeds,
-                              const uint32_t *__restrict__ d_nseg)
+                              const uint32_t *__restrict__ d_nseg, uint32_t *__restrict__ lrep)
 {
 	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= nseg)
@@ -313,12 +386,73 @@ __global__ void k_weld_greedy(const uint32_t *__restrict__ starts, uint32_t nseg
 	}
 	const float tol = 0.00001f;
 	uint32_t b = starts[s], e = s + 1 < nseg ? starts[s + 1] : n;
+	const uint32_t len = e - b;
+	if (len <= (uint32_t)WG_REG) {
+		float k[WG_REG];
+		uint32_t v[WG_REG];
+		double px[WG_REG], py[WG_REG], pz[WG_REG];
+		int cl[WG_REG];
+#pragma unroll
+		for (int q = 0; q < WG_REG; q++) {
+			cl[q] = -1;
+			if ((uint32_t)q < len) {
+				k[q] = ord2f(keys[b + q]);
+				v[q] = vals[b + q];
+			}
+		}
+#pragma unroll
+		for (int q = 0; q < WG_REG; q++)
+			if ((uint32_t)q < len) {
+				const double *p = pts + (size_t)v[q] * 3;
+				px[q] = p[0];
+				py[q] = p[1];
+				pz[q] = p[2];
+			}
+		int ns = 0;
+		unsigned seedmask = 0;
+#pragma unroll
+		for (int i = 0; i < WG_REG; i++) {
+			if ((uint32_t)i >= len || cl[i] >= 0)
+				continue;
+			cl[i] = ns++;
+			seedmask |= 1u << i;
+			// the reference's window stops at the first key >= k[i] + tol;
+			// keys ascend and fl(k[j] - k[i]) ascends with them
+#pragma unroll
+			for (int j = i + 1; j < WG_REG; j++)
+				if ((uint32_t)j < len && (k[j] - k[i]) < tol &&
+				    dist3v(px[i], py[i], pz[i], px[j], py[j], pz[j]) < (double)tol)
+					cl[j] = cl[i];
+		}
+#pragma unroll
+		for (int q = 0; q < WG_REG; q++)
+			if ((uint32_t)q < len) {
+				lcls[b + q] = cl[q];
+				if (lrep) {
+					uint32_t hi = WG_NOSEED;
+					if ((seedmask >> q) & 1u) { // members follow their seed
+						hi = v[q];
+#pragma unroll
+						for (int j = q + 1; j < WG_REG; j++)
+							if ((uint32_t)j < len && cl[j] == cl[q])
+								hi = max(hi, v[j]);
+					}
+					lrep[b + q] = hi;
+				}
+			}
+		seeds[s] = (uint32_t)ns;
+		return;
+	}
 	for (uint32_t i = b; i < e; i++)
 		lcls[i] = -1;
 	int ns = 0;
+	bool stolen = false;
 	for (uint32_t i = b; i < e; i++) {
-		if (lcls[i] >= 0)
+		if (lcls[i] >= 0) {
+			if (lrep)
+				lrep[i] = WG_NOSEED;
 			continue;
+		}
 		const float k0 = ord2f(keys[i]);
 		const double *seed = pts + (size_t)vals[i] * 3;
 		// the seed is always in its own class: for a finite position its
@@ -326,30 +460,58 @@ __global__ void k_weld_greedy(const uint32_t *__restrict__ starts, uint32_t nseg
 		// nothing, and the reference would then leave it unassigned
 		// (old2new -1, src/meshify.c:66-79)
 		lcls[i] = ns;
+		uint32_t hi = vals[i];
 		for (uint32_t j = i + 1; j < e && (ord2f(keys[j]) - k0) < tol; j++)
-			if (dist3(seed, pts + (size_t)vals[j] * 3) < (double)tol)
+			if (dist3(seed, pts + (size_t)vals[j] * 3) < (double)tol) {
+				stolen |= lcls[j] >= 0;
 				lcls[j] = ns;
+				hi = max(hi, vals[j]);
+			}
+		if (lrep)
+			lrep[i] = hi;
 		ns++;
 	}
+	if (lrep && stolen) // members follow their seed, within its key window
+		for (uint32_t i = b; i < e; i++) {
+			if (lrep[i] == WG_NOSEED)
+				continue;
+			const int c = lcls[i];
+			const float k0 = ord2f(keys[i]);
+			uint32_t hi = vals[i];
+			for (uint32_t j = i + 1; j < e && (ord2f(keys[j]) - k0) < tol; j++)
+				if (lcls[j] == c)
+					hi = max(hi, vals[j]);
+			lrep[i] = hi;
+		}
 	seeds[s] = (uint32_t)ns;
 }
 
 // d_nnew (when given): the class count on the device; when it equals n no
 // vertex was merged and, as the reference then keeps its input unchanged
 // (src/meshify.c:82-87), every vertex keeps its own number
+// The representative of each class (the last member in index order provides
+// the position): from lrep at the seeds when given, else by atomics on a rep
+// array the caller filled with -1.
 __global__ void k_weld_assign(const uint32_t *__restrict__ vals, const int *__restrict__ lcls,
                               const uint32_t *__restrict__ flags, const uint32_t *__restrict__ segid,
                               const uint32_t *__restrict__ seedbase, uint32_t n, int *__restrict__ cls,
-                              int *__restrict__ rep, const uint32_t *__restrict__ d_nnew)
+                              int *__restrict__ rep, const uint32_t *__restrict__ d_nnew,
+                              const uint32_t *__restrict__ lrep)
 {
 	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
 	uint32_t sg = segid[i] + flags[i] - 1; // inclusive segment index
 	uint32_t v = vals[i];
-	const int c = (d_nnew && *d_nnew == n) ? (int)v : (int)seedbase[sg] + lcls[i];
+	const bool ident = d_nnew && *d_nnew == n;
+	const int c = ident ? (int)v : (int)seedbase[sg] + lcls[i];
 	cls[v] = c;
-	atomicMax(&rep[c], (int)v); // last member in index order provides the position
+	if (!lrep)
+		atomicMax(&rep[c], (int)v);
+	else if (ident)
+		rep[c] = (int)v;
+	else if (lrep[i] != WG_NOSEED)
+		rep[c] = (int)lrep[i];
 }
 
 __global__ void k_gather_pts(const double *__restrict__ pts, const int *__restrict__ rep, uint32_t n,
@@ -407,18 +569,18 @@ void launch_weld_segstarts(const uint32_t *flags, const uint32_t *segid, uint32_
 	hipLaunchKernelGGL(k_weld_segstarts, dim3(g1(n)), dim3(256), 0, s, flags, segid, n, starts);
 }
 void launch_weld_greedy(const uint32_t *starts, uint32_t nseg, uint32_t n, const uint32_t *keys,
-                        const uint32_t *vals, const double *pts, int *lcls, uint32_t *seeds, hipStream_t s,
-                        const uint32_t *d_nseg)
+                        const uint32_t *vals, const double *pts, int *lcls, uint32_t *seeds, uint32_t *lrep,
+                        hipStream_t s, const uint32_t *d_nseg)
 {
 	hipLaunchKernelGGL(k_weld_greedy, dim3(g1(nseg)), dim3(256), 0, s, starts, nseg, n, keys, vals, pts, lcls,
-	                   seeds, d_nseg);
+	                   seeds, d_nseg, lrep);
 }
 void launch_weld_assign(const uint32_t *vals, const int *lcls, const uint32_t *flags, const uint32_t *segid,
-                        const uint32_t *seedbase, uint32_t n, int *cls, int *rep, hipStream_t s,
-                        const uint32_t *d_nnew)
+                        const uint32_t *seedbase, const uint32_t *lrep, uint32_t n, int *cls, int *rep,
+                        hipStream_t s, const uint32_t *d_nnew)
 {
 	hipLaunchKernelGGL(k_weld_assign, dim3(g1(n)), dim3(256), 0, s, vals, lcls, flags, segid, seedbase, n, cls,
-	                   rep, d_nnew);
+	                   rep, d_nnew, lrep);
 }
 void launch_gather_pts(const double *pts, const int *rep, uint32_t n, double *out, hipStream_t s,
                        const uint32_t *d_n)
@@ -2019,28 +2181,6 @@ __global__ void k_atlas_weld_keys(const double *__restrict__ pts, uint32_t n, co
 	vtask[i] = (uint32_t)t;
 }
 
-// second sort key: the task of each key-sorted vertex; value = its position
-__global__ void k_atlas_task_keys(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ vtask, uint32_t n,
-                                  uint32_t *tk, uint32_t *pos)
-{
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
-	tk[i] = vtask[vals[i]];
-	pos[i] = i;
-}
-
-__global__ void k_atlas_weld_gather(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-                                    const uint32_t *__restrict__ pos, uint32_t n, uint32_t *keys2, uint32_t *vals2)
-{
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
-	uint32_t p = pos[i];
-	keys2[i] = keys[p];
-	vals2[i] = vals[p];
-}
-
 __global__ void k_atlas_weld_segflags(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ tk, uint32_t n,
                                       uint32_t *flags)
 {
@@ -2063,6 +2203,33 @@ __global__ void k_atlas_class_base(const uint32_t *__restrict__ tk, const uint32
 		return;
 	if (i == 0 || tk[i] != tk[i - 1])
 		cbase[tk[i]] = seedbase[segid[i]];
+}
+
+// Per task, from the class bases the assign pass left: its class count
+// (next non-empty task's base, or the total, minus its own), and whether
+// its weld merged nothing (class count = vertex count).  nextne[t]: the
+// next task with vertices, ntasks for none.  An empty task takes the next
+// base and count 0.
+__global__ void k_atlas_weld_tasks(uint32_t *__restrict__ cbase, const uint32_t *__restrict__ vbase,
+                                   const uint32_t *__restrict__ nextne, int ntasks,
+                                   const uint32_t *__restrict__ d_nnew, uint32_t *__restrict__ ncls,
+                                   uint8_t *__restrict__ ident)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= ntasks)
+		return;
+	const uint32_t nx = nextne[t];
+	const uint32_t next = nx < (uint32_t)ntasks ? cbase[nx] : *d_nnew;
+	const uint32_t nv = vbase[t + 1] - vbase[t];
+	if (!nv) {
+		cbase[t] = next; // no thread reads an empty task's base
+		ncls[t] = 0;
+		ident[t] = 0;
+		return;
+	}
+	const uint32_t c = next - cbase[t];
+	ncls[t] = c;
+	ident[t] = c == nv;
 }
 
 // a task whose weld merged nothing keeps its vertex order (src/meshify.c:82-87)
@@ -2103,18 +2270,6 @@ void launch_atlas_weld_keys(const double *pts, uint32_t n, const uint32_t *vbase
 		hipLaunchKernelGGL(k_atlas_weld_keys, dim3(g1(n)), dim3(256), 0, s, pts, n, vbase, ntasks, keys, vals,
 		                   vtask);
 }
-void launch_atlas_task_keys(const uint32_t *vals, const uint32_t *vtask, uint32_t n, uint32_t *tk, uint32_t *pos,
-                            hipStream_t s)
-{
-	if (n)
-		hipLaunchKernelGGL(k_atlas_task_keys, dim3(g1(n)), dim3(256), 0, s, vals, vtask, n, tk, pos);
-}
-void launch_atlas_weld_gather(const uint32_t *keys, const uint32_t *vals, const uint32_t *pos, uint32_t n,
-                              uint32_t *keys2, uint32_t *vals2, hipStream_t s)
-{
-	if (n)
-		hipLaunchKernelGGL(k_atlas_weld_gather, dim3(g1(n)), dim3(256), 0, s, keys, vals, pos, n, keys2, vals2);
-}
 void launch_atlas_weld_segflags(const uint32_t *keys, const uint32_t *tk, uint32_t n, uint32_t *flags,
                                 hipStream_t s)
 {
@@ -2126,6 +2281,13 @@ void launch_atlas_class_base(const uint32_t *tk, const uint32_t *segid, const ui
 {
 	if (n)
 		hipLaunchKernelGGL(k_atlas_class_base, dim3(g1(n)), dim3(256), 0, s, tk, segid, seedbase, n, cbase);
+}
+void launch_atlas_weld_tasks(uint32_t *cbase, const uint32_t *vbase, const uint32_t *nextne, int ntasks,
+                             const uint32_t *d_nnew, uint32_t *ncls, uint8_t *ident, hipStream_t s)
+{
+	if (ntasks)
+		hipLaunchKernelGGL(k_atlas_weld_tasks, dim3((ntasks + 255) / 256), dim3(256), 0, s, cbase, vbase, nextne,
+		                   ntasks, d_nnew, ncls, ident);
 }
 void launch_atlas_weld_identity(uint32_t n, const uint32_t *vtask, const uint32_t *vbase, const uint32_t *cbase,
                                 const uint8_t *ident, int *cls, int *rep, hipStream_t s)

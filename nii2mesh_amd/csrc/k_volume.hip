@@ -205,8 +205,51 @@ __device__ __forceinline__ int atlas_label_of(float v, int nlabel)
 }
 
 // per label: voxel count and box (stats[7*i] = count, min xyz, max xyz);
-// PRIV: accumulate in LDS (7 * (nlabel + 1) words) and flush once per block
-template <bool PRIV>
+// PRIV: accumulate in LDS (7 * (nlabel + 1) words) and flush once per block.
+// Each block reads one contiguous run of voxels, AS_UNROLL loads per thread
+// in flight, so it meets few labels and flushes few.  Within a wave the
+// voxels are consecutive along x: a run of lanes with one label in one row
+// is merged by its first lane (count = run length, x from its two ends, y
+// and z shared), one set of atomics per run.  I32: the volume has fewer
+// than 2^32 voxels, so coordinates take 32-bit division.
+constexpr int AS_UNROLL = 4;
+
+template <bool PRIV, bool I32>
+__device__ __forceinline__ void atlas_stats_voxel(int i, size_t v, int nx, int ny, uint32_t *__restrict__ s_st,
+                                                  uint32_t *__restrict__ stats)
+{
+	if (!__ballot(i != 0))
+		return;
+	uint32_t x, y, z;
+	if (I32) {
+		const uint32_t w = (uint32_t)v, r = w / (uint32_t)nx;
+		x = w - r * (uint32_t)nx;
+		y = r % (uint32_t)ny;
+		z = r / (uint32_t)ny;
+	} else {
+		x = (uint32_t)(v % nx);
+		y = (uint32_t)((v / nx) % ny);
+		z = (uint32_t)(v / ((size_t)nx * ny));
+	}
+	const int lane = lane_id();
+	const int iprev = __shfl_up(i, 1);
+	const bool head = lane == 0 || i != iprev || x == 0; // a run starts
+	const unsigned long long heads = __ballot(head);
+	if (!head || !i)
+		return;
+	const unsigned long long after = lane == N2M_WAVE - 1 ? 0ull : heads & (~0ull << (lane + 1));
+	const uint32_t len = (uint32_t)((after ? __ffsll((long long)after) - 1 : N2M_WAVE) - lane);
+	uint32_t *st = PRIV ? s_st + 7 * i : stats + 7 * i;
+	atomicAdd(&st[0], len);
+	atomicMin(&st[1], x);
+	atomicMin(&st[2], y);
+	atomicMin(&st[3], z);
+	atomicMax(&st[4], x + len - 1);
+	atomicMax(&st[5], y);
+	atomicMax(&st[6], z);
+}
+
+template <bool PRIV, bool I32>
 __global__ void __launch_bounds__(256)
 k_atlas_stats(const float *__restrict__ labels, int nx, int ny, int nz, int nlabel, uint32_t *__restrict__ stats)
 {
@@ -220,19 +263,20 @@ k_atlas_stats(const float *__restrict__ labels, int nx, int ny, int nz, int nlab
 		__syncthreads();
 	}
 	const size_t n = (size_t)nx * ny * nz;
-	for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
-		int i = atlas_label_of(labels[v], nlabel);
-		if (!i)
-			continue;
-		uint32_t x = (uint32_t)(v % nx), y = (uint32_t)((v / nx) % ny), z = (uint32_t)(v / ((size_t)nx * ny));
-		uint32_t *st = PRIV ? s_st + 7 * i : stats + 7 * i;
-		atomicAdd(&st[0], 1u);
-		atomicMin(&st[1], x);
-		atomicMin(&st[2], y);
-		atomicMin(&st[3], z);
-		atomicMax(&st[4], x);
-		atomicMax(&st[5], y);
-		atomicMax(&st[6], z);
+	constexpr size_t STEP = 256 * AS_UNROLL;
+	const size_t per = ((n + gridDim.x - 1) / gridDim.x + STEP - 1) / STEP * STEP;
+	const size_t v0 = blockIdx.x * per, v1 = v0 + per < n ? v0 + per : n;
+	for (size_t vb = v0; vb < v1; vb += STEP) { // uniform trip count: whole waves take part
+		float val[AS_UNROLL];
+#pragma unroll
+		for (int u = 0; u < AS_UNROLL; u++) {
+			const size_t v = vb + u * 256 + threadIdx.x;
+			val[u] = v < v1 ? labels[v] : 0.0f;
+		}
+#pragma unroll
+		for (int u = 0; u < AS_UNROLL; u++)
+			atlas_stats_voxel<PRIV, I32>(atlas_label_of(val[u], nlabel), vb + u * 256 + threadIdx.x, nx, ny, s_st,
+			                             stats);
 	}
 	if (!PRIV)
 		return;
@@ -251,14 +295,22 @@ k_atlas_stats(const float *__restrict__ labels, int nx, int ny, int nz, int nlab
 void launch_atlas_stats(const float *labels, int nx, int ny, int nz, int nlabel, uint32_t *stats, hipStream_t s)
 {
 	size_t n = (size_t)nx * ny * nz;
-	size_t b = (n + 255) / 256;
+	size_t b = (n + 1023) / 1024;
 	b = b > 2048 ? 2048 : (b < 1 ? 1 : b);
-	if (nlabel <= 1023)
-		hipLaunchKernelGGL(k_atlas_stats<true>, dim3((unsigned)b), dim3(256), (size_t)7 * (nlabel + 1) * 4, s, labels,
-		                   nx, ny, nz, nlabel, stats);
-	else
-		hipLaunchKernelGGL(k_atlas_stats<false>, dim3((unsigned)b), dim3(256), 0, s, labels, nx, ny, nz, nlabel,
+	const size_t lds = (size_t)7 * (nlabel + 1) * 4;
+	const bool i32 = n <= 0xffffffffull;
+	if (nlabel <= 1023 && i32)
+		hipLaunchKernelGGL((k_atlas_stats<true, true>), dim3((unsigned)b), dim3(256), lds, s, labels, nx, ny, nz, nlabel,
 		                   stats);
+	else if (nlabel <= 1023)
+		hipLaunchKernelGGL((k_atlas_stats<true, false>), dim3((unsigned)b), dim3(256), lds, s, labels, nx, ny, nz,
+		                   nlabel, stats);
+	else if (i32)
+		hipLaunchKernelGGL((k_atlas_stats<false, true>), dim3((unsigned)b), dim3(256), 0, s, labels, nx, ny, nz, nlabel,
+		                   stats);
+	else
+		hipLaunchKernelGGL((k_atlas_stats<false, false>), dim3((unsigned)b), dim3(256), 0, s, labels, nx, ny, nz,
+		                   nlabel, stats);
 }
 
 // the binarised label (src/nii2mesh.c:557-563) read inside a task's crop
@@ -278,15 +330,18 @@ struct LabelSrc {
 // Crop tile shapes the atlas autotuner chooses from (x, y, z-chunk).
 #define N2M_ATLAS_TILES(X) X(0, 64, 8, 32) X(1, 32, 8, 16) X(2, 16, 16, 16) X(3, 32, 8, 32)
 
-// smooth (or just binarise) every task's crop; ranges[2*t] = ord min/max
+// smooth (or just binarise) every task's crop; ranges[2*t] = ord min/max.
+// LOOP: persistent, the grid walks the `nwork` items of the block map in
+// strides of its own size; otherwise workgroup b takes item b.
 template <int TX, int TY, int ZC>
-__global__ void __launch_bounds__(SM_BS)
-k_atlas_smooth(const float *__restrict__ labels, int gnx, int gny, const AtlasTask *__restrict__ tasks,
-               const uint32_t *__restrict__ blockTask, int smooth, float *__restrict__ crops,
-               uint32_t *__restrict__ ranges)
+__device__ __forceinline__ void atlas_smooth_item(const float *__restrict__ labels, int gnx, int gny,
+                                                  const AtlasTask *__restrict__ tasks,
+                                                  const uint32_t *__restrict__ blockTask, unsigned b, int smooth,
+                                                  float *__restrict__ crops, uint32_t *__restrict__ ranges)
 {
-	const AtlasTask t = tasks[blockTask[blockIdx.x]];
-	const int local = blockIdx.x - t.first_block;
+	const int ti = (int)blockTask[b];
+	const AtlasTask t = tasks[ti];
+	const int local = (int)b - t.first_block;
 	LabelSrc src{labels, gnx, (size_t)gnx * gny, t.ox, t.oy, t.oz, t.lo, t.hi};
 	float *out = crops + t.off;
 	float lmin = __builtin_inff(), lmax = -__builtin_inff();
@@ -294,8 +349,7 @@ k_atlas_smooth(const float *__restrict__ labels, int gnx, int gny, const AtlasTa
 		const int tx = (t.nx + TX - 1) / TX, ty = (t.ny + TY - 1) / TY;
 		const int bx = local % tx, by = (local / tx) % ty, bz = local / (tx * ty);
 		const int z0 = bz * ZC, z1 = min(z0 + ZC, t.nz);
-		smooth_tile<TX, TY>(src, out, t.nx, t.ny, t.nz, bx * TX, by * TY, z0, z1, IsoMasks{}, nullptr, 0, lmin,
-		                    lmax);
+		smooth_tile<TX, TY>(src, out, t.nx, t.ny, t.nz, bx * TX, by * TY, z0, z1, IsoMasks{}, nullptr, 0, lmin, lmax);
 	} else {
 		const size_t n = (size_t)t.nx * t.ny * t.nz;
 		const size_t v = (size_t)local * SM_BS + threadIdx.x;
@@ -307,7 +361,23 @@ k_atlas_smooth(const float *__restrict__ labels, int gnx, int gny, const AtlasTa
 			lmax = o;
 		}
 	}
-	reduce_range(lmin, lmax, &ranges[2 * blockTask[blockIdx.x]]);
+	reduce_range(lmin, lmax, &ranges[2 * ti]);
+}
+
+template <int TX, int TY, int ZC, bool LOOP>
+__global__ void __launch_bounds__(SM_BS)
+k_atlas_smooth(const float *__restrict__ labels, int gnx, int gny, const AtlasTask *__restrict__ tasks,
+               const uint32_t *__restrict__ blockTask, unsigned nwork, int smooth, float *__restrict__ crops,
+               uint32_t *__restrict__ ranges)
+{
+	if (!LOOP) {
+		atlas_smooth_item<TX, TY, ZC>(labels, gnx, gny, tasks, blockTask, blockIdx.x, smooth, crops, ranges);
+		return;
+	}
+	for (unsigned b = blockIdx.x; b < nwork; b += gridDim.x) {
+		atlas_smooth_item<TX, TY, ZC>(labels, gnx, gny, tasks, blockTask, b, smooth, crops, ranges);
+		__syncthreads(); // the next item reuses the LDS
+	}
 }
 
 int atlas_tile_count()
@@ -334,15 +404,19 @@ void atlas_tile_shape(int tile, int shape[3])
 }
 
 void launch_atlas_smooth(const float *labels, int gnx, int gny, int gnz, const AtlasTask *tasks,
-                         const uint32_t *blockTask, unsigned nblocks, int smooth, int tile, float *crops,
-                         uint32_t *ranges, hipStream_t s)
+                         const uint32_t *blockTask, unsigned nblocks, unsigned grid, int smooth, int tile,
+                         float *crops, uint32_t *ranges, hipStream_t s)
 {
 	if (!nblocks)
 		return;
 #define N2M_LAUNCH_TILE(i, tx, ty, zc)                                                                             \
 	if (tile == i) {                                                                                              \
-		hipLaunchKernelGGL((k_atlas_smooth<tx, ty, zc>), dim3(nblocks), dim3(SM_BS), 0, s, labels, gnx, gny, tasks, \
-		                   blockTask, smooth, crops, ranges);                                                      \
+		if (grid < nblocks)                                                                                        \
+			hipLaunchKernelGGL((k_atlas_smooth<tx, ty, zc, true>), dim3(grid), dim3(SM_BS), 0, s, labels, gnx, gny, \
+			                   tasks, blockTask, nblocks, smooth, crops, ranges);                                   \
+		else                                                                                                       \
+			hipLaunchKernelGGL((k_atlas_smooth<tx, ty, zc, false>), dim3(nblocks), dim3(SM_BS), 0, s, labels, gnx,  \
+			                   gny, tasks, blockTask, nblocks, smooth, crops, ranges);                              \
 		return;                                                                                                    \
 	}
 	N2M_ATLAS_TILES(N2M_LAUNCH_TILE)
@@ -351,15 +425,16 @@ void launch_atlas_smooth(const float *labels, int gnx, int gny, int gnz, const A
 
 // darken the voxels on the volume's faces, bounding box of >= iso and the
 // minimum of the result, per task (src/meshify.c:347-371)
-__global__ void __launch_bounds__(256)
-k_atlas_apply(float *__restrict__ crops, int gnx, int gny, int gnz, const AtlasTask *__restrict__ tasks,
-              const uint32_t *__restrict__ blockTask, int *__restrict__ bbox, uint32_t *__restrict__ vmin)
+__device__ __forceinline__ void atlas_apply_item(float *__restrict__ crops, int gnx, int gny, int gnz,
+                                                 const AtlasTask *__restrict__ tasks,
+                                                 const uint32_t *__restrict__ blockTask, unsigned b,
+                                                 int *__restrict__ bbox, uint32_t *__restrict__ vmin)
 {
 	__shared__ int s_red[6][4];
 	__shared__ uint32_t s_min[4];
-	const int ti = blockTask[blockIdx.x];
+	const int ti = (int)blockTask[b];
 	const AtlasTask t = tasks[ti];
-	const int local = blockIdx.x - t.first_block;
+	const int local = (int)b - t.first_block;
 	const size_t n = (size_t)t.nx * t.ny * t.nz;
 	float lmin = __builtin_inff();
 	int bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, bmax[3] = {-1, -1, -1};
@@ -419,12 +494,51 @@ k_atlas_apply(float *__restrict__ crops, int gnx, int gny, int gnz, const AtlasT
 	}
 }
 
-void launch_atlas_apply(float *crops, int gnx, int gny, int gnz, const AtlasTask *tasks, const uint32_t *blockTask,
-                        unsigned nblocks, int *bbox, uint32_t *vmin, hipStream_t s)
+template <bool LOOP>
+__global__ void __launch_bounds__(256)
+k_atlas_apply(float *__restrict__ crops, int gnx, int gny, int gnz, const AtlasTask *__restrict__ tasks,
+              const uint32_t *__restrict__ blockTask, unsigned nwork, int *__restrict__ bbox,
+              uint32_t *__restrict__ vmin)
 {
-	if (nblocks)
-		hipLaunchKernelGGL(k_atlas_apply, dim3(nblocks), dim3(256), 0, s, crops, gnx, gny, gnz, tasks, blockTask,
-		                   bbox, vmin);
+	if (!LOOP) {
+		atlas_apply_item(crops, gnx, gny, gnz, tasks, blockTask, blockIdx.x, bbox, vmin);
+		return;
+	}
+	for (unsigned b = blockIdx.x; b < nwork; b += gridDim.x) { // persistent, as k_atlas_smooth
+		atlas_apply_item(crops, gnx, gny, gnz, tasks, blockTask, b, bbox, vmin);
+		__syncthreads(); // the next item rewrites the LDS partials
+	}
+}
+
+__global__ void k_atlas_iso(AtlasTask *__restrict__ tasks, int ntasks, const uint32_t *__restrict__ ranges, float req)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= ntasks)
+		return;
+	const float mn = ord2f(ranges[2 * t]), mx = ord2f(ranges[2 * t + 1]);
+	float iso = __builtin_nanf(""), edge = __builtin_nanf("");
+	bool fb;
+	atlas_iso(req, mn, mx, &iso, &edge, &fb);
+	tasks[t].iso = iso;
+	tasks[t].mn = mn;
+	tasks[t].edgeMax = edge;
+}
+
+void launch_atlas_iso(AtlasTask *tasks, int ntasks, const uint32_t *ranges, float req, hipStream_t s)
+{
+	if (ntasks)
+		hipLaunchKernelGGL(k_atlas_iso, dim3((ntasks + 255) / 256), dim3(256), 0, s, tasks, ntasks, ranges, req);
+}
+
+void launch_atlas_apply(float *crops, int gnx, int gny, int gnz, const AtlasTask *tasks, const uint32_t *blockTask,
+                        unsigned nblocks, unsigned grid, int *bbox, uint32_t *vmin, hipStream_t s)
+{
+	if (nblocks && grid < nblocks)
+		hipLaunchKernelGGL(k_atlas_apply<true>, dim3(grid), dim3(256), 0, s, crops, gnx, gny, gnz, tasks, blockTask,
+		                   nblocks, bbox, vmin);
+	else if (nblocks)
+		hipLaunchKernelGGL(k_atlas_apply<false>, dim3(nblocks), dim3(256), 0, s, crops, gnx, gny, gnz, tasks,
+		                   blockTask, nblocks, bbox, vmin);
 }
 
 void launch_smooth(const float *in, float *out, int nx, int ny, int nz, uint32_t *range, const float *isos,
@@ -471,15 +585,19 @@ k_minmax(const float *__restrict__ img, size_t n, int vec4, uint32_t *range)
 			omin = min(omin, s_red[0][w]);
 			omax = max(omax, s_red[1][w]);
 		}
-		atomicMin(&range[0], omin);
-		atomicMax(&range[1], omax);
+		// same-address atomics serialise across the chip: only a block that
+		// improves on what it reads issues one
+		if (omin < ld_agent(&range[0]))
+			atomicMin(&range[0], omin);
+		if (omax > ld_agent(&range[1]))
+			atomicMax(&range[1], omax);
 	}
 }
 
 void launch_minmax(const float *img, size_t n, uint32_t *range, hipStream_t s)
 {
 	size_t blocks = (n / 4 + 255) / 256;
-	blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
+	blocks = blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
 	int vec4 = ((uintptr_t)img & 15) == 0; // float4 loads need 16-byte alignment
 	hipLaunchKernelGGL(k_minmax, dim3((unsigned)blocks), dim3(256), 0, s, img, n, vec4, range);
 }

@@ -22,6 +22,23 @@ inline float ord2f_host(uint32_t o)
 	return f;
 }
 
+// The isolevel meshify() takes for an intensity range [mn, mx] when asked
+// for `req` (src/meshify.c:312-319) and its edge value (:347); false for no
+// variability.  One definition for the host and the device, so both derive
+// the same values.
+__host__ __device__ inline bool atlas_iso(float req, float mn, float mx, float *iso, float *edgeMax, bool *fallback)
+{
+	if (mn == mx)
+		return false;
+	float isolevel = req;
+	*fallback = (isolevel <= mn) || (isolevel > mx);
+	if (*fallback)
+		isolevel = 0.5 * (mn + mx);
+	*iso = isolevel;
+	*edgeMax = 0.75 * (mn + isolevel);
+	return true;
+}
+
 // One label of the batched atlas (src/nii2mesh.c:551-573): its padded crop
 // of the volume and the per-label values meshify() derives.
 struct AtlasTask {
@@ -91,13 +108,18 @@ void launch_hist_bins(const float *img, size_t n, float mn, float scl, int nbins
 
 // atlas volume stages (k_volume.hip)
 void launch_atlas_stats(const float *labels, int nx, int ny, int nz, int nlabel, uint32_t *stats, hipStream_t s);
+// The atlas stages are persistent: `grid` workgroups (at most nblocks) walk
+// the nblocks items of the block map.
 void launch_atlas_smooth(const float *labels, int gnx, int gny, int gnz, const AtlasTask *tasks,
-                         const uint32_t *blockTask, unsigned nblocks, int smooth, int tile, float *crops,
-                         uint32_t *ranges, hipStream_t s);
+                         const uint32_t *blockTask, unsigned nblocks, unsigned grid, int smooth, int tile,
+                         float *crops, uint32_t *ranges, hipStream_t s);
 int atlas_tile_count();
 void atlas_tile_shape(int tile, int shape[3]);
 void launch_atlas_apply(float *crops, int gnx, int gny, int gnz, const AtlasTask *tasks, const uint32_t *blockTask,
-                        unsigned nblocks, int *bbox, uint32_t *vmin, hipStream_t s);
+                        unsigned nblocks, unsigned grid, int *bbox, uint32_t *vmin, hipStream_t s);
+// iso, mn, edgeMax of every task from its range (ranges[2t], ord min/max),
+// as atlas_iso; a task without variability gets a NaN iso
+void launch_atlas_iso(AtlasTask *tasks, int ntasks, const uint32_t *ranges, float req, hipStream_t s);
 unsigned atlas_smooth_blocks(int nx, int ny, int nz, int smooth, int tile);
 unsigned atlas_apply_blocks(int nx, int ny, int nz);
 void launch_binarize(const float *labels, size_t n, float lo, float hi, float *out, uint32_t *count, hipStream_t s);
@@ -156,17 +178,18 @@ void launch_adj_centre(const unsigned long long *vrec, const uint32_t *crec, uin
 void launch_adj_edge(const unsigned long long *vrec, uint32_t nmc, const double *gm, const float *img,
                      const AdjGeo &g, double *dvol, hipStream_t s);
 
-// atlas marching cubes (k_mc.hip): one workgroup per block of a task
+// atlas marching cubes (k_mc.hip): persistent, `grid` workgroups walk the
+// nblocks items of the block map
 void launch_atlas_classic_count(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
-                                unsigned nblocks, uint8_t *cfgs, uint32_t *blockSums, hipStream_t s);
+                                unsigned nblocks, unsigned grid, uint8_t *cfgs, uint32_t *blockSums, hipStream_t s);
 void launch_atlas_classic_emit(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
-                               unsigned nblocks, const uint8_t *cfgs, const uint32_t *blockOffs, double *pts,
-                               hipStream_t s);
+                               unsigned nblocks, unsigned grid, const uint8_t *cfgs, const uint32_t *blockOffs,
+                               double *pts, hipStream_t s);
 void launch_atlas_lw_classify(const float *crops, const AtlasTask *tasks, const uint32_t *blockTask,
-                              unsigned nblocks, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
-                              size_t stride, hipStream_t s);
+                              unsigned nblocks, unsigned grid, int originalMC, uint8_t *pflags, uint16_t *crow,
+                              uint32_t *segcnt, size_t stride, hipStream_t s);
 void launch_atlas_lw_emit(const float *crops, const AtlasTask *tasks, int ntasks, const uint32_t *blockTask,
-                          unsigned nblocks, int originalMC, const uint8_t *pflags, const uint16_t *crow,
+                          unsigned nblocks, unsigned grid, int originalMC, const uint8_t *pflags, const uint16_t *crow,
                           const uint32_t *segcnt, const uint32_t *bases, size_t stride, uint32_t totalE,
                           double *pts, int *tris, hipStream_t s);
 int lw_seg_len();
@@ -178,18 +201,31 @@ void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total,
 size_t radix_counts_words(size_t n);
 bool radix_sort_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int bits,
                     uint32_t *counts, uint32_t *scan_scratch, hipStream_t s);
+// one tile of a segmented radix sort: keys [start, start + len), digit d's
+// count at counts[cbase + d * cstride]
+struct RsTile {
+	uint32_t start, len, cbase, cstride;
+};
+// tiles that sort every segment [sb[k], sb[k + 1]) on its own (the counts
+// take 256 words per tile)
+std::vector<RsTile> radix_seg_tiles(const std::vector<uint32_t> &sb);
+bool radix_sort_seg_u32(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, size_t n, int bits,
+                        const RsTile *desc, size_t ntiles, uint32_t *counts, uint32_t *scan_scratch,
+                        hipStream_t s);
 void launch_weld_keys(const double *pts, uint32_t n, uint32_t *keys, uint32_t *vals, hipStream_t s);
 void launch_weld_segflags(const uint32_t *keys, uint32_t n, uint32_t *flags, hipStream_t s);
 void launch_weld_segstarts(const uint32_t *flags, const uint32_t *segid, uint32_t n, uint32_t *starts,
                            hipStream_t s);
 // d_nseg / d_nnew / d_n: counts left on the device (no host round trip);
 // the launches then cover the upper bound n
+// lrep: n words written by the greedy pass (each seed's representative), from
+// which the assign pass sets rep[] with plain stores (rep needs no fill)
 void launch_weld_greedy(const uint32_t *starts, uint32_t nseg, uint32_t n, const uint32_t *keys,
-                        const uint32_t *vals, const double *pts, int *lcls, uint32_t *seeds, hipStream_t s,
-                        const uint32_t *d_nseg = nullptr);
+                        const uint32_t *vals, const double *pts, int *lcls, uint32_t *seeds, uint32_t *lrep,
+                        hipStream_t s, const uint32_t *d_nseg = nullptr);
 void launch_weld_assign(const uint32_t *vals, const int *lcls, const uint32_t *flags, const uint32_t *segid,
-                        const uint32_t *seedbase, uint32_t n, int *cls, int *rep, hipStream_t s,
-                        const uint32_t *d_nnew = nullptr);
+                        const uint32_t *seedbase, const uint32_t *lrep, uint32_t n, int *cls, int *rep,
+                        hipStream_t s, const uint32_t *d_nnew = nullptr);
 void launch_gather_pts(const double *pts, const int *rep, uint32_t n, double *out, hipStream_t s,
                        const uint32_t *d_n = nullptr);
 // wpos[v] = pts[rep[cls[v]]]: the welded position of every pre-weld vertex v
@@ -277,14 +313,12 @@ void launch_long_border_mark(const uint32_t *v, const uint32_t *id, uint32_t n, 
 // batched atlas weld (k_mesh.hip)
 void launch_atlas_weld_keys(const double *pts, uint32_t n, const uint32_t *vbase, int ntasks, uint32_t *keys,
                             uint32_t *vals, uint32_t *vtask, hipStream_t s);
-void launch_atlas_task_keys(const uint32_t *vals, const uint32_t *vtask, uint32_t n, uint32_t *tk, uint32_t *pos,
-                            hipStream_t s);
-void launch_atlas_weld_gather(const uint32_t *keys, const uint32_t *vals, const uint32_t *pos, uint32_t n,
-                              uint32_t *keys2, uint32_t *vals2, hipStream_t s);
 void launch_atlas_weld_segflags(const uint32_t *keys, const uint32_t *tk, uint32_t n, uint32_t *flags,
                                 hipStream_t s);
 void launch_atlas_class_base(const uint32_t *tk, const uint32_t *segid, const uint32_t *seedbase, uint32_t n,
                              uint32_t *cbase, hipStream_t s);
+void launch_atlas_weld_tasks(uint32_t *cbase, const uint32_t *vbase, const uint32_t *nextne, int ntasks,
+                             const uint32_t *d_nnew, uint32_t *ncls, uint8_t *ident, hipStream_t s);
 void launch_atlas_weld_identity(uint32_t n, const uint32_t *vtask, const uint32_t *vbase, const uint32_t *cbase,
                                 const uint8_t *ident, int *cls, int *rep, hipStream_t s);
 void launch_gather_u32(const uint32_t *src, const uint32_t *idx, uint32_t n, uint32_t *dst, hipStream_t s);

@@ -197,6 +197,38 @@ def test_atlas_every_tile(gpu, oracle, cubes, tile, monkeypatch):
     compare(got, want)
 
 
+_PERSIST_WANT = {}
+
+
+@pytest.mark.parametrize('bpc', [0, 1, 4])
+@pytest.mark.parametrize('cubes', [0, 1])
+def test_atlas_persistent_grids(gpu, oracle, cubes, bpc, monkeypatch):
+    """Every stage launched persistent at one blocks-per-CU choice (index
+    into 0 / 1 / 2 / 4 / 8; 0 is one workgroup per item) on crops with more
+    work items than a 1-per-CU grid has workgroups, so the workgroups loop:
+    the oracle's meshes, HC included."""
+    monkeypatch.setenv('N2M_ATLAS_BPC', str(bpc))
+    vol = small_atlas((96, 96, 96), 24, 5)
+    if cubes not in _PERSIST_WANT:
+        _PERSIST_WANT[cubes] = expect(oracle, vol, cubes, 0, True, False, 3, None)
+    rc, got = run_gpu(gpu, vol, cubes, 0, True, False, 3, None, False)
+    assert rc == 0
+    compare(got, _PERSIST_WANT[cubes])
+
+
+def test_atlas_tuner_report(gpu):
+    """After a batched run every stage it launched reports a tuned shape."""
+    vol = small_atlas((40, 42, 44), 9, 3)
+    rc, _ = run_gpu(gpu, vol, 1, 0, True, False, 0, None, False)
+    assert rc == 0
+    t = gpu.atlas_tuned(0)
+    assert set(t) == {'smooth', 'apply', 'classic_count', 'classic_emit', 'lewiner_classify', 'lewiner_emit'}
+    for stage in ('smooth', 'apply', 'lewiner_classify', 'lewiner_emit'):
+        tile, bpc = t[stage]
+        assert bpc in (0, 1, 2, 4, 8), (stage, t[stage])
+        assert (tile >= 0) == (stage == 'smooth'), (stage, t[stage])
+
+
 @pytest.mark.parametrize('cubes', [0, 1])
 def test_atlas_host_entry(gpu, oracle, cubes):
     """n2m_atlas (host arrays, the CLI's form) gives the oracle's meshes."""

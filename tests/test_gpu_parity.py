@@ -127,6 +127,28 @@ def test_unify_nan_rows(gpu, oracle, first_nan):
     assert (len(p1) == n + k) if first_nan else (len(p1) < n + k)
 
 
+@pytest.mark.parametrize('fillers', [0, 8])
+def test_unify_claimed_twice(gpu, oracle, fillers):
+    """A vertex claimed by a seed and then by a later seed of the same key
+    window ends in the later class (src/meshify.c:66-79); the earlier class
+    keeps only its own members' positions.  Seed i = 1 claims j = 3 (8e-6
+    apart); i' = 2 is 1.07e-5 from i, so it seeds its own class, and claims
+    j (8.6e-6 apart).  `fillers` more vertices in the same key segment, far
+    apart in 3-D, make the segment longer than the short-segment path."""
+    import nii2mesh_amd.abi as abi
+    pts = [(0.0, 0.0, 0.0), (10.0, 0.0, 0.0), (10.0000065, 8.5e-6, 0.0), (10.000008, 0.0, 0.0)]
+    for k in range(1, fillers + 1):
+        r, th = 10.0 - 5e-6 * k, 0.1 * k
+        pts.append((r * np.cos(th), r * np.sin(th), 0.0))
+    pts = np.array(pts, np.float64)
+    tris = np.array([[0, 1, 2], [1, 2, 3]] + [[0, 1, 3 + k] for k in range(1, fillers + 1)], np.int32)
+    gu = abi.bind_unify(gpu.load(), 'n2m_unify_vertices')
+    p1, t1 = gu(pts, tris)
+    p2, t2 = oracle.unify(pts, tris)
+    assert _eq(p1, p2) and _eq(t1, t2)
+    assert len(p2) == len(pts) - 1 and t2[1, 1] == t2[1, 2]  # j went to i''s class
+
+
 @pytest.mark.parametrize('flavour', ['classic', 'lewiner'])
 @pytest.mark.parametrize('lock', [True, False])
 def test_hc_smooth(gpu, oracle, flavour, lock):
