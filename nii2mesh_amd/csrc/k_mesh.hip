@@ -368,122 +368,209 @@ constexpr int WG_REG = 8;
 // seed claimed; a segment where that happened takes the maxima again over
 // the final classes.
 constexpr uint32_t WG_NOSEED = 0xffffffffu;
+// one segment of at most WG_REG vertices, by one lane from registers
+__device__ __forceinline__ void weld_segment_regs(uint32_t s, uint32_t b, uint32_t len,
+                                                  const uint32_t *__restrict__ keys,
+                                                  const uint32_t *__restrict__ vals, const double *__restrict__ pts,
+                                                  int *__restrict__ lcls, uint32_t *__restrict__ seeds,
+                                                  uint32_t *__restrict__ lrep)
+{
+	const float tol = 0.00001f;
+	float k[WG_REG];
+	uint32_t v[WG_REG];
+	double px[WG_REG], py[WG_REG], pz[WG_REG];
+	int cl[WG_REG];
+#pragma unroll
+	for (int q = 0; q < WG_REG; q++) {
+		cl[q] = -1;
+		if ((uint32_t)q < len) {
+			k[q] = ord2f(keys[b + q]);
+			v[q] = vals[b + q];
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < WG_REG; q++)
+		if ((uint32_t)q < len) {
+			const double *p = pts + (size_t)v[q] * 3;
+			px[q] = p[0];
+			py[q] = p[1];
+			pz[q] = p[2];
+		}
+	int ns = 0;
+	unsigned seedmask = 0;
+#pragma unroll
+	for (int i = 0; i < WG_REG; i++) {
+		if ((uint32_t)i >= len || cl[i] >= 0)
+			continue;
+		cl[i] = ns++;
+		seedmask |= 1u << i;
+		// the reference's window stops at the first key >= k[i] + tol;
+		// keys ascend and fl(k[j] - k[i]) ascends with them
+#pragma unroll
+		for (int j = i + 1; j < WG_REG; j++)
+			if ((uint32_t)j < len && (k[j] - k[i]) < tol &&
+			    dist3v(px[i], py[i], pz[i], px[j], py[j], pz[j]) < (double)tol)
+				cl[j] = cl[i];
+	}
+#pragma unroll
+	for (int q = 0; q < WG_REG; q++)
+		if ((uint32_t)q < len) {
+			lcls[b + q] = cl[q];
+			if (lrep) {
+				uint32_t hi = WG_NOSEED;
+				if ((seedmask >> q) & 1u) { // members follow their seed
+					hi = v[q];
+#pragma unroll
+					for (int j = q + 1; j < WG_REG; j++)
+						if ((uint32_t)j < len && cl[j] == cl[q])
+							hi = max(hi, v[j]);
+				}
+				lrep[b + q] = hi;
+			}
+		}
+	seeds[s] = (uint32_t)ns;
+}
+
+// one segment of any length, by one lane (the reference's loop as written)
+__device__ __forceinline__ void weld_segment_serial(uint32_t s, uint32_t b, uint32_t e,
+                                                    const uint32_t *__restrict__ keys,
+                                                    const uint32_t *__restrict__ vals,
+                                                    const double *__restrict__ pts, int *__restrict__ lcls,
+                                                    uint32_t *__restrict__ seeds, uint32_t *__restrict__ lrep)
+{
+	const float tol = 0.00001f;
+	for (uint32_t i = b; i < e; i++)
+	lcls[i] = -1;
+	int ns = 0;
+	bool stolen = false;
+	for (uint32_t i = b; i < e; i++) {
+	if (lcls[i] >= 0) {
+		if (lrep)
+			lrep[i] = WG_NOSEED;
+		continue;
+	}
+	const float k0 = ord2f(keys[i]);
+	const double *seed = pts + (size_t)vals[i] * 3;
+	// the seed is always in its own class: for a finite position its
+	// distance to itself is 0 < tol anyway; a NaN position matches
+	// nothing, and the reference would then leave it unassigned
+	// (old2new -1, src/meshify.c:66-79)
+	lcls[i] = ns;
+	uint32_t hi = vals[i];
+	for (uint32_t j = i + 1; j < e && (ord2f(keys[j]) - k0) < tol; j++)
+		if (dist3(seed, pts + (size_t)vals[j] * 3) < (double)tol) {
+			stolen |= lcls[j] >= 0;
+			lcls[j] = ns;
+			hi = max(hi, vals[j]);
+		}
+	if (lrep)
+		lrep[i] = hi;
+	ns++;
+	}
+	if (lrep && stolen) // members follow their seed, within its key window
+	for (uint32_t i = b; i < e; i++) {
+		if (lrep[i] == WG_NOSEED)
+			continue;
+		const int c = lcls[i];
+		const float k0 = ord2f(keys[i]);
+		uint32_t hi = vals[i];
+		for (uint32_t j = i + 1; j < e && (ord2f(keys[j]) - k0) < tol; j++)
+			if (lcls[j] == c)
+				hi = max(hi, vals[j]);
+		lrep[i] = hi;
+	}
+	seeds[s] = (uint32_t)ns;
+}
+
+// One segment of WG_REG < len <= 64 vertices by the whole wave: lane j holds
+// vertex j; the seeds are taken in order, and for each the lanes after it
+// test themselves against it at once.  The same comparisons as the serial
+// loop (a later seed overwrites an earlier claim), with the loads of the
+// segment in flight together instead of one dependent chain per test.
+__device__ __forceinline__ void weld_segment_wave(uint32_t s, uint32_t b, uint32_t len,
+                                                  const uint32_t *__restrict__ keys,
+                                                  const uint32_t *__restrict__ vals, const double *__restrict__ pts,
+                                                  int *__restrict__ lcls, uint32_t *__restrict__ seeds,
+                                                  uint32_t *__restrict__ lrep)
+{
+	const float tol = 0.00001f;
+	const int j = lane_id();
+	const bool in = (uint32_t)j < len;
+	float kj = 0.f;
+	uint32_t vj = 0;
+	double px = 0., py = 0., pz = 0.;
+	if (in) {
+		kj = ord2f(keys[b + j]);
+		vj = vals[b + j];
+		const double *p = pts + (size_t)vj * 3;
+		px = p[0];
+		py = p[1];
+		pz = p[2];
+	}
+	int cl = -1, ns = 0;
+	unsigned long long seedm = 0;
+	for (int i = 0; i < (int)len; i++) {
+		if (__shfl(cl, i) >= 0)
+			continue;
+		seedm |= 1ull << i;
+		const float ki = __shfl(kj, i);
+		const double ax = __shfl(px, i), ay = __shfl(py, i), az = __shfl(pz, i);
+		if (j == i || (in && j > i && (kj - ki) < tol && dist3v(ax, ay, az, px, py, pz) < (double)tol))
+			cl = ns;
+		ns++;
+	}
+	// representative of each seed's final class: its largest member id
+	uint32_t hi = WG_NOSEED;
+	for (unsigned long long m = seedm; m; m &= m - 1) {
+		const int i = __ffsll((long long)m) - 1;
+		const int c = __shfl(cl, i);
+		const uint32_t mx = wave_max_u32(in && cl == c ? vj : 0u);
+		if (j == i)
+			hi = mx;
+	}
+	if (in) {
+		lcls[b + j] = cl;
+		if (lrep)
+			lrep[b + j] = hi;
+	}
+	if (j == 0)
+		seeds[s] = (uint32_t)ns;
+}
+
+// lrep (when given): at each seed's sorted position the largest vertex id
+// whose final class is the seed's (the member whose position the reference
+// keeps, src/meshify.c:100-101), WG_NOSEED elsewhere, so that the class
+// representatives need no atomics.  A seed can claim a vertex an earlier
+// seed claimed; a segment where that happened takes the maxima again over
+// the final classes.  One lane per segment; segments longer than WG_REG
+// (and at most a wave) are taken by the lane's whole wave in turn.
 __global__ void k_weld_greedy(const uint32_t *__restrict__ starts, uint32_t nseg, uint32_t n,
                               const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
                               const double *__restrict__ pts, int *__restrict__ lcls, uint32_t *__restrict__ seeds,
                               const uint32_t *__restrict__ d_nseg, uint32_t *__restrict__ lrep)
 {
-	uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= nseg)
-		return;
-	if (d_nseg) {
+	const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (d_nseg && s < nseg) {
 		const uint32_t ns_dev = *d_nseg;
-		if (s >= ns_dev) {
+		if (s >= ns_dev)
 			seeds[s] = 0u;
-			return;
-		}
 		nseg = ns_dev;
 	}
-	const float tol = 0.00001f;
-	uint32_t b = starts[s], e = s + 1 < nseg ? starts[s + 1] : n;
-	const uint32_t len = e - b;
-	if (len <= (uint32_t)WG_REG) {
-		float k[WG_REG];
-		uint32_t v[WG_REG];
-		double px[WG_REG], py[WG_REG], pz[WG_REG];
-		int cl[WG_REG];
-#pragma unroll
-		for (int q = 0; q < WG_REG; q++) {
-			cl[q] = -1;
-			if ((uint32_t)q < len) {
-				k[q] = ord2f(keys[b + q]);
-				v[q] = vals[b + q];
-			}
-		}
-#pragma unroll
-		for (int q = 0; q < WG_REG; q++)
-			if ((uint32_t)q < len) {
-				const double *p = pts + (size_t)v[q] * 3;
-				px[q] = p[0];
-				py[q] = p[1];
-				pz[q] = p[2];
-			}
-		int ns = 0;
-		unsigned seedmask = 0;
-#pragma unroll
-		for (int i = 0; i < WG_REG; i++) {
-			if ((uint32_t)i >= len || cl[i] >= 0)
-				continue;
-			cl[i] = ns++;
-			seedmask |= 1u << i;
-			// the reference's window stops at the first key >= k[i] + tol;
-			// keys ascend and fl(k[j] - k[i]) ascends with them
-#pragma unroll
-			for (int j = i + 1; j < WG_REG; j++)
-				if ((uint32_t)j < len && (k[j] - k[i]) < tol &&
-				    dist3v(px[i], py[i], pz[i], px[j], py[j], pz[j]) < (double)tol)
-					cl[j] = cl[i];
-		}
-#pragma unroll
-		for (int q = 0; q < WG_REG; q++)
-			if ((uint32_t)q < len) {
-				lcls[b + q] = cl[q];
-				if (lrep) {
-					uint32_t hi = WG_NOSEED;
-					if ((seedmask >> q) & 1u) { // members follow their seed
-						hi = v[q];
-#pragma unroll
-						for (int j = q + 1; j < WG_REG; j++)
-							if ((uint32_t)j < len && cl[j] == cl[q])
-								hi = max(hi, v[j]);
-					}
-					lrep[b + q] = hi;
-				}
-			}
-		seeds[s] = (uint32_t)ns;
-		return;
+	const bool mine = s < nseg;
+	uint32_t b = 0, len = 0;
+	if (mine) {
+		b = starts[s];
+		len = (s + 1 < nseg ? starts[s + 1] : n) - b;
+		if (len <= (uint32_t)WG_REG)
+			weld_segment_regs(s, b, len, keys, vals, pts, lcls, seeds, lrep);
+		else if (len > (uint32_t)N2M_WAVE)
+			weld_segment_serial(s, b, b + len, keys, vals, pts, lcls, seeds, lrep);
 	}
-	for (uint32_t i = b; i < e; i++)
-		lcls[i] = -1;
-	int ns = 0;
-	bool stolen = false;
-	for (uint32_t i = b; i < e; i++) {
-		if (lcls[i] >= 0) {
-			if (lrep)
-				lrep[i] = WG_NOSEED;
-			continue;
-		}
-		const float k0 = ord2f(keys[i]);
-		const double *seed = pts + (size_t)vals[i] * 3;
-		// the seed is always in its own class: for a finite position its
-		// distance to itself is 0 < tol anyway; a NaN position matches
-		// nothing, and the reference would then leave it unassigned
-		// (old2new -1, src/meshify.c:66-79)
-		lcls[i] = ns;
-		uint32_t hi = vals[i];
-		for (uint32_t j = i + 1; j < e && (ord2f(keys[j]) - k0) < tol; j++)
-			if (dist3(seed, pts + (size_t)vals[j] * 3) < (double)tol) {
-				stolen |= lcls[j] >= 0;
-				lcls[j] = ns;
-				hi = max(hi, vals[j]);
-			}
-		if (lrep)
-			lrep[i] = hi;
-		ns++;
+	for (unsigned long long m = __ballot(mine && len > (uint32_t)WG_REG && len <= (uint32_t)N2M_WAVE); m;
+	     m &= m - 1) {
+		const int o = __ffsll((long long)m) - 1;
+		weld_segment_wave(__shfl(s, o), __shfl(b, o), __shfl(len, o), keys, vals, pts, lcls, seeds, lrep);
 	}
-	if (lrep && stolen) // members follow their seed, within its key window
-		for (uint32_t i = b; i < e; i++) {
-			if (lrep[i] == WG_NOSEED)
-				continue;
-			const int c = lcls[i];
-			const float k0 = ord2f(keys[i]);
-			uint32_t hi = vals[i];
-			for (uint32_t j = i + 1; j < e && (ord2f(keys[j]) - k0) < tol; j++)
-				if (lcls[j] == c)
-					hi = max(hi, vals[j]);
-			lrep[i] = hi;
-		}
-	seeds[s] = (uint32_t)ns;
 }
 
 // d_nnew (when given): the class count on the device; when it equals n no

@@ -127,14 +127,15 @@ def test_unify_nan_rows(gpu, oracle, first_nan):
     assert (len(p1) == n + k) if first_nan else (len(p1) < n + k)
 
 
-@pytest.mark.parametrize('fillers', [0, 8])
+@pytest.mark.parametrize('fillers', [0, 8, 70])
 def test_unify_claimed_twice(gpu, oracle, fillers):
     """A vertex claimed by a seed and then by a later seed of the same key
     window ends in the later class (src/meshify.c:66-79); the earlier class
     keeps only its own members' positions.  Seed i = 1 claims j = 3 (8e-6
     apart); i' = 2 is 1.07e-5 from i, so it seeds its own class, and claims
     j (8.6e-6 apart).  `fillers` more vertices in the same key segment, far
-    apart in 3-D, make the segment longer than the short-segment path."""
+    apart in 3-D, make the segment take the wave path (8: 11 vertices) or
+    the one-lane path (70: 73 vertices, longer than a wave)."""
     import nii2mesh_amd.abi as abi
     pts = [(0.0, 0.0, 0.0), (10.0, 0.0, 0.0), (10.0000065, 8.5e-6, 0.0), (10.000008, 0.0, 0.0)]
     for k in range(1, fillers + 1):
