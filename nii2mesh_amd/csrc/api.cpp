@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <functional>
 #include <limits>
+#include <condition_variable>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -2720,24 +2721,30 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 			range[1] = r[1];
 		}
 		// The isolevels are meshed by one host thread and context (stream and
-		// scratch; C->workers, kept between calls) each: the meshing stages
-		// are mostly bound by latency, so they overlap well.  The HC passes,
-		// bound by bandwidth, then run one after another on this context.
+		// scratch) each: the meshing stages are mostly bound by latency, so
+		// they overlap well.  This thread meshes the last isolevel on this
+		// context and worker threads the others on C->workers (kept between
+		// calls): with one stream fewer, every meshing stream gets a hardware
+		// queue of its own (HIP maps streams onto 4 queues per process by
+		// default, and two streams on one queue run one after the other).
 		// N2M_PIPE_THREADS=0 meshes the isolevels one after another.
+		//
+		// HC with threads: a mesh whose meshing ends while another isolevel is
+		// still being meshed is smoothed at once on its own stream (timing
+		// bracket "hc_ring_side": it shares the GPU).  The mesh meshed last
+		// (in practice the largest: its weld takes longest) prepares its
+		// smoothing (relabelling, CSR, rings, border flags) while those run,
+		// then waits for them, so that its iterations ("hc_ring") have the
+		// GPU to themselves.  N2M_PIPE_HC=1: every mesh is smoothed right
+		// after its meshing, the HC passes one at a time (a mutex);
+		// N2M_PIPE_HC=2: the same, concurrently.
 		const char *pt = getenv("N2M_PIPE_THREADS");
 		const bool threaded = niso > 1 && !(pt && *pt == '0');
-		// N2M_PIPE_HC=1 (with threads): a worker smooths its mesh as soon as
-		// it is meshed, while other isolevels are still being meshed; a mutex
-		// keeps the HC passes of different meshes from overlapping on the GPU
-		// (each waits for the previous one to finish).  By default all meshes
-		// are smoothed here after the meshing: the step is within 1 % either
-		// way, and the HC kernels then run on a GPU of their own.
-		// N2M_PIPE_HC=2: as 1, but the HC passes of different meshes run
-		// concurrently (one stream each).
 		const char *ph = getenv("N2M_PIPE_HC");
-		const bool hc_in_worker = threaded && hcIter > 0 && ph && (*ph == '1' || *ph == '2');
-		const bool hc_serial = !(ph && *ph == '2');
-		std::mutex hc_mu;
+		const int hc_mode = ph && *ph == '1' ? 1 : (ph && *ph == '2' ? 2 : 0);
+		std::mutex hc_mu, st_mu;
+		std::condition_variable st_cv;
+		int finished = 0, side_pending = 0;
 		std::vector<char> done(niso, 0);
 		auto mesh_one = [&](n2m_ctx *D, int k) {
 			n2m_params P = *prm;
@@ -2750,110 +2757,110 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 			VolOut vo;
 			if (volume_stages(D, v, dim, P, smoothed, k < nmask ? masks + k * nwords : nullptr, vo,
 			                  smoothed ? sm : d_img) &&
-			    surface_stages(D, v, dim, vo, P, m)) {
+			    surface_stages(D, v, dim, vo, P, m))
 				done[k] = 1;
-				if (hc_in_worker) {
-					D->sync();
-					std::unique_lock<std::mutex> lock(hc_mu, std::defer_lock);
-					if (hc_serial)
-						lock.lock();
-					smooth_hc_device(D, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
-					                 true);
-					D->sync();
+			D->sync();
+		};
+		auto smooth_k = [&](n2m_ctx *D, int k, const char *stat, const std::function<void()> *before) {
+			Mesh &m = C->meshes[k];
+			smooth_hc_device(D, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter, true,
+			                 before, stat);
+			D->sync();
+		};
+		// one isolevel on a thread of its own: mesh it, then smooth it
+		auto run_iso = [&](n2m_ctx *D, int k) {
+			try {
+				mesh_one(D, k);
+			} catch (...) {
+				std::lock_guard<std::mutex> g(st_mu);
+				finished++;
+				st_cv.notify_all();
+				throw;
+			}
+			bool last, side = false;
+			{
+				std::lock_guard<std::mutex> g(st_mu);
+				last = ++finished == niso;
+				if (!last && hc_mode == 0 && done[k] && hcIter > 0) {
+					side = true;
+					side_pending++;
 				}
 			}
-			D->sync();
+			st_cv.notify_all();
+			if (!done[k] || hcIter <= 0)
+				return;
+			if (hc_mode != 0) {
+				std::unique_lock<std::mutex> lock(hc_mu, std::defer_lock);
+				if (hc_mode == 1)
+					lock.lock();
+				smooth_k(D, k, "hc_ring", nullptr);
+			} else if (side) {
+				try {
+					smooth_k(D, k, "hc_ring_side", nullptr);
+				} catch (...) {
+					std::lock_guard<std::mutex> g(st_mu);
+					side_pending--;
+					st_cv.notify_all();
+					throw;
+				}
+				std::lock_guard<std::mutex> g(st_mu);
+				side_pending--;
+				st_cv.notify_all();
+			} else { // the last one: its iterations wait for the side passes
+				const std::function<void()> wait = [&] {
+					std::unique_lock<std::mutex> l(st_mu);
+					st_cv.wait(l, [&] { return side_pending == 0; });
+				};
+				smooth_k(D, k, "hc_ring", &wait);
+			}
 		};
 		if (threaded) {
 			C->sync(); // the shared smoothed volume is complete
-			while ((int)C->workers.size() < niso)
+			const int kmain = niso - 1;
+			while ((int)C->workers.size() < niso - 1)
 				C->workers.push_back(ctx_new());
 			std::vector<std::string> err(niso);
 			std::vector<std::thread> th;
-			for (int k = 0; k < niso; k++) {
+			for (int k = 0; k < kmain; k++) {
 				n2m_ctx *D = C->workers[k];
 				D->timing = C->timing;
 				D->timing_only = C->timing_only;
 				th.emplace_back([&, D, k] {
 					try {
 						HIPX(hipSetDevice(C->device));
-						mesh_one(D, k);
+						run_iso(D, k);
 					} catch (const std::exception &e) {
 						err[k] = e.what();
 					}
 				});
+			}
+			try {
+				run_iso(C, kmain);
+			} catch (const std::exception &e) {
+				err[kmain] = e.what();
 			}
 			for (auto &t : th)
 				t.join();
-			for (int k = 0; k < niso; k++) {
+			for (int k = 0; k < kmain; k++)
 				C->absorb(*C->workers[k]);
+			for (int k = 0; k < niso; k++)
 				if (!err[k].empty())
 					throw std::runtime_error(err[k]);
-			}
 		} else {
 			for (int k = 0; k < niso; k++)
 				mesh_one(C, k);
-		}
-		int ok = 0, big = -1;
-		for (int k = 0; k < niso; k++)
-			if (done[k]) {
-				ok++;
-				if (big < 0 || C->meshes[k].nt > C->meshes[big].nt)
+			int big = -1;
+			for (int k = 0; k < niso; k++)
+				if (done[k] && (big < 0 || C->meshes[k].nt > C->meshes[big].nt))
 					big = k;
-			}
-		if (hcIter > 0 && !hc_in_worker && big >= 0) {
-			// The largest mesh is smoothed on this context.  With worker
-			// contexts, the other meshes are smoothed on theirs while its
-			// preparation (relabelling, CSR, rings, border flags) runs, and
-			// they have finished before its iterations are enqueued, so the
-			// iterations of the largest mesh have the GPU to themselves.
-			// Their timing brackets are "hc_ring_side" (they share the GPU),
-			// the largest mesh's "hc_ring".
-			std::vector<std::thread> th;
-			std::vector<std::string> err(niso);
-			for (int k = 0; k < niso; k++) {
-				if (!done[k] || k == big)
-					continue;
-				Mesh &m = C->meshes[k];
-				if (!threaded) {
-					smooth_hc_device(C, (double *)m.pts.p, (const int *)m.tris.p, m.np, m.nt, alpha, beta, hcIter,
-					                 true, nullptr, "hc_ring_side");
-					continue;
-				}
-				n2m_ctx *D = C->workers[k];
-				th.emplace_back([&, D, k] {
-					try {
-						HIPX(hipSetDevice(C->device));
-						Mesh &mk = C->meshes[k];
-						smooth_hc_device(D, (double *)mk.pts.p, (const int *)mk.tris.p, mk.np, mk.nt, alpha, beta,
-						                 hcIter, true, nullptr, "hc_ring_side");
-						D->sync();
-					} catch (const std::exception &e) {
-						err[k] = e.what();
-					}
-				});
-			}
-			const std::function<void()> join = [&] {
-				for (auto &t : th)
-					t.join();
-				th.clear();
-			};
-			Mesh &mb = C->meshes[big];
-			try {
-				smooth_hc_device(C, (double *)mb.pts.p, (const int *)mb.tris.p, mb.np, mb.nt, alpha, beta, hcIter,
-				                 true, &join);
-			} catch (...) {
-				join();
-				throw;
-			}
-			join();
-			if (threaded)
-				for (int k = 0; k < niso; k++) {
-					C->absorb(*C->workers[k]);
-					if (!err[k].empty())
-						throw std::runtime_error(err[k]);
-				}
+			if (hcIter > 0)
+				for (int k = 0; k < niso; k++)
+					if (done[k])
+						smooth_k(C, k, k == big ? "hc_ring" : "hc_ring_side", nullptr);
 		}
+		int ok = 0;
+		for (int k = 0; k < niso; k++)
+			ok += done[k];
 		C->sync();
 		return ok == niso ? EXIT_SUCCESS : EXIT_FAILURE;
 	});
