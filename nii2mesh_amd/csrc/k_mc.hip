@@ -82,40 +82,58 @@ __device__ __forceinline__ uint32_t classic_count_block(const float *__restrict_
 	return block_sum_u32<MC_BS>(nt, scratch);
 }
 
-// emit the triangle corners of cube block `blk`, first triangle at `off`
+// emit the triangle corners of cube block `blk`, first triangle at `off`.
+// The block's triangles, not its cubes, are spread over the threads: the
+// scan of the per-cube counts lists each triangle's cube and rank in LDS,
+// then thread j builds triangles j, j + MC_BS, ... (the surface cubes are a
+// few of the block's, so one thread per cube left most lanes idle while a
+// few built up to 5 triangles each).  Triangle q is written at off + q,
+// its corners in the cube's table order, as before.
 __device__ __forceinline__ void classic_emit_block(const float *__restrict__ img, const ClsGeo &g,
                                                    const uint8_t *__restrict__ cfgs, size_t blk, uint32_t off0,
                                                    double *__restrict__ pts)
 {
 	__shared__ uint32_t scratch[MC_BS / N2M_WAVE + 1];
-	size_t ncube = (size_t)g.cx * g.cy * g.cz;
-	size_t c = blk * MC_BS + threadIdx.x;
-	int cfg = c < ncube ? cfgs[c] : 0;
-	uint64_t w = n2m_classic_tri[cfg];
-	uint32_t nt = classic_ntri(w), tot;
-	uint32_t off = block_excl_scan<MC_BS>(nt, scratch, &tot) + off0;
-	if (!nt)
-		return;
-	int x = g.lo0 + (int)(c % g.cx);
-	size_t r = c / g.cx;
-	int y = g.lo1 + (int)(r % g.cy), z = g.lo2 + (int)(r / g.cy);
+	__shared__ uint8_t s_cube[5 * MC_BS]; // the cube (block-local) of each triangle
+	__shared__ uint8_t s_rank[5 * MC_BS]; // its rank within the cube
+	const size_t ncube = (size_t)g.cx * g.cy * g.cz;
+	const size_t c0 = blk * MC_BS;
+	uint32_t tot;
+	{
+		const size_t c = c0 + threadIdx.x;
+		const int cfg = c < ncube ? cfgs[c] : 0;
+		const uint32_t nt = classic_ntri(n2m_classic_tri[cfg]);
+		const uint32_t off = block_excl_scan<MC_BS>(nt, scratch, &tot);
+		for (uint32_t k = 0; k < nt; k++) {
+			s_cube[off + k] = (uint8_t)threadIdx.x;
+			s_rank[off + k] = (uint8_t)k;
+		}
+		__syncthreads();
+	}
 	const size_t nxy = (size_t)g.nx * g.ny;
-	const float *b = img + (size_t)x + (size_t)y * g.nx + (size_t)z * nxy;
-	double val[8];
-#pragma unroll
-	for (int p = 0; p < 8; p++)
-		val[p] = b[c_cx[p] + (size_t)c_cy[p] * g.nx + (size_t)c_cz[p] * nxy];
 	const double iso = g.iso;
-	double *o = pts + (size_t)off * 9;
-	for (uint32_t k = 0; k < 3 * nt; k++) {
-		int e = (int)((w >> (4 * k)) & 0xF);
-		int a = c_ea[e], bb = c_eb[e];
-		double mu = (iso - val[a]) / (val[bb] - val[a]);
-		double ax = x + g.ox + c_cx[a], ay = y + g.oy + c_cy[a], az = z + g.oz + c_cz[a];
-		double bx = x + g.ox + c_cx[bb], by = y + g.oy + c_cy[bb], bz = z + g.oz + c_cz[bb];
-		o[3 * k + 0] = ax + mu * (bx - ax);
-		o[3 * k + 1] = ay + mu * (by - ay);
-		o[3 * k + 2] = az + mu * (bz - az);
+	for (uint32_t q = threadIdx.x; q < tot; q += MC_BS) {
+		const size_t c = c0 + s_cube[q];
+		const int kk = s_rank[q];
+		const uint64_t w = n2m_classic_tri[cfgs[c]];
+		const int x = g.lo0 + (int)(c % g.cx);
+		const size_t r = c / g.cx;
+		const int y = g.lo1 + (int)(r % g.cy), z = g.lo2 + (int)(r / g.cy);
+		const float *b = img + (size_t)x + (size_t)y * g.nx + (size_t)z * nxy;
+		double *o = pts + ((size_t)off0 + q) * 9;
+#pragma unroll
+		for (int j = 0; j < 3; j++) {
+			const int e = (int)((w >> (4 * (3 * kk + j))) & 0xF);
+			const int a = c_ea[e], bb = c_eb[e];
+			const double va = b[c_cx[a] + (size_t)c_cy[a] * g.nx + (size_t)c_cz[a] * nxy];
+			const double vb = b[c_cx[bb] + (size_t)c_cy[bb] * g.nx + (size_t)c_cz[bb] * nxy];
+			const double mu = (iso - va) / (vb - va);
+			const double ax = x + g.ox + c_cx[a], ay = y + g.oy + c_cy[a], az = z + g.oz + c_cz[a];
+			const double bx = x + g.ox + c_cx[bb], by = y + g.oy + c_cy[bb], bz = z + g.oz + c_cz[bb];
+			o[3 * j + 0] = ax + mu * (bx - ax);
+			o[3 * j + 1] = ay + mu * (by - ay);
+			o[3 * j + 2] = az + mu * (bz - az);
+		}
 	}
 }
 
