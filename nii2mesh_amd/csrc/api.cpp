@@ -202,6 +202,7 @@ enum Slot {
 	S_AT_VTASK,
 	S_AT_RSTILES,
 	S_LREP,
+	S_HC_HUGE,
 	S_AT_NEXT,
 	S_AT_CBASE,
 	S_AT_IDENT,
@@ -219,17 +220,6 @@ enum Slot {
 	S_ENC_A,
 	S_ENC_B,
 	S_ENC_C,
-	S_LG_OFF,
-	S_LG_FLAG,
-	S_LG_K0,
-	S_LG_V0,
-	S_LG_K1,
-	S_LG_V1,
-	S_LG_EK0,
-	S_LG_EV0,
-	S_LG_EK1,
-	S_LG_EV1,
-	S_LG_RC,
 	S_Q_ND,
 	S_Q_Q,
 	S_Q_ERR,
@@ -1075,29 +1065,7 @@ struct Csr {
 	uint32_t *start;
 	int2 *pairs;
 	uint32_t *ref;
-	// references of the high-valence vertices (more than 16 each), sorted by
-	// vertex and, within a vertex, by corner: nlong entries
-	uint32_t nlong = 0;
-	const uint32_t *lv = nullptr, *lc = nullptr;
 };
-
-int bits_for(uint32_t n) // radix bits that cover the values 0..n-1
-{
-	int b = 8;
-	while (b < 32 && ((uint64_t)1 << b) < n)
-		b += 8;
-	return b;
-}
-
-// stable sort of n (key, value) pairs held in k0 / v0 (k1 / v1 the other
-// buffers); returns where the result is
-std::pair<uint32_t *, uint32_t *> sort_pairs(n2m_ctx *C, uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1,
-                                             size_t n, int bits)
-{
-	uint32_t *rc = C->buf<uint32_t>(S_LG_RC, radix_counts_words(n));
-	const bool in1 = radix_sort_u32(k0, v0, k1, v1, n, bits, rc, scan_scratch(C, radix_counts_words(n)), C->stream);
-	return in1 ? std::make_pair(k1, v1) : std::make_pair(k0, v0);
-}
 
 // sorted_ref: ref holds each vertex's corners in ascending order (by
 // default only the pairs are ordered)
@@ -1117,45 +1085,18 @@ Csr build_csr(n2m_ctx *C, const int *tris, uint32_t nv, uint32_t nt, bool sorted
 	launch_csr_fill(tris, nt, c.start, rank, c.ref, s);
 	launch_csr_pairs(tris, nv, c.start, c.ref, c.pairs, s, sorted_ref);
 	C->tend();
-	// lists longer than 16: ordered all at once (k_csr_pairs left them)
-	uint32_t *loff = C->buf<uint32_t>(S_LG_OFF, (size_t)nv + 1);
-	launch_long_count(c.start, nv, loff, s);
-	const uint32_t L = scan(C, loff, loff, (size_t)nv + 1, true);
-	if (L) {
-		const uint32_t n3 = 3 * nt;
-		uint32_t *pos = C->buf<uint32_t>(S_LG_FLAG, n3);
-		launch_long_flag(tris, n3, c.start, pos, s);
-		scan_excl_u32(pos, pos, n3, nullptr, scan_scratch(C, n3), s);
-		uint32_t *k0 = C->buf<uint32_t>(S_LG_K0, L), *v0 = C->buf<uint32_t>(S_LG_V0, L);
-		uint32_t *k1 = C->buf<uint32_t>(S_LG_K1, L), *v1 = C->buf<uint32_t>(S_LG_V1, L);
-		launch_long_compact(tris, n3, c.start, pos, k0, v0, s);
-		auto kv = sort_pairs(C, k0, v0, k1, v1, L, bits_for(nv));
-		launch_long_place(tris, kv.first, kv.second, L, c.start, loff, c.ref, c.pairs, s);
-		C->check();
-		c.nlong = L;
-		c.lv = kv.first;
-		c.lc = kv.second;
-	}
+	// lists longer than 16 (k_csr_pairs left them): one wave each
+	launch_long_sort(tris, nv, c.start, c.ref, c.pairs, C->buf<uint32_t>(S_HC_HUGE, long_huge_words(nt)), s);
+	C->check();
 	return c;
 }
 
 // border flags of the high-valence vertices (k_border / k_border_pairs
-// skip them): entries (id, vertex) sorted by (vertex, id), then the ids
-// that occur once
-void border_long(n2m_ctx *C, const Csr &c, const int *tris, uint32_t nv, bool literal, uint8_t *border)
+// skip them), one wave per list
+void border_long(n2m_ctx *C, const Csr &c, const int *tris, uint32_t nv, uint32_t nt, bool literal, uint8_t *border)
 {
-	if (!c.nlong)
-		return;
-	hipStream_t s = C->stream;
-	const size_t n = (size_t)c.nlong * (literal ? 3 : 2);
-	uint32_t *ek0 = C->buf<uint32_t>(S_LG_EK0, n), *ev0 = C->buf<uint32_t>(S_LG_EV0, n);
-	uint32_t *ek1 = C->buf<uint32_t>(S_LG_EK1, n), *ev1 = C->buf<uint32_t>(S_LG_EV1, n);
-	launch_long_border_entries(tris, c.lv, c.lc, c.nlong, literal ? 1 : 0, ek0, ev0, s);
-	const int bits = bits_for(nv);
-	auto a = sort_pairs(C, ek0, ev0, ek1, ev1, n, bits); // by id
-	uint32_t *ok = a.first == ek0 ? ek1 : ek0, *ov = a.second == ev0 ? ev1 : ev0;
-	auto b = sort_pairs(C, a.second, a.first, ov, ok, n, bits); // then by vertex (stable)
-	launch_long_border_mark(b.first, b.second, (uint32_t)n, literal ? 1 : 0, border, s);
+	launch_long_border(tris, nv, c.start, c.ref, literal ? 1 : 0, border,
+	                   C->buf<uint32_t>(S_HC_HUGE, long_huge_words(nt)), C->stream);
 	C->check();
 }
 
@@ -1215,7 +1156,7 @@ uint8_t *border_flags(n2m_ctx *C, const Csr &c, const int *tris, uint32_t nv, ui
 	} else {
 		launch_border_pairs(c.start, c.pairs, nv, border, s);
 	}
-	border_long(C, c, tris, nv, repeats, border);
+	border_long(C, c, tris, nv, nt, repeats, border);
 	C->tend();
 	return border;
 }

@@ -1743,128 +1743,267 @@ __global__ void k_hc_store(const d3 *__restrict__ p, const d3 *__restrict__ o, c
 // id counted once for that vertex (src/quadric.c:188-215).
 // ---------------------------------------------------------------------------
 // lcnt[v] = the list length of a long vertex, else 0 (v <= nv: a 0 sentinel)
-__global__ void k_long_count(const uint32_t *__restrict__ start, uint32_t nv, uint32_t *__restrict__ lcnt)
+// ---------------------------------------------------------------------------
+// High-valence lists (more than CP_MAX references), one wave per list: the
+// lanes of a thread-per-vertex launch that hold such a vertex hand it to
+// their whole wave in turn.  A list of up to LW_K * 64 elements is held in
+// registers (element i in slot i / 64 of lane i % 64) and every element is
+// compared with all the others through shuffles; a longer one is compared
+// from global memory (the same counts, more loads).
+// ---------------------------------------------------------------------------
+constexpr int LW_K = 4;
+
+// counts, for each element a of lane slots r[], the elements below it (less)
+// and equal to it (eq) among the E elements of the wave
+__device__ __forceinline__ void wave_rank_eq(const uint32_t (&r)[LW_K], uint32_t E, uint32_t (&less)[LW_K],
+                                             uint32_t (&eq)[LW_K])
+{
+#pragma unroll
+	for (int k = 0; k < LW_K; k++)
+		less[k] = eq[k] = 0;
+#pragma unroll
+	for (int si = 0; si < LW_K; si++) {
+		if ((uint32_t)si * N2M_WAVE >= E)
+			break;
+		const uint32_t nl = E - si * N2M_WAVE < (uint32_t)N2M_WAVE ? E - si * N2M_WAVE : (uint32_t)N2M_WAVE;
+		for (uint32_t li = 0; li < nl; li++) {
+			const uint32_t x = __shfl(r[si], (int)li);
+#pragma unroll
+			for (int k = 0; k < LW_K; k++) {
+				less[k] += x < r[k];
+				eq[k] += x == r[k];
+			}
+		}
+	}
+}
+
+// Sorts each long list of ref ascending in place and expands its pairs
+// (what k_csr_pairs does for the short lists).  A list longer than the
+// register path is appended to `huge` (huge[0] = count, the vertices
+// after it) for k_huge_sort.
+__global__ void k_long_sort(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restrict__ start,
+                            uint32_t *__restrict__ ref, int2 *__restrict__ pairs, uint32_t *__restrict__ huge)
 {
 	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-	if (v > nv)
-		return;
-	const uint32_t n = v < nv ? start[v + 1] - start[v] : 0u;
-	lcnt[v] = n > (uint32_t)CP_MAX ? n : 0u;
-}
-
-__device__ __forceinline__ bool long_corner(const int *__restrict__ tris, const uint32_t *__restrict__ start,
-                                            uint32_t i)
-{
-	const uint32_t v = (uint32_t)tris[i];
-	return start[v + 1] - start[v] > (uint32_t)CP_MAX;
-}
-
-__global__ void k_long_flag(const int *__restrict__ tris, uint32_t n3, const uint32_t *__restrict__ start,
-                            uint32_t *__restrict__ flag)
-{
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n3)
-		flag[i] = long_corner(tris, start, i) ? 1u : 0u;
-}
-
-// pos: exclusive scan of the flags; key = vertex, val = corner
-__global__ void k_long_compact(const int *__restrict__ tris, uint32_t n3, const uint32_t *__restrict__ start,
-                               const uint32_t *__restrict__ pos, uint32_t *__restrict__ key, uint32_t *__restrict__ val)
-{
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n3 && long_corner(tris, start, i)) {
-		key[pos[i]] = (uint32_t)tris[i];
-		val[pos[i]] = i;
-	}
-}
-
-// sorted by vertex, corners ascending within a vertex; loff = exclusive scan
-// of k_long_count, so vertex v's entries start at loff[v]
-__global__ void k_long_place(const int *__restrict__ tris, const uint32_t *__restrict__ key,
-                             const uint32_t *__restrict__ val, uint32_t L, const uint32_t *__restrict__ start,
-                             const uint32_t *__restrict__ loff, uint32_t *__restrict__ ref, int2 *__restrict__ pairs)
-{
-	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= L)
-		return;
-	const uint32_t v = key[j], c = val[j], at = start[v] + (j - loff[v]);
-	ref[at] = c;
-	pairs[at] = corner_pair(tris, c);
-}
-
-// border entries (key id, value vertex): the two pair ids of each reference,
-// or (literal) the three ids of its triangle
-__global__ void k_long_border_entries(const int *__restrict__ tris, const uint32_t *__restrict__ key,
-                                      const uint32_t *__restrict__ val, uint32_t L, int literal,
-                                      uint32_t *__restrict__ ek, uint32_t *__restrict__ ev)
-{
-	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= L)
-		return;
-	const uint32_t v = key[j], c = val[j];
-	if (literal) {
-		const uint32_t t = c / 3;
-		for (int m = 0; m < 3; m++) {
-			ek[3 * (size_t)j + m] = (uint32_t)tris[3 * t + m];
-			ev[3 * (size_t)j + m] = v;
+	const uint32_t deg = v < nv ? start[v + 1] - start[v] : 0u;
+	const int lane = lane_id();
+	for (unsigned long long m = __ballot(deg > (uint32_t)CP_MAX); m; m &= m - 1) {
+		const int o = __ffsll((long long)m) - 1;
+		const uint32_t u = __shfl(v, o), d = __shfl(deg, o), b = start[u];
+		if (d > (uint32_t)(LW_K * N2M_WAVE)) {
+			if (lane == 0)
+				huge[1 + atomicAdd(&huge[0], 1u)] = u;
+			continue;
 		}
-	} else {
-		const int2 q = corner_pair(tris, c);
-		ek[2 * (size_t)j] = (uint32_t)q.x;
-		ek[2 * (size_t)j + 1] = (uint32_t)q.y;
-		ev[2 * (size_t)j] = v;
-		ev[2 * (size_t)j + 1] = v;
+		uint32_t r[LW_K], less[LW_K], eq[LW_K];
+#pragma unroll
+		for (int k = 0; k < LW_K; k++) {
+			const uint32_t i = k * N2M_WAVE + lane;
+			r[k] = i < d ? ref[b + i] : 0xffffffffu;
+		}
+		wave_rank_eq(r, d, less, eq);
+#pragma unroll
+		for (int k = 0; k < LW_K; k++) {
+			const uint32_t i = k * N2M_WAVE + lane;
+			if (i < d) { // the references are distinct corners: ranks are a permutation
+				ref[b + less[k]] = r[k];
+				pairs[b + less[k]] = corner_pair(tris, r[k]);
+			}
+		}
 	}
 }
 
-// entries sorted by (vertex, id): one that occurs once marks the vertex
-// (pair form) or the id (literal form) as border
-__global__ void k_long_border_mark(const uint32_t *__restrict__ v, const uint32_t *__restrict__ id, uint32_t n,
-                                   int literal, uint8_t *__restrict__ border)
+// Border flags of the long lists (k_border and k_border_pairs leave them):
+// the ids of a list's entries -- the two pair ids of each reference, or
+// (literal) the three ids of its triangle -- are counted within the list;
+// one that occurs once marks the vertex (pair form) or the id (literal).
+// Lists with more entries than the register path go to `huge`.
+__device__ __forceinline__ uint32_t long_entry(const int *__restrict__ tris, const uint32_t *__restrict__ ref,
+                                               uint32_t b, uint32_t e, int per)
 {
-	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= n)
+	const uint32_t r = ref[b + e / per], m = e % per;
+	if (per == 3)
+		return (uint32_t)tris[3 * (r / 3) + m];
+	const int2 q = corner_pair(tris, r);
+	return (uint32_t)(m ? q.y : q.x);
+}
+
+__global__ void k_long_border(const int *__restrict__ tris, uint32_t nv, const uint32_t *__restrict__ start,
+                              const uint32_t *__restrict__ ref, int literal, uint8_t *__restrict__ border,
+                              uint32_t *__restrict__ huge)
+{
+	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t deg = v < nv ? start[v + 1] - start[v] : 0u;
+	const int lane = lane_id(), per = literal ? 3 : 2;
+	for (unsigned long long m = __ballot(deg > (uint32_t)CP_MAX); m; m &= m - 1) {
+		const int o = __ffsll((long long)m) - 1;
+		const uint32_t u = __shfl(v, o), d = __shfl(deg, o), b = start[u], E = d * per;
+		if (E > (uint32_t)(LW_K * N2M_WAVE)) {
+			if (lane == 0)
+				huge[1 + atomicAdd(&huge[0], 1u)] = u;
+			continue;
+		}
+		uint32_t r[LW_K], less[LW_K], eq[LW_K];
+#pragma unroll
+		for (int k = 0; k < LW_K; k++) {
+			const uint32_t i = k * N2M_WAVE + lane;
+			r[k] = i < E ? long_entry(tris, ref, b, i, per) : 0xffffffffu;
+		}
+		wave_rank_eq(r, E, less, eq);
+		bool once = false; // this lane holds an entry that occurs once
+#pragma unroll
+		for (int k = 0; k < LW_K; k++)
+			if (k * N2M_WAVE + lane < E && eq[k] == 1) {
+				once = true;
+				if (literal)
+					border[r[k]] = 1;
+			}
+		if (!literal && __ballot(once) && lane == 0)
+			border[u] = 1;
+	}
+}
+
+// The lists left in `huge`, one workgroup each (a grid-stride loop over the
+// count the list kernels left): up to HG_CAP values are sorted in LDS
+// (bitonic); a longer list compares every pair from global memory.
+constexpr int HG_BS = 1024, HG_CAP = 16384;
+
+__device__ void lds_bitonic(uint32_t *s, uint32_t n, uint32_t npad)
+{
+	for (uint32_t i = n + threadIdx.x; i < npad; i += HG_BS)
+		s[i] = 0xffffffffu;
+	__syncthreads();
+	for (uint32_t k = 2; k <= npad; k <<= 1)
+		for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+			for (uint32_t i = threadIdx.x; i < npad; i += HG_BS) {
+				const uint32_t x = i ^ j;
+				if (x > i) {
+					const uint32_t a = s[i], c = s[x];
+					if ((a > c) == ((i & k) == 0)) {
+						s[i] = c;
+						s[x] = a;
+					}
+				}
+			}
+			__syncthreads();
+		}
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t n)
+{
+	uint32_t p = 1;
+	while (p < n)
+		p <<= 1;
+	return p;
+}
+
+__global__ void __launch_bounds__(HG_BS)
+k_huge_sort(const int *__restrict__ tris, const uint32_t *__restrict__ start, uint32_t *__restrict__ ref,
+            int2 *__restrict__ pairs, const uint32_t *__restrict__ huge)
+{
+	__shared__ uint32_t s[HG_CAP];
+	const uint32_t nh = huge[0];
+	for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+		const uint32_t u = huge[1 + h], b = start[u], d = start[u + 1] - b;
+		if (d <= (uint32_t)HG_CAP) {
+			for (uint32_t i = threadIdx.x; i < d; i += HG_BS)
+				s[i] = ref[b + i];
+			lds_bitonic(s, d, pow2_at_least(d));
+			for (uint32_t i = threadIdx.x; i < d; i += HG_BS) {
+				ref[b + i] = s[i];
+				pairs[b + i] = corner_pair(tris, s[i]);
+			}
+		} else { // ranks over a copy of the list kept in pairs[]
+			for (uint32_t i = threadIdx.x; i < d; i += HG_BS)
+				pairs[b + i] = make_int2((int)ref[b + i], 0);
+			__threadfence_block();
+			__syncthreads();
+			for (uint32_t i = threadIdx.x; i < d; i += HG_BS) {
+				const uint32_t x = (uint32_t)pairs[b + i].x;
+				uint32_t less = 0;
+				for (uint32_t j = 0; j < d; j++)
+					less += (uint32_t)pairs[b + j].x < x;
+				ref[b + less] = x;
+			}
+			__threadfence_block();
+			__syncthreads();
+			for (uint32_t i = threadIdx.x; i < d; i += HG_BS)
+				pairs[b + i] = corner_pair(tris, ref[b + i]);
+		}
+		__syncthreads();
+	}
+}
+
+__global__ void __launch_bounds__(HG_BS)
+k_huge_border(const int *__restrict__ tris, const uint32_t *__restrict__ start, const uint32_t *__restrict__ ref,
+              int literal, uint8_t *__restrict__ border, const uint32_t *__restrict__ huge)
+{
+	__shared__ uint32_t s[HG_CAP];
+	__shared__ uint32_t s_once;
+	const int per = literal ? 3 : 2;
+	const uint32_t nh = huge[0];
+	for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+		const uint32_t u = huge[1 + h], b = start[u], E = (start[u + 1] - b) * per;
+		if (threadIdx.x == 0)
+			s_once = 0;
+		__syncthreads();
+		if (E <= (uint32_t)HG_CAP) {
+			for (uint32_t i = threadIdx.x; i < E; i += HG_BS)
+				s[i] = long_entry(tris, ref, b, i, per);
+			lds_bitonic(s, E, pow2_at_least(E));
+			for (uint32_t i = threadIdx.x; i < E; i += HG_BS) {
+				const uint32_t x = s[i];
+				if ((i == 0 || s[i - 1] != x) && (i + 1 == E || s[i + 1] != x)) {
+					if (literal)
+						border[x] = 1;
+					else
+						s_once = 1;
+				}
+			}
+		} else {
+			for (uint32_t i = threadIdx.x; i < E; i += HG_BS) {
+				const uint32_t x = long_entry(tris, ref, b, i, per);
+				uint32_t n = 0;
+				for (uint32_t j = 0; j < E && n < 2; j++)
+					n += long_entry(tris, ref, b, j, per) == x;
+				if (n == 1) {
+					if (literal)
+						border[x] = 1;
+					else
+						s_once = 1;
+				}
+			}
+		}
+		__syncthreads();
+		if (threadIdx.x == 0 && !literal && s_once)
+			border[u] = 1;
+		__syncthreads();
+	}
+}
+
+// vertices with more than 256 references, or more than 256 border entries
+// (85 references of 3), plus the count
+size_t long_huge_words(size_t nt) { return 2 + 3 * nt / 85; }
+
+void launch_long_sort(const int *tris, uint32_t nv, const uint32_t *start, uint32_t *ref, int2 *pairs,
+                      uint32_t *huge, hipStream_t s)
+{
+	if (!nv)
 		return;
-	const bool first = j == 0 || v[j - 1] != v[j] || id[j - 1] != id[j];
-	const bool last = j + 1 == n || v[j + 1] != v[j] || id[j + 1] != id[j];
-	if (first && last)
-		border[literal ? id[j] : v[j]] = 1;
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, huge);
+	hipLaunchKernelGGL(k_long_sort, dim3(g1(nv)), dim3(256), 0, s, tris, nv, start, ref, pairs, huge);
+	hipLaunchKernelGGL(k_huge_sort, dim3(64), dim3(HG_BS), 0, s, tris, start, ref, pairs, (const uint32_t *)huge);
 }
-
-void launch_long_count(const uint32_t *start, uint32_t nv, uint32_t *lcnt, hipStream_t s)
+void launch_long_border(const int *tris, uint32_t nv, const uint32_t *start, const uint32_t *ref, int literal,
+                        uint8_t *border, uint32_t *huge, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_long_count, dim3(g1((size_t)nv + 1)), dim3(256), 0, s, start, nv, lcnt);
+	if (!nv)
+		return;
+	hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, huge);
+	hipLaunchKernelGGL(k_long_border, dim3(g1(nv)), dim3(256), 0, s, tris, nv, start, ref, literal, border, huge);
+	hipLaunchKernelGGL(k_huge_border, dim3(64), dim3(HG_BS), 0, s, tris, start, ref, literal, border,
+	                   (const uint32_t *)huge);
 }
-void launch_long_flag(const int *tris, uint32_t n3, const uint32_t *start, uint32_t *flag, hipStream_t s)
-{
-	if (n3)
-		hipLaunchKernelGGL(k_long_flag, dim3(g1(n3)), dim3(256), 0, s, tris, n3, start, flag);
-}
-void launch_long_compact(const int *tris, uint32_t n3, const uint32_t *start, const uint32_t *pos, uint32_t *key,
-                         uint32_t *val, hipStream_t s)
-{
-	if (n3)
-		hipLaunchKernelGGL(k_long_compact, dim3(g1(n3)), dim3(256), 0, s, tris, n3, start, pos, key, val);
-}
-void launch_long_place(const int *tris, const uint32_t *key, const uint32_t *val, uint32_t L, const uint32_t *start,
-                       const uint32_t *loff, uint32_t *ref, int2 *pairs, hipStream_t s)
-{
-	if (L)
-		hipLaunchKernelGGL(k_long_place, dim3(g1(L)), dim3(256), 0, s, tris, key, val, L, start, loff, ref, pairs);
-}
-void launch_long_border_entries(const int *tris, const uint32_t *key, const uint32_t *val, uint32_t L, int literal,
-                                uint32_t *ek, uint32_t *ev, hipStream_t s)
-{
-	if (L)
-		hipLaunchKernelGGL(k_long_border_entries, dim3(g1(L)), dim3(256), 0, s, tris, key, val, L, literal, ek, ev);
-}
-void launch_long_border_mark(const uint32_t *v, const uint32_t *id, uint32_t n, int literal, uint8_t *border,
-                             hipStream_t s)
-{
-	if (n)
-		hipLaunchKernelGGL(k_long_border_mark, dim3(g1(n)), dim3(256), 0, s, v, id, n, literal, border);
-}
-
 void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, uint32_t *rank, hipStream_t s)
 {
 	hipLaunchKernelGGL(k_csr_count, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, cnt, rank);

@@ -300,16 +300,15 @@ void launch_relabel_tris(const int *tris, const uint32_t *inv, uint32_t nt, int 
 void launch_tris_repeat(const int *tris, uint32_t nt, uint32_t *flag, hipStream_t s);
 void launch_border_pairs(const uint32_t *start, const int2 *pairs, uint32_t nv, uint8_t *border, hipStream_t s);
 // high-valence vertices (more than 16 references)
-void launch_long_count(const uint32_t *start, uint32_t nv, uint32_t *lcnt, hipStream_t s);
-void launch_long_flag(const int *tris, uint32_t n3, const uint32_t *start, uint32_t *flag, hipStream_t s);
-void launch_long_compact(const int *tris, uint32_t n3, const uint32_t *start, const uint32_t *pos, uint32_t *key,
-                         uint32_t *val, hipStream_t s);
-void launch_long_place(const int *tris, const uint32_t *key, const uint32_t *val, uint32_t L, const uint32_t *start,
-                       const uint32_t *loff, uint32_t *ref, int2 *pairs, hipStream_t s);
-void launch_long_border_entries(const int *tris, const uint32_t *key, const uint32_t *val, uint32_t L, int literal,
-                                uint32_t *ek, uint32_t *ev, hipStream_t s);
-void launch_long_border_mark(const uint32_t *v, const uint32_t *id, uint32_t n, int literal, uint8_t *border,
-                             hipStream_t s);
+// the lists longer than 16 references, one wave each (the longest one
+// workgroup each): sorted in place with their pairs expanded; their border
+// flags (literal: the once-counted ids).  huge: 1 + (3 nt / 257 + 1) words
+// of scratch.
+void launch_long_sort(const int *tris, uint32_t nv, const uint32_t *start, uint32_t *ref, int2 *pairs,
+                      uint32_t *huge, hipStream_t s);
+void launch_long_border(const int *tris, uint32_t nv, const uint32_t *start, const uint32_t *ref, int literal,
+                        uint8_t *border, uint32_t *huge, hipStream_t s);
+size_t long_huge_words(size_t nt);
 // batched atlas weld (k_mesh.hip)
 void launch_atlas_weld_keys(const double *pts, uint32_t n, const uint32_t *vbase, int ntasks, uint32_t *keys,
                             uint32_t *vals, uint32_t *vtask, hipStream_t s);

@@ -589,14 +589,16 @@ def fan_mesh(n_apex, seed, repeats=False):
     return t[order], p
 
 
+@pytest.mark.parametrize('apex', [2000, 12000])
 @pytest.mark.parametrize('repeats', [False, True])
 @pytest.mark.parametrize('lock', [True, False])
-def test_hc_high_valence_fan(gpu, oracle, repeats, lock):
-    """A vertex in 12000 triangles (and others of valence 20 and 14): its
-    sorted reference list and its border flags come from the sort-based
-    pass, not one lane per list; HC and the plain Laplacian equal the
-    oracle's bit for bit."""
-    t, p = fan_mesh(12000, 3, repeats)
+def test_hc_high_valence_fan(gpu, oracle, repeats, lock, apex):
+    """A vertex in `apex` triangles (and others of valence 20 and 14): the
+    valence-20 list is sorted and counted by one wave, the apex's by one
+    workgroup (an LDS sort; with 12000 its 24000 or 36000 border entries
+    exceed the LDS and are compared from global memory); HC and the plain
+    Laplacian equal the oracle's bit for bit."""
+    t, p = fan_mesh(apex, 3, repeats)
     a, b = p.copy(), p.copy()
     gpu.lib().laplacian_smoothHC(a, t, 0.1, 0.5, 5, lock)
     oracle.laplacian_smoothHC(b, t, 0.1, 0.5, 5, lock)
