@@ -203,6 +203,8 @@ enum Slot {
 	S_AT_RSTILES,
 	S_LREP,
 	S_HC_HUGE,
+	S_SIGNP,
+	S_SIGNN,
 	S_AT_NEXT,
 	S_AT_CBASE,
 	S_AT_IDENT,
@@ -590,6 +592,9 @@ struct VolOut {
 	float iso;
 	int lo[3], hi[3];
 	float vmin; // min of the final volume (Lewiner padding)
+	// the apply stage's Lewiner sign bits of the final volume (null when not
+	// written): lw_clamp(v - iso) > 0 / < 0, ceil(nx / 32) words per row
+	const uint32_t *sgp = nullptr, *sgn = nullptr;
 };
 
 // spec_bits (when given) holds the threshold mask of P.isolevel, written by
@@ -682,10 +687,19 @@ bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P
 	uint32_t minit = 0xffffffffu;
 	HIPX(hipMemcpyAsync(C->d_small + SM_BBOX, init, 6 * sizeof(int), hipMemcpyHostToDevice, s));
 	HIPX(hipMemcpyAsync(C->d_small + SM_VMIN, &minit, 4, hipMemcpyHostToDevice, s));
+	uint32_t *sgp = nullptr, *sgn = nullptr;
+	if (P.cubes == N2M_CUBES_LEWINER) { // the sign bits the classify pass reads
+		const size_t sw = (size_t)((NX + 31) / 32) * NY * NZ;
+		sgp = C->buf<uint32_t>(S_SIGNP, sw);
+		sgn = C->buf<uint32_t>(S_SIGNN, sw);
+	}
 	C->tbegin("apply", 8.0 * nvox);
-	launch_apply(in, vol, NX, NY, NZ, isolevel, mn, edgeMax, P.fillBubbles ? keep : nullptr, dil, P.fillBubbles ? 1 : 0,
-	             (int *)(C->d_small + SM_BBOX), C->d_small + SM_VMIN, s);
+	const bool signs = launch_apply(in, vol, NX, NY, NZ, isolevel, mn, edgeMax, P.fillBubbles ? keep : nullptr, dil,
+	                                P.fillBubbles ? 1 : 0, (int *)(C->d_small + SM_BBOX), C->d_small + SM_VMIN, s, sgp,
+	                                sgn);
 	C->tend();
+	out.sgp = signs ? sgp : nullptr;
+	out.sgn = signs ? sgn : nullptr;
 	uint32_t *bb = C->fetch(SM_BBOX, 7);
 	if ((P.onlyLargest || P.fillBubbles) && P.verbose)
 		printf("voxel clustering (largest cluster, bubbles): %ld ms\n", (long)round(now_ms() - t0));
@@ -710,7 +724,8 @@ struct McOut {
 };
 
 bool marching_cubes(n2m_ctx *C, const float *vol, const int dim[3], const int lo[3], const int hi[3], float iso,
-                    int cubes, int originalMC, float vmin, McOut &o, const int *origin = nullptr)
+                    int cubes, int originalMC, float vmin, McOut &o, const int *origin = nullptr,
+                    const uint32_t *sgp = nullptr, const uint32_t *sgn = nullptr)
 {
 	static const int zero[3] = {0, 0, 0};
 	if (!origin)
@@ -755,8 +770,13 @@ bool marching_cubes(n2m_ctx *C, const float *vol, const int dim[3], const int lo
 	uint32_t *cnt = C->buf<uint32_t>(S_SEGCNT, 3 * nseg);
 	uint32_t *base = C->buf<uint32_t>(S_SEGBASE, 3 * nseg);
 	float mnv = vmin - iso;
-	C->tbegin("mc_lewiner_classify", 4.0 * npts + 2.0 * npts + 3.0 * npts / 8);
-	launch_lw_classify(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, nullptr, cr, cnt, s, fm);
+	if (sgp && sgn) {
+		C->tbegin("mc_lewiner_classify", 2.0 * npts / 8 + 3.0 * npts / 8);
+		launch_lw_classify_bits(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, sgp, sgn, cr, cnt, fm, s);
+	} else {
+		C->tbegin("mc_lewiner_classify", 4.0 * npts + 2.0 * npts + 3.0 * npts / 8);
+		launch_lw_classify(vol, NX, NY, NZ, lo, hi, iso, mnv, originalMC, nullptr, cr, cnt, s, fm);
+	}
 	C->tend();
 	uint32_t tot[3];
 	scan3(C, cnt, base, nseg, tot);
@@ -988,7 +1008,8 @@ bool surface_stages(n2m_ctx *C, const float *vol, const int dim[3], const VolOut
 {
 	double t0 = now_ms();
 	McOut mc;
-	if (!marching_cubes(C, vol, dim, vo.lo, vo.hi, vo.iso, P.cubes, P.originalMC, vo.vmin, mc))
+	if (!marching_cubes(C, vol, dim, vo.lo, vo.hi, vo.iso, P.cubes, P.originalMC, vo.vmin, mc, nullptr, vo.sgp,
+	                    vo.sgn))
 		return false;
 	if (P.verbose) {
 		C->sync();

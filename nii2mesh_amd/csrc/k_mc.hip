@@ -554,6 +554,148 @@ k_lw_classify(const float *__restrict__ img, LwGeo g, uint8_t *__restrict__ pfla
 	}
 }
 
+// The whole-volume pass from sign bits.  P / N hold the bits
+// lw_clamp(v - iso) > 0 and < 0 of every voxel in linear order (the apply
+// stage writes them; rows of whole 32-voxel words).  A wave takes a chunk of
+// 64 points of a row as 64-bit words: the three crossing masks
+// (c0 < 0 ? c1 > 0 : c1 < 0, NaN-exact with the two bits) and the mask of
+// cubes with corners on both sides come from a few word operations; only
+// the lanes of those cubes build a case, and only the cases whose tiling
+// depends on face or interior tests read the eight corner values.
+struct SignBits {
+	const uint32_t *P, *N;
+	size_t nwords;         // words of each array (nx * ny * nz / 32)
+	uint32_t padP, padN;   // a padding word (the signs of mnv, lw_clamp applied)
+	int nx;
+	long long nxy;
+	// 65 bits of `a` from the linear index of sub-box point (x, y, z), as
+	// lw_data indexes (bits past the volume take the padding)
+	__device__ __forceinline__ unsigned long long win(const uint32_t *__restrict__ a, uint32_t pad, const LwGeo &g,
+	                                                  int x, int y, int z, uint32_t &b64) const
+	{
+		const long long j = (long long)g.lo0 + x + (long long)(y + g.lo1) * nx + (long long)(z + g.lo2) * nxy;
+		const size_t w = (size_t)(j >> 5);
+		const int sh = (int)(j & 31);
+		const uint32_t w0 = w < nwords ? a[w] : pad, w1 = w + 1 < nwords ? a[w + 1] : pad,
+		               w2 = w + 2 < nwords ? a[w + 2] : pad;
+		const unsigned long long lo = (unsigned long long)w0 | ((unsigned long long)w1 << 32);
+		b64 = (w2 >> sh) & 1u;
+		return sh ? (lo >> sh) | ((unsigned long long)w2 << (64 - sh)) : lo;
+	}
+};
+
+// the tilings whose choice reads corner values (face / interior tests)
+__device__ __forceinline__ bool lw_case_tests(int cs)
+{
+	return cs == 3 || cs == 4 || cs == 6 || cs == 7 || cs == 10 || cs == 12 || cs == 13;
+}
+
+// Each lane takes one 64-point chunk: the 8 lanes of an aligned group are
+// the (up to) 8 chunks of one row segment (LW_SEG = 8 x 64), so a wave
+// covers 8 row segments and the segment counts are a reduction over the
+// group.  The surface cubes of a chunk (a few) are then visited one by one.
+static_assert(LW_SEG == 8 * N2M_WAVE, "8 chunks per row segment");
+
+__global__ void __launch_bounds__(LW_BS)
+k_lw_classify_bits(const float *__restrict__ img, LwGeo g, SignBits sb, uint16_t *__restrict__ crow,
+                   uint32_t *__restrict__ segcnt /* 3 arrays of nseg*sy*sz */, unsigned long long *__restrict__ fmask)
+{
+	__shared__ uint8_t s_cs[256], s_cf[256], s_ntri[N2M_LW_NROWS], s_centre[N2M_LW_NROWS];
+	s_cs[threadIdx.x] = n2m_lw_case[threadIdx.x];
+	s_cf[threadIdx.x] = n2m_lw_config[threadIdx.x];
+	for (int i = threadIdx.x; i < N2M_LW_NROWS; i += LW_BS) {
+		s_ntri[i] = n2m_lw_row_ntri[i];
+		s_centre[i] = n2m_lw_row_centre[i];
+	}
+	__syncthreads();
+	const size_t ns = (size_t)g.nseg * g.sy * g.sz;
+	const size_t sr = ((size_t)blockIdx.x * LW_BS + threadIdx.x) / 8; // row segment
+	const int k = threadIdx.x & 7;                                     // chunk within it
+	if (sr >= ns)
+		return; // whole 8-lane groups
+	const int seg = (int)(sr % g.nseg);
+	const size_t yz = sr / g.nseg;
+	const int y = (int)(yz % g.sy), z = (int)(yz / g.sy);
+	const int x0 = seg * LW_SEG + k * N2M_WAVE;
+	const bool hy = y < g.sy - 1, hz = z < g.sz - 1, cuberow = hy && hz;
+	uint32_t nv = 0, nt = 0, nc = 0;
+	if (x0 < g.sx) {
+		// bit i <-> point x0 + i of rows A (y, z), B (y + 1, z), C (y, z + 1),
+		// D (y + 1, z + 1); the "x" words hold the x + 1 neighbours
+		uint32_t t0, t1;
+		const unsigned long long pA = sb.win(sb.P, sb.padP, g, x0, y, z, t0),
+		                         nA = sb.win(sb.N, sb.padN, g, x0, y, z, t1);
+		const unsigned long long pAx = (pA >> 1) | ((unsigned long long)t0 << 63),
+		                         nAx = (nA >> 1) | ((unsigned long long)t1 << 63);
+		unsigned long long pB = pA, nB = nA, pC = pA, nC = nA, pD = pA, pBx = pAx, pCx = pAx, pDx = pAx;
+		if (hy) {
+			pB = sb.win(sb.P, sb.padP, g, x0, y + 1, z, t0);
+			pBx = (pB >> 1) | ((unsigned long long)t0 << 63);
+			nB = sb.win(sb.N, sb.padN, g, x0, y + 1, z, t1);
+		}
+		if (hz) {
+			pC = sb.win(sb.P, sb.padP, g, x0, y, z + 1, t0);
+			pCx = (pC >> 1) | ((unsigned long long)t0 << 63);
+			nC = sb.win(sb.N, sb.padN, g, x0, y, z + 1, t1);
+		}
+		if (cuberow) {
+			pD = sb.win(sb.P, sb.padP, g, x0, y + 1, z + 1, t0);
+			pDx = (pD >> 1) | ((unsigned long long)t0 << 63);
+		}
+		const int npt = min(N2M_WAVE, g.sx - x0), nxn = min(N2M_WAVE, g.sx - 1 - x0); // points; with an x + 1
+		const unsigned long long valid = npt >= N2M_WAVE ? ~0ull : ((1ull << npt) - 1ull);
+		const unsigned long long validx = nxn <= 0 ? 0ull : (nxn >= N2M_WAVE ? ~0ull : ((1ull << nxn) - 1ull));
+		const unsigned long long fx = ((nA & pAx) | (~nA & nAx)) & validx;
+		const unsigned long long fy = hy ? ((nA & pB) | (~nA & nB)) & valid : 0ull;
+		const unsigned long long fz = hz ? ((nA & pC) | (~nA & nC)) & valid : 0ull;
+		unsigned long long surf = 0, mc = 0;
+		if (cuberow) {
+			const unsigned long long all1 = pA & pAx & pB & pBx & pC & pCx & pD & pDx;
+			const unsigned long long any1 = pA | pAx | pB | pBx | pC | pCx | pD | pDx;
+			surf = any1 & ~all1 & validx;
+		}
+		const size_t rowbase = (size_t)g.sx * ((size_t)y + (size_t)g.sy * z);
+		for (unsigned long long m = surf; m; m &= m - 1) {
+			const int i = __ffsll((long long)m) - 1, x = x0 + i;
+			const int lut = (int)((pA >> i) & 1ull) | ((int)((pAx >> i) & 1ull) << 1) |
+			                ((int)((pBx >> i) & 1ull) << 2) | ((int)((pB >> i) & 1ull) << 3) |
+			                ((int)((pC >> i) & 1ull) << 4) | ((int)((pCx >> i) & 1ull) << 5) |
+			                ((int)((pDx >> i) & 1ull) << 6) | ((int)((pD >> i) & 1ull) << 7);
+			float c[8];
+			if (!g.originalMC && lw_case_tests(s_cs[lut])) {
+#pragma unroll
+				for (int p = 0; p < 8; p++)
+					c[p] = lw_data(img, g, x + c_cx[p], y + c_cy[p], z + c_cz[p]);
+			}
+			const int row = lw_select(c, lut, g.originalMC, s_cs, s_cf);
+			if (row >= 0) {
+				nt += s_ntri[row];
+				nc += (!g.originalMC && s_centre[row]) ? 1u : 0u;
+				crow[rowbase + x] = (uint16_t)(row + 1);
+				mc |= 1ull << i;
+			}
+		}
+		const int nchunk = (g.sx + N2M_WAVE - 1) / N2M_WAVE;
+		unsigned long long *o = fmask + ((size_t)(y + (size_t)g.sy * z) * nchunk + (size_t)(x0 / N2M_WAVE)) * LW_FMW;
+		o[0] = fx;
+		o[1] = fy;
+		o[2] = fz;
+		o[3] = mc;
+		nv = (uint32_t)(__popcll(fx) + __popcll(fy) + __popcll(fz));
+	}
+#pragma unroll
+	for (int d = 1; d < 8; d <<= 1) {
+		nv += __shfl_xor(nv, d);
+		nt += __shfl_xor(nt, d);
+		nc += __shfl_xor(nc, d);
+	}
+	if (k == 0) {
+		segcnt[sr] = nv;
+		segcnt[ns + sr] = nt;
+		segcnt[2 * ns + sr] = nc;
+	}
+}
+
 // edge vertex position in sub-box coordinates, src/MarchingCubes.c:935-1026
 __device__ __forceinline__ void lw_edge_pos(const float *__restrict__ img, const LwGeo &g, int x, int y, int z,
                                             int a, float *p)
@@ -1046,6 +1188,26 @@ void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3
 	LwGeo g;
 	lw_geo(g, nx, ny, nz, lo, hi, iso, mnv, originalMC);
 	hipLaunchKernelGGL(k_lw_classify, dim3(g.nseg, g.sy, (g.sz + LW_ZC - 1) / LW_ZC), dim3(LW_BS), 0, s, img, g, pflags,
+	                   crow, segcnt, fmask);
+}
+
+void launch_lw_classify_bits(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
+                             float mnv, int originalMC, const uint32_t *P, const uint32_t *N, uint16_t *crow,
+                             uint32_t *segcnt, unsigned long long *fmask, hipStream_t s)
+{
+	LwGeo g;
+	lw_geo(g, nx, ny, nz, lo, hi, iso, mnv, originalMC);
+	SignBits sb;
+	sb.P = P;
+	sb.N = N;
+	sb.nwords = (size_t)nx * ny * nz / 32; // nx % 32 == 0
+	sb.nx = nx;
+	sb.nxy = (long long)nx * ny;
+	const float m = fabsf(mnv) < FLT_EPSILON ? FLT_EPSILON : mnv; // lw_clamp
+	sb.padP = m > 0.f ? 0xffffffffu : 0u;
+	sb.padN = m < 0.f ? 0xffffffffu : 0u;
+	const size_t lanes = 8 * (size_t)g.nseg * g.sy * g.sz;
+	hipLaunchKernelGGL(k_lw_classify_bits, dim3((unsigned)((lanes + LW_BS - 1) / LW_BS)), dim3(LW_BS), 0, s, img, g, sb,
 	                   crow, segcnt, fmask);
 }
 

@@ -561,7 +561,12 @@ __global__ void k_weld_greedy(const uint32_t *__restrict__ starts, uint32_t nseg
 	if (mine) {
 		b = starts[s];
 		len = (s + 1 < nseg ? starts[s + 1] : n) - b;
-		if (len <= (uint32_t)WG_REG)
+		if (len == 1) { // a vertex alone in its key window is its own class
+			lcls[b] = 0;
+			if (lrep)
+				lrep[b] = vals[b];
+			seeds[s] = 1u;
+		} else if (len <= (uint32_t)WG_REG)
 			weld_segment_regs(s, b, len, keys, vals, pts, lcls, seeds, lrep);
 		else if (len > (uint32_t)N2M_WAVE)
 			weld_segment_serial(s, b, b + len, keys, vals, pts, lcls, seeds, lrep);

@@ -9,6 +9,8 @@
 //
 // Bit volumes: one uint32 word per 32 voxels of an x-row, W = ceil(nx/32)
 // words per row, word index wx + W*(y + ny*z), bit b <-> x = 32*wx + b.
+#include <float.h>
+
 #include "n2m_device.cuh"
 #include "n2m_internal.h"
 
@@ -805,7 +807,7 @@ void launch_dilate_bits(const uint32_t *in, uint32_t *out, int nx, int ny, int n
 __global__ void __launch_bounds__(256)
 k_apply(const float *src, float *img, int nx, int ny, int nz, int W, int vec4, float iso, float mn,
         float edgeMax, const uint32_t *__restrict__ keep, const uint32_t *__restrict__ dil, int fill,
-        int *__restrict__ bbox, uint32_t *__restrict__ vmin)
+        int *__restrict__ bbox, uint32_t *__restrict__ vmin, uint32_t *__restrict__ sgp, uint32_t *__restrict__ sgn)
 {
 	__shared__ int s_red[6][4];
 	__shared__ uint32_t s_min[4];
@@ -856,6 +858,32 @@ k_apply(const float *src, float *img, int nx, int ny, int nz, int W, int vec4, f
 				v.z = voxel(x + 2, y, z, face_row, v.z, kw, dw, k_on, d_on);
 				v.w = voxel(x + 3, y, z, face_row, v.w, kw, dw, k_on, d_on);
 				line[q] = v;
+				if (sgp) {
+					// sign bits of the Lewiner data lw_clamp(v - iso): the 8
+					// lanes of an aligned group hold one 32-voxel word of the
+					// row (nx % 32 == 0 and the span a multiple of 8, so the
+					// group shares its row and its trip count)
+					const float vv[4] = {v.x, v.y, v.z, v.w};
+					uint32_t bp = 0, bn = 0;
+#pragma unroll
+					for (int u = 0; u < 4; u++) {
+						float d = vv[u] - iso;
+						d = fabsf(d) < FLT_EPSILON ? FLT_EPSILON : d;
+						bp |= (uint32_t)(d > 0.f) << u;
+						bn |= (uint32_t)(d < 0.f) << u;
+					}
+					bp <<= 4 * (q & 7);
+					bn <<= 4 * (q & 7);
+#pragma unroll
+					for (int m = 1; m < 8; m <<= 1) {
+						bp |= __shfl_xor(bp, m);
+						bn |= __shfl_xor(bn, m);
+					}
+					if ((q & 7) == 0) {
+						sgp[(size_t)W * row + (x >> 5)] = bp;
+						sgn[(size_t)W * row + (x >> 5)] = bn;
+					}
+				}
 			}
 		}
 	} else {
@@ -906,16 +934,26 @@ k_apply(const float *src, float *img, int nx, int ny, int nz, int W, int vec4, f
 	}
 }
 
-void launch_apply(const float *src, float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax,
-                  const uint32_t *keep, const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s)
+bool apply_sign_bits_ok(const float *src, const float *img, int nx)
+{
+	const float *in = src ? src : img;
+	return nx % 32 == 0 && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(img)) & 15u) == 0;
+}
+
+bool launch_apply(const float *src, float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax,
+                  const uint32_t *keep, const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s,
+                  uint32_t *sgp, uint32_t *sgn)
 {
 	const int W = (nx + 31) / 32;
 	const float *in = src ? src : img;
 	const bool vec4 = nx % 4 == 0 && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(img)) & 15u) == 0;
+	if (!sgp || !sgn || !apply_sign_bits_ok(src, img, nx))
+		sgp = sgn = nullptr;
 	const size_t rows = (size_t)ny * nz;
 	const size_t rpi = vec4 && nx / 4 < 256 ? 256 / (nx / 4) : 1;
 	hipLaunchKernelGGL(k_apply, dim3(row_blocks((rows + rpi - 1) / rpi)), dim3(256), 0, s, in, img, nx, ny, nz, W,
-	                   vec4 ? 1 : 0, iso, mn, edgeMax, keep, dil, fill, bbox, vmin);
+	                   vec4 ? 1 : 0, iso, mn, edgeMax, keep, dil, fill, bbox, vmin, sgp, sgn);
+	return sgp != nullptr;
 }
 
 // ---------------------------------------------------------------------------

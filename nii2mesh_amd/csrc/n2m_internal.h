@@ -87,8 +87,12 @@ void launch_mask_bits(const float *img, int nx, int ny, int nz, float iso, int n
                       hipStream_t s);
 void launch_dilate_bits(const uint32_t *in, uint32_t *out, int nx, int ny, int nz, hipStream_t s);
 // img = apply(src) (src null: in place); every voxel of img is written
-void launch_apply(const float *src, float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax,
-                  const uint32_t *keep, const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s);
+// sgp / sgn (when given and nx % 32 == 0 with 16-byte aligned volumes):
+// the Lewiner sign bits of the result, lw_clamp(v - iso) > 0 / < 0, ceil(nx
+// / 32) words per row; returns whether they were written
+bool launch_apply(const float *src, float *img, int nx, int ny, int nz, float iso, float mn, float edgeMax,
+                  const uint32_t *keep, const uint32_t *dil, int fill, int *bbox, uint32_t *vmin, hipStream_t s,
+                  uint32_t *sgp = nullptr, uint32_t *sgn = nullptr);
 void launch_hist_range(const float *img, size_t n, uint32_t *stats, hipStream_t s);
 // mesh output encoding (k_encode.hip)
 void launch_f64_to_f32(const double *in, size_t n, float *out, hipStream_t s);
@@ -154,6 +158,11 @@ size_t lw_nseg_total(int nx, int ny, int nz, const int lo[3], const int hi[3]);
 // pass A of the whole-volume Lewiner cubes: point flags to pflags (one byte
 // each), or, when fmask is given, to per-axis bit masks (lw_fmask_words
 // words); tiling rows to crow, segment counts to segcnt
+// launch_lw_classify (fmask form) from the sign bits the apply stage wrote
+// (P: lw_clamp(v - iso) > 0, N: < 0; per volume row, ceil(nx / 32) words)
+void launch_lw_classify_bits(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
+                             float mnv, int originalMC, const uint32_t *P, const uint32_t *N, uint16_t *crow,
+                             uint32_t *segcnt, unsigned long long *fmask, hipStream_t s);
 void launch_lw_classify(const float *img, int nx, int ny, int nz, const int lo[3], const int hi[3], float iso,
                         float mnv, int originalMC, uint8_t *pflags, uint16_t *crow, uint32_t *segcnt,
                         hipStream_t s, unsigned long long *fmask = nullptr);

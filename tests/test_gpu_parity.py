@@ -354,6 +354,39 @@ def test_meshify_blobs(gpu, oracle, shape, flags, flavour):
     assert _eq(ra[2], rb[2])
 
 
+def _sign_bits_volume(kind, shape, seed):
+    if kind == 'blobs':
+        return support.blobs(shape, seed, noise=3.0), None
+    if kind == 'face':  # an object cut by the x = nx - 1 face: the sub-box reaches nx (the wrapped reads)
+        img = support.blobs(shape, seed, noise=1.0)
+        img[:, :, -3:] += 80.0
+        return img, None
+    if kind == 'ties':  # integers at an integer isolevel: values equal to iso, FLT_EPSILON clamps
+        rng = np.random.default_rng(seed)
+        return rng.integers(0, 4, size=shape).astype(np.float32), 2.0
+    img = support.nan_volume(shape, seed)  # 'nan'
+    return img, float(np.float32(np.nanpercentile(img, 60)))
+
+
+@pytest.mark.parametrize('shape', [(9, 11, 32), (20, 23, 64), (14, 18, 96)])
+@pytest.mark.parametrize('kind', ['blobs', 'face', 'ties', 'nan'])
+@pytest.mark.parametrize('flags', [(True, True, True), (False, False, False), (True, False, True)])
+@pytest.mark.parametrize('o1', [0, 1])
+def test_meshify_lewiner_sign_bits(gpu, oracle, shape, kind, flags, o1):
+    """Lewiner meshify on volumes whose rows are whole 32-voxel words, where
+    the classify pass reads the apply stage's sign bits instead of the
+    values: the same meshes and mutated volume as the oracle."""
+    img, iso = _sign_bits_volume(kind, shape, 30 + shape[2])
+    if iso is None:
+        iso = float(np.float32(np.percentile(img, 62)))
+    a, b = img.copy(), img.copy()
+    ra = gpu.lib(1).meshify(a, iso, *flags, originalMC=o1)
+    rb = oracle.meshify(b, iso, *flags, originalMC=o1, cubes=1)
+    assert ra[0] == rb[0]
+    assert support.nan_eq(a, b)
+    assert _eq(ra[1], rb[1]) and support.nan_eq(ra[2], rb[2])
+
+
 @pytest.mark.parametrize('flags', [(True, True, True), (False, False, False), (False, True, False)])
 @pytest.mark.parametrize('flavour', ['classic', 'lewiner'])
 def test_meshify_nan_volume(gpu, oracle, flags, flavour):
