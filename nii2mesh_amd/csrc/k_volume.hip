@@ -68,17 +68,36 @@ __device__ __forceinline__ void smooth_tile(const Src &src, float *__restrict__ 
 		for (int k = 0; k < 5; k++)
 			ring[c][k] = 0.f;
 	const int x = x0 + tx;
+	// the plane's halo tile is loaded into registers one plane ahead, so its
+	// memory latency overlaps the previous plane's passes
+	constexpr int NIN = (TY + 4) * (TX + 4), NL = (NIN + SM_BS - 1) / SM_BS;
+	float pre[NL];
+	auto fetch = [&](int zz) {
+#pragma unroll
+		for (int q = 0; q < NL; q++) {
+			const int i = q * SM_BS + threadIdx.x;
+			const int ly = i / (TX + 4), lx = i % (TX + 4);
+			const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+			float v = 0.f;
+			if (i < NIN && zz >= 0 && zz < nz && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+				v = src(gx, gy, zz);
+			pre[q] = v;
+		}
+	};
+	fetch(z0 - 2);
 	for (int zz = z0 - 2; zz <= z1 + 1; zz++) {
 		const bool have = zz >= 0 && zz < nz;
 		if (have) {
-			for (int i = threadIdx.x; i < (TY + 4) * (TX + 4); i += SM_BS) {
-				int ly = i / (TX + 4), lx = i % (TX + 4);
-				int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
-				float v = 0.f;
-				if (gx >= 0 && gx < nx && gy >= 0 && gy < ny)
-					v = src(gx, gy, zz);
-				s_in[ly][lx] = v;
+#pragma unroll
+			for (int q = 0; q < NL; q++) {
+				const int i = q * SM_BS + threadIdx.x;
+				if (i < NIN)
+					s_in[i / (TX + 4)][i % (TX + 4)] = pre[q];
 			}
+		}
+		if (zz + 1 <= z1 + 1)
+			fetch(zz + 1);
+		if (have) {
 			__syncthreads();
 			for (int i = threadIdx.x; i < (TY + 4) * TX; i += SM_BS) {
 				int ly = i / TX, lx = i % TX;
