@@ -1263,25 +1263,39 @@ __device__ __forceinline__ void stage_halo(double *lds, const d3 *__restrict__ p
 	__syncthreads();
 }
 
-// row s of an LDS array of W-double rows (W = 4: x, y as one 16-byte read)
+// row s of an LDS array of W-double rows (W = 4: x, y as one 16-byte read),
+// or, for W = 2, of the split form of NR rows: NR 16-byte (x, y) pairs, then
+// NR z values.  Random rows of the split form are read about a third faster
+// than 32-byte rows: its 16-byte reads fall on every bank group, those of
+// 32-byte rows on half of them.
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) dbl2 lds_dbl2;
-template <int W> __device__ __forceinline__ d3 lds_row_w(const lds_double *L, uint32_t s)
+template <int W, int NR = 0> __device__ __forceinline__ d3 lds_row_w(const lds_double *L, uint32_t s)
 {
-	if constexpr (W == 4) {
-		const dbl2 xy = ((const lds_dbl2 *)L)[2 * s];
+	if constexpr (W == 4 || W == 2) {
+		const dbl2 xy = ((const lds_dbl2 *)L)[W == 4 ? 2 * s : s];
 		d3 r;
 		r.x = xy.x;
 		r.y = xy.y;
-		r.z = L[4 * s + 2];
+		r.z = W == 4 ? L[4 * s + 2] : L[2 * NR + 2 * s];
 		return r;
 	} else {
 		return lds_row(L, s);
 	}
 }
 
+// writes row s of the split form of NR rows
+template <int NR> __device__ __forceinline__ void lds_put_split(double *L, uint32_t s, const d3 v)
+{
+	dbl2 xy;
+	xy.x = v.x;
+	xy.y = v.y;
+	((dbl2 *)L)[s] = xy;
+	L[2 * NR + 2 * s] = v.z;
+}
+
 // laplace_at() over LDS slots: same terms, same order
-template <int K, int W = 3>
+template <int K, int W = 3, int NR = 0>
 __device__ __forceinline__ d3 laplace_slots(const SlotRefs<K> &r, const uint32_t *__restrict__ slots,
                                             const lds_double *L, const d3 self)
 {
@@ -1292,7 +1306,7 @@ __device__ __forceinline__ d3 laplace_slots(const SlotRefs<K> &r, const uint32_t
 #pragma unroll
 	for (int k = 0; k < K; k++) {
 		if ((uint32_t)k < n) {
-			const d3 P = lds_row_w<W>(L, r.q[k] & 0xffffu), Q = lds_row_w<W>(L, r.q[k] >> 16);
+			const d3 P = lds_row_w<W, NR>(L, r.q[k] & 0xffffu), Q = lds_row_w<W, NR>(L, r.q[k] >> 16);
 			sx = sx + (P.x + Q.x);
 			sy = sy + (P.y + Q.y);
 			sz = sz + (P.z + Q.z);
@@ -1300,7 +1314,7 @@ __device__ __forceinline__ d3 laplace_slots(const SlotRefs<K> &r, const uint32_t
 	}
 	for (uint32_t i = r.b + K; i < r.e; i++) {
 		const uint32_t s = slots[i];
-		const d3 P = lds_row_w<W>(L, s & 0xffffu), Q = lds_row_w<W>(L, s >> 16);
+		const d3 P = lds_row_w<W, NR>(L, s & 0xffffu), Q = lds_row_w<W, NR>(L, s >> 16);
 		sx = sx + (P.x + Q.x);
 		sy = sy + (P.y + Q.y);
 		sz = sz + (P.z + Q.z);
@@ -1609,7 +1623,7 @@ k_ring_build(const uint32_t *__restrict__ start, const int2 *__restrict__ pairs,
 }
 
 // one HC iteration (hc_a then hc_b) in two-ring form; pout != p.  LDS rows
-// are padded to 4 doubles (read as 16 + 8 bytes).
+// in split form (an (x, y) array and a z array, read as 16 + 8 bytes).
 template <int RB>
 __global__ void __launch_bounds__(RT)
 k_hc_ring(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots, const uint16_t *__restrict__ roff,
@@ -1617,8 +1631,8 @@ k_hc_ring(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots
           const d3 *__restrict__ o, const d3 *__restrict__ o1, const d3 *__restrict__ p, d3 *__restrict__ pout,
           uint32_t nv, double alpha, double alpha1, double beta, double beta1)
 {
-	constexpr int W = 4;
-	__shared__ double lds[W * RROWS];
+	constexpr int W = 2;
+	__shared__ double lds[4 * RROWS];
 	constexpr int RCAP = RROWS - RB;
 	const uint32_t blk = blockIdx.x, base = blk * RB, tid = threadIdx.x;
 	const uint2 nn = rcnt[blk];
@@ -1658,25 +1672,18 @@ k_hc_ring(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots
 	}
 	// stage block (the own rows, already in registers), ring-1 and ring-2
 	// positions
-	if (tid < RB) {
-		lds[W * tid] = pv.x;
-		lds[W * tid + 1] = pv.y;
-		lds[W * tid + 2] = pv.z;
-	}
-	for (uint32_t i = tid; i < n1 + n2; i += RT) {
-		const d3 q = p[rg[i]];
-		lds[W * (RB + i)] = q.x;
-		lds[W * (RB + i) + 1] = q.y;
-		lds[W * (RB + i) + 2] = q.z;
-	}
+	if (tid < RB)
+		lds_put_split<RROWS>(lds, tid, pv);
+	for (uint32_t i = tid; i < n1 + n2; i += RT)
+		lds_put_split<RROWS>(lds, RB + i, p[rg[i]]);
 	__syncthreads();
 	const lds_double *L = (const lds_double *)lds;
 	// hc_a on row tid and, for rings longer than RT - RB, on row tid + RT
 	d3 Lp = {0.0, 0.0, 0.0}, bb = {0.0, 0.0, 0.0}, bb2 = {0.0, 0.0, 0.0};
 	if (own || r1a) {
 		if (!own)
-			pv = lds_row_w<W>(L, tid);
-		Lp = laplace_slots<HK, W>(r, sl, L, pv);
+			pv = lds_row_w<W, RROWS>(L, tid);
+		Lp = laplace_slots<HK, W, RROWS>(r, sl, L, pv);
 		bb.x = Lp.x - (ov.x * alpha + pv.x * alpha1);
 		bb.y = Lp.y - (ov.y * alpha + pv.y * alpha1);
 		bb.z = Lp.z - (ov.z * alpha + pv.z * alpha1);
@@ -1691,28 +1698,22 @@ k_hc_ring(const uint32_t *__restrict__ start, const uint32_t *__restrict__ slots
 		for (int k = 0; k < HK; k++)
 			r2.q[k] = r2.b + k < r2.e ? rs[r2.b + k] : 0u;
 		const d3 o2 = ob[u2 - RB];
-		const d3 pv2 = lds_row_w<W>(L, u2);
-		const d3 L2 = laplace_slots<HK, W>(r2, rs, L, pv2);
+		const d3 pv2 = lds_row_w<W, RROWS>(L, u2);
+		const d3 L2 = laplace_slots<HK, W, RROWS>(r2, rs, L, pv2);
 		bb2.x = L2.x - (o2.x * alpha + pv2.x * alpha1);
 		bb2.y = L2.y - (o2.y * alpha + pv2.y * alpha1);
 		bb2.z = L2.z - (o2.z * alpha + pv2.z * alpha1);
 	}
 	__syncthreads(); // every position row has been read
-	if (own || r1a) {
-		lds[W * tid] = bb.x;
-		lds[W * tid + 1] = bb.y;
-		lds[W * tid + 2] = bb.z;
-	}
-	if (r2a) {
-		lds[W * u2] = bb2.x;
-		lds[W * u2 + 1] = bb2.y;
-		lds[W * u2 + 2] = bb2.z;
-	}
+	if (own || r1a)
+		lds_put_split<RROWS>(lds, tid, bb);
+	if (r2a)
+		lds_put_split<RROWS>(lds, u2, bb2);
 	__syncthreads();
 	if (!own)
 		return;
 	// hc_b on the block's vertices: their pairs reach block and ring 1 only
-	const d3 Lb = laplace_slots<HK, W>(r, sl, L, bb);
+	const d3 Lb = laplace_slots<HK, W, RROWS>(r, sl, L, bb);
 	d3 out;
 	out.x = Lp.x - (bb.x * beta + Lb.x * beta1);
 	out.y = Lp.y - (bb.y * beta + Lb.y * beta1);
