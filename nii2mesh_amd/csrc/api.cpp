@@ -302,6 +302,8 @@ struct n2m_ctx {
 	hipEvent_t cur_a = nullptr;
 	double cur_bytes = 0;
 	std::vector<n2m_ctx *> workers; // per-isolevel contexts of the pipeline's threaded meshing
+	void *scan_zero_p = nullptr;    // the scan scratch allocation last zeroed
+	size_t scan_zero_cap = 0;
 
 	void *get(Slot s, size_t bytes)
 	{
@@ -556,7 +558,16 @@ void set_mesh_count(n2m_ctx *c, int n)
 uint32_t *scan_scratch(n2m_ctx *C, size_t n)
 {
 	size_t w = scan_scratch_words(n) + 16;
-	return C->buf<uint32_t>(S_SCAN, w > ((size_t)1 << 21) ? w : ((size_t)1 << 21));
+	uint32_t *p = C->buf<uint32_t>(S_SCAN, w > ((size_t)1 << 21) ? w : ((size_t)1 << 21));
+	// the single-pass scan keeps the scratch's leading words zero between
+	// scans: zero them once per allocation, stream-ordered before the scans
+	const DevBuf &b = C->bufs[S_SCAN];
+	if (b.p != C->scan_zero_p || b.cap != C->scan_zero_cap) {
+		HIPX(hipMemsetAsync(p, 0, 4 * scan_scratch_zero_words(), C->stream));
+		C->scan_zero_p = b.p;
+		C->scan_zero_cap = b.cap;
+	}
+	return p;
 }
 
 // three exclusive scans of n elements (in + k*n -> out + k*n) with one host
@@ -598,9 +609,12 @@ struct VolOut {
 };
 
 // spec_bits (when given) holds the threshold mask of P.isolevel, written by
-// the smoothing; it is recomputed there if the range check moves the isolevel
+// the smoothing; it is recomputed there if the range check moves the isolevel.
+// known_range (when given) holds the ordered min / max words of the input on
+// the host, which saves the device words' round trip.
 bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P, bool have_range,
-                   uint32_t *spec_bits, VolOut &out, const float *src = nullptr)
+                   uint32_t *spec_bits, VolOut &out, const float *src = nullptr,
+                   const uint32_t *known_range = nullptr)
 {
 	// src (when given) holds the voxels; the stages read it and the apply
 	// stage writes every voxel of vol, so vol needs no copy of src
@@ -608,14 +622,14 @@ bool volume_stages(n2m_ctx *C, float *vol, const int dim[3], const n2m_params &P
 	const int NX = dim[0], NY = dim[1], NZ = dim[2];
 	const size_t nvox = (size_t)NX * NY * NZ;
 	hipStream_t s = C->stream;
-	if (!have_range) {
+	if (!have_range && !known_range) {
 		uint32_t init[2] = {0xffffffffu, 0u};
 		HIPX(hipMemcpyAsync(C->d_small + SM_RANGE, init, sizeof init, hipMemcpyHostToDevice, s));
 		C->tbegin("minmax", 4.0 * nvox);
 		launch_minmax(in, nvox, C->d_small + SM_RANGE, s);
 		C->tend();
 	}
-	uint32_t *r = C->fetch(SM_RANGE, 2);
+	const uint32_t *r = known_range ? known_range : C->fetch(SM_RANGE, 2);
 	float mn = ord2f_host(r[0]), mx = ord2f_host(r[1]);
 	if (mn == mx) {
 		printf("Error: No variability in image intensity.\n");
@@ -2712,13 +2726,11 @@ int n2m_ctx_pipeline(n2m_ctx *C, const float *d_img, const int dim[3], const n2m
 			n2m_params P = *prm;
 			P.isolevel = isolevels[k];
 			float *v = D->buf<float>(S_VOL, nvox); // written whole by the apply stage
-			if (smoothed)
-				HIPX(hipMemcpyAsync(D->d_small + SM_RANGE, range, sizeof range, hipMemcpyHostToDevice, D->stream));
 			Mesh &m = C->meshes[k];
 			m.nt = m.np = 0;
 			VolOut vo;
 			if (volume_stages(D, v, dim, P, smoothed, k < nmask ? masks + k * nwords : nullptr, vo,
-			                  smoothed ? sm : d_img) &&
+			                  smoothed ? sm : d_img, smoothed ? range : nullptr) &&
 			    surface_stages(D, v, dim, vo, P, m))
 				done[k] = 1;
 			D->sync();

@@ -12,7 +12,8 @@
 namespace n2m {
 
 // ---------------------------------------------------------------------------
-// Exclusive scan of uint32 (multi-level: reduce, scan the partials, apply).
+// Exclusive scan of uint32: one workgroup for one tile, one pass with
+// decoupled look-back up to SP_MAX_TILES tiles, else three levels.
 // ---------------------------------------------------------------------------
 constexpr int SC_BS = 256, SC_IPT = 8, SC_TILE = SC_BS * SC_IPT;
 
@@ -57,6 +58,18 @@ k_scan_apply(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, size_t
 		*total = run;
 }
 
+// Scans of up to SP_MAX_TILES tiles run in one pass (k_scan_1p); longer ones
+// reduce, scan the partial sums and apply (three launches: there the serial
+// chain of look-backs costs more than the two extra launches).  The first
+// SP_WORDS words of the scratch belong to the single-pass scan: two counters
+// (tickets handed out, workgroups finished), then one 64-bit word per tile
+// (status << 32 | sum).  They are zero between scans: the caller zeroes them
+// once when the scratch is allocated, and the last workgroup of every
+// single-pass scan clears what that scan used.  The partial sums of the
+// multi-level form follow them.
+constexpr int SP_MAX_TILES = 128, SP_WORDS = 320; // SP_WORDS >= 2 + 2 * SP_MAX_TILES
+static_assert(SP_WORDS >= 2 + 2 * SP_MAX_TILES, "single-pass scan scratch");
+
 size_t scan_scratch_words(size_t n)
 {
 	size_t w = 0;
@@ -64,7 +77,97 @@ size_t scan_scratch_words(size_t n)
 		n = (n + SC_TILE - 1) / SC_TILE;
 		w += n;
 	}
-	return w + 1;
+	return SP_WORDS + w + 1;
+}
+size_t scan_scratch_zero_words() { return SP_WORDS; }
+
+constexpr uint32_t SP_AGG = 1u, SP_INC = 2u; // tile status: own sum / inclusive prefix
+constexpr uint32_t SP_SPIN_MAX = 1u << 20;   // bound on the polls of one predecessor
+
+// Exclusive scan in one pass (decoupled look-back): each workgroup takes the
+// next tile number from a counter, so every tile below it belongs to a
+// workgroup that has already started; it publishes its tile's sum, adds the
+// sums (or the inclusive prefix, where published) of the tiles before it,
+// and publishes its inclusive prefix.  A predecessor's sum is published
+// before that workgroup waits for anything, so every wait ends.
+__global__ void __launch_bounds__(SC_BS)
+k_scan_1p(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, uint32_t *scr, uint32_t ntiles)
+{
+	__shared__ uint32_t scratch[SC_BS / N2M_WAVE + 1];
+	__shared__ uint32_t s_tile, s_prefix, s_last;
+	uint32_t *ctr = scr;
+	unsigned long long *flags = reinterpret_cast<unsigned long long *>(scr + 2);
+	if (threadIdx.x == 0)
+		s_tile = atomicAdd(&ctr[0], 1u);
+	__syncthreads();
+	const uint32_t tile = s_tile;
+	const size_t base = tile * (size_t)SC_TILE + (size_t)threadIdx.x * SC_IPT;
+	uint32_t v[SC_IPT], sum = 0;
+#pragma unroll
+	for (int k = 0; k < SC_IPT; k++) {
+		const size_t i = base + k;
+		v[k] = i < n ? in[i] : 0u;
+		sum += v[k];
+	}
+	uint32_t agg;
+	uint32_t run = block_excl_scan<SC_BS>(sum, scratch, &agg);
+	// the look-back, by the first wave: lane l reads the word of tile
+	// j0 - l, so one round trip covers 64 predecessors; it stops at the
+	// nearest tile with an inclusive prefix (tile 0 always has one)
+	if (threadIdx.x < N2M_WAVE) {
+		const int lane = (int)threadIdx.x;
+		uint32_t prefix = 0;
+		if (tile > 0) {
+			if (lane == 0)
+				__hip_atomic_store(&flags[tile], ((unsigned long long)SP_AGG << 32) | agg, __ATOMIC_RELAXED,
+				                   __HIP_MEMORY_SCOPE_AGENT);
+			for (long long j0 = (long long)tile - 1;; j0 -= N2M_WAVE) {
+				const long long j = j0 - lane;
+				unsigned long long f;
+				uint32_t spins = 0;
+				bool all_ready;
+				do {
+					f = j >= 0 ? __hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+					           : ((unsigned long long)SP_INC << 32);
+					all_ready = __all((f >> 32) != 0);
+				} while (!all_ready && ++spins < SP_SPIN_MAX);
+				const unsigned long long inc = __ballot((f >> 32) == SP_INC);
+				const int stop = inc ? __ffsll((long long)inc) - 1 : N2M_WAVE - 1;
+				prefix += wave_sum<uint32_t>(lane <= stop ? (uint32_t)f : 0u);
+				if (inc || !all_ready)
+					break;
+			}
+		}
+		if (lane == 0) {
+			__hip_atomic_store(&flags[tile], ((unsigned long long)SP_INC << 32) | (prefix + agg), __ATOMIC_RELAXED,
+			                   __HIP_MEMORY_SCOPE_AGENT);
+			s_prefix = prefix;
+		}
+	}
+	__syncthreads();
+	run += s_prefix;
+#pragma unroll
+	for (int k = 0; k < SC_IPT; k++) {
+		const size_t i = base + k;
+		if (i < n)
+			out[i] = run;
+		run += v[k];
+	}
+	if (total && tile == ntiles - 1 && threadIdx.x == SC_BS - 1)
+		*total = run;
+	// the last workgroup to finish clears the counters and the tile words:
+	// every other one has finished its look-back by then
+	if (threadIdx.x == 0)
+		s_last = atomicAdd(&ctr[1], 1u) == ntiles - 1;
+	__syncthreads();
+	if (s_last) {
+		for (uint32_t t = threadIdx.x; t < ntiles; t += SC_BS)
+			flags[t] = 0ull;
+		if (threadIdx.x == 0) {
+			ctr[0] = 0u;
+			ctr[1] = 0u;
+		}
+	}
 }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
@@ -94,6 +197,23 @@ __global__ void __launch_bounds__(SC_SBS) k_scan_sums(uint32_t *__restrict__ a, 
 	}
 }
 
+// the multi-level form; sums: scan_scratch_words(n) - SP_WORDS words
+static void scan_levels(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, uint32_t *sums, hipStream_t s)
+{
+	const size_t nb = (n + SC_TILE - 1) / SC_TILE;
+	if (nb == 1) {
+		hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SC_BS), 0, s, in, out, n, (const uint32_t *)nullptr, total);
+		return;
+	}
+	hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_BS), 0, s, in, n, sums);
+	if (nb <= (size_t)SC_SMALL)
+		hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_SBS), 0, s, sums, (uint32_t)nb);
+	else
+		scan_levels(sums, sums, nb, nullptr, sums + nb, s);
+	hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SC_BS), 0, s, in, out, n, (const uint32_t *)sums,
+	                   total);
+}
+
 void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, uint32_t *scratch,
                    hipStream_t s)
 {
@@ -102,19 +222,12 @@ void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total,
 			hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, total);
 		return;
 	}
-	size_t nb = (n + SC_TILE - 1) / SC_TILE;
-	if (nb == 1) {
-		hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SC_BS), 0, s, in, out, n, (const uint32_t *)nullptr, total);
-		return;
-	}
-	uint32_t *sums = scratch;
-	hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_BS), 0, s, in, n, sums);
-	if (nb <= (size_t)SC_SMALL)
-		hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_SBS), 0, s, sums, (uint32_t)nb);
+	const size_t nb = (n + SC_TILE - 1) / SC_TILE;
+	if (nb > 1 && nb <= (size_t)SP_MAX_TILES)
+		hipLaunchKernelGGL(k_scan_1p, dim3((unsigned)nb), dim3(SC_BS), 0, s, in, out, n, total, scratch,
+		                   (uint32_t)nb);
 	else
-		scan_excl_u32(sums, sums, nb, nullptr, scratch + nb, s);
-	hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SC_BS), 0, s, in, out, n, (const uint32_t *)sums,
-	                   total);
+		scan_levels(in, out, n, total, scratch + SP_WORDS, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -205,6 +205,7 @@ int lw_seg_len();
 
 // k_mesh.hip
 size_t scan_scratch_words(size_t n);
+size_t scan_scratch_zero_words(); // leading words kept zero between scans
 void scan_excl_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total, uint32_t *scratch,
                    hipStream_t s);
 size_t radix_counts_words(size_t n);
