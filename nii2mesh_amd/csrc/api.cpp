@@ -343,9 +343,27 @@ struct n2m_ctx {
 	uint8_t *h_stage = nullptr;
 	size_t stage_cap = 0, stage_used = 0;
 
+	// Waits for the stream.  It polls for up to kSpinNs first: a blocking
+	// wait sleeps until an interrupt and wakes some tens of microseconds
+	// after the work ends, and most waits here are the host round trips of
+	// a few small counts, which end sooner than that.
+	static constexpr long long kSpinNs = 200000;
 	void sync()
 	{
-		HIPX(hipStreamSynchronize(stream));
+		timespec t0, t;
+		clock_gettime(CLOCK_MONOTONIC, &t0);
+		for (;;) {
+			const hipError_t e = hipStreamQuery(stream);
+			if (e == hipSuccess)
+				break;
+			if (e != hipErrorNotReady)
+				HIPX(e);
+			clock_gettime(CLOCK_MONOTONIC, &t);
+			if ((t.tv_sec - t0.tv_sec) * 1000000000LL + (t.tv_nsec - t0.tv_nsec) > kSpinNs) {
+				HIPX(hipStreamSynchronize(stream));
+				break;
+			}
+		}
 		stage_used = 0;
 	}
 	void check() { HIPX(hipGetLastError()); }
