@@ -1376,21 +1376,22 @@ __device__ __forceinline__ void stage_halo(double *lds, const d3 *__restrict__ p
 	__syncthreads();
 }
 
-// row s of an LDS array of W-double rows (W = 4: x, y as one 16-byte read),
-// or, for W = 2, of the split form of NR rows: NR 16-byte (x, y) pairs, then
-// NR z values.  Random rows of the split form are read about a third faster
-// than 32-byte rows: its 16-byte reads fall on every bank group, those of
-// 32-byte rows on half of them.
+// row s of an LDS array of 3-double rows (W = 3), or, for W = 2, of the
+// split form of NR rows: NR 16-byte (x, y) pairs, then the z values at a
+// 16-byte stride (so that both reads share one address).  Random rows of
+// the split form are read about a third faster than 32-byte (x, y, z, pad)
+// rows: its 16-byte reads fall on every bank group, those of 32-byte rows
+// on half of them.
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) dbl2 lds_dbl2;
 template <int W, int NR = 0> __device__ __forceinline__ d3 lds_row_w(const lds_double *L, uint32_t s)
 {
-	if constexpr (W == 4 || W == 2) {
-		const dbl2 xy = ((const lds_dbl2 *)L)[W == 4 ? 2 * s : s];
+	if constexpr (W == 2) {
+		const dbl2 xy = ((const lds_dbl2 *)L)[s];
 		d3 r;
 		r.x = xy.x;
 		r.y = xy.y;
-		r.z = W == 4 ? L[4 * s + 2] : L[2 * NR + 2 * s];
+		r.z = L[2 * NR + 2 * s];
 		return r;
 	} else {
 		return lds_row(L, s);
