@@ -22,6 +22,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <exception>
 #include <utility>
 #include <vector>
 
@@ -302,6 +303,14 @@ struct n2m_ctx {
 	hipEvent_t cur_a = nullptr;
 	double cur_bytes = 0;
 	std::vector<n2m_ctx *> workers; // per-isolevel contexts of the pipeline's threaded meshing
+	// large copies between pageable host memory and the device (xfer_big):
+	// one stream, two pinned chunk buffers and two events per host thread
+	struct XferLane {
+		hipStream_t st = nullptr;
+		uint8_t *buf[2] = {nullptr, nullptr};
+		hipEvent_t ev[2] = {nullptr, nullptr};
+	};
+	std::vector<XferLane> xfer;
 	void *scan_zero_p = nullptr;    // the scan scratch allocation last zeroed
 	size_t scan_zero_cap = 0;
 
@@ -530,8 +539,117 @@ void ctx_free(n2m_ctx *c)
 		(void)hipHostFree(c->h_thr);
 	if (c->h_stage)
 		(void)hipHostFree(c->h_stage);
+	for (auto &x : c->xfer) {
+		if (x.st)
+			(void)hipStreamSynchronize(x.st);
+		for (int b = 0; b < 2; b++) {
+			if (x.buf[b])
+				(void)hipHostFree(x.buf[b]);
+			if (x.ev[b])
+				(void)hipEventDestroy(x.ev[b]);
+		}
+		if (x.st)
+			(void)hipStreamDestroy(x.st);
+	}
 	(void)hipStreamDestroy(c->stream);
 	delete c;
+}
+
+// ---------------------------------------------------------------------------
+// Large copies between pageable host memory and the device.  A plain copy
+// from or into pageable memory runs at 10-15 GB/s here: its host side is
+// one thread, and the pages of freshly allocated memory fault in on first
+// touch.  Pinned DMA runs at about 54 GB/s, and a host copy spread over
+// eight threads keeps up with it.  So the range is split over kXferLanes host
+// threads, each with its own stream and two pinned chunk buffers: the DMA of
+// one chunk overlaps the host copy of the other.  Synchronous: the copy is
+// complete on return.  Work queued on the context's stream before the call
+// is ordered before it.
+// ---------------------------------------------------------------------------
+constexpr size_t kXferMin = (size_t)32 << 20, kXferChunk = (size_t)4 << 20;
+constexpr int kXferLanes = 8;
+
+void xfer_lanes(n2m_ctx *C, int n)
+{
+	while ((int)C->xfer.size() < n) {
+		n2m_ctx::XferLane x;
+		HIPX(hipStreamCreateWithFlags(&x.st, hipStreamNonBlocking));
+		C->xfer.push_back(x); // owned by the context from here (freed by ctx_free)
+		n2m_ctx::XferLane &y = C->xfer.back();
+		for (int b = 0; b < 2; b++) {
+			HIPX(hipHostMalloc((void **)&y.buf[b], kXferChunk, hipHostMallocDefault));
+			HIPX(hipEventCreateWithFlags(&y.ev[b], hipEventDisableTiming));
+		}
+	}
+}
+
+// d2h: dst host, src device; else dst device, src host
+void xfer_big(n2m_ctx *C, void *dst, const void *src, size_t bytes, bool d2h)
+{
+	const hipMemcpyKind kind = d2h ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+	if (bytes < kXferMin) {
+		HIPX(hipMemcpyAsync(dst, src, bytes, kind, C->stream));
+		C->sync();
+		return;
+	}
+	unsigned hc = std::thread::hardware_concurrency();
+	const int nl = (int)std::min<size_t>({(size_t)kXferLanes, hc ? (size_t)hc : 1, bytes / kXferChunk});
+	xfer_lanes(C, nl);
+	hipEvent_t ready = C->ev(); // the context's queued work comes first
+	HIPX(hipEventRecord(ready, C->stream));
+	// lane k copies [k * per, min((k + 1) * per, bytes)), per a multiple of the chunk
+	const size_t per = ((bytes + nl - 1) / nl + kXferChunk - 1) / kXferChunk * kXferChunk;
+	std::vector<std::exception_ptr> err(nl);
+	auto lane = [&](int k) {
+		try {
+			if (k > 0) // a new host thread starts on device 0
+				HIPX(hipSetDevice(C->device));
+			n2m_ctx::XferLane &x = C->xfer[k];
+			const size_t a = (size_t)k * per, b = std::min(bytes, a + per);
+			if (a >= b)
+				return;
+			HIPX(hipStreamWaitEvent(x.st, ready, 0));
+			const size_t nc = (b - a + kXferChunk - 1) / kXferChunk;
+			auto len = [&](size_t c) { return std::min(kXferChunk, b - a - c * kXferChunk); };
+			auto off = [&](size_t c) { return a + c * kXferChunk; };
+			if (d2h) {
+				auto issue = [&](size_t c) {
+					HIPX(hipMemcpyAsync(x.buf[c & 1], (const uint8_t *)src + off(c), len(c), kind, x.st));
+					HIPX(hipEventRecord(x.ev[c & 1], x.st));
+				};
+				issue(0);
+				if (nc > 1)
+					issue(1);
+				for (size_t c = 0; c < nc; c++) {
+					HIPX(hipEventSynchronize(x.ev[c & 1]));
+					memcpy((uint8_t *)dst + off(c), x.buf[c & 1], len(c));
+					if (c + 2 < nc)
+						issue(c + 2);
+				}
+			} else {
+				for (size_t c = 0; c < nc; c++) {
+					if (c >= 2) // the buffer's previous chunk has left it
+						HIPX(hipEventSynchronize(x.ev[c & 1]));
+					memcpy(x.buf[c & 1], (const uint8_t *)src + off(c), len(c));
+					HIPX(hipMemcpyAsync((uint8_t *)dst + off(c), x.buf[c & 1], len(c), kind, x.st));
+					HIPX(hipEventRecord(x.ev[c & 1], x.st));
+				}
+				HIPX(hipStreamSynchronize(x.st));
+			}
+		} catch (...) {
+			err[k] = std::current_exception();
+		}
+	};
+	std::vector<std::thread> th;
+	for (int k = 1; k < nl; k++)
+		th.emplace_back(lane, k);
+	lane(0);
+	for (auto &t : th)
+		t.join();
+	C->evpool.push_back(ready);
+	for (auto &e : err)
+		if (e)
+			std::rethrow_exception(e);
 }
 
 // one context per host thread for the drop-in entry points (meshify is
@@ -1384,7 +1502,9 @@ template <typename T> T *host_copy_out(n2m_ctx *C, const void *d, size_t count)
 	T *h = (T *)malloc(count ? count * sizeof(T) : 1);
 	if (!h)
 		fail("out of host memory");
-	if (count)
+	if (count * sizeof(T) >= kXferMin)
+		xfer_big(C, h, d, count * sizeof(T), true);
+	else if (count)
 		HIPX(hipMemcpyAsync(h, d, count * sizeof(T), hipMemcpyDeviceToHost, C->stream));
 	return h;
 }
@@ -2901,10 +3021,9 @@ int n2m_ctx_download(n2m_ctx *C, int mesh, vec3i *t, vec3d *p)
 			fail("no mesh %d", mesh);
 		Mesh &m = C->meshes[mesh];
 		if (m.nt)
-			HIPX(hipMemcpyAsync(t, m.tris.p, (size_t)12 * m.nt, hipMemcpyDeviceToHost, C->stream));
+			xfer_big(C, t, m.tris.p, (size_t)12 * m.nt, true);
 		if (m.np)
-			HIPX(hipMemcpyAsync(p, m.pts.p, (size_t)24 * m.np, hipMemcpyDeviceToHost, C->stream));
-		C->sync();
+			xfer_big(C, p, m.pts.p, (size_t)24 * m.np, true);
 		return EXIT_SUCCESS;
 	});
 }
@@ -2984,7 +3103,7 @@ int n2m_atlas(const float *img, short dim[3], int originalMC, bool preSmooth, bo
 		const int d[3] = {dim[0], dim[1], dim[2]};
 		const size_t nvox = (size_t)d[0] * d[1] * d[2];
 		float *dl = C->buf<float>(S_AT_LABELS, nvox);
-		HIPX(hipMemcpyAsync(dl, img, 4 * nvox, hipMemcpyHostToDevice, C->stream));
+		xfer_big(C, dl, img, 4 * nvox, false);
 		n2m_params P{cubes_flavour(), originalMC, 0.5f, preSmooth, 0, fillBubbles, verbose};
 		int rc = atlas_run(C, dl, d, P, postSmooth, srow, false);
 		const int n = (int)C->atlas.size();
@@ -3185,7 +3304,7 @@ int meshify(float *img, short dim[3], int originalMC, float isolevel, vec3i **t,
 		int d[3] = {dim[0], dim[1], dim[2]};
 		const size_t nvox = (size_t)d[0] * d[1] * d[2];
 		float *src = C->buf<float>(S_VOL, nvox);
-		HIPX(hipMemcpyAsync(src, img, 4 * nvox, hipMemcpyHostToDevice, C->stream));
+		xfer_big(C, src, img, 4 * nvox, false);
 		n2m_params P{cubes_flavour(), originalMC, isolevel, preSmooth, onlyLargest, fillBubbles, verbose};
 		Mesh m;
 		struct Free {
@@ -3200,7 +3319,7 @@ int meshify(float *img, short dim[3], int originalMC, float isolevel, vec3i **t,
 		} fr{m};
 		bool ok = meshify_device(C, src, src, d, P, m);
 		// the reference leaves img smoothed, filled and darkened
-		HIPX(hipMemcpyAsync(img, src, 4 * nvox, hipMemcpyDeviceToHost, C->stream));
+		xfer_big(C, img, src, 4 * nvox, true);
 		if (!ok) {
 			C->sync();
 			return EXIT_FAILURE;
@@ -3224,12 +3343,11 @@ void laplacian_smoothHC(vec3d *verts, vec3i *tris, int nvert, int ntri, double a
 		n2m_ctx *C = tl_ctx.get();
 		double *dp = C->buf<double>(S_PTS1, 3 * (size_t)nvert);
 		int *dt = C->buf<int>(S_TRIS1, 3 * (size_t)(ntri > 0 ? ntri : 1));
-		HIPX(hipMemcpyAsync(dp, verts, (size_t)24 * nvert, hipMemcpyHostToDevice, C->stream));
+		xfer_big(C, dp, verts, (size_t)24 * nvert, false);
 		if (ntri > 0)
-			HIPX(hipMemcpyAsync(dt, tris, (size_t)12 * ntri, hipMemcpyHostToDevice, C->stream));
+			xfer_big(C, dt, tris, (size_t)12 * ntri, false);
 		smooth_hc_device(C, dp, dt, nvert, ntri, alpha, beta, iter, lockEdges);
-		HIPX(hipMemcpyAsync(verts, dp, (size_t)24 * nvert, hipMemcpyDeviceToHost, C->stream));
-		C->sync();
+		xfer_big(C, verts, dp, (size_t)24 * nvert, true);
 		return 0;
 	});
 }
