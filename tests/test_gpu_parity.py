@@ -285,6 +285,26 @@ def test_hc_halo_overflow_falls_back(gpu, oracle):
     assert np.array_equal(c, d)
 
 
+def test_hc_large_host_transfer(gpu, oracle):
+    """laplacian_smoothHC on host arrays of more than 32 MiB each (a
+    1205 x 1203 grid with a rough height field: 1.45 M vertices, 2.89 M
+    triangles, sizes no multiple of the 4 MiB staging chunk): the vertices
+    and triangles go up, and the vertices come back, through the pinned
+    multi-threaded staging; the result is the oracle's, bit for bit."""
+    nx, ny = 1205, 1203
+    rng = np.random.default_rng(23)
+    y, x = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    pts = np.ascontiguousarray(np.stack([x, y, rng.normal(size=(ny, nx)) * 0.3], -1).reshape(-1, 3))
+    v = np.arange(nx * ny, dtype=np.int32).reshape(ny, nx)
+    a0, b0, c0, d0 = v[:-1, :-1].ravel(), v[:-1, 1:].ravel(), v[1:, :-1].ravel(), v[1:, 1:].ravel()
+    tris = np.ascontiguousarray(np.concatenate([np.stack([a0, b0, d0], 1), np.stack([a0, d0, c0], 1)]), np.int32)
+    assert pts.nbytes > (32 << 20) and tris.nbytes > (32 << 20) and pts.nbytes % (4 << 20) != 0
+    a, b = pts.copy(), pts.copy()
+    gpu.lib().laplacian_smoothHC(a, tris, 0.1, 0.5, 2, True)
+    oracle.laplacian_smoothHC(b, tris, 0.1, 0.5, 2, True)
+    assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize('case', ['no_iterations', 'no_triangles', 'one_triangle', 'isolated_vertices',
                                   'ragged_blocks'])
 def test_hc_small_and_ragged(gpu, oracle, case):
