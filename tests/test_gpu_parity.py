@@ -101,6 +101,27 @@ def test_unify_and_degenerate(gpu, oracle, shape, flavour):
     assert _eq(d1, d2)
 
 
+@pytest.mark.parametrize('n', [2048, 2049, 128 * 2048, 128 * 2048 + 1])
+def test_unify_scan_tile_boundaries(gpu, oracle, n):
+    """Weld and degenerate removal on n points and n triangles at the scan's
+    size boundaries: one 2048-word tile, two tiles, the largest single-pass
+    scan (128 tiles) and the smallest three-level one (129 tiles).  Points
+    on a 40^3 integer lattice, so most of them are welded."""
+    import nii2mesh_amd.abi as abi
+    rng = np.random.default_rng(n)
+    pts = np.ascontiguousarray(rng.integers(0, 40, size=(n, 3)).astype(np.float64))
+    tris = np.ascontiguousarray(rng.integers(0, n, size=(n, 3)), np.int32)
+    L = gpu.load()
+    gu = abi.bind_unify(L, 'n2m_unify_vertices')
+    gd = abi.bind_degenerate(L, 'n2m_remove_degenerate_triangles')
+    p1, t1 = gu(pts, tris)
+    p2, t2 = oracle.unify(pts, tris)
+    assert len(p1) < n and _eq(p1, p2) and _eq(t1, t2)
+    d1 = gd(p1, t1)
+    d2 = oracle.degenerate(p2, t2)
+    assert _eq(d1, d2)
+
+
 @pytest.mark.parametrize('first_nan', [False, True])
 def test_unify_nan_rows(gpu, oracle, first_nan):
     """Weld with vertices at NaN positions: each is a class of its own (its
