@@ -567,7 +567,7 @@ void ctx_free(n2m_ctx *c)
 // is ordered before it.
 // ---------------------------------------------------------------------------
 constexpr size_t kXferMin = (size_t)32 << 20, kXferChunk = (size_t)4 << 20;
-constexpr int kXferLanes = 8;
+constexpr int kXferLanes = 8, kMaxXferLanes = 32; // N2M_XFER_LANES overrides the default
 
 void xfer_lanes(n2m_ctx *C, int n)
 {
@@ -593,7 +593,9 @@ void xfer_big(n2m_ctx *C, void *dst, const void *src, size_t bytes, bool d2h)
 		return;
 	}
 	unsigned hc = std::thread::hardware_concurrency();
-	const int nl = (int)std::min<size_t>({(size_t)kXferLanes, hc ? (size_t)hc : 1, bytes / kXferChunk});
+	const char *el = getenv("N2M_XFER_LANES");
+	const size_t want = el && atoi(el) > 0 ? (size_t)atoi(el) : (size_t)kXferLanes;
+	const int nl = (int)std::min<size_t>({want, (size_t)kMaxXferLanes, hc ? (size_t)hc : 1, bytes / kXferChunk});
 	xfer_lanes(C, nl);
 	hipEvent_t ready = C->ev(); // the context's queued work comes first
 	HIPX(hipEventRecord(ready, C->stream));
