@@ -1233,6 +1233,17 @@ bool meshify_device(n2m_ctx *C, const float *src, float *vol, const int dim[3], 
 	return surface_stages(C, vol, dim, vo, P, m);
 }
 
+// Vertex renumbering used by the smoothers: Morton order of the positions.
+// perm[new] = old.  Returns false (identity) for small meshes.  The
+// relabelled triangles (tris) are written by build_csr's count pass, from
+// src (the triangles as given) and inv (old -> new).
+struct Relabel {
+	bool on = false;
+	const uint32_t *perm = nullptr, *inv = nullptr;
+	const int *src = nullptr;
+	int *tris = nullptr;
+};
+
 // CSR of a device mesh (tris int[3*nt]) for smoothing
 struct Csr {
 	uint32_t *start;
@@ -1242,7 +1253,9 @@ struct Csr {
 
 // sorted_ref: ref holds each vertex's corners in ascending order (by
 // default only the pairs are ordered)
-Csr build_csr(n2m_ctx *C, const int *tris, uint32_t nv, uint32_t nt, bool sorted_ref = false)
+// With R (on), tris is R->tris, which the count pass writes from R->src.
+Csr build_csr(n2m_ctx *C, const int *tris, uint32_t nv, uint32_t nt, bool sorted_ref = false,
+              const Relabel *R = nullptr)
 {
 	hipStream_t s = C->stream;
 	Csr c;
@@ -1252,8 +1265,12 @@ Csr build_csr(n2m_ctx *C, const int *tris, uint32_t nv, uint32_t nt, bool sorted
 	c.pairs = C->buf<int2>(S_HC_PAIRS, (size_t)3 * nt + 1);
 	uint32_t *rank = C->buf<uint32_t>(S_HC_RANK, (size_t)3 * nt + 1);
 	HIPX(hipMemsetAsync(cnt, 0, 4 * ((size_t)nv + 1), s));
-	C->tbegin("csr_build", 12.0 * nt * 4 + 8.0 * nv);
-	launch_csr_count(tris, nt, cnt, rank, s);
+	const bool rl = R && R->on;
+	C->tbegin("csr_build", 12.0 * nt * 4 + 8.0 * nv + (rl ? 12.0 * nt : 0.0));
+	if (rl)
+		launch_csr_count(R->src, nt, cnt, rank, s, R->inv, R->tris);
+	else
+		launch_csr_count(tris, nt, cnt, rank, s);
 	scan_excl_u32(cnt, c.start, nv + 1, nullptr, scan_scratch(C, nv + 1), s);
 	launch_csr_fill(tris, nt, c.start, rank, c.ref, s);
 	launch_csr_pairs(tris, nv, c.start, c.ref, c.pairs, s, sorted_ref);
@@ -1280,21 +1297,13 @@ void check_tris(const int *tris, int ntri, int nvert)
 			fail("triangle %d references vertex %d (mesh has %d vertices)", i / 3, tris[i], nvert);
 }
 
-// Vertex renumbering used by the smoothers: Morton order of the positions.
-// perm[new] = old.  Returns false (identity) for small meshes.
-struct Relabel {
-	bool on = false;
-	const uint32_t *perm = nullptr;
-	const int *tris = nullptr;
-};
-
 Relabel morton_relabel(n2m_ctx *C, const double *pts, const int *tris, uint32_t nv, uint32_t nt)
 {
 	Relabel r;
 	if (nv < 32768)
 		return r;
 	hipStream_t s = C->stream;
-	C->tbegin("hc_relabel", 24.0 * nv * 2 + 16.0 * 4 * nv + 24.0 * nt);
+	C->tbegin("hc_relabel", 24.0 * nv * 2 + 16.0 * 4 * nv);
 	uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
 	HIPX(hipMemcpyAsync(C->d_small + SM_PBOX, init, sizeof init, hipMemcpyHostToDevice, s));
 	launch_pts_bbox(pts, nv, C->d_small + SM_PBOX, s);
@@ -1306,10 +1315,10 @@ Relabel morton_relabel(n2m_ctx *C, const double *pts, const int *tris, uint32_t 
 	r.perm = in1 ? v1 : v0;
 	uint32_t *inv = C->buf<uint32_t>(S_HC_INV, nv);
 	launch_invert_perm(r.perm, nv, inv, s);
-	int *t2 = C->buf<int>(S_HC_TRIS, (size_t)3 * (nt ? nt : 1));
-	launch_relabel_tris(tris, inv, nt, t2, s);
 	C->tend();
-	r.tris = t2;
+	r.inv = inv;
+	r.src = tris;
+	r.tris = C->buf<int>(S_HC_TRIS, (size_t)3 * (nt ? nt : 1)); // written by build_csr
 	r.on = true;
 	return r;
 }
@@ -1405,7 +1414,7 @@ void smooth_hc_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uin
 	hipStream_t s = C->stream;
 	Relabel R = morton_relabel(C, pts, tris, nv, nt);
 	const int *T = R.on ? R.tris : tris;
-	Csr c = build_csr(C, T, nv, nt);
+	Csr c = build_csr(C, T, nv, nt, false, &R);
 	// one launch per iteration (two-ring form) where every block's rings fit,
 	// else the halo pair of launches, else the window pair
 	const size_t n3 = 3 * (size_t)nv;
@@ -1466,7 +1475,7 @@ void laplace_device(n2m_ctx *C, double *pts, const int *tris, uint32_t nv, uint3
 		return;
 	hipStream_t s = C->stream;
 	Relabel R = morton_relabel(C, pts, tris, nv, nt);
-	Csr c = build_csr(C, R.on ? R.tris : tris, nv, nt);
+	Csr c = build_csr(C, R.on ? R.tris : tris, nv, nt, false, &R);
 	const Halo H = build_halo(C, c, nv, nt);
 	const size_t n3 = 3 * (size_t)nv;
 	double *p = C->buf<double>(S_HC_P, n3), *q = C->buf<double>(S_HC_PN, n3);

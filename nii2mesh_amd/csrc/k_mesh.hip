@@ -928,10 +928,18 @@ void launch_compact_tris(const int *tris, const uint32_t *keep, const uint32_t *
 // run of equal vertices takes its ranks with one atomic: consecutive
 // triangles share vertices, so this cuts the atomics about 2.5 times.
 __global__ void __launch_bounds__(256)
-k_csr_count(const int *__restrict__ tris, uint32_t nt, uint32_t *__restrict__ cnt, uint32_t *__restrict__ rank)
+k_csr_count(const int *__restrict__ tris, uint32_t nt, uint32_t *__restrict__ cnt, uint32_t *__restrict__ rank,
+            const uint32_t *__restrict__ inv, int *__restrict__ tout)
 {
 	const uint32_t n = 3 * nt, i = blockIdx.x * blockDim.x + threadIdx.x, lane = lane_id();
-	uint32_t key = i < n ? (uint32_t)tris[i] : 0xffffffffu, src = lane; // vertex ids are < 2^32 - 1
+	uint32_t key = 0xffffffffu, src = lane; // vertex ids are < 2^32 - 1
+	if (i < n) {
+		key = (uint32_t)tris[i];
+		if (inv) { // the relabelling of the triangles, written on the way
+			key = inv[key];
+			tout[i] = (int)key;
+		}
+	}
 	for (uint32_t k = 2; k <= N2M_WAVE; k <<= 1)
 		for (uint32_t j = k >> 1; j > 0; j >>= 1) {
 			const uint32_t ok = __shfl_xor(key, j, N2M_WAVE), os = __shfl_xor(src, j, N2M_WAVE);
@@ -2124,9 +2132,10 @@ void launch_long_border(const int *tris, uint32_t nv, const uint32_t *start, con
 	hipLaunchKernelGGL(k_huge_border, dim3(64), dim3(HG_BS), 0, s, tris, start, ref, literal, border,
 	                   (const uint32_t *)huge);
 }
-void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, uint32_t *rank, hipStream_t s)
+void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, uint32_t *rank, hipStream_t s,
+                      const uint32_t *inv, int *tout)
 {
-	hipLaunchKernelGGL(k_csr_count, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, cnt, rank);
+	hipLaunchKernelGGL(k_csr_count, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, nt, cnt, rank, inv, tout);
 }
 void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, const uint32_t *rank, uint32_t *ref,
                      hipStream_t s)
@@ -2363,14 +2372,6 @@ __global__ void k_invert_perm(const uint32_t *__restrict__ perm, uint32_t n, uin
 		inv[perm[i]] = i;
 }
 
-__global__ void k_relabel_tris(const int *__restrict__ tris, const uint32_t *__restrict__ inv, uint32_t n3,
-                               int *__restrict__ out)
-{
-	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n3)
-		out[i] = (int)inv[tris[i]];
-}
-
 // 1 if any triangle repeats a vertex (then the general border rule applies)
 __global__ void k_tris_repeat(const int *__restrict__ tris, uint32_t nt, uint32_t *flag)
 {
@@ -2445,10 +2446,6 @@ void launch_morton(const double *pts, uint32_t n, const uint32_t *bb, uint32_t *
 void launch_invert_perm(const uint32_t *perm, uint32_t n, uint32_t *inv, hipStream_t s)
 {
 	hipLaunchKernelGGL(k_invert_perm, dim3(g1(n)), dim3(256), 0, s, perm, n, inv);
-}
-void launch_relabel_tris(const int *tris, const uint32_t *inv, uint32_t nt, int *out, hipStream_t s)
-{
-	hipLaunchKernelGGL(k_relabel_tris, dim3(g1((size_t)3 * nt)), dim3(256), 0, s, tris, inv, 3 * nt, out);
 }
 void launch_tris_repeat(const int *tris, uint32_t nt, uint32_t *flag, hipStream_t s)
 {

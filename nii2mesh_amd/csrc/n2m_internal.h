@@ -252,7 +252,10 @@ void launch_compact_tris(const int *tris, const uint32_t *keep, const uint32_t *
                          hipStream_t s);
 // CSR of the references: count (with each corner's arrival rank), scan,
 // fill at start + rank, then sort each vertex's list and expand to pairs
-void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, uint32_t *rank, hipStream_t s);
+// inv given: counts the references of inv[tris[i]] and writes those ids to
+// tout (the relabelled triangles)
+void launch_csr_count(const int *tris, uint32_t nt, uint32_t *cnt, uint32_t *rank, hipStream_t s,
+                      const uint32_t *inv = nullptr, int *tout = nullptr);
 void launch_csr_fill(const int *tris, uint32_t nt, const uint32_t *start, const uint32_t *rank, uint32_t *ref,
                      hipStream_t s);
 void launch_csr_pairs(const int *tris, uint32_t nv, const uint32_t *start, uint32_t *ref, int2 *pairs,
@@ -306,7 +309,6 @@ void launch_pts_bbox(const double *pts, uint32_t n, uint32_t *bb, hipStream_t s)
 void launch_morton(const double *pts, uint32_t n, const uint32_t *bb, uint32_t *keys, uint32_t *vals,
                    hipStream_t s);
 void launch_invert_perm(const uint32_t *perm, uint32_t n, uint32_t *inv, hipStream_t s);
-void launch_relabel_tris(const int *tris, const uint32_t *inv, uint32_t nt, int *out, hipStream_t s);
 void launch_tris_repeat(const int *tris, uint32_t nt, uint32_t *flag, hipStream_t s);
 void launch_border_pairs(const uint32_t *start, const int2 *pairs, uint32_t nv, uint8_t *border, hipStream_t s);
 // high-valence vertices (more than 16 references)
